@@ -1,0 +1,180 @@
+"""espec_amd -- MI355X (gfx950) wave-packet propagation behind eSPec's interfaces.
+
+Python host mirror of the C ABI in include/espec_hip.h.  Each function keeps
+the name, argument meaning and error behaviour of the eSPec subroutine it
+replaces (PSPOFFT, PSODFFT, PLNZ, FFT, AU, LNZ, DSTEVX, LANCZS, FCORREL,
+HWHM, SPECTRUMTD, ...): arrays are split real/imaginary float64 vectors,
+times in fs, and a nonzero INFO is raised as ``EspecError``.
+
+The library is loaded from espec_amd/libespec_hip.so (built in-tree by
+``python -m espec_amd.build``).  There is no CPU fallback: if the library or
+a GPU is missing, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libespec_hip.so")
+
+FP64, MIXED = 0x0, 0x1
+EXACT, REF = 0x0, 0x2
+SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
+AMU = 1836.152667539    # electron masses per amu (espec.f:614)
+A0A = 0.5291772083      # angstrom per bohr (espec.f:67)
+
+_lib = None
+
+
+class EspecError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        msg = _load().espec_strerror(code).decode() if _lib else str(code)
+        super().__init__(f"{where}: INFO={code} ({msg})")
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} not built; run `python -m espec_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    vp, dp, ip = C.c_void_p, C.c_double, C.c_int
+    P = C.POINTER
+    L.espec_strerror.restype = C.c_char_p
+    L.espec_strerror.argtypes = [ip]
+    L.espec_version.restype = C.c_char_p
+    L.espec_device_count.restype = ip
+    prop = [ip, i, dp, dp, dp, ip, d, d, d, d, d, d, d, vp, ip, P(ip), ip]
+    L.espec_pspofft.argtypes = prop
+    L.espec_digt.argtypes = [ip, i, d, d, d, P(dp), ip]
+    L.espec_spo_plan_create.argtypes = [P(vp), ip, i, ip, dp, ip, d, d, d, d, vp, ip, ip]
+    L.espec_spo_plan_run.argtypes = [vp, ip]
+    L.espec_spo_plan_sync.argtypes = [vp]
+    L.espec_spo_plan_get.argtypes = [vp, ip, vp, vp, vp, ip, P(ip), vp, vp]
+    L.espec_spo_plan_last_timing.argtypes = [vp, P(dp), P(ip)]
+    L.espec_spo_plan_destroy.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def lib():
+    return _load()
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _np3(np_):
+    a = np.ones(3, dtype=np.int32)
+    a[: len(np_)] = np_
+    return a
+
+
+def _v3(x, fill=1.0):
+    a = np.full(3, fill, dtype=np.float64)
+    a[: len(x)] = x
+    return a
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise EspecError(rc, where)
+
+
+def nsteps(ti, tf, dt):
+    """Trip count of DO T=TI+DT,TF+DT,DT (pspofft.f:110)."""
+    c = ((tf + dt) - (ti + dt) + dt) / dt
+    return max(int(c), 0)
+
+
+def device_count():
+    return _load().espec_device_count()
+
+
+def pspofft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT):
+    """PSPOFFT (pspofft.f:1-150): returns dict(cr, ci, norm, u, v).
+
+    cr/ci are the LNZVC(NC,MPR)/LNZVC(NC,MPI) columns: sample 0 is C(TI),
+    then one sample every lc steps."""
+    L = _load()
+    maxs = nsteps(ti, tf, dt) // lc + 1
+    cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
+    u, v = _f64(u1).copy(), _f64(v1).copy()
+    ns = C.c_int(0)
+    rc = L.espec_pspofft(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v,
+                         _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags)
+    _check(rc, "PSPOFFT")
+    k = ns.value
+    return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
+
+
+def digt(np_, sh, sm, flags=EXACT):
+    """DIGT (dkeft.f:204): kinetic grid p^2/2m and B = 1/N."""
+    L = _load()
+    tc = np.zeros(int(np.prod(np_)))
+    b = C.c_double()
+    _check(L.espec_digt(len(np_), _np3(np_), _v3(sh), _v3(sm), tc, C.byref(b), flags), "DIGT")
+    return tc, b.value
+
+
+class SpoPlan:
+    """Device-resident batch of split-operator packets on one grid (no
+    reference counterpart: replaces the job farm of tools&scrips/parallel.x)."""
+
+    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=MIXED):
+        L = self._L = _load()
+        vpot = _f64(vpot)
+        n = int(np.prod(np_))
+        self.n, self.np = n, list(np_)
+        self.npk = vpot.size // n
+        self.maxsamp = maxsamp
+        self._h = C.c_void_p()
+        self._u = _f64(u)
+        self._v = _f64(v) if v is not None else None
+        rc = L.espec_spo_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, dt, lc,
+                                     _v3(sh), _v3(sm), vpot.ravel(), self._u.ravel(),
+                                     self._v.ctypes.data if self._v is not None else None,
+                                     maxsamp, flags)
+        _check(rc, "spo_plan_create")
+
+    def run(self, nsteps):
+        _check(self._L.espec_spo_plan_run(self._h, nsteps), "spo_plan_run")
+
+    def sync(self):
+        _check(self._L.espec_spo_plan_sync(self._h), "spo_plan_sync")
+
+    def timing(self):
+        ms, nl = C.c_double(), C.c_int()
+        _check(self._L.espec_spo_plan_last_timing(self._h, C.byref(ms), C.byref(nl)), "timing")
+        return ms.value, nl.value
+
+    def get(self, pk=0, psi=True):
+        cr, ci, nr = np.zeros(self.maxsamp), np.zeros(self.maxsamp), np.zeros(self.maxsamp)
+        u, v = np.zeros(self.n), np.zeros(self.n)
+        ns = C.c_int()
+        _check(self._L.espec_spo_plan_get(self._h, pk, cr.ctypes.data, ci.ctypes.data,
+                                          nr.ctypes.data, self.maxsamp, C.byref(ns),
+                                          u.ctypes.data if psi else None,
+                                          v.ctypes.data if psi else None), "spo_plan_get")
+        k = ns.value
+        out = dict(cr=cr[:k], ci=ci[:k], norm=nr[:k])
+        if psi:
+            out.update(u=u, v=v)
+        return out
+
+    def close(self):
+        if self._h:
+            self._L.espec_spo_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,337 @@
+// Split-operator plans and the PSPOFFT / DIGT entry points.
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "api_common.hpp"
+#include "spo_lds.hpp"
+#include "spo_pass.hpp"
+
+using namespace espec;
+
+struct espec_spo_plan {
+    int ndim = 1, np[3] = {1, 1, 1}, n = 0, npk = 0, lc = 1, maxsamp = 0, flags = 0;
+    int nt = 64;
+    bool mixed = false, ref = false, lds = true;
+    double dt = 0, b = 1;
+    long long steps_done = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf x, psi0, vph, kin, tw, perm, cbuf;
+    SpoLdsArgs la{};
+    SpoPassPlan pp{};  // multi-pass plan for grids that do not fit in LDS
+    float last_ms = 0;
+    int last_launches = 0;
+
+    ~espec_spo_plan()
+    {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+        spo_pass_release(pp);
+    }
+};
+
+namespace {
+
+void fill_axis(DevAxis &d, const FftPlan &p, int es, int nl, int lmod, int la, int lb, int tw0)
+{
+    d.n = p.n;
+    d.ln = ilog2(p.n);
+    d.nst = p.nst;
+    d.es = es;
+    d.nl = nl;
+    d.lnl = ilog2(nl);
+    d.lmod = lmod;
+    d.la = la;
+    d.lb = lb;
+    d.tw0 = tw0;
+    for (int s = 0; s < p.nst; ++s) {
+        d.rad[s] = p.rad[s];
+        d.lm[s] = p.lm[s];
+        d.lL[s] = p.lL[s];
+        d.off[s] = p.off[s];
+    }
+}
+
+// Line geometry of axis a of an ndim grid (x fastest).
+void axis_geometry(int ndim, const int *np, int a, int &es, int &nl, int &lmod, int &la, int &lb)
+{
+    int n = ntot(ndim, np);
+    es = 1;
+    for (int k = 0; k < a; ++k) es *= np[k];
+    nl = n / np[a];
+    lmod = es;          // lines inside one slab of the lower axes
+    la = 1;
+    lb = es * np[a];    // jump over the transformed axis
+}
+
+template <typename R>
+int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const double *vpot,
+                 const double *u, const double *v)
+{
+    const int n = P.n, npk = P.npk;
+    const double cst1 = P.dt / kShfs, cst2 = P.dt / (2.0 * kShfs);
+    std::vector<FftPlan> fp;
+    std::vector<R> tw;
+    std::vector<int> perm;
+    for (int a = 0; a < P.ndim; ++a) {
+        fp.push_back(make_fft_plan(P.np[a], P.ref));
+        int es, nl, lmod, la, lb;
+        axis_geometry(P.ndim, P.np, a, es, nl, lmod, la, lb);
+        fill_axis(P.la.ax[a], fp[a], es, nl, lmod, la, lb, (int)(tw.size() / 2));
+        for (size_t i = 0; i < fp[a].twc.size(); ++i) {
+            tw.push_back((R)fp[a].twc[i]);
+            tw.push_back((R)fp[a].tws[i]);
+        }
+        P.la.perm_off[a] = (int)perm.size();
+        perm.insert(perm.end(), fp[a].perm.begin(), fp[a].perm.end());
+    }
+    // Kinetic phase: natural order for the reference kernel; for the fused
+    // kernel, position of frequency k is sum_a perm_a[k_a] * stride_a.
+    std::vector<double> tc = kinetic_grid(P.ndim, P.np, sh, sm, P.ref);
+    std::vector<double> tcs(n);
+    if (P.ref || !P.lds) {
+        tcs = tc;
+    } else {
+        for (int k = 0; k < n; ++k) {
+            int r = k, pos = 0, stride = 1;
+            for (int a = 0; a < P.ndim; ++a) {
+                pos += fp[a].perm[r % P.np[a]] * stride;
+                r /= P.np[a];
+                stride *= P.np[a];
+            }
+            tcs[pos] = tc[k];
+        }
+    }
+    std::vector<R> kin = phase_table<R>(tcs.data(), n, cst1);
+    std::vector<R> vph((size_t)2 * n * npk);
+    for (int p = 0; p < npk; ++p) {
+        std::vector<R> t = phase_table<R>(vpot + (size_t)p * n, n, cst2);
+        std::memcpy(vph.data() + (size_t)2 * n * p, t.data(), sizeof(R) * 2 * n);
+    }
+    std::vector<R> xs((size_t)2 * n * npk);
+    for (int p = 0; p < npk; ++p) {
+        std::vector<R> t = interleave<R>(u + (size_t)p * n, v ? v + (size_t)p * n : nullptr, n);
+        std::memcpy(xs.data() + (size_t)2 * n * p, t.data(), sizeof(R) * 2 * n);
+    }
+    ESPEC_HIP_TRY(P.x.upload(xs.data(), sizeof(R) * xs.size()));
+    ESPEC_HIP_TRY(P.psi0.upload(xs.data(), sizeof(R) * xs.size()));
+    ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(R) * vph.size()));
+    ESPEC_HIP_TRY(P.kin.upload(kin.data(), sizeof(R) * kin.size()));
+    ESPEC_HIP_TRY(P.tw.upload(tw.data(), sizeof(R) * (tw.size() ? tw.size() : 2)));
+    ESPEC_HIP_TRY(P.perm.upload(perm.data(), sizeof(int) * perm.size()));
+    ESPEC_HIP_TRY(P.cbuf.alloc(sizeof(double) * 3 * (size_t)P.maxsamp * npk));
+    ESPEC_HIP_TRY(hipMemset(P.cbuf.p, 0, P.cbuf.bytes));
+    if (!P.lds) {
+        std::vector<std::vector<double>> taxes;
+        for (int a = 0; a < P.ndim; ++a) taxes.push_back(kinetic_axis(P.np[a], sh[a], sm[a], false));
+        return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, P.x.p, P.psi0.p,
+                              P.vph.p, P.cbuf.as<double>(), P.maxsamp);
+    }
+    return ESPEC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *espec_strerror(int code)
+{
+    switch (code) {
+    case ESPEC_OK: return "ok";
+    case ESPEC_DRIFT: return "energy or norm drift above TOL";
+    case ESPEC_ERR_ARG: return "invalid argument";
+    case ESPEC_ERR_HIP: return "HIP runtime error";
+    case ESPEC_ERR_NOMEM: return "device allocation failed";
+    case ESPEC_ERR_DIAG: return "tridiagonal eigensolve failed";
+    case ESPEC_ERR_UNSUPPORTED: return "unsupported on the device";
+    default: return "unknown error";
+    }
+}
+
+int espec_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+const char *espec_version(void) { return "espec-mi355x 0.1 (gfx950)"; }
+
+int espec_digt(int ndim, const int *np, const double *sh, const double *sm, double *tc,
+               double *b, int flags)
+{
+    if (ndim < 1 || ndim > 3 || !np || !sh || !sm || !tc) return ESPEC_ERR_ARG;
+    std::vector<double> t = kinetic_grid(ndim, np, sh, sm, (flags & ESPEC_REF) != 0);
+    std::memcpy(tc, t.data(), sizeof(double) * t.size());
+    if (b) *b = 1.0 / ntot(ndim, np);
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk, double dt,
+                          int lc, const double *sh, const double *sm, const double *vpot,
+                          const double *u, const double *v, int maxsamp, int flags)
+{
+    if (!out || ndim < 1 || ndim > 3 || !np || npk < 1 || lc < 1 || !sh || !sm || !vpot || !u ||
+        maxsamp < 1)
+        return ESPEC_ERR_ARG;
+    *out = nullptr;
+    for (int a = 0; a < ndim; ++a)
+        if (!is_pow2(np[a]) || np[a] < 8 || np[a] > 65536) return ESPEC_ERR_ARG;
+    std::unique_ptr<espec_spo_plan> P(new espec_spo_plan);
+    P->ndim = ndim;
+    for (int a = 0; a < ndim; ++a) P->np[a] = np[a];
+    P->n = ntot(ndim, np);
+    P->npk = npk;
+    P->dt = dt;
+    P->lc = lc;
+    P->maxsamp = maxsamp;
+    P->flags = flags;
+    P->mixed = (flags & ESPEC_MIXED) != 0;
+    P->ref = (flags & ESPEC_REF) != 0;
+    if (P->ref && P->mixed) return ESPEC_ERR_UNSUPPORTED;
+    P->b = 1.0 / P->n;
+    P->nt = P->n <= 1024 ? 64 : 256;
+    P->lds = spo_lds_bytes(P->n, P->mixed, P->ref, P->nt) <= 160 * 1024 - 1024;
+    if (!P->lds && P->ref) return ESPEC_ERR_UNSUPPORTED;
+    ESPEC_HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    ESPEC_HIP_TRY(hipEventCreate(&P->ev0));
+    ESPEC_HIP_TRY(hipEventCreate(&P->ev1));
+    int rc = P->mixed ? build_tables<float>(*P, sh, sm, vpot, u, v)
+                      : build_tables<double>(*P, sh, sm, vpot, u, v);
+    if (rc) return rc;
+    SpoLdsArgs &a = P->la;
+    a.ndim = ndim;
+    a.n = P->n;
+    a.npk = npk;
+    a.tw = P->tw.p;
+    a.perm = P->perm.as<int>();
+    a.x = P->x.p;
+    a.psi0 = P->psi0.p;
+    a.vph = P->vph.p;
+    a.kin = P->kin.p;
+    a.b = P->b;
+    a.lc = lc;
+    a.maxsamp = maxsamp;
+    a.cbuf = P->cbuf.as<double>();
+    // C(TI): sample 0 (pspofft.f:86-90).
+    ESPEC_HIP_TRY(launch_spo_sample(P->x.p, P->psi0.p, P->n, npk, P->mixed, a.cbuf, maxsamp, 0,
+                                    P->stream));
+    ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+    *out = P.release();
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_run(espec_spo_plan *P, int nsteps)
+{
+    if (!P || nsteps < 0) return ESPEC_ERR_ARG;
+    if (nsteps == 0) return ESPEC_OK;
+    ESPEC_HIP_TRY(hipEventRecord(P->ev0, P->stream));
+    int launches = 0;
+    if (P->lds) {
+        // Keep single launches short: about 2^24 point-steps per packet.
+        const int chunk = std::max(1, (1 << 24) / P->n);
+        int done = 0;
+        while (done < nsteps) {
+            int k = std::min(chunk, nsteps - done);
+            P->la.nsteps = k;
+            P->la.step0 = P->steps_done + done;
+            ESPEC_HIP_TRY(launch_spo_lds(P->la, P->mixed, P->ref, P->nt, P->stream));
+            ++launches;
+            done += k;
+        }
+    } else {
+        int rc = spo_pass_run(P->pp, nsteps, P->steps_done, P->lc, P->stream, &launches);
+        if (rc) return rc;
+    }
+    ESPEC_HIP_TRY(hipEventRecord(P->ev1, P->stream));
+    P->steps_done += nsteps;
+    P->last_launches = launches;
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_sync(espec_spo_plan *P)
+{
+    if (!P) return ESPEC_ERR_ARG;
+    ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, P->ev0, P->ev1) == hipSuccess) P->last_ms = ms;
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_last_timing(espec_spo_plan *P, double *ms, int *launches)
+{
+    if (!P) return ESPEC_ERR_ARG;
+    if (ms) *ms = P->last_ms;
+    if (launches) *launches = P->last_launches;
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_get(espec_spo_plan *P, int pk, double *cr, double *ci, double *nrm,
+                       int maxsamp, int *nsamp, double *u, double *v)
+{
+    if (!P || pk < 0 || pk >= P->npk) return ESPEC_ERR_ARG;
+    ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+    long long have = P->steps_done / P->lc + 1;
+    int ns = (int)std::min<long long>(have, std::min(maxsamp, P->maxsamp));
+    if (nsamp) *nsamp = ns;
+    if (ns > 0 && (cr || ci || nrm)) {
+        std::vector<double> c((size_t)3 * ns);
+        ESPEC_HIP_TRY(hipMemcpy(c.data(), P->cbuf.as<double>() + (size_t)3 * P->maxsamp * pk,
+                                sizeof(double) * c.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < ns; ++i) {
+            if (cr) cr[i] = c[3 * i];
+            if (ci) ci[i] = c[3 * i + 1];
+            if (nrm) nrm[i] = c[3 * i + 2];
+        }
+    }
+    if (u || v) {
+        const size_t n = P->n;
+        if (P->mixed) {
+            std::vector<float> h(2 * n);
+            ESPEC_HIP_TRY(hipMemcpy(h.data(), P->x.as<float>() + 2 * n * pk, sizeof(float) * 2 * n,
+                                    hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) {
+                if (u) u[i] = h[2 * i];
+                if (v) v[i] = h[2 * i + 1];
+            }
+        } else {
+            std::vector<double> h(2 * n);
+            ESPEC_HIP_TRY(hipMemcpy(h.data(), P->x.as<double>() + 2 * n * pk, sizeof(double) * 2 * n,
+                                    hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) {
+                if (u) u[i] = h[2 * i];
+                if (v) v[i] = h[2 * i + 1];
+            }
+        }
+    }
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_destroy(espec_spo_plan *P)
+{
+    if (P) {
+        (void)hipStreamSynchronize(P->stream);
+        delete P;
+    }
+    return ESPEC_OK;
+}
+
+int espec_pspofft(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                  const double *sh, const double *sm, double *u1, double *v1, const double *vpot,
+                  double *cr, double *ci, double *nrm, int maxsamp, int *nsamp, int flags)
+{
+    if (!u1 || !v1 || !cr || !ci || maxsamp < 1) return ESPEC_ERR_ARG;
+    // DO T=TI+DT,TF+DT,DT (pspofft.f:110).
+    const int nsteps = fortran_trip(ti + dt, tf + dt, dt);
+    espec_spo_plan *P = nullptr;
+    int rc = espec_spo_plan_create(&P, ndim, np, 1, dt, lc, sh, sm, vpot, u1, v1, maxsamp, flags);
+    if (rc) return rc;
+    rc = espec_spo_plan_run(P, nsteps);
+    if (!rc) rc = espec_spo_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u1, v1);
+    espec_spo_plan_destroy(P);
+    return rc;
+}
+
+}  // extern "C"

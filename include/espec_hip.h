@@ -1,0 +1,193 @@
+/*
+ * espec_hip.h -- C ABI of libespec_hip.so, the MI355X (gfx950) wave-packet
+ * propagation library that stands in for eSPec's hot-path subroutines.
+ *
+ * Every entry point takes plain host pointers and sizes (no device or
+ * framework types) so that the Fortran host can bind it with
+ * ISO_C_BINDING.  The arrays follow the reference's conventions: psi is
+ * split into real and imaginary REAL*8 vectors U, V; multi-dimensional
+ * grids are flattened with x fastest (gvpot.f:80); times are in fs and
+ * converted with 2.4188843243e-2 fs per a.u. (pspofft.f:76); masses in
+ * electron masses; lengths in bohr.
+ *
+ * Return value: 0 on success (INFO = 0); 1 reproduces the reference's
+ * drift flag (pspofft.f:137-142); >= 2 are library errors, see
+ * espec_strerror().  No entry point ever exits the process.
+ *
+ * Flags select the arithmetic:
+ *   ESPEC_FP64   complex128 state, FP64 everything;
+ *   ESPEC_MIXED  complex64 state and FFT/stencil arithmetic, FP64 norm,
+ *                C(t), Lanczos alpha/beta and Gram coefficients;
+ *   ESPEC_REF    the reference's exact semantics (f32-rounded 2*pi in the
+ *                FFT twiddles, the accumulated kinetic grid, the
+ *                non-unitary kinetic and Lanczos recombination updates)
+ *                so that FP64 results match the reference build;
+ *   ESPEC_EXACT  the same algorithms with those defects removed.
+ */
+#ifndef ESPEC_HIP_H
+#define ESPEC_HIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESPEC_FP64  0x0
+#define ESPEC_MIXED 0x1
+#define ESPEC_EXACT 0x0
+#define ESPEC_REF   0x2
+
+enum {
+    ESPEC_OK = 0,
+    ESPEC_DRIFT = 1,          /* energy/norm drift above TOL (reference INFO=1) */
+    ESPEC_ERR_ARG = 2,        /* invalid argument (size, dimension, radix) */
+    ESPEC_ERR_HIP = 3,        /* HIP runtime failure */
+    ESPEC_ERR_NOMEM = 4,      /* device allocation failed */
+    ESPEC_ERR_DIAG = 5,       /* tridiagonal eigensolve failed (sil.f:118-121) */
+    ESPEC_ERR_UNSUPPORTED = 6 /* combination not implemented on the device */
+};
+
+const char *espec_strerror(int code);
+/* Number of HIP devices the library sees (0 when no GPU). */
+int espec_device_count(void);
+/* Library version string. */
+const char *espec_version(void);
+
+/* ---------------------------------------------------------------------
+ * Split-operator FFT propagation.  Replaces PSPOFFT (pspofft.f:1-3),
+ * PSPOFFTAUX (spofft.f:3-4), DIGT (dkeft.f:204) and the C(t) sampling of
+ * FCORREL (fcorrel.f:1) inside the time loop.
+ *   ndim, np[ndim]     grid (DIM .1D/.2D/.3D and NP); powers of two
+ *   dt, ti, tf         time step, initial and final time in fs; the step
+ *                      count is the reference REAL DO trip count of
+ *                      DO T=TI+DT,TF+DT,DT (pspofft.f:110)
+ *   lc                 C(t) every lc steps (LC argument)
+ *   sh[ndim], sm[ndim] grid step (bohr) and mass (m_e) per axis
+ *   u1, v1 [N]         in: psi(TI); out: psi after the last step
+ *   vpot [N]           potential (hartree)
+ *   cr, ci [maxsamp]   out: C(t) samples, sample 0 = C(TI); these are the
+ *                      LNZVC(NC,MPR) / LNZVC(NC,MPI) columns of the reference
+ *   nrm [maxsamp]      out (may be NULL): sum |psi|^2 at each sample
+ *   nsamp              out: samples written
+ * ------------------------------------------------------------------- */
+int espec_pspofft(int ndim, const int *np, double dt, double ti, double tf,
+                  int lc, const double *sh, const double *sm, double *u1,
+                  double *v1, const double *vpot, double *cr, double *ci,
+                  double *nrm, int maxsamp, int *nsamp, int flags);
+
+/* Replaces PSODFFT (psodfft.f:1-3) / PSODFFTAUX (sodfft.f:3-4): second-
+ * order-differencing leapfrog with the FFT kinetic operator, including
+ * the two half-strength start-up steps (psodfft.f:120-123).  Arguments as
+ * espec_pspofft; u1/v1 return psi at the last step. */
+int espec_psodfft(int ndim, const int *np, double dt, double ti, double tf,
+                  int lc, const double *sh, const double *sm, double *u1,
+                  double *v1, const double *vpot, double *cr, double *ci,
+                  double *nrm, int maxsamp, int *nsamp, int flags);
+
+/* Replaces FFT(A,B,NTOT,N,NSPAN,ISN) (fft3d.f:1): one variate of an in-place
+ * multivariate transform, isn > 0 -> exp(+i ...), unnormalised; n a power
+ * of two.  Runs on the device. */
+int espec_fft(double *a, double *b, int ntot, int n, int nspan, int isn,
+              int flags);
+
+/* Replaces DIGT (dkeft.f:204): kinetic grid tc[N] = p^2/(2m) on the FFT
+ * momentum grid and b = 1/N (host-side table; ESPEC_REF reproduces the
+ * accumulated 1-D grid with 2*pi = 6.283185307). */
+int espec_digt(int ndim, const int *np, const double *sh, const double *sm,
+               double *tc, double *b, int flags);
+
+/* ---------------------------------------------------------------------
+ * Short-iterative Lanczos propagation.  Replaces PLNZ (plnz.f:1-4) with
+ * PLNZAUX (sil.f:3-5), LNZ (lnz.f:4-5), REORT (reort.f:1), the DSTEVX
+ * tridiagonal eigensolve (dstevx.f:5) and the DGEMV projections.
+ *   mp      Krylov order (MP), 2..32
+ *   abstol  DSTEVX tolerance: <= 0 selects the QL path (dsteqr.f), > 0 the
+ *           bisection path whose eigenvalues are midpoints of Sturm
+ *           intervals of width < abstol (dstebz.f)
+ *   shm[ndim] FD kinetic prefactor SHM per axis (espec.f:630)
+ * ------------------------------------------------------------------- */
+int espec_plnz(int ndim, const int *np, int mp, double abstol, double dt,
+               double ti, double tf, int lc, const double *shm, double *u,
+               double *v, const double *vpot, double *cr, double *ci,
+               double *nrm, int maxsamp, int *nsamp, int flags);
+
+/* Replaces AU(DIM,N,NP,SHM,VPOT,U,WORK,VAR) (au.f:3) -> AU_1D (au1d.f:1),
+ * AU_2D (au2di.f:1), AU_3D (au3d.f:1): hu = H u, 4th-order FD kinetic with
+ * Dirichlet truncation plus V.  Runs on the device in FP64. */
+int espec_au(int ndim, const int *np, const double *shm, const double *vpot,
+             const double *u, double *hu);
+
+/* Replaces LNZ (lnz.f:4): m Lanczos steps from start with one classical
+ * Gram-Schmidt pass (REORT) against the first nreort vectors.  al[m],
+ * bt[m]; basis (n x nreort, column-major) receives the Lanczos vectors. */
+int espec_lnz(int ndim, const int *np, int m, int nreort, const double *shm,
+              const double *vpot, const double *start, double *al,
+              double *bt, double *basis);
+
+/* Replaces the DSTEVX('V','A',...) call of sil.f:82 for an m x m
+ * symmetric tridiagonal (d[m], e[m-1]); w ascending, z column-major.
+ * Runs on the device (one wavefront). */
+int espec_dstevx(int m, const double *d, const double *e, double abstol,
+                 double *w, double *z);
+
+/* Replaces LANCZS (lanczs.f:1-3): eigenpairs il..iu (1-based) of the FD
+ * Hamiltonian from the RAN start vector (rans.f:3) with an n-step Lanczos
+ * and partial reorthogonalisation against nreort vectors; eigenvectors
+ * rebuilt by replaying the recurrence.  eigvc is n x (iu-il+1). */
+int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort,
+                 const double *shm, const double *vpot, double *eigvl,
+                 double *eigvc);
+
+/* Replaces FCORREL(N,U1,U2) (fcorrel.f:1): sum u1*u2 in FP64 on device. */
+int espec_fcorrel(int n, const double *u1, const double *u2, double *out);
+
+/* ---------------------------------------------------------------------
+ * Correlation -> spectrum (espec.f:1546-1594).
+ * espec_assemble_corr: espec.f:1546-1566 mirror of 2^(mf-1) samples into
+ *   the 2^mf-point C(t) on t = -TF + (i-1)*DTF.
+ * espec_hwhm: HWHM (hwhm.f:1).
+ * espec_spectrumtd: SPECTRUMTD (spectrumtd.f:1): iwin 1 = .SG window,
+ *   0 = none; e/w/wi receive the printed (E, W, FFTi) columns in print
+ *   order; the transform is FFTNT (cfft.f:6) on the device.
+ * espec_lc_rule: LC and the *CHANGE final-time rule (espec.f:1324-1353).
+ * ------------------------------------------------------------------- */
+int espec_assemble_corr(int mf, const double *cr, const double *ci,
+                        double *u, double *v);
+int espec_hwhm(int mf, double dt, double tf, double tal, double de,
+               double *u, double *v);
+int espec_spectrumtd(int iwin, int mf, double dt, double tf, double wp,
+                     double de, double *u, double *v, double *e, double *w,
+                     double *wi, int flags);
+int espec_lc_rule(double ti, double *tf, double dt, int mf, int change,
+                  int *lc);
+
+/* ---------------------------------------------------------------------
+ * Device-resident batched split-operator plans (no reference counterpart:
+ * they replace the job farm of tools&scrips/parallel.x, which runs one
+ * espec.x per packet).  A plan owns device copies of psi, psi0, the phase
+ * tables and the C(t) buffer for npk packets on the same grid; run()
+ * advances all of them nsteps with inputs already resident in HBM.
+ *   vpot  [npk][N], u/v [npk][N]  host arrays copied once at create
+ * ------------------------------------------------------------------- */
+typedef struct espec_spo_plan espec_spo_plan;
+int espec_spo_plan_create(espec_spo_plan **plan, int ndim, const int *np,
+                          int npk, double dt, int lc, const double *sh,
+                          const double *sm, const double *vpot,
+                          const double *u, const double *v, int maxsamp,
+                          int flags);
+/* Enqueue nsteps steps on the plan's stream (asynchronous). */
+int espec_spo_plan_run(espec_spo_plan *plan, int nsteps);
+int espec_spo_plan_sync(espec_spo_plan *plan);
+/* Copy out C(t) of packet pk (samples taken so far) and its psi. */
+int espec_spo_plan_get(espec_spo_plan *plan, int pk, double *cr, double *ci,
+                       double *nrm, int maxsamp, int *nsamp, double *u,
+                       double *v);
+/* Kernel time (ms) of the last run() measured with HIP events on the
+ * plan's stream, and the number of kernel launches it made. */
+int espec_spo_plan_last_timing(espec_spo_plan *plan, double *ms,
+                               int *launches);
+int espec_spo_plan_destroy(espec_spo_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

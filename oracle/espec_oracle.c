@@ -1,0 +1,993 @@
+/*
+ * espec_oracle.c -- CPU FP64 restatement of the eSPec wave-packet hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product); see
+ * espec_oracle.h for the contract and the two modes.  Written from the
+ * mathematics of each reference routine; the steps that must stay alike
+ * (loop order of the split-operator step, sampling rule, Lanczos
+ * recurrence order) follow the cited reference lines so that rounding
+ * stays within 1e-12 of the reference build.
+ */
+#include "espec_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double TWOPI_EXACT = 6.283185307179586476925287;
+
+/* fft3d.f:119 RAD=6.2831853071796 is a default-REAL literal: f32. */
+static double fft_rad(int mode)
+{
+    return mode == ORC_REF ? (double)(float)6.2831853071796 : TWOPI_EXACT;
+}
+
+static int ntot_of(int ndim, const int *np)
+{
+    int n = 1;
+    for (int i = 0; i < ndim; ++i) n *= np[i];
+    return n;
+}
+
+/* ------------------------------------------------------------------ FFT */
+
+/* fft3d.f:124-173: square factors of 16 -> 4, then for a remaining 8 a
+ * square factor 2, the square-free middle factor, then the square factors
+ * mirrored. */
+int orc_singleton_factors(int n, int *fac)
+{
+    int sq[40], ns = 0, k = n, mid = 0, nf = 0;
+    while (k % 16 == 0) { sq[ns++] = 4; k /= 16; }
+    if (k > 4) {              /* k == 8 */
+        sq[ns++] = 2;
+        k /= 4;
+        mid = k;
+    } else if (k != 1) {
+        mid = k;
+    }
+    for (int i = 0; i < ns; ++i) fac[nf++] = sq[i];
+    if (mid) fac[nf++] = mid;
+    for (int i = ns - 1; i >= 0; --i) fac[nf++] = sq[i];
+    return nf;
+}
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* Decimation-in-frequency transform of one contiguous line (re, im) of
+ * length n with Singleton's factor order.  Stage s of length L applies
+ * the radix-r DFT to x[b + j + p*m], m = L/r, then multiplies output q by
+ * the twiddle of angle q*j*RAD/L (fft3d.f:175-177, 420-449).  Singleton's
+ * radix-2 stage reuses the twiddle of j for m-j through cos -> -cos
+ * (fft3d.f:206-211); with the f32 RAD that reflection is not the same
+ * angle, so ORC_REF keeps it.  The digit-reversed result is permuted to
+ * natural order (fft3d.f:800-950). */
+static void dif_line(double *re, double *im, int n, int isn, int mode,
+                     double *tr, double *ti)
+{
+    int fac[40];
+    int nf = orc_singleton_factors(n, fac);
+    double rad = fft_rad(mode);
+    if (isn < 0) rad = -rad;
+    int L = n;
+    for (int s = 0; s < nf; ++s) {
+        int r = fac[s], m = L / r;
+        double phi = rad / (double)L;
+        for (int j = 0; j < m; ++j) {
+            double c1, s1, c2 = 1, s2 = 0, c3 = 1, s3 = 0;
+            if (r == 2 && mode == ORC_REF && 2 * j > m) {
+                c1 = -cos((double)(m - j) * phi);
+                s1 = sin((double)(m - j) * phi);
+            } else {
+                c1 = cos((double)j * phi);
+                s1 = sin((double)j * phi);
+            }
+            if (r == 4) {
+                c2 = cos(2.0 * j * phi); s2 = sin(2.0 * j * phi);
+                c3 = cos(3.0 * j * phi); s3 = sin(3.0 * j * phi);
+            }
+            for (int b = 0; b < n; b += L) {
+                int k0 = b + j;
+                if (r == 2) {
+                    int k1 = k0 + m;
+                    double ar = re[k0] - re[k1], ai = im[k0] - im[k1];
+                    re[k0] += re[k1];
+                    im[k0] += im[k1];
+                    re[k1] = c1 * ar - s1 * ai;
+                    im[k1] = s1 * ar + c1 * ai;
+                } else { /* r == 4 */
+                    int k1 = k0 + m, k2 = k1 + m, k3 = k2 + m;
+                    double akpr = re[k0] + re[k2], akmr = re[k0] - re[k2];
+                    double akpi = im[k0] + im[k2], akmi = im[k0] - im[k2];
+                    double ajpr = re[k1] + re[k3], ajmr = re[k1] - re[k3];
+                    double ajpi = im[k1] + im[k3], ajmi = im[k1] - im[k3];
+                    double x1r, x1i, x3r, x3i;
+                    if (isn >= 0) {
+                        x1r = akmr - ajmi; x1i = akmi + ajmr;
+                        x3r = akmr + ajmi; x3i = akmi - ajmr;
+                    } else {
+                        x1r = akmr + ajmi; x1i = akmi - ajmr;
+                        x3r = akmr - ajmi; x3i = akmi + ajmr;
+                    }
+                    double x2r = akpr - ajpr, x2i = akpi - ajpi;
+                    re[k0] = akpr + ajpr;
+                    im[k0] = akpi + ajpi;
+                    re[k1] = x1r * c1 - x1i * s1; im[k1] = x1r * s1 + x1i * c1;
+                    re[k2] = x2r * c2 - x2i * s2; im[k2] = x2r * s2 + x2i * c2;
+                    re[k3] = x3r * c3 - x3i * s3; im[k3] = x3r * s3 + x3i * c3;
+                }
+            }
+        }
+        L = m;
+    }
+    /* X[k] sits at sum_s q_s * n/(r_1..r_s), k = q_1 + r_1 q_2 + ... */
+    for (int k = 0; k < n; ++k) {
+        int kk = k, span = n, pos = 0;
+        for (int s = 0; s < nf; ++s) {
+            span /= fac[s];
+            pos += (kk % fac[s]) * span;
+            kk /= fac[s];
+        }
+        tr[k] = re[pos];
+        ti[k] = im[pos];
+    }
+    memcpy(re, tr, sizeof(double) * n);
+    memcpy(im, ti, sizeof(double) * n);
+}
+
+/* Direct DFT for lengths that are not powers of two (angles from the same
+ * RAD); the hot path only uses powers of two (espec.f:260-277). */
+static void dft_line(double *re, double *im, int n, int isn, int mode,
+                     double *tr, double *ti)
+{
+    double rad = fft_rad(mode);
+    if (isn < 0) rad = -rad;
+    for (int k = 0; k < n; ++k) {
+        double sr = 0, si = 0;
+        for (int j = 0; j < n; ++j) {
+            double a = rad * (double)((long)j * k % n) / (double)n;
+            double c = cos(a), s = sin(a);
+            sr += re[j] * c - im[j] * s;
+            si += re[j] * s + im[j] * c;
+        }
+        tr[k] = sr;
+        ti[k] = si;
+    }
+    memcpy(re, tr, sizeof(double) * n);
+    memcpy(im, ti, sizeof(double) * n);
+}
+
+/* fft3d.f:1 -- one variate of a multivariate transform: lines of length n
+ * whose elements are nspan/n apart. */
+void orc_fft(double *a, double *b, int ntot, int n, int nspan, int isn, int mode)
+{
+    if (n < 2) return;
+    int stride = nspan / n;
+    double *lr = malloc(sizeof(double) * 4 * n);
+    double *li = lr + n, *tr = lr + 2 * n, *ti = lr + 3 * n;
+    int nlines = ntot / n;
+    for (int l = 0; l < nlines; ++l) {
+        int base = (l % stride) + (l / stride) * nspan;
+        for (int j = 0; j < n; ++j) {
+            lr[j] = a[base + j * stride];
+            li[j] = b[base + j * stride];
+        }
+        if (is_pow2(n)) dif_line(lr, li, n, isn, mode, tr, ti);
+        else dft_line(lr, li, n, isn, mode, tr, ti);
+        for (int j = 0; j < n; ++j) {
+            a[base + j * stride] = lr[j];
+            b[base + j * stride] = li[j];
+        }
+    }
+    free(lr);
+}
+
+static void fft_all(int ndim, const int *np, double *u, double *v, int isn, int mode)
+{
+    int n = ntot_of(ndim, np), nf = 1;
+    for (int i = 0; i < ndim; ++i) {
+        nf *= np[i];
+        orc_fft(u, v, n, np[i], nf, isn, mode);
+    }
+}
+
+/* cfft.f:6-106 FFTNT: radix-2 DIF with exp(-i a), a = (IA-1)*TWOPI/N,
+ * bit reversal; itype=-1 conjugates in and out and scales by 1/N.  TWOPI
+ * is a default-REAL literal (cfft.f:35). */
+void orc_fftnt(double *x, double *y, int n, int m, int itype, int mode)
+{
+    if (n == 1) return;
+    double twopi = mode == ORC_REF ? (double)(float)6.283185307179586476925287
+                                   : TWOPI_EXACT;
+    if (itype == -1)
+        for (int i = 0; i < n; ++i) y[i] = -y[i];
+    double p = twopi / n;
+    int n2 = n;
+    for (int k = 0; k < m; ++k) {
+        int n1 = n2;
+        n2 /= 2;
+        int ie = n / n1, ia = 1;
+        for (int j = 0; j < n2; ++j) {
+            double a = (double)(ia - 1) * p;
+            double c = cos(a), s = sin(a);
+            ia += ie;
+            for (int i = j; i < n; i += n1) {
+                int l = i + n2;
+                double xt = x[i] - x[l];
+                x[i] = x[i] + x[l];
+                double yt = y[i] - y[l];
+                y[i] = y[i] + y[l];
+                x[l] = c * xt + s * yt;
+                y[l] = c * yt - s * xt;
+            }
+        }
+    }
+    /* in-place bit reversal */
+    for (int i = 0, j = 0; i < n - 1; ++i) {
+        if (i < j) {
+            double t = x[j]; x[j] = x[i]; x[i] = t;
+            t = y[j]; y[j] = y[i]; y[i] = t;
+        }
+        int k = n / 2;
+        while (k <= j) { j -= k; k /= 2; }
+        j += k;
+    }
+    if (itype == -1)
+        for (int i = 0; i < n; ++i) {
+            y[i] = -y[i] / n;
+            x[i] = x[i] / n;
+        }
+}
+
+/* ------------------------------------------------------- kinetic grid */
+
+/* dkeft.f:204-282.  ORC_REF accumulates p by repeated +DP with the
+ * 10-digit 2*pi (dkeft.f:241, 272-282) and flips the sign at N/2; the
+ * reference's 2-D/3-D branches do not fill a grid (dkeft.f:283-340), so
+ * for ndim > 1 ORC_REF builds each axis with the 1-D recipe and sums.
+ * ORC_EXACT uses p_k = (k - N [k >= N/2]) 2 pi / (N h). */
+void orc_digt(int ndim, const int *np, const double *sh, const double *sm,
+              double *tc, double *b, int mode)
+{
+    int n = ntot_of(ndim, np);
+    *b = 1.0 / n;
+    double *ax[3];
+    for (int a = 0; a < ndim; ++a) {
+        int na = np[a];
+        ax[a] = malloc(sizeof(double) * na);
+        if (mode == ORC_REF) {
+            double dp = 6.283185307 / (na * sh[a]);
+            double p1 = 0.0;
+            for (int i = 0; i < na / 2; ++i) {
+                ax[a][i] = p1 * p1 / (2.0 * sm[a]);
+                p1 = p1 + dp;
+            }
+            p1 = -p1;
+            for (int i = na / 2; i < na; ++i) {
+                ax[a][i] = p1 * p1 / (2.0 * sm[a]);
+                p1 = p1 + dp;
+            }
+        } else {
+            double dp = TWOPI_EXACT / (na * sh[a]);
+            for (int i = 0; i < na; ++i) {
+                double p = (double)(i < na / 2 ? i : i - na) * dp;
+                ax[a][i] = p * p / (2.0 * sm[a]);
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        int r = i;
+        double t = 0;
+        for (int a = 0; a < ndim; ++a) {
+            t += ax[a][r % np[a]];
+            r /= np[a];
+        }
+        tc[i] = t;
+    }
+    for (int a = 0; a < ndim; ++a) free(ax[a]);
+}
+
+/* -------------------------------------------------- propagator steps */
+
+/* spofft.f:120-133 and 178-191: V half-phase fused with the swap of the
+ * two halves of the flattened array; post != 0 also scales by b. */
+static void vphase_swap(int n, double cst2, double b, int post, double *u,
+                        double *v, const double *vpot)
+{
+    int nh = n / 2;
+    for (int i = 0; i < nh; ++i) {
+        /* spofft.f:120-133 (pre) / 178-191 (post, reading the swapped
+         * positions and scaling by b). */
+        double th1 = cst2 * vpot[post ? i + nh : i];
+        double c1 = cos(th1), s1 = sin(th1);
+        double th2 = cst2 * vpot[post ? i : i + nh];
+        double c2 = cos(th2), s2 = sin(th2);
+        double uaux, vaux;
+        if (post) {
+            uaux = c1 * b * u[i] + s1 * b * v[i];
+            vaux = c1 * b * v[i] - s1 * b * u[i];
+            u[i] = c2 * b * u[i + nh] + s2 * b * v[i + nh];
+            v[i] = c2 * b * v[i + nh] - s2 * b * u[i + nh];
+        } else {
+            uaux = c1 * u[i] + s1 * v[i];
+            vaux = c1 * v[i] - s1 * u[i];
+            u[i] = c2 * u[i + nh] + s2 * v[i + nh];
+            v[i] = c2 * v[i + nh] - s2 * u[i + nh];
+        }
+        u[i + nh] = uaux;
+        v[i + nh] = vaux;
+    }
+}
+
+void orc_spo_step(int ndim, const int *np, double b, double cst1, double cst2,
+                  double *u, double *v, const double *tc, const double *vpot,
+                  int mode)
+{
+    int n = ntot_of(ndim, np);
+    vphase_swap(n, cst2, b, 0, u, v, vpot);
+    fft_all(ndim, np, u, v, +1, mode);
+    for (int i = 0; i < n; ++i) {
+        double th = cst1 * tc[i];
+        double c = cos(th), s = sin(th);
+        if (mode == ORC_REF) {
+            /* spofft.f:152-153: V is updated from the new U. */
+            u[i] = c * u[i] + s * v[i];
+            v[i] = c * v[i] - s * u[i];
+        } else {
+            double ur = c * u[i] + s * v[i];
+            v[i] = c * v[i] - s * u[i];
+            u[i] = ur;
+        }
+    }
+    fft_all(ndim, np, u, v, -1, mode);
+    vphase_swap(n, cst2, b, 1, u, v, vpot);
+}
+
+/* sodfft.f:3-129 without absorber. */
+void orc_sod_step(int ndim, const int *np, double b, double cst, double *u1,
+                  double *u2, double *v1, double *v2, const double *tc,
+                  const double *vpot, double *wr, double *wi, int mode)
+{
+    int n = ntot_of(ndim, np), nh = n / 2;
+    for (int i = 0; i < n; ++i) {
+        wr[i] = u1[i] + cst * vpot[i] * v2[i];
+        wi[i] = v1[i] - cst * vpot[i] * u2[i];
+        u1[i] = u2[i];
+        v1[i] = v2[i];
+    }
+    for (int i = 0; i < nh; ++i) {
+        double t = u2[i]; u2[i] = u2[i + nh]; u2[i + nh] = t;
+        t = v2[i]; v2[i] = v2[i + nh]; v2[i + nh] = t;
+    }
+    fft_all(ndim, np, u2, v2, +1, mode);
+    for (int i = 0; i < n; ++i) {
+        u2[i] = tc[i] * u2[i];
+        v2[i] = tc[i] * v2[i];
+    }
+    fft_all(ndim, np, u2, v2, -1, mode);
+    for (int i = 0; i < nh; ++i) {
+        double uaux = wr[i + nh] + cst * b * v2[i];
+        double vaux = wi[i + nh] - cst * b * u2[i];
+        u2[i] = wr[i] + cst * b * v2[i + nh];
+        v2[i] = wi[i] - cst * b * u2[i + nh];
+        u2[i + nh] = uaux;
+        v2[i + nh] = vaux;
+    }
+}
+
+/* Fortran 77 trip count of DO T=M1,M2,M3: MAX(INT((M2-M1+M3)/M3),0). */
+static int trip(double m1, double m2, double m3)
+{
+    double c = (m2 - m1 + m3) / m3;
+    int k = (int)c;
+    return k > 0 ? k : 0;
+}
+
+int orc_nsteps(double ti, double tf, double dt) { return trip(ti + dt, tf + dt, dt); }
+
+double orc_fcorrel(int n, const double *a, const double *b)
+{
+    double s = 0;
+    for (int i = 0; i < n; ++i) s = s + a[i] * b[i];
+    return s;
+}
+
+double orc_ecnorm(int n, const double *a) { return sqrt(orc_fcorrel(n, a, a)); }
+
+/* eigenergfft.f:1-103 EFFT4: <V> + <T> through a forward/inverse pair on
+ * the half-swapped copy, scaled by b = 1/N. */
+double orc_efft4(int ndim, const int *np, const double *u, const double *v,
+                 const double *tc, const double *vpot, int mode)
+{
+    int n = ntot_of(ndim, np), nh = n / 2;
+    double b = 1.0 / n;
+    double su1 = 0, sv1 = 0, su2 = 0, sv2 = 0;
+    for (int i = 0; i < n; ++i) {
+        su1 = su1 + u[i] * vpot[i] * u[i];
+        sv1 = sv1 + v[i] * vpot[i] * v[i];
+    }
+    double *wu = malloc(sizeof(double) * 2 * n), *wv = wu + n;
+    for (int i = 0; i < nh; ++i) {
+        wu[i] = u[i + nh]; wv[i] = v[i + nh];
+        wu[i + nh] = u[i]; wv[i + nh] = v[i];
+    }
+    fft_all(ndim, np, wu, wv, +1, mode);
+    for (int i = 0; i < n; ++i) { wu[i] *= tc[i]; wv[i] *= tc[i]; }
+    fft_all(ndim, np, wu, wv, -1, mode);
+    for (int i = 0; i < nh; ++i) {
+        su2 = su2 + u[i] * b * wu[i + nh];
+        sv2 = sv2 + v[i] * b * wv[i + nh];
+        su2 = su2 + u[i + nh] * b * wu[i];
+        sv2 = sv2 + v[i + nh] * b * wv[i];
+    }
+    free(wu);
+    return su1 + su2 + sv1 + sv2;
+}
+
+static void corr(int n, const double *u0, const double *v0, const double *u,
+                 const double *v, double *cr, double *ci)
+{
+    *cr = orc_fcorrel(n, u0, u) + orc_fcorrel(n, v0, v);
+    *ci = orc_fcorrel(n, u0, v) - orc_fcorrel(n, v0, u);
+}
+
+/* pspofft.f:72-145 with PRTCRL='.NO' (no print-time energy checks). */
+int orc_pspofft(int ndim, const int *np, double dt, double ti, double tf,
+                int lc, const double *sh, const double *sm, double *u1,
+                double *v1, const double *vpot, double *cr, double *ci,
+                int maxsamp, double *e, double *nrm, int mode)
+{
+    int n = ntot_of(ndim, np);
+    double cst1 = dt / ORC_SHFS, cst2 = dt / (2.0 * ORC_SHFS), b;
+    double *u0 = malloc(sizeof(double) * 3 * n), *v0 = u0 + n, *tc = u0 + 2 * n;
+    memcpy(u0, u1, sizeof(double) * n);
+    memcpy(v0, v1, sizeof(double) * n);
+    orc_digt(ndim, np, sh, sm, tc, &b, mode);
+    int nc = 0, mc = 0;
+    if (maxsamp > 0) {
+        corr(n, u0, v0, u1, v1, &cr[0], &ci[0]);
+        if (e) e[0] = orc_efft4(ndim, np, u1, v1, tc, vpot, mode);
+        if (nrm) nrm[0] = orc_fcorrel(n, u1, u1) + orc_fcorrel(n, v1, v1);
+    }
+    int ns = orc_nsteps(ti, tf, dt);
+    for (int s = 0; s < ns; ++s) {
+        orc_spo_step(ndim, np, b, cst1, cst2, u1, v1, tc, vpot, mode);
+        if (++mc == lc) {
+            ++nc;
+            mc = 0;
+            if (nc < maxsamp) {
+                corr(n, u0, v0, u1, v1, &cr[nc], &ci[nc]);
+                if (e) e[nc] = orc_efft4(ndim, np, u1, v1, tc, vpot, mode);
+                if (nrm) nrm[nc] = orc_fcorrel(n, u1, u1) + orc_fcorrel(n, v1, v1);
+            }
+        }
+    }
+    free(u0);
+    return nc + 1;
+}
+
+/* psodfft.f:75-200 with PRTCRL='.NO': two half-strength start-up steps
+ * from psi0, then leapfrog from TI+2DT. */
+int orc_psodfft(int ndim, const int *np, double dt, double ti, double tf,
+                int lc, const double *sh, const double *sm, double *u1,
+                double *v1, const double *vpot, double *cr, double *ci,
+                int maxsamp, int mode)
+{
+    int n = ntot_of(ndim, np);
+    double cst = 2.0 * dt / ORC_SHFS, b;
+    double *w = malloc(sizeof(double) * 7 * n);
+    double *u0 = w, *v0 = w + n, *u2 = w + 2 * n, *v2 = w + 3 * n, *tc = w + 4 * n;
+    double *wr = w + 5 * n, *wi = w + 6 * n;
+    memcpy(u0, u1, sizeof(double) * n);
+    memcpy(u2, u1, sizeof(double) * n);
+    memcpy(v0, v1, sizeof(double) * n);
+    memcpy(v2, v1, sizeof(double) * n);
+    orc_digt(ndim, np, sh, sm, tc, &b, mode);
+    int nc = 0, mc;
+    if (maxsamp > 0) corr(n, u0, v0, u2, v2, &cr[0], &ci[0]);
+    orc_sod_step(ndim, np, b, cst / 2.0, u1, u2, v1, v2, tc, vpot, wr, wi, mode);
+    orc_sod_step(ndim, np, b, cst / 2.0, u1, u2, v1, v2, tc, vpot, wr, wi, mode);
+    mc = 1;
+    if (mc == lc) {
+        ++nc;
+        mc = 0;
+        if (nc < maxsamp) corr(n, u0, v0, u2, v2, &cr[nc], &ci[nc]);
+    }
+    memcpy(u1, u0, sizeof(double) * n);
+    memcpy(v1, v0, sizeof(double) * n);
+    int ns = trip(ti + 2 * dt, tf + dt, dt);
+    for (int s = 0; s < ns; ++s) {
+        orc_sod_step(ndim, np, b, cst, u1, u2, v1, v2, tc, vpot, wr, wi, mode);
+        if (++mc == lc) {
+            ++nc;
+            mc = 0;
+            if (nc < maxsamp) corr(n, u0, v0, u2, v2, &cr[nc], &ci[nc]);
+        }
+    }
+    /* The reference leaves psi(t) in (U2, V2) and psi(t-dt) in (U1, V1);
+     * return psi(t) in u1/v1 for the caller. */
+    memcpy(u1, u2, sizeof(double) * n);
+    memcpy(v1, v2, sizeof(double) * n);
+    free(w);
+    return nc + 1;
+}
+
+/* --------------------------------------------------- FD Hamiltonian */
+
+/* au1d.f:33-67, au2di.f:1-239, au3d.f:56-113: per axis
+ * shm_a (u[-2] - 16 u[-1] + 30 u[0] - 16 u[+1] + u[+2]) plus V u, with
+ * neighbours outside the grid taken as zero. */
+void orc_au(int ndim, const int *np, const double *shm, const double *vpot,
+            const double *u, double *hu)
+{
+    int n = ntot_of(ndim, np);
+    int stride[3] = {1, 1, 1};
+    for (int a = 1; a < ndim; ++a) stride[a] = stride[a - 1] * np[a - 1];
+    for (int i = 0; i < n; ++i) {
+        double acc = vpot[i] * u[i];
+        for (int a = 0; a < ndim; ++a) {
+            int ia = (i / stride[a]) % np[a], s = stride[a];
+            double t = 30.0 * u[i];
+            if (ia >= 2) t += u[i - 2 * s];
+            if (ia >= 1) t -= 16.0 * u[i - s];
+            if (ia + 1 < np[a]) t -= 16.0 * u[i + s];
+            if (ia + 2 < np[a]) t += u[i + 2 * s];
+            acc += shm[a] * t;
+        }
+        hu[i] = acc;
+    }
+}
+
+double orc_eigenerg(int ndim, const int *np, const double *shm,
+                    const double *u, const double *vpot)
+{
+    int n = ntot_of(ndim, np);
+    double *w = malloc(sizeof(double) * n);
+    orc_au(ndim, np, shm, vpot, u, w);
+    double s = orc_fcorrel(n, u, w);
+    free(w);
+    return s;
+}
+
+/* ----------------------------------------------------------- Lanczos */
+
+/* reort.f:54-80: one classical Gram-Schmidt pass. */
+void orc_reort(int n, int mcols, const double *basis, double *w)
+{
+    double rv[64];
+    for (int j = 0; j < mcols; ++j) {
+        double s = 0;
+        for (int i = 0; i < n; ++i) s = s + w[i] * basis[(size_t)j * n + i];
+        rv[j] = s;
+    }
+    for (int j = 0; j < mcols; ++j)
+        for (int i = 0; i < n; ++i) w[i] = w[i] - rv[j] * basis[(size_t)j * n + i];
+}
+
+/* lnz.f:65-119. */
+int orc_lnz(int ndim, const int *np, int m, int nreort, const double *shm,
+            const double *vpot, const double *start, double *al, double *bt,
+            double *basis)
+{
+    int n = ntot_of(ndim, np), it = 0;
+    double *w = malloc(sizeof(double) * 3 * n), *vi = w + n, *v0 = w + 2 * n;
+    memcpy(w, start, sizeof(double) * n);
+    memset(vi, 0, sizeof(double) * n);
+    memset(basis, 0, sizeof(double) * (size_t)n * nreort);
+    double bt0 = orc_ecnorm(n, w);
+    if (bt0 == 0) { free(w); return 0; }
+    for (int I = 1; I <= m; ++I) {
+        for (int j = 0; j < n; ++j) {
+            v0[j] = vi[j];
+            vi[j] = w[j] / bt0;
+            if (I <= nreort) basis[(size_t)(I - 1) * n + j] = vi[j];
+        }
+        if (I <= nreort) it = I;
+        orc_au(ndim, np, shm, vpot, vi, w);
+        double s = 0;
+        for (int j = 0; j < n; ++j) {
+            w[j] = w[j] - bt0 * v0[j];
+            s = s + vi[j] * w[j];
+        }
+        al[I - 1] = s;
+        for (int j = 0; j < n; ++j) w[j] = w[j] - al[I - 1] * vi[j];
+        orc_reort(n, it, basis, w);
+        bt[I - 1] = orc_ecnorm(n, w);
+        bt0 = bt[I - 1];
+    }
+    free(w);
+    return m;
+}
+
+/* Implicit QL with Wilkinson shifts on a symmetric tridiagonal matrix
+ * (the method of dsteqr.f for ABSTOL <= 0), eigenvectors accumulated from
+ * the identity, then sorted ascending. */
+int orc_tridiag_eig(int m, const double *d_in, const double *e_in, double *w,
+                    double *z)
+{
+    double *d = w, *e = malloc(sizeof(double) * (m + 1));
+    memcpy(d, d_in, sizeof(double) * m);
+    for (int i = 0; i < m - 1; ++i) e[i] = e_in[i];
+    e[m - 1] = 0;
+    for (int i = 0; i < m * m; ++i) z[i] = 0;
+    for (int i = 0; i < m; ++i) z[i * m + i] = 1;
+    for (int l = 0; l < m; ++l) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < m - 1; ++mm) {
+                double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+                if (fabs(e[mm]) <= 2.2204460492503131e-16 * dd)
+                    break;
+            }
+            if (mm != l) {
+                if (++iter > 60) { free(e); return l + 1; }
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                double s = 1, c = 1, p = 0;
+                int i;
+                for (i = mm - 1; i >= l; --i) {
+                    double f = s * e[i], bb = c * e[i];
+                    r = hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0) {
+                        d[i + 1] -= p;
+                        e[mm] = 0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * bb;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - bb;
+                    for (int k = 0; k < m; ++k) {
+                        double zf = z[(i + 1) * m + k];
+                        z[(i + 1) * m + k] = s * z[i * m + k] + c * zf;
+                        z[i * m + k] = c * z[i * m + k] - s * zf;
+                    }
+                }
+                if (r == 0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[mm] = 0;
+            }
+        } while (mm != l);
+    }
+    /* selection sort ascending, carrying vectors */
+    for (int i = 0; i < m - 1; ++i) {
+        int k = i;
+        for (int j = i + 1; j < m; ++j)
+            if (d[j] < d[k]) k = j;
+        if (k != i) {
+            double t = d[i]; d[i] = d[k]; d[k] = t;
+            for (int r = 0; r < m; ++r) {
+                t = z[i * m + r]; z[i * m + r] = z[k * m + r]; z[k * m + r] = t;
+            }
+        }
+    }
+    free(e);
+    return 0;
+}
+
+/* Sturm count: number of eigenvalues < x (dlaebz.f, IJOB=1 recurrence
+ * d_j - e_{j-1}^2 / q - x with the pivot guard). */
+static int sturm(int m, const double *d, const double *e2, double pivmin, double x)
+{
+    int cnt = 0;
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q <= 0) ++cnt;
+    for (int j = 1; j < m; ++j) {
+        q = d[j] - e2[j - 1] / q - x;
+        if (fabs(q) < pivmin) q = -pivmin;
+        if (q <= 0) ++cnt;
+    }
+    return cnt;
+}
+
+/* dstebz.f RANGE='A' for one unreduced block: Gershgorin interval widened
+ * as dstebz.f does, bisection of each eigenvalue's interval until its
+ * width is below max(abstol, pivmin, reltol*max|end|), midpoint returned.
+ * PARITY UNPINNED against dstebz.f's exact interval sequence beyond the
+ * tolerance: used for ABSTOL > 0 only. */
+int orc_tridiag_bisect(int m, const double *d, const double *e, double abstol,
+                       double *w)
+{
+    const double ulp = 2.2204460492503131e-16, safemn = 2.2250738585072014e-308;
+    const double fudge = 2.1, reltol = ulp * 2.0;
+    double *e2 = malloc(sizeof(double) * (m + 1));
+    double pivmin = 1.0;
+    for (int j = 0; j < m - 1; ++j) {
+        e2[j] = e[j] * e[j];
+        if (e2[j] > pivmin) pivmin = e2[j];
+    }
+    pivmin = pivmin * safemn;
+    double gl = d[0], gu = d[0];
+    for (int j = 0; j < m; ++j) {
+        double r = (j > 0 ? fabs(e[j - 1]) : 0) + (j < m - 1 ? fabs(e[j]) : 0);
+        if (d[j] - r < gl) gl = d[j] - r;
+        if (d[j] + r > gu) gu = d[j] + r;
+    }
+    double tnorm = fmax(fabs(gl), fabs(gu));
+    gl = gl - fudge * tnorm * ulp * m - fudge * 2.0 * pivmin;
+    gu = gu + fudge * tnorm * ulp * m + fudge * pivmin;
+    double atoli = abstol <= 0 ? ulp * tnorm : abstol;
+    for (int k = 0; k < m; ++k) {
+        double a = gl, bb = gu;
+        for (int it = 0; it < 200; ++it) {
+            double tmp1 = fabs(bb - a), tmp2 = fmax(fabs(bb), fabs(a));
+            if (tmp1 < fmax(atoli, fmax(pivmin, reltol * tmp2))) break;
+            double c = 0.5 * (a + bb);
+            if (sturm(m, d, e2, pivmin, c) > k) bb = c;
+            else a = c;
+        }
+        w[k] = 0.5 * (a + bb);
+    }
+    free(e2);
+    return 0;
+}
+
+/* DGEMV 'T' (dgemv.f): y_j = sum_i A(i,j) x_i. */
+static void gemv_t(int rows, int cols, const double *A, int lda, const double *x, double *y)
+{
+    for (int j = 0; j < cols; ++j) {
+        double t = 0;
+        for (int i = 0; i < rows; ++i) t = t + A[(size_t)j * lda + i] * x[i];
+        y[j] = t;
+    }
+}
+
+/* DGEMV 'N' (dgemv.f): y = sum_j x_j A(:,j), column by column. */
+static void gemv_n(int rows, int cols, const double *A, int lda, const double *x, double *y)
+{
+    for (int i = 0; i < rows; ++i) y[i] = 0;
+    for (int j = 0; j < cols; ++j) {
+        double t = x[j];
+        for (int i = 0; i < rows; ++i) y[i] = y[i] + t * A[(size_t)j * lda + i];
+    }
+}
+
+static int tridiag(int mp, const double *al, const double *bt, double abstol,
+                   double *lam, double *q)
+{
+    int info = orc_tridiag_eig(mp, al, bt, lam, q);
+    if (info == 0 && abstol > 0) info = orc_tridiag_bisect(mp, al, bt, abstol, lam);
+    return info;
+}
+
+/* sil.f:70-181 without absorber: Lanczos on Re psi and on Im psi
+ * separately, each projected, rotated by exp(-i lambda cst), and
+ * recombined.  ORC_REF keeps sil.f:98-99, where -sin multiplies the
+ * already cos-scaled coefficient. */
+int orc_plnz_step(int ndim, const int *np, int mp, double abstol, double cst,
+                  const double *shm, const double *vpot, double *u, double *v,
+                  int mode)
+{
+    int n = ntot_of(ndim, np), info;
+    double *basis = malloc(sizeof(double) * (size_t)n * mp);
+    double *wv = malloc(sizeof(double) * 2 * n), *wu = wv + n;
+    double al[64], bt[64], lam[64], q[64 * 64], c[64], u2[64], ur[64], wvs[64], work[64];
+
+    orc_lnz(ndim, np, mp, mp, shm, vpot, u, al, bt, basis);
+    info = tridiag(mp, al, bt, abstol, lam, q);
+    if (info) goto done;
+    gemv_t(n, mp, basis, n, u, c);
+    gemv_t(mp, mp, q, mp, c, u2);
+    for (int i = 0; i < mp; ++i) {
+        double ang = lam[i] * cst;
+        if (mode == ORC_REF) {
+            ur[i] = cos(ang) * u2[i];
+            wvs[i] = -sin(ang) * ur[i];
+        } else {
+            ur[i] = cos(ang) * u2[i];
+            wvs[i] = -sin(ang) * u2[i];
+        }
+    }
+    gemv_n(mp, mp, q, mp, ur, work);
+    gemv_n(n, mp, basis, n, work, u);
+    gemv_n(mp, mp, q, mp, wvs, work);
+    gemv_n(n, mp, basis, n, work, wv);
+
+    orc_lnz(ndim, np, mp, mp, shm, vpot, v, al, bt, basis);
+    info = tridiag(mp, al, bt, abstol, lam, q);
+    if (info) goto done;
+    gemv_t(n, mp, basis, n, v, c);
+    gemv_t(mp, mp, q, mp, c, u2);
+    for (int i = 0; i < mp; ++i) {
+        double ang = lam[i] * cst;
+        wvs[i] = sin(ang) * u2[i];
+        ur[i] = cos(ang) * u2[i];
+    }
+    gemv_n(mp, mp, q, mp, ur, work);
+    gemv_n(n, mp, basis, n, work, v);
+    gemv_n(mp, mp, q, mp, wvs, work);
+    gemv_n(n, mp, basis, n, work, wu);
+    for (int i = 0; i < n; ++i) {
+        u[i] = u[i] + wu[i];
+        v[i] = wv[i] + v[i];
+    }
+done:
+    free(basis);
+    free(wv);
+    return info;
+}
+
+/* plnz.f:72-140 with PRTCRL='.NO'. */
+int orc_plnz(int ndim, const int *np, int mp, double abstol, double dt,
+             double ti, double tf, int lc, const double *shm, double *u,
+             double *v, const double *vpot, double *cr, double *ci,
+             int maxsamp, int mode)
+{
+    int n = ntot_of(ndim, np);
+    double cst = dt / ORC_SHFS;
+    double *u0 = malloc(sizeof(double) * 2 * n), *v0 = u0 + n;
+    memcpy(u0, u, sizeof(double) * n);
+    memcpy(v0, v, sizeof(double) * n);
+    int nc = 0, mc = 0;
+    if (maxsamp > 0) corr(n, u0, v0, u, v, &cr[0], &ci[0]);
+    int ns = orc_nsteps(ti, tf, dt);
+    for (int s = 0; s < ns; ++s) {
+        if (orc_plnz_step(ndim, np, mp, abstol, cst, shm, vpot, u, v, mode)) {
+            free(u0);
+            return -1;
+        }
+        if (++mc == lc) {
+            ++nc;
+            mc = 0;
+            if (nc < maxsamp) corr(n, u0, v0, u, v, &cr[nc], &ci[nc]);
+        }
+    }
+    free(u0);
+    return nc + 1;
+}
+
+/* rans.f:32-39. */
+double orc_ran(int *nseed)
+{
+    *nseed = (int)(((long long)*nseed * 1029 + 221591) % 1048576);
+    return 1.0 * *nseed / 1048576;
+}
+
+/* lanczs.f:76-183. */
+int orc_lanczs(int ndim, const int *np, int il, int iu, int nreort,
+               const double *shm, const double *vpot, double *eigvl,
+               double *eigvc)
+{
+    int n = ntot_of(ndim, np), ns = iu - il + 1;
+    double *al = malloc(sizeof(double) * 2 * n), *bt = al + n;
+    double *start = malloc(sizeof(double) * 6 * n);
+    double *wk1 = start + n, *wk2 = start + 2 * n, *v0 = start + 3 * n, *vi = start + 4 * n;
+    double *basis = malloc(sizeof(double) * (size_t)n * nreort);
+    double *lam = malloc(sizeof(double) * n), *q = malloc(sizeof(double) * (size_t)n * n);
+    int seed = 1;
+    for (int i = 0; i < n; ++i) start[i] = orc_ran(&seed);
+    orc_lnz(ndim, np, n, nreort, shm, vpot, start, al, bt, basis);
+    int info = orc_tridiag_eig(n, al, bt, lam, q);
+    if (info) goto out;
+    for (int k = 0; k < ns; ++k) {
+        double *qk = q + (size_t)(il - 1 + k) * n;
+        /* dstein.f normalises each eigenvector so that its largest
+         * component is positive. */
+        int jm = 0;
+        for (int j = 1; j < n; ++j)
+            if (fabs(qk[j]) > fabs(qk[jm])) jm = j;
+        double sg = qk[jm] < 0 ? -1.0 : 1.0;
+        eigvl[k] = lam[il - 1 + k];
+        memset(vi, 0, sizeof(double) * n);
+        memset(wk2, 0, sizeof(double) * n);
+        memset(basis, 0, sizeof(double) * (size_t)n * nreort);
+        seed = 1;
+        for (int i = 0; i < n; ++i) wk1[i] = orc_ran(&seed);
+        double bt0 = orc_ecnorm(n, wk1);
+        int it = 0;
+        for (int J = 1; J <= n; ++J) {
+            for (int i = 0; i < n; ++i) {
+                v0[i] = wk2[i];
+                wk2[i] = wk1[i] / bt0;
+                if (J <= nreort) basis[(size_t)(J - 1) * n + i] = wk2[i];
+                vi[i] = vi[i] + sg * qk[J - 1] * wk2[i];
+            }
+            if (J <= nreort) it = J;
+            orc_au(ndim, np, shm, vpot, wk2, wk1);
+            double s = 0;
+            for (int i = 0; i < n; ++i) {
+                wk1[i] = wk1[i] - bt0 * v0[i];
+                s = s + wk2[i] * wk1[i];
+            }
+            for (int i = 0; i < n; ++i) wk1[i] = wk1[i] - s * wk2[i];
+            orc_reort(n, it, basis, wk1);
+            bt0 = orc_ecnorm(n, wk1);
+        }
+        memcpy(eigvc + (size_t)k * n, vi, sizeof(double) * n);
+    }
+out:
+    free(al); free(start); free(basis); free(lam); free(q);
+    return info;
+}
+
+/* -------------------------------------------------------- spectrum */
+
+int orc_lc_rule(double ti, double *tf, double dt, int mf, int change)
+{
+    for (;;) {
+        double sc = 2.0 * (*tf - ti) / dt;
+        int nc = (int)(sc / pow(2.0, mf));
+        if (nc < 0) return 2147483645;
+        if (fabs(sc - nc * pow(2.0, mf)) >= 1.0e-10 && change) {
+            *tf = nc * pow(2.0, mf - 1.0) * dt;
+            continue;
+        }
+        return nc;
+    }
+}
+
+void orc_assemble_corr(int mf, const double *cr, const double *ci, double *u,
+                       double *v)
+{
+    int mc = (1 << mf) / 2;
+    /* 1-based indices as in espec.f:1552-1566 */
+    double *U = u - 1, *V = v - 1;
+    for (int i = 1; i <= 2 * mc; ++i) { U[i] = 0; V[i] = 0; }
+    for (int i = 1; i <= mc; ++i) {
+        U[mc + i] = U[mc + i] + cr[i - 1];
+        U[mc - i + 2] = U[mc + i];
+        V[mc + i] = V[mc + i] + ci[i - 1];
+        V[mc - i + 2] = -V[mc + i];
+    }
+    U[1] = U[1] + cr[mc - 1];
+    V[1] = V[1] + ci[mc - 1];
+    V[1] = -V[1];
+}
+
+void orc_hwhm(int mf, double dt, double tf, double tal, double de, double *u,
+              double *v)
+{
+    int n = 1 << mf;
+    double dem = -de * 1.5192676, ta = tal * 1.5192676;
+    for (int i = 1; i <= n; ++i) {
+        double t = -tf + (i - 1.0) * dt;
+        double cs = cos(dem * t), sn = sin(dem * t), cstl = exp(-fabs(t) * ta);
+        double ui = u[i - 1];
+        u[i - 1] = cs * u[i - 1] - sn * v[i - 1];
+        v[i - 1] = cs * v[i - 1] + sn * ui;
+        u[i - 1] = cstl * u[i - 1];
+        v[i - 1] = cstl * v[i - 1];
+    }
+}
+
+void orc_spectrumtd(int iwin, int mf, double dt, double tf, double wp,
+                    double de, double *u, double *v, double *e_out,
+                    double *w_out, double *wi_out, int mode)
+{
+    int n = 1 << mf;
+    double enc = 1.0 / (n * dt) * 1.519829846003e-1;
+    enc = enc * 27.2113834;
+    double en = -n / 2.0 * enc;
+    if (de < 0) de = 0;
+    if (iwin == 1) {
+        double tmax = n * dt;
+        double lg = log(1.0 / wp);
+        double taux = tmax * tmax / (lg * lg);
+        for (int i = 1; i <= n; ++i) {
+            double t = -tf + (i - 1.0) * dt;
+            double cstl = exp(-(t * t) / taux);
+            u[i - 1] = cstl * u[i - 1];
+            v[i - 1] = cstl * v[i - 1];
+        }
+    }
+    for (int i = 0; i < n / 2; ++i) {
+        double t = u[i]; u[i] = u[n / 2 + i]; u[n / 2 + i] = t;
+        t = v[i]; v[i] = v[n / 2 + i]; v[n / 2 + i] = t;
+    }
+    orc_fftnt(u, v, n, mf, -1, mode);
+    int k = 0;
+    for (int i = n / 2; i < n; ++i, ++k) {
+        e_out[k] = en + de; w_out[k] = u[i]; wi_out[k] = v[i];
+        en = en + enc;
+    }
+    for (int i = 0; i < n / 2; ++i, ++k) {
+        e_out[k] = en + de; w_out[k] = u[i]; wi_out[k] = v[i];
+        en = en + enc;
+    }
+}

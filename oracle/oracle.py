@@ -1,0 +1,366 @@
+"""ctypes bindings for the checkers under oracle/ (test infrastructure only).
+
+``Oracle`` loads ``_ref/liboracle.so``, the FP64 C restatement of the eSPec
+hot path (espec_oracle.c).  ``Reference`` loads ``_ref/libespec_ref.so``, the
+reference's own Fortran compiled from its sources plus ref_harness.f90, which
+exists only where the reference checkout was present at build time.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this module; the product library never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+EXACT, REF = 0, 1
+SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
+
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_dp = C.c_double
+_ip = C.c_int
+
+
+def _f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _np3(np_):
+    a = np.ones(3, dtype=np.int32)
+    a[: len(np_)] = np_
+    return a
+
+
+def _v3(x, fill=1.0):
+    a = np.full(3, fill, dtype=np.float64)
+    a[: len(x)] = x
+    return a
+
+
+class Oracle:
+    """FP64 restatement; every method mirrors one reference routine."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(REF_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            raise OSError(f"oracle library missing: {path} (run `make -C oracle oracle`)")
+        L = self.lib = C.CDLL(path)
+        L.orc_fft.argtypes = [_d, _d, _ip, _ip, _ip, _ip, _ip]
+        L.orc_singleton_factors.argtypes = [_ip, _i]
+        L.orc_singleton_factors.restype = _ip
+        L.orc_fftnt.argtypes = [_d, _d, _ip, _ip, _ip, _ip]
+        L.orc_digt.argtypes = [_ip, _i, _d, _d, _d, C.POINTER(_dp), _ip]
+        L.orc_spo_step.argtypes = [_ip, _i, _dp, _dp, _dp, _d, _d, _d, _d, _ip]
+        L.orc_nsteps.argtypes = [_dp, _dp, _dp]
+        L.orc_nsteps.restype = _ip
+        L.orc_pspofft.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
+                                  _d, _d, _ip, C.c_void_p, C.c_void_p, _ip]
+        L.orc_pspofft.restype = _ip
+        L.orc_psodfft.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
+                                  _d, _d, _ip, _ip]
+        L.orc_psodfft.restype = _ip
+        L.orc_fcorrel.argtypes = [_ip, _d, _d]
+        L.orc_fcorrel.restype = _dp
+        L.orc_efft4.argtypes = [_ip, _i, _d, _d, _d, _d, _ip]
+        L.orc_efft4.restype = _dp
+        L.orc_eigenerg.argtypes = [_ip, _i, _d, _d, _d]
+        L.orc_eigenerg.restype = _dp
+        L.orc_au.argtypes = [_ip, _i, _d, _d, _d, _d]
+        L.orc_lnz.argtypes = [_ip, _i, _ip, _ip, _d, _d, _d, _d, _d, _d]
+        L.orc_lnz.restype = _ip
+        L.orc_reort.argtypes = [_ip, _ip, _d, _d]
+        L.orc_tridiag_eig.argtypes = [_ip, _d, _d, _d, _d]
+        L.orc_tridiag_eig.restype = _ip
+        L.orc_tridiag_bisect.argtypes = [_ip, _d, _d, _dp, _d]
+        L.orc_tridiag_bisect.restype = _ip
+        L.orc_plnz_step.argtypes = [_ip, _i, _ip, _dp, _dp, _d, _d, _d, _d, _ip]
+        L.orc_plnz_step.restype = _ip
+        L.orc_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d,
+                               _d, _d, _d, _ip, _ip]
+        L.orc_plnz.restype = _ip
+        L.orc_ran.argtypes = [C.POINTER(_ip)]
+        L.orc_ran.restype = _dp
+        L.orc_lanczs.argtypes = [_ip, _i, _ip, _ip, _ip, _d, _d, _d, _d]
+        L.orc_lanczs.restype = _ip
+        L.orc_lc_rule.argtypes = [_dp, C.POINTER(_dp), _dp, _ip, _ip]
+        L.orc_lc_rule.restype = _ip
+        L.orc_assemble_corr.argtypes = [_ip, _d, _d, _d, _d]
+        L.orc_hwhm.argtypes = [_ip, _dp, _dp, _dp, _dp, _d, _d]
+        L.orc_spectrumtd.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp, _d, _d, _d, _d,
+                                     _d, _ip]
+
+    # -- FFT family ----------------------------------------------------
+    def fft(self, a, b, ntot, n, nspan, isn, mode=REF):
+        a, b = _f64(a).copy(), _f64(b).copy()
+        self.lib.orc_fft(a, b, ntot, n, nspan, isn, mode)
+        return a, b
+
+    def singleton_factors(self, n):
+        f = np.zeros(40, dtype=np.int32)
+        k = self.lib.orc_singleton_factors(n, f)
+        return list(f[:k])
+
+    def fftnt(self, x, y, m, itype, mode=REF):
+        x, y = _f64(x).copy(), _f64(y).copy()
+        self.lib.orc_fftnt(x, y, len(x), m, itype, mode)
+        return x, y
+
+    def digt(self, np_, sh, sm, mode=REF):
+        n = int(np.prod(np_))
+        tc = np.zeros(n)
+        b = _dp()
+        self.lib.orc_digt(len(np_), _np3(np_), _v3(sh), _v3(sm), tc, C.byref(b), mode)
+        return tc, b.value
+
+    def spo_step(self, np_, b, cst1, cst2, u, v, tc, vpot, mode=REF):
+        u, v = _f64(u).copy(), _f64(v).copy()
+        self.lib.orc_spo_step(len(np_), _np3(np_), b, cst1, cst2, u, v, _f64(tc),
+                              _f64(vpot), mode)
+        return u, v
+
+    def nsteps(self, ti, tf, dt):
+        return self.lib.orc_nsteps(ti, tf, dt)
+
+    # -- propagators ---------------------------------------------------
+    def pspofft(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, mode=REF, energies=False):
+        ns = self.nsteps(ti, tf, dt) // max(lc, 1) + 2
+        cr, ci = np.zeros(ns), np.zeros(ns)
+        e, nr = np.zeros(ns), np.zeros(ns)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        k = self.lib.orc_pspofft(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm),
+                                 u, v, _f64(vpot), cr, ci, ns,
+                                 e.ctypes.data if energies else None,
+                                 nr.ctypes.data if energies else None, mode)
+        out = dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
+        if energies:
+            out.update(e=e[:k], norm=nr[:k])
+        return out
+
+    def psodfft(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, mode=REF):
+        ns = self.nsteps(ti, tf, dt) // max(lc, 1) + 3
+        cr, ci = np.zeros(ns), np.zeros(ns)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        k = self.lib.orc_psodfft(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm),
+                                 u, v, _f64(vpot), cr, ci, ns, mode)
+        return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
+
+    def plnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, mode=REF):
+        ns = self.nsteps(ti, tf, dt) // max(lc, 1) + 2
+        cr, ci = np.zeros(ns), np.zeros(ns)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        k = self.lib.orc_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc,
+                              _v3(shm, 0.0), u, v, _f64(vpot), cr, ci, ns, mode)
+        if k < 0:
+            raise RuntimeError("diagonalization error")
+        return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
+
+    def plnz_step(self, np_, mp, abstol, cst, shm, u, v, vpot, mode=REF):
+        u, v = _f64(u).copy(), _f64(v).copy()
+        info = self.lib.orc_plnz_step(len(np_), _np3(np_), mp, abstol, cst,
+                                      _v3(shm, 0.0), _f64(vpot), u, v, mode)
+        if info:
+            raise RuntimeError("diagonalization error")
+        return u, v
+
+    # -- observables and operators --------------------------------------
+    def fcorrel(self, a, b):
+        return self.lib.orc_fcorrel(len(a), _f64(a), _f64(b))
+
+    def efft4(self, np_, u, v, tc, vpot, mode=REF):
+        return self.lib.orc_efft4(len(np_), _np3(np_), _f64(u), _f64(v), _f64(tc),
+                                  _f64(vpot), mode)
+
+    def eigenerg(self, np_, shm, u, vpot):
+        return self.lib.orc_eigenerg(len(np_), _np3(np_), _v3(shm, 0.0), _f64(u), _f64(vpot))
+
+    def au(self, np_, shm, vpot, u):
+        hu = np.zeros(int(np.prod(np_)))
+        self.lib.orc_au(len(np_), _np3(np_), _v3(shm, 0.0), _f64(vpot), _f64(u), hu)
+        return hu
+
+    def lnz(self, np_, m, nreort, shm, vpot, start):
+        n = int(np.prod(np_))
+        al, bt = np.zeros(m), np.zeros(m)
+        basis = np.zeros(n * nreort)
+        self.lib.orc_lnz(len(np_), _np3(np_), m, nreort, _v3(shm, 0.0), _f64(vpot),
+                         _f64(start), al, bt, basis)
+        return al, bt, basis.reshape(nreort, n)
+
+    def reort(self, basis_cols, w):
+        basis_cols = _f64(basis_cols)
+        w = _f64(w).copy()
+        self.lib.orc_reort(len(w), basis_cols.shape[0], basis_cols.ravel(), w)
+        return w
+
+    def tridiag_eig(self, d, e):
+        m = len(d)
+        w, z = np.zeros(m), np.zeros(m * m)
+        info = self.lib.orc_tridiag_eig(m, _f64(d), _f64(np.append(e, 0.0)), w, z)
+        return info, w, z.reshape(m, m).T  # columns are eigenvectors
+
+    def tridiag_bisect(self, d, e, abstol):
+        m = len(d)
+        w = np.zeros(m)
+        self.lib.orc_tridiag_bisect(m, _f64(d), _f64(np.append(e, 0.0)), abstol, w)
+        return w
+
+    def ran(self, nseed, count):
+        s = _ip(nseed)
+        out = [self.lib.orc_ran(C.byref(s)) for _ in range(count)]
+        return np.array(out), s.value
+
+    def lanczs(self, np_, il, iu, nreort, shm, vpot):
+        n = int(np.prod(np_))
+        ns = iu - il + 1
+        ev, vc = np.zeros(ns), np.zeros(ns * n)
+        info = self.lib.orc_lanczs(len(np_), _np3(np_), il, iu, nreort, _v3(shm, 0.0),
+                                   _f64(vpot), ev, vc)
+        return info, ev, vc.reshape(ns, n)
+
+    # -- correlation to spectrum -----------------------------------------
+    def lc_rule(self, ti, tf, dt, mf, change=True):
+        t = _dp(tf)
+        lc = self.lib.orc_lc_rule(ti, C.byref(t), dt, mf, int(change))
+        return lc, t.value
+
+    def assemble_corr(self, mf, cr, ci):
+        n = 1 << mf
+        u, v = np.zeros(n), np.zeros(n)
+        self.lib.orc_assemble_corr(mf, _f64(cr), _f64(ci), u, v)
+        return u, v
+
+    def hwhm(self, mf, dt, tf, tal, de, u, v):
+        u, v = _f64(u).copy(), _f64(v).copy()
+        self.lib.orc_hwhm(mf, dt, tf, tal, de, u, v)
+        return u, v
+
+    def spectrumtd(self, iwin, mf, dt, tf, wp, de, u, v, mode=REF):
+        n = 1 << mf
+        u, v = _f64(u).copy(), _f64(v).copy()
+        e, w, wi = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.lib.orc_spectrumtd(iwin, mf, dt, tf, wp, de, u, v, e, w, wi, mode)
+        return e, w, wi
+
+
+class Reference:
+    """The reference's own Fortran (compiled from its sources) via ref_harness.f90."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(REF_DIR, "libespec_ref.so")
+        if not os.path.exists(path):
+            raise OSError(f"reference build missing: {path}")
+        L = self.lib = C.CDLL(path)
+        P = C.POINTER
+        L.ref_pspofft.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _d,
+                                  _d, _ip, P(_ip)]
+        L.ref_psodfft.argtypes = L.ref_pspofft.argtypes
+        L.ref_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d,
+                               _d, _d, _ip, P(_ip)]
+        L.ref_fft.argtypes = [_d, _d, _ip, _ip, _ip, _ip]
+        L.ref_fftnt.argtypes = [_d, _d, _ip, _ip, _ip]
+        L.ref_digt.argtypes = [_ip, _i, _d, _d, _d, P(_dp)]
+        L.ref_spo_step.argtypes = [_ip, _i, _dp, _dp, _dp, _d, _d, _d, _d]
+        L.ref_au.argtypes = [_ip, _i, _d, _d, _d, _d]
+        L.ref_efft4.argtypes = [_ip, _i, _d, _d, _d, _d]
+        L.ref_efft4.restype = _dp
+        L.ref_lnz.argtypes = [_ip, _i, _ip, _ip, _d, _d, _d, _d, _d, _d, _ip]
+        L.ref_dstevx.argtypes = [_ip, _d, _d, _dp, _d, _d, P(_ip)]
+        L.ref_hwhm.argtypes = [_ip, _dp, _dp, _dp, _dp, _d, _d]
+        L.ref_spectrumtd.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp, _d, _d]
+        L.ref_ran.argtypes = [P(_ip)]
+        L.ref_ran.restype = _dp
+        L.ref_lanczs.argtypes = [_ip, _i, _ip, _ip, _ip, _dp, _d, _d, _d, _d, _ip, P(_ip)]
+
+    def _prop(self, fn, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, extra):
+        ns = Oracle.__new__(Oracle)  # only for nsteps arithmetic below
+        nst = int(((tf + dt) - (ti + dt) + dt) / dt)
+        nsamp = nst // max(lc, 1) + 1 + extra
+        cr, ci = np.zeros(nsamp), np.zeros(nsamp)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        info = _ip(0)
+        fn(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v, _f64(vpot), cr, ci,
+           nsamp, C.byref(info))
+        return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
+
+    def pspofft(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot):
+        return self._prop(self.lib.ref_pspofft, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, 0)
+
+    def psodfft(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot):
+        return self._prop(self.lib.ref_psodfft, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, 1)
+
+    def plnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot):
+        nst = int(((tf + dt) - (ti + dt) + dt) / dt)
+        nsamp = nst // max(lc, 1) + 1
+        cr, ci = np.zeros(nsamp), np.zeros(nsamp)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        info = _ip(0)
+        self.lib.ref_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0),
+                          u, v, _f64(vpot), cr, ci, nsamp, C.byref(info))
+        return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
+
+    def fft(self, a, b, ntot, n, nspan, isn):
+        a, b = _f64(a).copy(), _f64(b).copy()
+        self.lib.ref_fft(a, b, ntot, n, nspan, isn)
+        return a, b
+
+    def fftnt(self, x, y, m, itype):
+        x, y = _f64(x).copy(), _f64(y).copy()
+        self.lib.ref_fftnt(x, y, len(x), m, itype)
+        return x, y
+
+    def digt(self, np_, sh, sm):
+        tc = np.zeros(int(np.prod(np_)))
+        b = _dp()
+        self.lib.ref_digt(len(np_), _np3(np_), _v3(sh), _v3(sm), tc, C.byref(b))
+        return tc, b.value
+
+    def spo_step(self, np_, b, cst1, cst2, u, v, tc, vpot):
+        u, v = _f64(u).copy(), _f64(v).copy()
+        self.lib.ref_spo_step(len(np_), _np3(np_), b, cst1, cst2, u, v, _f64(tc), _f64(vpot))
+        return u, v
+
+    def au(self, np_, shm, vpot, u):
+        hu = np.zeros(int(np.prod(np_)))
+        self.lib.ref_au(len(np_), _np3(np_), _v3(shm, 0.0), _f64(vpot), _f64(u), hu)
+        return hu
+
+    def efft4(self, np_, u, v, tc, vpot):
+        return self.lib.ref_efft4(len(np_), _np3(np_), _f64(u), _f64(v), _f64(tc), _f64(vpot))
+
+    def lnz(self, np_, m, nreort, shm, vpot, start):
+        n = int(np.prod(np_))
+        al, bt = np.zeros(m), np.zeros(m)
+        basis = np.zeros(n * nreort)
+        self.lib.ref_lnz(len(np_), _np3(np_), m, nreort, _v3(shm, 0.0), _f64(vpot),
+                         _f64(start), al, bt, basis, n)
+        return al, bt, basis.reshape(nreort, n)
+
+    def dstevx(self, d, e, abstol):
+        m = len(d)
+        w, z = np.zeros(m), np.zeros(m * m)
+        info = _ip(0)
+        self.lib.ref_dstevx(m, _f64(d).copy(), _f64(np.append(e, 0.0)).copy(), abstol, w, z,
+                            C.byref(info))
+        return info.value, w, z.reshape(m, m).T
+
+    def hwhm(self, mf, dt, tf, tal, de, u, v):
+        u, v = _f64(u).copy(), _f64(v).copy()
+        self.lib.ref_hwhm(mf, dt, tf, tal, de, u, v)
+        return u, v
+
+    def ran(self, nseed, count):
+        s = _ip(nseed)
+        out = [self.lib.ref_ran(C.byref(s)) for _ in range(count)]
+        return np.array(out), s.value
+
+    def lanczs(self, np_, il, iu, nreort, abstol, shm, vpot):
+        n = int(np.prod(np_))
+        ns = iu - il + 1
+        ev, vc = np.zeros(ns), np.zeros(n * ns)
+        info = _ip(0)
+        self.lib.ref_lanczs(len(np_), _np3(np_), il, iu, nreort, abstol, _v3(shm, 0.0),
+                            _f64(vpot), ev, vc, n, C.byref(info))
+        return info.value, ev, vc.reshape(ns, n)

@@ -1,0 +1,320 @@
+! Test-only C-callable harness around the eSPec reference Fortran.
+!
+! This file is compiled together with the reference sources, where they
+! lie under the reference checkout, into oracle/_ref/libespec_ref.so (see
+! oracle/Makefile).  It is checker infrastructure: only the tests and the
+! CPU-baseline leg of bench.py load it, never the product library.
+!
+! Each wrapper turns plain C arguments into the reference's Fortran
+! 77 calling convention (character flags, LNZVC column offsets, work
+! arrays) and calls the reference subroutine unchanged.
+module espec_ref_harness
+  use iso_c_binding
+  implicit none
+contains
+
+  subroutine dimstr(ndim, s)
+    integer(c_int), intent(in) :: ndim
+    character(len=8), intent(out) :: s
+    if (ndim == 1) then
+      s = '.1D     '
+    else if (ndim == 2) then
+      s = '.2D     '
+    else
+      s = '.3D     '
+    end if
+  end subroutine dimstr
+
+  ! PSPOFFT (pspofft.f:1-150).  C(t) samples go to cr/ci (length >= nsamp).
+  subroutine ref_pspofft(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, &
+                         cr, ci, nsamp, info) bind(C, name='ref_pspofft')
+    integer(c_int), value :: ndim, lc, nsamp
+    integer(c_int) :: np(3)
+    real(c_double), value :: dt, ti, tf
+    real(c_double) :: sh(3), sm(3), u1(*), v1(*), vpot(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims
+    integer :: n, mxdct, i
+    real(8), allocatable :: tc(:), vabc(:), lnzvc(:,:), w1(:), w2(:)
+    real(8) :: shm(3), xi(3), xf(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(tc(n), vabc(n), lnzvc(mxdct, 2), w1(n), w2(n))
+    lnzvc = 0d0
+    vabc = 1d0
+    shm = 0d0; xi = 0d0; xf = 0d0; var = 0d0
+    call dimstr(ndim, dims)
+    call PSPOFFT(.false., dims, '.NO     ', '.NONE   ', info, lc, 0, 2, 1, &
+                 mxdct, n, dt, ti, tf, 1d30, np, shm, sh, sm, u1, v1, xi, xf, &
+                 tc, vpot, vabc, lnzvc, w1, w2, var)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, 1)
+      ci(i) = lnzvc(i, 2)
+    end do
+    deallocate(tc, vabc, lnzvc, w1, w2)
+  end subroutine ref_pspofft
+
+  ! PSODFFT (psodfft.f:1-211).
+  subroutine ref_psodfft(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, &
+                         cr, ci, nsamp, info) bind(C, name='ref_psodfft')
+    integer(c_int), value :: ndim, lc, nsamp
+    integer(c_int) :: np(3)
+    real(c_double), value :: dt, ti, tf
+    real(c_double) :: sh(3), sm(3), u1(*), v1(*), vpot(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims
+    integer :: n, mxdct, i
+    real(8), allocatable :: tc(:), vabc(:), lnzvc(:,:), u2(:), v2(:), wr(:), wi(:)
+    real(8) :: shm(3), xi(3), xf(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(tc(n), vabc(n), lnzvc(mxdct, 2), u2(n), v2(n), wr(n), wi(n))
+    lnzvc = 0d0
+    vabc = 1d0
+    shm = 0d0; xi = 0d0; xf = 0d0; var = 0d0
+    call dimstr(ndim, dims)
+    call PSODFFT(.false., dims, '.NO     ', '.NONE   ', info, lc, 0, 2, 1, &
+                 mxdct, n, dt, ti, tf, 1d30, np, shm, sh, sm, u1, u2, v1, v2, &
+                 xi, xf, tc, vpot, vabc, wr, wi, var, lnzvc)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, 1)
+      ci(i) = lnzvc(i, 2)
+    end do
+    deallocate(tc, vabc, lnzvc, u2, v2, wr, wi)
+  end subroutine ref_psodfft
+
+  ! PLNZ (plnz.f:1-156); Krylov order mp, DSTEVX tolerance abstol.
+  subroutine ref_plnz(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, &
+                      cr, ci, nsamp, info) bind(C, name='ref_plnz')
+    integer(c_int), value :: ndim, mp, lc, nsamp
+    integer(c_int) :: np(3)
+    real(c_double), value :: abstol, dt, ti, tf
+    real(c_double) :: shm(3), u(*), v(*), vpot(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims
+    integer :: n, mxdct, i
+    integer, allocatable :: iwork(:)
+    real(8), allocatable :: vabc(:), lnzvc(:,:), wu(:), wv(:), wp(:), work(:)
+    real(8) :: xi(3), xf(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(iwork(5*mp + 16), vabc(n), lnzvc(mxdct, mp + 2), wu(n), wv(n), &
+             wp(n), work(max(n, 5*mp + 16)))
+    lnzvc = 0d0
+    vabc = 1d0
+    xi = 0d0; xf = 0d0; var = 0d0
+    call dimstr(ndim, dims)
+    call PLNZ(.false., .false., dims, '.NO     ', '.NONE   ', info, lc, mp, &
+              mp, mp + 2, mp + 1, mxdct, n, abstol, dt, ti, tf, 1d30, np, &
+              iwork, shm, vpot, vabc, u, v, xi, xf, wu, wv, wp, work, var, lnzvc)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, mp + 1)
+      ci(i) = lnzvc(i, mp + 2)
+    end do
+    deallocate(iwork, vabc, lnzvc, wu, wv, wp, work)
+  end subroutine ref_plnz
+
+  ! FFT (fft3d.f:1): Singleton multivariate in-place transform.
+  subroutine ref_fft(a, b, ntot, n, nspan, isn) bind(C, name='ref_fft')
+    real(c_double) :: a(*), b(*)
+    integer(c_int), value :: ntot, n, nspan, isn
+    call FFT(a, b, ntot, n, nspan, isn)
+  end subroutine ref_fft
+
+  ! FFTNT (cfft.f:6): radix-2 transform used by the spectrum.
+  subroutine ref_fftnt(x, y, n, m, itype) bind(C, name='ref_fftnt')
+    real(c_double) :: x(*), y(*)
+    integer(c_int), value :: n, m, itype
+    call FFTNT(x, y, n, m, itype)
+  end subroutine ref_fftnt
+
+  ! DIGT (dkeft.f:204): kinetic-energy grid.
+  subroutine ref_digt(ndim, np, sh, sm, tc, b) bind(C, name='ref_digt')
+    integer(c_int), value :: ndim
+    integer(c_int) :: np(3)
+    real(c_double) :: sh(3), sm(3), tc(*), b
+    character(len=8) :: dims
+    integer :: nd, n, i
+    real(8) :: b1, b2
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    call dimstr(ndim, dims)
+    call DIGT(dims, nd, n, b, b1, b2, np, sh, sm, tc)
+  end subroutine ref_digt
+
+  ! PSPOFFTAUX (spofft.f:3): one split-operator step, no absorber.
+  subroutine ref_spo_step(ndim, np, b, cst1, cst2, u, v, tc, vpot) &
+      bind(C, name='ref_spo_step')
+    integer(c_int), value :: ndim
+    integer(c_int) :: np(3)
+    real(c_double), value :: b, cst1, cst2
+    real(c_double) :: u(*), v(*), tc(*), vpot(*)
+    integer :: n, i
+    real(8) :: vabc(1)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    call PSPOFFTAUX(.false., '.NONE   ', ndim, n, b, b, b, cst1, cst2, np, &
+                    u, v, tc, vpot, vabc)
+  end subroutine ref_spo_step
+
+  ! AU (au.f:3): 4th-order finite-difference H*u.
+  subroutine ref_au(ndim, np, shm, vpot, u, hu) bind(C, name='ref_au')
+    integer(c_int), value :: ndim
+    integer(c_int) :: np(3)
+    real(c_double) :: shm(3), vpot(*), u(*), hu(*)
+    character(len=8) :: dims
+    integer :: n, i
+    real(8) :: var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    var = 0d0
+    call dimstr(ndim, dims)
+    call AU(dims, n, np, shm, vpot, u, hu, var)
+  end subroutine ref_au
+
+  ! EFFT4 (eigenergfft.f:1): energy through the FFT kinetic operator.
+  function ref_efft4(ndim, np, u, v, tc, vpot) result(e) bind(C, name='ref_efft4')
+    integer(c_int), value :: ndim
+    integer(c_int) :: np(3)
+    real(c_double) :: u(*), v(*), tc(*), vpot(*)
+    real(c_double) :: e
+    integer :: n, i
+    real(8), allocatable :: wu(:), wv(:)
+    real(8) :: b
+    real(8), external :: EFFT4
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    b = 1d0/n
+    allocate(wu(n), wv(n))
+    e = EFFT4(ndim, n, np, b, u, v, wu, wv, tc, vpot)
+    deallocate(wu, wv)
+  end function ref_efft4
+
+  ! LNZ (lnz.f:4) followed by nothing: returns alpha, beta and the basis.
+  subroutine ref_lnz(ndim, np, m, nreort, shm, vpot, start, al, bt, basis, ldb) &
+      bind(C, name='ref_lnz')
+    integer(c_int), value :: ndim, m, nreort, ldb
+    integer(c_int) :: np(3)
+    real(c_double) :: shm(3), vpot(*), start(*), al(*), bt(*), basis(ldb, *)
+    character(len=8) :: dims
+    integer :: n, i
+    real(8), allocatable :: v0(:), vi(:), work(:)
+    real(8) :: var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    allocate(v0(n), vi(n), work(n))
+    vi(1:n) = start(1:n)
+    var = 0d0
+    call dimstr(ndim, dims)
+    call LNZ(.false., dims, n, m, nreort, nreort, ldb, np, shm, vpot, v0, vi, &
+             work, al, bt, var, basis)
+    deallocate(v0, vi, work)
+  end subroutine ref_lnz
+
+  ! DSTEVX (dstevx.f): all eigenpairs of the m x m tridiagonal (d, e).
+  subroutine ref_dstevx(m, d, e, abstol, w, z, info) bind(C, name='ref_dstevx')
+    integer(c_int), value :: m
+    real(c_double), value :: abstol
+    real(c_double) :: d(*), e(*), w(*), z(m, *)
+    integer(c_int) :: info
+    integer :: nf, il, iu
+    integer, allocatable :: iwork(:)
+    real(8), allocatable :: work(:)
+    real(8) :: vl, vu
+    ! The vendored DSTEVX drops LAPACK's IFAIL argument (dstevx.f:5-6).
+    allocate(iwork(5*m), work(5*m))
+    vl = 0d0; vu = 0d0; il = 1; iu = m
+    call DSTEVX('V', 'A', m, d, e, vl, vu, il, iu, abstol, nf, w, z, m, work, &
+                iwork, info)
+    deallocate(iwork, work)
+  end subroutine ref_dstevx
+
+  ! HWHM (hwhm.f:1) Lorentzian damping of the assembled correlation.
+  subroutine ref_hwhm(m, dt, tf, tal, de, u, v) bind(C, name='ref_hwhm')
+    integer(c_int), value :: m
+    real(c_double), value :: dt, tf, tal, de
+    real(c_double) :: u(*), v(*)
+    real(8) :: t2
+    t2 = tal
+    call HWHM(m, dt, tf, t2, de, u, v)
+  end subroutine ref_hwhm
+
+  ! SPECTRUMTD (spectrumtd.f:1) with the .SG window (iwin=1) or none (0).
+  subroutine ref_spectrumtd(iwin, m, dt, tf, wp, de, u, v) &
+      bind(C, name='ref_spectrumtd')
+    integer(c_int), value :: iwin, m
+    real(c_double), value :: dt, tf, wp, de
+    real(c_double) :: u(*), v(*)
+    real(8) :: de2
+    de2 = de
+    if (iwin == 1) then
+      call SPECTRUMTD('.SG     ', m, dt, tf, wp, de2, u, v)
+    else
+      call SPECTRUMTD('.NONE   ', m, dt, tf, wp, de2, u, v)
+    end if
+  end subroutine ref_spectrumtd
+
+  ! RAN (rans.f:3): the linear congruential start-vector generator.
+  function ref_ran(nseed) result(r) bind(C, name='ref_ran')
+    integer(c_int) :: nseed
+    real(c_double) :: r
+    real(8), external :: RAN
+    r = RAN(nseed)
+  end function ref_ran
+
+  ! LANCZS (lanczs.f:1): eigenstates il..iu of H from the RAN start vector.
+  subroutine ref_lanczs(ndim, np, il, iu, nreort, abstol, shm, vpot, eigvl, &
+                        eigvc, ldv, info) bind(C, name='ref_lanczs')
+    integer(c_int), value :: ndim, il, iu, nreort, ldv
+    integer(c_int) :: np(3)
+    real(c_double), value :: abstol
+    real(c_double) :: shm(3), vpot(*), eigvl(*), eigvc(ldv, *)
+    integer(c_int) :: info
+    character(len=8) :: dims
+    integer :: n, i, nseed, nr
+    integer, allocatable :: iwk(:)
+    real(8), allocatable :: al(:), bt(:), v0(:), vi(:), wk1(:), wk2(:), &
+                            lnzvc(:,:), lanc(:,:), ev(:)
+    real(8) :: var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    allocate(iwk(5*n + 16), al(n), bt(n), v0(n), vi(n), wk1(5*n), wk2(5*n), &
+             lnzvc(ldv, max(nreort, iu - il + 1)), lanc(ldv, max(nreort, iu - il + 1)), &
+             ev(n))
+    var = 0d0
+    nseed = 1
+    nr = nreort
+    call dimstr(ndim, dims)
+    ! lanczs.f's LNZVC dummy is Krylov storage and its EIGVC dummy the
+    ! tridiagonal eigenvectors (espec.f:918-920 passes them swapped).
+    call LANCZS(.true., dims, il, iu, info, nr, nr, n, nseed, ldv, abstol, iwk, &
+                np, al, bt, ev, shm, v0, vi, vpot, wk1, wk2, var, lnzvc, lanc)
+    do i = 1, iu - il + 1
+      eigvl(i) = ev(i)
+      eigvc(1:n, i) = lnzvc(1:n, i)
+    end do
+    deallocate(iwk, al, bt, v0, vi, wk1, wk2, lnzvc, lanc, ev)
+  end subroutine ref_lanczs
+
+end module espec_ref_harness
