@@ -4,7 +4,7 @@
 //
 // Step (spofft.f:112-191):  psi <- B e^{-iV dt/2} F^-1 e^{-iT dt} F e^{-iV dt/2} psi
 //
-// spo_lds_fast (ESPEC_EXACT, FP64 or FP32): forward DIF stages leave the
+// spo_lds_fast (ESPEC_EXACT, FP64 or mixed): forward DIF stages leave the
 // spectrum digit-reversed, the kinetic phase is applied in that order and
 // the inverse DIT stages restore natural order, so no permutation pass is
 // needed.  The V phases are fused into the outermost butterfly stage (the
@@ -22,42 +22,52 @@
 
 namespace espec {
 
-template <typename R, int NT>
+// S is the storage precision of psi and psi0 in HBM (float for ESPEC_MIXED,
+// double for ESPEC_FP64); LDS, twiddles, phase tables and all arithmetic
+// are FP64, so the mixed path rounds psi to complex64 once per launch.
+template <typename S, int NT>
 __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
 {
-    using V = V2<R>;
+    using V = double2;
+    using VS = V2<S>;
     extern __shared__ double smem_d[];
     V *x = reinterpret_cast<V *>(smem_d);
-    double *red = reinterpret_cast<double *>(x + a.n);
+    V *vl = x + a.n;  // V phase table, staged in LDS when a.vph_lds
+    double *red = reinterpret_cast<double *>(x + (a.vph_lds ? 2 : 1) * a.n);
     const int pk = blockIdx.x;
     const long long po = (long long)pk * a.n;
-    V *gx = reinterpret_cast<V *>(a.x) + po;
-    const V *__restrict__ g0 = reinterpret_cast<const V *>(a.psi0) + po;
-    const V *__restrict__ gv = reinterpret_cast<const V *>(a.vph) + po;
+    VS *gx = reinterpret_cast<VS *>(a.x) + po;
+    const VS *__restrict__ g0 = reinterpret_cast<const VS *>(a.psi0) + po;
+    const V *__restrict__ gvg = reinterpret_cast<const V *>(a.vph) + po;
     const V *__restrict__ gk = reinterpret_cast<const V *>(a.kin);
     const V *__restrict__ tw = reinterpret_cast<const V *>(a.tw);
     const int n = a.n;
-    const R b = (R)a.b;
+    const double b = a.b;
     const DevAxis &ax0 = a.ax[0];
     const DevAxis &axl = a.ax[a.ndim - 1];
 
-    for (int i = threadIdx.x; i < n; i += NT) x[i] = gx[i];
+    for (int i = threadIdx.x; i < n; i += NT) {
+        VS t = gx[i];
+        x[i] = {(double)t.x, (double)t.y};
+        if (a.vph_lds) vl[i] = gvg[i];
+    }
     __syncthreads();
+    const V *gv = a.vph_lds ? vl : gvg;
 
-    fused_outer<R, NT, false, true>(x, ax0, tw, [&](int i, V &v) { v = cmulc(v, gv[i]); });
+    fused_outer<double, NT, false, true>(x, ax0, tw, [&](int i, V &v) { v = cmulc(v, gv[i]); });
     for (int st = 0; st < a.nsteps; ++st) {
         for (int d = 0; d < a.ndim; ++d) {
             const DevAxis &ax = a.ax[d];
             const int s0 = d == 0 ? 1 : 0;
             const int s1 = d == a.ndim - 1 ? ax.nst - 1 : ax.nst;
-            for (int s = s0; s < s1; ++s) dif_stage<R, NT, 1>(x, ax, s, tw);
+            for (int s = s0; s < s1; ++s) dif_stage<double, NT, 1>(x, ax, s, tw);
         }
-        fused_inner<R, NT>(x, axl, [&](int i, V &v) { v = cmulc(v, gk[i]); });
+        fused_inner<double, NT>(x, axl, [&](int i, V &v) { v = cmulc(v, gk[i]); });
         for (int d = a.ndim - 1; d >= 0; --d) {
             const DevAxis &ax = a.ax[d];
             const int s0 = d == 0 ? 1 : 0;
             const int s1 = d == a.ndim - 1 ? ax.nst - 1 : ax.nst;
-            for (int s = s1 - 1; s >= s0; --s) dit_stage<R, NT, -1>(x, ax, s, tw);
+            for (int s = s1 - 1; s >= s0; --s) dit_stage<double, NT, -1>(x, ax, s, tw);
         }
         const long long g = a.step0 + st + 1;
         const bool samp = (g % a.lc) == 0;
@@ -68,20 +78,20 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
             v.x *= b;
             v.y *= b;
             if (samp) {
-                V p = g0[i];
+                VS p = g0[i];
                 cr += (double)p.x * v.x + (double)p.y * v.y;
                 ci += (double)p.x * v.y - (double)p.y * v.x;
-                nr += (double)v.x * v.x + (double)v.y * v.y;
+                nr += v.x * v.x + v.y * v.y;
             }
             return w;
         };
         if (st + 1 < a.nsteps)
-            fused_outer<R, NT, true, true>(x, ax0, tw, [&](int i, V &v) {
+            fused_outer<double, NT, true, true>(x, ax0, tw, [&](int i, V &v) {
                 V w = post(i, v);
                 v = cmulc(v, w);
             });
         else
-            fused_outer<R, NT, true, false>(x, ax0, tw, [&](int i, V &v) { post(i, v); });
+            fused_outer<double, NT, true, false>(x, ax0, tw, [&](int i, V &v) { post(i, v); });
         if (samp) {
             block_sum3<NT>(cr, ci, nr, red);
             const long long nc = g / a.lc;
@@ -93,7 +103,7 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
             }
         }
     }
-    for (int i = threadIdx.x; i < n; i += NT) gx[i] = x[i];
+    for (int i = threadIdx.x; i < n; i += NT) gx[i] = {(S)x[i].x, (S)x[i].y};
 }
 
 template <int NT>
@@ -199,10 +209,9 @@ __global__ __launch_bounds__(NT) void spo_sample(const V2<R> *x, const V2<R> *ps
     }
 }
 
-size_t spo_lds_bytes(int n, bool mixed, bool ref, int nt)
+size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt)
 {
-    size_t v = mixed ? 8 : 16;
-    return v * n * (ref ? 2 : 1) + sizeof(double) * 3 * (nt / 64);
+    return sizeof(double2) * n * ((ref || vph_lds) ? 2 : 1) + sizeof(double) * 3 * (nt / 64);
 }
 
 template <typename K>
@@ -219,7 +228,7 @@ static hipError_t launch_lds(K kern, int grid, int nt, size_t lds, hipStream_t s
 
 hipError_t launch_spo_lds(const SpoLdsArgs &a, bool mixed, bool ref, int nt, hipStream_t s)
 {
-    size_t lds = spo_lds_bytes(a.n, mixed, ref, nt);
+    size_t lds = spo_lds_bytes(a.n, ref, !ref && a.vph_lds, nt);
     if (ref) {
         if (mixed) return hipErrorInvalidValue;
         return nt == 64 ? launch_lds(spo_lds_ref<64>, a.npk, 64, lds, s, a)

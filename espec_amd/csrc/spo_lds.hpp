@@ -7,20 +7,21 @@ namespace espec {
 struct SpoLdsArgs {
     int ndim, n, npk;
     DevAxis ax[3];
-    const void *tw;        // V2<R> twiddles, all axes
+    const void *tw;        // double2 twiddles, all axes
     const int *perm;       // digit-reversal maps, all axes
     int perm_off[3];
-    void *x;               // [npk][n] state
-    const void *psi0;      // [npk][n] initial state for C(t)
-    const void *vph;       // [npk][n] (cos, sin) of cst2 V, one table per packet
-    const void *kin;       // [n] (cos, sin) of cst1 T, shared (digit-reversed for the fast kernel)
+    void *x;               // [npk][n] state (complex64 mixed, complex128 FP64)
+    const void *psi0;      // [npk][n] initial state for C(t), same precision
+    const void *vph;       // [npk][n] double2 (cos, sin) of cst2 V, one table per packet
+    const void *kin;       // [n] double2 (cos, sin) of cst1 T, shared (digit-reversed for the fast kernel)
     double b;              // 1/N
     int nsteps, lc, maxsamp;
     long long step0;       // steps done before this launch
     double *cbuf;          // [npk][maxsamp][3] = (Re C, Im C, |psi|^2)
+    int vph_lds;           // stage the V phase table in LDS (fast kernel)
 };
 
-size_t spo_lds_bytes(int n, bool mixed, bool ref, int nt);
+size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt);
 hipError_t launch_spo_lds(const SpoLdsArgs &a, bool mixed, bool ref, int nt, hipStream_t s);
 hipError_t launch_spo_sample(const void *x, const void *psi0, int n, int npk, bool mixed,
                              double *cbuf, int maxsamp, int idx, hipStream_t s);

@@ -23,63 +23,69 @@ struct PassArgs {
     int n, npk, np0, xblocks;
     DevAxis axx;          // row block: lines = rows, es = 1
     DevAxis axs;          // strided tile: lines = tile columns, es = tile
-    const void *tw;
-    const void *kx;       // x kinetic phases (digit-reversed order)
-    const void *ks;       // strided-axis kinetic phases
-    void *x;
-    const void *psi0, *vph;
+    const double2 *tw;
+    const double2 *kx;    // x kinetic phases (digit-reversed order)
+    const double2 *ks;    // strided-axis kinetic phases
+    void *x;              // state, storage precision
+    const void *psi0;     // initial state, storage precision
+    const void *vpot;     // potential, storage precision
     double *part;
     long long *dstep;
-    double b;
+    double b, cst2;
     int lc, maxsamp;
     int S, L, ltile, tiles, last;
 };
 
-template <typename R, int NT, bool INV, bool FWD>
+// S: storage precision of psi, psi0 and V in HBM; LDS and arithmetic FP64.
+template <typename S, int NT, bool INV, bool FWD>
 __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
 {
-    using V = V2<R>;
+    using V = double2;
+    using VS = V2<S>;
     extern __shared__ double smem_d[];
     V *x = reinterpret_cast<V *>(smem_d);
     const int cnt = a.axx.nl * a.np0;
     double *red = reinterpret_cast<double *>(x + cnt);
     const int pk = blockIdx.y, blk = blockIdx.x;
     const long long po = (long long)pk * a.n + (long long)blk * cnt;
-    V *gx = reinterpret_cast<V *>(a.x) + po;
-    const V *__restrict__ gv = reinterpret_cast<const V *>(a.vph) + po;
-    const V *__restrict__ g0 = reinterpret_cast<const V *>(a.psi0) + po;
-    const V *__restrict__ tw = reinterpret_cast<const V *>(a.tw);
-    const V *__restrict__ kx = reinterpret_cast<const V *>(a.kx);
-    const R b = (R)a.b;
+    VS *gx = reinterpret_cast<VS *>(a.x) + po;
+    const S *__restrict__ gv = reinterpret_cast<const S *>(a.vpot) + po;
+    const VS *__restrict__ g0 = reinterpret_cast<const VS *>(a.psi0) + po;
+    const double b = a.b;
 
-    for (int i = threadIdx.x; i < cnt; i += NT) x[i] = gx[i];
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+        VS t = gx[i];
+        x[i] = {(double)t.x, (double)t.y};
+    }
     __syncthreads();
     const long long g = INV ? *a.dstep : 0;  // the step this pass completes
     const bool samp = INV && (g % a.lc) == 0;
     if (INV)
-        for (int s = a.axx.nst - 1; s >= 1; --s) dit_stage<R, NT, -1>(x, a.axx, s, tw);
+        for (int s = a.axx.nst - 1; s >= 1; --s) dit_stage<double, NT, -1>(x, a.axx, s, a.tw);
     double cr = 0, ci = 0, nr = 0;
-    fused_outer<R, NT, INV, FWD>(x, a.axx, tw, [&](int i, V &v) {
-        V w = gv[i];
+    fused_outer<double, NT, INV, FWD>(x, a.axx, a.tw, [&](int i, V &v) {
+        double sn, cs;
+        sincos(a.cst2 * (double)gv[i], &sn, &cs);
+        const V w = {cs, sn};
         if (INV) {
             v = cmulc(v, w);
             v.x *= b;
             v.y *= b;
             if (samp) {
-                V p = g0[i];
+                VS p = g0[i];
                 cr += (double)p.x * v.x + (double)p.y * v.y;
                 ci += (double)p.x * v.y - (double)p.y * v.x;
-                nr += (double)v.x * v.x + (double)v.y * v.y;
+                nr += v.x * v.x + v.y * v.y;
             }
         }
         if (FWD) v = cmulc(v, w);
     });
     if (FWD) {
-        for (int s = 1; s < a.axx.nst; ++s) dif_stage<R, NT, 1>(x, a.axx, s, tw);
-        for (int i = threadIdx.x; i < cnt; i += NT) x[i] = cmulc(x[i], kx[i & (a.np0 - 1)]);
+        for (int s = 1; s < a.axx.nst; ++s) dif_stage<double, NT, 1>(x, a.axx, s, a.tw);
+        for (int i = threadIdx.x; i < cnt; i += NT) x[i] = cmulc(x[i], a.kx[i & (a.np0 - 1)]);
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < cnt; i += NT) gx[i] = x[i];
+    for (int i = threadIdx.x; i < cnt; i += NT) gx[i] = {(S)x[i].x, (S)x[i].y};
     if (samp) {
         block_sum3<NT>(cr, ci, nr, red);
         const long long nc = g / a.lc;
@@ -92,10 +98,11 @@ __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
     }
 }
 
-template <typename R, int NT>
+template <typename S, int NT>
 __global__ __launch_bounds__(NT) void pass_strided(PassArgs a)
 {
-    using V = V2<R>;
+    using V = double2;
+    using VS = V2<S>;
     extern __shared__ double smem_d[];
     V *x = reinterpret_cast<V *>(smem_d);
     const int tile = 1 << a.ltile;
@@ -104,16 +111,18 @@ __global__ __launch_bounds__(NT) void pass_strided(PassArgs a)
     const int spt = a.S >> a.ltile;  // tiles across the lower axes
     const long long base =
         (long long)pk * a.n + (long long)(t / spt) * a.S * a.L + (long long)(t % spt) * tile;
-    V *gx = reinterpret_cast<V *>(a.x) + base;
-    const V *__restrict__ tw = reinterpret_cast<const V *>(a.tw);
-    const V *__restrict__ ks = reinterpret_cast<const V *>(a.ks);
+    VS *gx = reinterpret_cast<VS *>(a.x) + base;
 
-    for (int i = threadIdx.x; i < cnt; i += NT) x[i] = gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))];
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+        VS v = gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))];
+        x[i] = {(double)v.x, (double)v.y};
+    }
     __syncthreads();
-    for (int s = 0; s < a.axs.nst - 1; ++s) dif_stage<R, NT, 1>(x, a.axs, s, tw);
-    fused_inner<R, NT>(x, a.axs, [&](int i, V &v) { v = cmulc(v, ks[i >> a.ltile]); });
-    for (int s = a.axs.nst - 2; s >= 0; --s) dit_stage<R, NT, -1>(x, a.axs, s, tw);
-    for (int i = threadIdx.x; i < cnt; i += NT) gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))] = x[i];
+    for (int s = 0; s < a.axs.nst - 1; ++s) dif_stage<double, NT, 1>(x, a.axs, s, a.tw);
+    fused_inner<double, NT>(x, a.axs, [&](int i, V &v) { v = cmulc(v, a.ks[i >> a.ltile]); });
+    for (int s = a.axs.nst - 2; s >= 0; --s) dit_stage<double, NT, -1>(x, a.axs, s, a.tw);
+    for (int i = threadIdx.x; i < cnt; i += NT)
+        gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))] = {(S)x[i].x, (S)x[i].y};
     if (a.last && t == 0 && pk == 0 && threadIdx.x == 0) *a.dstep += 1;
 }
 
@@ -178,7 +187,7 @@ template <typename R>
 hipError_t rows(const SpoPassPlan &p, const PassArgs &a, bool inv, bool fwd, hipStream_t s)
 {
     const int nt = 64 * p.rows_per_wg;
-    size_t lds = sizeof(V2<R>) * p.rows_per_wg * p.np[0] + sizeof(double) * 3 * (nt / 64);
+    size_t lds = sizeof(double2) * p.rows_per_wg * p.np[0] + sizeof(double) * 3 * (nt / 64);
     dim3 grid(p.xblocks, p.npk);
 #define ESPEC_ROWS(NT_)                                                                  \
     if (nt == NT_) {                                                                     \
@@ -197,14 +206,14 @@ template <typename R>
 hipError_t strided(const SpoPassPlan &p, PassArgs a, int axis, bool last, hipStream_t s)
 {
     a.axs = p.axs[axis];
-    a.ks = static_cast<const V2<R> *>(p.kax) + p.kax_off[axis];
+    a.ks = static_cast<const double2 *>(p.kax) + p.kax_off[axis];
     a.S = 1;
     for (int k = 0; k < axis; ++k) a.S *= p.np[k];
     a.L = p.np[axis];
     a.ltile = ilog2(p.tile);
     a.tiles = p.n / (p.tile * a.L);
     a.last = last ? 1 : 0;
-    size_t lds = sizeof(V2<R>) * p.tile * a.L;
+    size_t lds = sizeof(double2) * p.tile * a.L;
     return launch(pass_strided<R, 256>, dim3(a.tiles, p.npk), 256, lds, s, a);
 }
 
@@ -217,11 +226,12 @@ PassArgs base_args(const SpoPassPlan &p)
     a.np0 = p.np[0];
     a.xblocks = p.xblocks;
     a.axx = p.axx;
-    a.tw = p.tw;
-    a.kx = static_cast<const V2<R> *>(p.kax) + p.kax_off[0];
+    a.tw = static_cast<const double2 *>(p.tw);
+    a.kx = static_cast<const double2 *>(p.kax) + p.kax_off[0];
     a.x = p.x;
     a.psi0 = p.psi0;
-    a.vph = p.vph;
+    a.vpot = p.vpot;
+    a.cst2 = p.cst2;
     a.part = p.part;
     a.dstep = p.dstep;
     a.b = p.b;
@@ -299,9 +309,10 @@ int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *l
 
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
-                   double cst1, void *x, const void *psi0, const void *vph, double *cbuf,
-                   int maxsamp)
+                   double cst1, double cst2, void *x, const void *psi0, const void *vpot,
+                   double *cbuf, int maxsamp)
 {
+    p.cst2 = cst2;
     p.ndim = ndim;
     p.n = 1;
     for (int a = 0; a < ndim; ++a) {
@@ -313,7 +324,7 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
     p.maxsamp = maxsamp;
     p.x = x;
     p.psi0 = psi0;
-    p.vph = vph;
+    p.vpot = vpot;
     p.cbuf = cbuf;
     p.b = 1.0 / p.n;
     p.tile = mixed ? 16 : 8;  // 128-byte rows of the strided tiles
@@ -344,14 +355,8 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
         }
     }
     DevBuf btw, bk, bpart, bstep;
-    if (mixed) {
-        std::vector<float> twf(tw.begin(), tw.end()), kf(kax.begin(), kax.end());
-        ESPEC_HIP_TRY(btw.upload(twf.data(), sizeof(float) * twf.size()));
-        ESPEC_HIP_TRY(bk.upload(kf.data(), sizeof(float) * kf.size()));
-    } else {
-        ESPEC_HIP_TRY(btw.upload(tw.data(), sizeof(double) * tw.size()));
-        ESPEC_HIP_TRY(bk.upload(kax.data(), sizeof(double) * kax.size()));
-    }
+    ESPEC_HIP_TRY(btw.upload(tw.data(), sizeof(double) * tw.size()));
+    ESPEC_HIP_TRY(bk.upload(kax.data(), sizeof(double) * kax.size()));
     ESPEC_HIP_TRY(bpart.alloc(sizeof(double) * 3 * (size_t)maxsamp * npk * p.xblocks));
     ESPEC_HIP_TRY(hipMemset(bpart.p, 0, bpart.bytes));
     ESPEC_HIP_TRY(bstep.alloc(sizeof(long long)));

@@ -15,14 +15,14 @@ struct SpoPassPlan {
     DevAxis axx{};         // LDS geometry of a row-pass block
     DevAxis axs[3]{};      // LDS geometry of strided-pass tiles (axes 1, 2)
     void *x = nullptr;
-    const void *psi0 = nullptr, *vph = nullptr;
+    const void *psi0 = nullptr, *vpot = nullptr;  // storage precision
     double *cbuf = nullptr;
-    void *tw = nullptr;    // twiddles, all axes
-    void *kax = nullptr;   // per-axis kinetic phases in digit-reversed order
+    void *tw = nullptr;    // double2 twiddles, all axes
+    void *kax = nullptr;   // double2 per-axis kinetic phases in digit-reversed order
     int kax_off[3] = {};
     double *part = nullptr;   // [maxsamp][npk][xblocks][3] row-pass partial sums
     long long *dstep = nullptr;  // steps completed, advanced on the device
-    double b = 1;
+    double b = 1, cst2 = 0;
     int lc = 1;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -31,8 +31,8 @@ struct SpoPassPlan {
 
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
-                   double cst1, void *x, const void *psi0, const void *vph, double *cbuf,
-                   int maxsamp);
+                   double cst1, double cst2, void *x, const void *psi0, const void *vpot,
+                   double *cbuf, int maxsamp);
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s,
                  int *launches);
 void spo_pass_release(SpoPassPlan &p);

@@ -66,14 +66,16 @@ void axis_geometry(int ndim, const int *np, int a, int &es, int &nl, int &lmod, 
     lb = es * np[a];    // jump over the transformed axis
 }
 
-template <typename R>
+// S is the storage precision of psi (float: ESPEC_MIXED); every table the
+// kernels multiply by (twiddles, V and T phases) is FP64.
+template <typename S>
 int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const double *vpot,
                  const double *u, const double *v)
 {
     const int n = P.n, npk = P.npk;
     const double cst1 = P.dt / kShfs, cst2 = P.dt / (2.0 * kShfs);
     std::vector<FftPlan> fp;
-    std::vector<R> tw;
+    std::vector<double> tw;
     std::vector<int> perm;
     for (int a = 0; a < P.ndim; ++a) {
         fp.push_back(make_fft_plan(P.np[a], P.ref));
@@ -81,17 +83,37 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         axis_geometry(P.ndim, P.np, a, es, nl, lmod, la, lb);
         fill_axis(P.la.ax[a], fp[a], es, nl, lmod, la, lb, (int)(tw.size() / 2));
         for (size_t i = 0; i < fp[a].twc.size(); ++i) {
-            tw.push_back((R)fp[a].twc[i]);
-            tw.push_back((R)fp[a].tws[i]);
+            tw.push_back(fp[a].twc[i]);
+            tw.push_back(fp[a].tws[i]);
         }
         P.la.perm_off[a] = (int)perm.size();
         perm.insert(perm.end(), fp[a].perm.begin(), fp[a].perm.end());
+    }
+    std::vector<S> xs((size_t)2 * n * npk);
+    for (int p = 0; p < npk; ++p) {
+        std::vector<S> t = interleave<S>(u + (size_t)p * n, v ? v + (size_t)p * n : nullptr, n);
+        std::memcpy(xs.data() + (size_t)2 * n * p, t.data(), sizeof(S) * 2 * n);
+    }
+    ESPEC_HIP_TRY(P.x.upload(xs.data(), sizeof(S) * xs.size()));
+    ESPEC_HIP_TRY(P.psi0.upload(xs.data(), sizeof(S) * xs.size()));
+    ESPEC_HIP_TRY(P.cbuf.alloc(sizeof(double) * 3 * (size_t)P.maxsamp * npk));
+    ESPEC_HIP_TRY(hipMemset(P.cbuf.p, 0, P.cbuf.bytes));
+    if (!P.lds) {
+        // Multi-pass: the potential itself in storage precision; the
+        // kernels form exp(-i cst2 V) in FP64.
+        std::vector<S> vs((size_t)n * npk);
+        for (size_t i = 0; i < vs.size(); ++i) vs[i] = (S)vpot[i];
+        ESPEC_HIP_TRY(P.vph.upload(vs.data(), sizeof(S) * vs.size()));
+        std::vector<std::vector<double>> taxes;
+        for (int a = 0; a < P.ndim; ++a) taxes.push_back(kinetic_axis(P.np[a], sh[a], sm[a], false));
+        return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, cst2, P.x.p,
+                              P.psi0.p, P.vph.p, P.cbuf.as<double>(), P.maxsamp);
     }
     // Kinetic phase: natural order for the reference kernel; for the fused
     // kernel, position of frequency k is sum_a perm_a[k_a] * stride_a.
     std::vector<double> tc = kinetic_grid(P.ndim, P.np, sh, sm, P.ref);
     std::vector<double> tcs(n);
-    if (P.ref || !P.lds) {
+    if (P.ref) {
         tcs = tc;
     } else {
         for (int k = 0; k < n; ++k) {
@@ -104,31 +126,16 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
             tcs[pos] = tc[k];
         }
     }
-    std::vector<R> kin = phase_table<R>(tcs.data(), n, cst1);
-    std::vector<R> vph((size_t)2 * n * npk);
+    std::vector<double> kin = phase_table<double>(tcs.data(), n, cst1);
+    std::vector<double> vph((size_t)2 * n * npk);
     for (int p = 0; p < npk; ++p) {
-        std::vector<R> t = phase_table<R>(vpot + (size_t)p * n, n, cst2);
-        std::memcpy(vph.data() + (size_t)2 * n * p, t.data(), sizeof(R) * 2 * n);
+        std::vector<double> t = phase_table<double>(vpot + (size_t)p * n, n, cst2);
+        std::memcpy(vph.data() + (size_t)2 * n * p, t.data(), sizeof(double) * 2 * n);
     }
-    std::vector<R> xs((size_t)2 * n * npk);
-    for (int p = 0; p < npk; ++p) {
-        std::vector<R> t = interleave<R>(u + (size_t)p * n, v ? v + (size_t)p * n : nullptr, n);
-        std::memcpy(xs.data() + (size_t)2 * n * p, t.data(), sizeof(R) * 2 * n);
-    }
-    ESPEC_HIP_TRY(P.x.upload(xs.data(), sizeof(R) * xs.size()));
-    ESPEC_HIP_TRY(P.psi0.upload(xs.data(), sizeof(R) * xs.size()));
-    ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(R) * vph.size()));
-    ESPEC_HIP_TRY(P.kin.upload(kin.data(), sizeof(R) * kin.size()));
-    ESPEC_HIP_TRY(P.tw.upload(tw.data(), sizeof(R) * (tw.size() ? tw.size() : 2)));
+    ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(double) * vph.size()));
+    ESPEC_HIP_TRY(P.kin.upload(kin.data(), sizeof(double) * kin.size()));
+    ESPEC_HIP_TRY(P.tw.upload(tw.data(), sizeof(double) * (tw.size() ? tw.size() : 2)));
     ESPEC_HIP_TRY(P.perm.upload(perm.data(), sizeof(int) * perm.size()));
-    ESPEC_HIP_TRY(P.cbuf.alloc(sizeof(double) * 3 * (size_t)P.maxsamp * npk));
-    ESPEC_HIP_TRY(hipMemset(P.cbuf.p, 0, P.cbuf.bytes));
-    if (!P.lds) {
-        std::vector<std::vector<double>> taxes;
-        for (int a = 0; a < P.ndim; ++a) taxes.push_back(kinetic_axis(P.np[a], sh[a], sm[a], false));
-        return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, P.x.p, P.psi0.p,
-                              P.vph.p, P.cbuf.as<double>(), P.maxsamp);
-    }
     return ESPEC_OK;
 }
 
@@ -193,7 +200,11 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
     if (P->ref && P->mixed) return ESPEC_ERR_UNSUPPORTED;
     P->b = 1.0 / P->n;
     P->nt = P->n <= 1024 ? 64 : 256;
-    P->lds = spo_lds_bytes(P->n, P->mixed, P->ref, P->nt) <= 160 * 1024 - 1024;
+    // LDS-resident when the FP64 grid fits in one workgroup's LDS; the V
+    // phase table is staged next to it when that fits too.
+    const size_t lds_cap = 160 * 1024 - 1024;
+    P->lds = spo_lds_bytes(P->n, P->ref, false, P->nt) <= lds_cap;
+    P->la.vph_lds = (!P->ref && spo_lds_bytes(P->n, false, true, P->nt) <= lds_cap) ? 1 : 0;
     if (!P->lds && P->ref) return ESPEC_ERR_UNSUPPORTED;
     ESPEC_HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     ESPEC_HIP_TRY(hipEventCreate(&P->ev0));
