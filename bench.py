@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Throughput of the split-operator FFT propagation on one MI355X.
+
+Workload: the 256x256 coupled-Morse surface (two Morse modes, De=0.17 Eh,
+a=1.0/bohr, masses 1836 and 918 m_e, bilinear coupling 0.005 q1 q2), product
+Gaussian packet (sigma 0.2 bohr, displaced +0.3 bohr), dt = 1.0 a.u., C(t)
+every step.  A "step" is one split-operator time step of the packet.
+
+Reported: grid-point-steps/s of the mixed path (complex64 state in HBM,
+FP64 arithmetic in the kernels) as `value`, the FP64 path beside it, the
+largest |C(t) - C_oracle(t)| over the timed run, a roofline object for the
+dominant kernel (HIP events on the plan's stream) and the FP64 C oracle
+timed on the host as the CPU baseline.  Inputs are resident in HBM before
+the timed region starts; only the propagation loop is timed.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 100]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "grid-point-steps/s, mixed vs FP64 SO-FFT on 1 GPU at |dC(t)|<=1e-6"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_run(E, c, flags, steps, warmup):
+    """Create a plan (inputs go to HBM), warm up, restart from psi0, time
+    `steps` steps between synchronisations.  Returns (seconds, C(t), plan)."""
+    plan = E.SpoPlan(c.np, c.dt_fs, 1, c.sh, c.sm, c.vpot, c.u, None, maxsamp=steps + 1,
+                     flags=flags)
+    if warmup:
+        plan.run(warmup)
+        plan.sync()
+        plan.reset()
+    plan.sync()
+    t0 = time.perf_counter()
+    plan.run(steps)
+    plan.sync()
+    t1 = time.perf_counter()
+    res = plan.get(0, psi=False)
+    return t1 - t0, res, plan
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{name}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
+    ap.add_argument("--profile-steps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the propagation of one packet (or a batch of "
+              "independent packets) runs on a single MI355X and no multi-GPU path is "
+              "built; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import espec_amd as E
+    from espec_amd import configs
+
+    c = configs.config2(nsteps=args.steps)
+    npts = c.n
+    K, W = args.steps, args.warmup
+
+    log(f"[bench] mixed path: {c.meta['name']} {K} steps (warmup {W})")
+    sec_mix, res_mix, plan_mix = timed_run(E, c, E.MIXED | E.EXACT, K, W)
+    ev_ms_mix, launches = plan_mix.timing()
+    log(f"[bench] mixed: {sec_mix * 1e3:.2f} ms host, {ev_ms_mix:.2f} ms events, {launches} launches")
+    log(f"[bench] FP64 path")
+    sec_64, res_64, plan_64 = timed_run(E, c, E.FP64 | E.EXACT, K, W)
+    log(f"[bench] fp64: {sec_64 * 1e3:.2f} ms host")
+
+    # Per-kernel durations (HIP events on the plan's stream, no graph).
+    plan_mix.reset()
+    prof = plan_mix.profile(args.profile_steps)
+    plan_mix.reset()
+    names = ["pass_rows (x pass: V phases, C(t), x FFT pair)", "pass_strided (y pass: y FFT pair)"]
+    kinds = [dict(kernel=names[i] if len(prof) > 1 else "spo_lds_fast", ms=m, bytes=b)
+             for i, (m, b) in enumerate(prof)]
+    dom = max(kinds, key=lambda k: k["ms"])
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3)
+    log(f"[bench] kernels: " + ", ".join(f"{k['kernel'].split()[0]} {k['ms'] * 1e3:.2f} us" for k in kinds))
+
+    ms_bw, by_bw = E.bw_probe(1 << 30, 10)
+    copy_bw = by_bw / (ms_bw * 1e-3)
+    log(f"[bench] copy kernel: {copy_bw / 1e9:.0f} GB/s")
+
+    out = {
+        "metric": METRIC,
+        "value": npts * K / sec_mix,
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": sec_mix * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage": "complex64",
+        "data": "synthetic",
+        "config": {"workload": "256x256 coupled-Morse split-operator FFT (2-D), mixed path",
+                   "grid": c.np, "dt_au": 1.0, "lc": 1, "steps": K,
+                   "launches_per_run": launches},
+        "fp64": {"value": npts * K / sec_64, "ms_per_step": sec_64 * 1e3 / K,
+                 "mixed_over_fp64": sec_64 / sec_mix},
+        "kernels": kinds,
+        "copy_kernel_GBps": copy_bw / 1e9,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom["kernel"],
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": pmc_traffic("config2_mixed"),
+        },
+    }
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle, EXACT
+        O = Oracle()
+        tf = (K - 1) * c.dt_fs  # the DO loop runs (TF-TI)/DT + 1 steps
+        log(f"[bench] CPU oracle (FP64 C, 1 thread): {K} steps of the same packet")
+        t0 = time.perf_counter()
+        o = O.pspofft(c.np, c.dt_fs, 0.0, tf, 1, c.sh, c.sm, c.u, c.v, c.vpot, EXACT)
+        t_cpu = time.perf_counter() - t0
+        n = min(len(o["cr"]), len(res_mix["cr"]))
+        dmix = max(np.abs(o["cr"][:n] - res_mix["cr"][:n]).max(),
+                   np.abs(o["ci"][:n] - res_mix["ci"][:n]).max())
+        d64 = max(np.abs(o["cr"][:n] - res_64["cr"][:n]).max(),
+                  np.abs(o["ci"][:n] - res_64["ci"][:n]).max())
+        out["max_abs_dC"] = {"mixed": float(dmix), "fp64": float(d64), "samples": int(n)}
+        out["norm_drift"] = {"mixed": float(np.abs(res_mix["norm"][:n] - 1).max()),
+                             "fp64": float(np.abs(res_64["norm"][:n] - 1).max())}
+        out["cpu_baseline"] = {"value": npts * K / t_cpu, "unit": "grid-point-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"the same 256x256 packet, {K} steps, FP64 C oracle "
+                                         f"(oracle/espec_oracle.c), {t_cpu:.1f} s"}
+        log(f"[bench] CPU: {t_cpu:.1f} s; max|dC| mixed {dmix:.2e} fp64 {d64:.2e}")
+    plan_mix.close()
+    plan_64.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

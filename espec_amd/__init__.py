@@ -58,6 +58,9 @@ def _load():
     L.espec_spo_plan_get.argtypes = [vp, ip, vp, vp, vp, ip, P(ip), vp, vp]
     L.espec_spo_plan_last_timing.argtypes = [vp, P(dp), P(ip)]
     L.espec_spo_plan_destroy.argtypes = [vp]
+    L.espec_spo_plan_reset.argtypes = [vp]
+    L.espec_spo_plan_profile.argtypes = [vp, ip, d, d, ip, P(ip)]
+    L.espec_bw_probe.argtypes = [C.c_size_t, ip, P(dp), P(dp)]
     _lib = L
     return L
 
@@ -91,6 +94,13 @@ def nsteps(ti, tf, dt):
     """Trip count of DO T=TI+DT,TF+DT,DT (pspofft.f:110)."""
     c = ((tf + dt) - (ti + dt) + dt) / dt
     return max(int(c), 0)
+
+
+def bw_probe(nbytes=1 << 30, iters=20):
+    """Streaming copy kernel: (mean ms per launch, bytes moved per launch)."""
+    ms, by = C.c_double(), C.c_double()
+    _check(_load().espec_bw_probe(nbytes, iters, C.byref(ms), C.byref(by)), "bw_probe")
+    return ms.value, by.value
 
 
 def device_count():
@@ -167,6 +177,19 @@ class SpoPlan:
         if psi:
             out.update(u=u, v=v)
         return out
+
+    def reset(self):
+        """Restart from psi0 at step 0 (tables and captured graphs kept)."""
+        _check(self._L.espec_spo_plan_reset(self._h), "spo_plan_reset")
+
+    def profile(self, nsteps):
+        """Per-kernel HIP-event timing: list of (mean ms, algorithmic bytes)
+        per kernel kind (0: step kernel or row pass, 1: strided passes)."""
+        ms, by = np.zeros(4), np.zeros(4)
+        k = C.c_int()
+        _check(self._L.espec_spo_plan_profile(self._h, nsteps, ms, by, 4, C.byref(k)),
+               "spo_plan_profile")
+        return [(ms[i], by[i]) for i in range(k.value)]
 
     def close(self):
         if self._h:

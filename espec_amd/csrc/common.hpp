@@ -33,16 +33,33 @@ template <int SIGN, typename V> __device__ __forceinline__ V ctw(V a, V w)
 }
 
 // Stage plan of one axis on the device (see FftPlan).
+// All sizes are powers of two, so the geometry is kept as shift counts:
+// element e of line l sits at base(l) + (e << les), with
+// base(l) = ((l & (lmod-1)) << lsa) + ((l >> llmod) << lsb).
 struct DevAxis {
-    int n, ln, nst, es;  // length, log2 length, stages, element stride
-    int nl, lnl, lmod, la, lb;  // lines (power of two): base(l) = (l % lmod)*la + (l / lmod)*lb
-    int tw0;               // offset of this axis' twiddles in the table
+    int n, ln, nst, es, les;        // length, log2 length, stages, element stride, log2 stride
+    int nl, lnl, lmod, llmod, lsa, lsb;
+    int tw0;                        // offset of this axis' twiddles in the table
     int rad[kMaxStages], lm[kMaxStages], lL[kMaxStages], off[kMaxStages];
 };
 
 __device__ __forceinline__ int line_base(const DevAxis &a, int l)
 {
-    return (l % a.lmod) * a.la + (l / a.lmod) * a.lb;
+    return ((l & (a.lmod - 1)) << a.lsa) + ((l >> a.llmod) << a.lsb);
+}
+
+// Host-side: fill the shift form of a line geometry given as
+// base(l) = (l % lmod) * la + (l / lmod) * lb (la, lb powers of two or 0).
+inline void set_line_geometry(DevAxis &d, int es, int nl, int lmod, int la, int lb)
+{
+    d.es = es;
+    d.les = ilog2(es);
+    d.nl = nl;
+    d.lnl = ilog2(nl);
+    d.lmod = lmod;
+    d.llmod = ilog2(lmod);
+    d.lsa = la > 0 ? ilog2(la) : 0;
+    d.lsb = lb > 0 ? ilog2(lb) : 0;
 }
 
 // Wave-level FP64 sum (64 lanes).

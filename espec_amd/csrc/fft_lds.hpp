@@ -53,8 +53,8 @@ __device__ __forceinline__ Bfly bfly_at(const DevAxis &ax, int s, int g, int lnb
     Bfly b;
     b.j = bf & ((1 << lm) - 1);
     b.e0 = ((bf >> lm) << ax.lL[s]) + b.j;
-    b.base = line_base(ax, l) + b.e0 * ax.es;
-    b.step = (1 << lm) * ax.es;
+    b.base = line_base(ax, l) + (b.e0 << ax.les);
+    b.step = 1 << (lm + ax.les);
     return b;
 }
 
@@ -252,7 +252,7 @@ __device__ void permute_axis(const V2<R> *x, V2<R> *y, const DevAxis &ax, const 
             k = g >> ax.lnl;
         }
         int lb = line_base(ax, l);
-        y[lb + k * ax.es] = x[lb + perm[k] * ax.es];
+        y[lb + (k << ax.les)] = x[lb + (perm[k] << ax.les)];
     }
     __syncthreads();
 }

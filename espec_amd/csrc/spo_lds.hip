@@ -31,16 +31,19 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
     using V = double2;
     using VS = V2<S>;
     extern __shared__ double smem_d[];
+    // LDS: psi | V phases (vph_lds) | kinetic phases, twiddles (tab_lds) | reduction
     V *x = reinterpret_cast<V *>(smem_d);
-    V *vl = x + a.n;  // V phase table, staged in LDS when a.vph_lds
-    double *red = reinterpret_cast<double *>(x + (a.vph_lds ? 2 : 1) * a.n);
+    V *vl = x + a.n;
+    V *kl = vl + (a.vph_lds ? a.n : 0);
+    V *twl = kl + (a.tab_lds ? a.n : 0);
+    double *red = reinterpret_cast<double *>(twl + (a.tab_lds ? a.ntw : 0));
     const int pk = blockIdx.x;
     const long long po = (long long)pk * a.n;
     VS *gx = reinterpret_cast<VS *>(a.x) + po;
     const VS *__restrict__ g0 = reinterpret_cast<const VS *>(a.psi0) + po;
     const V *__restrict__ gvg = reinterpret_cast<const V *>(a.vph) + po;
-    const V *__restrict__ gk = reinterpret_cast<const V *>(a.kin);
-    const V *__restrict__ tw = reinterpret_cast<const V *>(a.tw);
+    const V *__restrict__ gkg = reinterpret_cast<const V *>(a.kin);
+    const V *__restrict__ twg = reinterpret_cast<const V *>(a.tw);
     const int n = a.n;
     const double b = a.b;
     const DevAxis &ax0 = a.ax[0];
@@ -50,9 +53,14 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
         VS t = gx[i];
         x[i] = {(double)t.x, (double)t.y};
         if (a.vph_lds) vl[i] = gvg[i];
+        if (a.tab_lds) kl[i] = gkg[i];
     }
+    if (a.tab_lds)
+        for (int j = threadIdx.x; j < a.ntw; j += NT) twl[j] = twg[j];
     __syncthreads();
     const V *gv = a.vph_lds ? vl : gvg;
+    const V *gk = a.tab_lds ? kl : gkg;
+    const V *tw = a.tab_lds ? twl : twg;
 
     fused_outer<double, NT, false, true>(x, ax0, tw, [&](int i, V &v) { v = cmulc(v, gv[i]); });
     for (int st = 0; st < a.nsteps; ++st) {
@@ -209,9 +217,11 @@ __global__ __launch_bounds__(NT) void spo_sample(const V2<R> *x, const V2<R> *ps
     }
 }
 
-size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt)
+size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt, bool tab_lds, int ntw)
 {
-    return sizeof(double2) * n * ((ref || vph_lds) ? 2 : 1) + sizeof(double) * 3 * (nt / 64);
+    size_t v = (ref || vph_lds) ? 2 : 1;
+    if (!ref && tab_lds) v += 1;
+    return sizeof(double2) * (n * v + (!ref && tab_lds ? ntw : 0)) + sizeof(double) * 3 * (nt / 64);
 }
 
 template <typename K>
@@ -228,7 +238,7 @@ static hipError_t launch_lds(K kern, int grid, int nt, size_t lds, hipStream_t s
 
 hipError_t launch_spo_lds(const SpoLdsArgs &a, bool mixed, bool ref, int nt, hipStream_t s)
 {
-    size_t lds = spo_lds_bytes(a.n, ref, !ref && a.vph_lds, nt);
+    size_t lds = spo_lds_bytes(a.n, ref, !ref && a.vph_lds, nt, a.tab_lds, a.ntw);
     if (ref) {
         if (mixed) return hipErrorInvalidValue;
         return nt == 64 ? launch_lds(spo_lds_ref<64>, a.npk, 64, lds, s, a)

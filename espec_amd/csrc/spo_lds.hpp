@@ -19,9 +19,11 @@ struct SpoLdsArgs {
     long long step0;       // steps done before this launch
     double *cbuf;          // [npk][maxsamp][3] = (Re C, Im C, |psi|^2)
     int vph_lds;           // stage the V phase table in LDS (fast kernel)
+    int tab_lds;           // stage kinetic phases and twiddles in LDS (fast kernel)
+    int ntw;               // twiddle entries, all axes
 };
 
-size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt);
+size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt, bool tab_lds = false, int ntw = 0);
 hipError_t launch_spo_lds(const SpoLdsArgs &a, bool mixed, bool ref, int nt, hipStream_t s);
 hipError_t launch_spo_sample(const void *x, const void *psi0, int n, int npk, bool mixed,
                              double *cbuf, int maxsamp, int idx, hipStream_t s);

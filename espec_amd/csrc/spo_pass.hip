@@ -5,13 +5,19 @@
 //               e^{-iV dt/2}  [forward x stages of step k+1]  e^{-iTx dt}
 //               -> state stored with x digit-reversed, y natural
 //   y pass    : forward y, e^{-iTy dt}, inverse y on tiles of contiguous
-//               x positions (coalesced 128-byte rows), y natural again
+//               x positions, y natural again
 // Since e^{-i(Tx+Ty)dt} = e^{-iTx dt} e^{-iTy dt}, the kinetic phase splits
 // per axis and each pass keeps one axis of the transform.  The state makes
 // one round trip through L2/MALL per pass; the V phases ride on the row
-// pass and the kinetic phases on the butterflies.  C(t) partial sums are
-// written per row block and reduced after the run (deterministic order).
+// pass and the kinetic phases on the butterflies.  Each kernel issues all
+// of its global loads (state, V, psi0, twiddles, kinetic phases) in one
+// prologue and then works from LDS only, so a pass costs one memory
+// latency rather than one per FFT stage.  C(t) partial sums are written
+// per row block and reduced after the run (deterministic order).
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "api_common.hpp"
 #include "fft_lds.hpp"
@@ -19,33 +25,50 @@
 
 namespace espec {
 
+constexpr int kMaxEpt = 16;  // elements per thread held in registers by the prologue
+
 struct PassArgs {
-    int n, npk, np0, xblocks;
-    DevAxis axx;          // row block: lines = rows, es = 1
-    DevAxis axs;          // strided tile: lines = tile columns, es = tile
-    const double2 *tw;
-    const double2 *kx;    // x kinetic phases (digit-reversed order)
-    const double2 *ks;    // strided-axis kinetic phases
-    void *x;              // state, storage precision
-    const void *psi0;     // initial state, storage precision
-    const void *vpot;     // potential, storage precision
+    int n, npk, np0, xblocks, cnt, ept;
+    DevAxis ax;            // LDS geometry of this pass (twiddles at offset 0)
+    const double2 *tw;     // this axis' twiddles, ntw entries
+    int ntw;
+    const double2 *kt;     // this axis' kinetic phases (digit-reversed order)
+    void *x;               // state, storage precision
+    const void *psi0;      // initial state, storage precision
+    const void *vpot;      // potential, storage precision
+    const double2 *vph;    // exp(-i cst2 V) as (cos, sin), FP64 (phase-table variant)
     double *part;
     long long *dstep;
     double b, cst2;
     int lc, maxsamp;
-    int S, L, ltile, tiles, last;
+    int S, L, ltile, last;
 };
 
+template <typename S>
+__device__ __forceinline__ double2 widen(V2<S> v)
+{
+    return {(double)v.x, (double)v.y};
+}
+
 // S: storage precision of psi, psi0 and V in HBM; LDS and arithmetic FP64.
-template <typename S, int NT, bool INV, bool FWD>
+// EPT: elements per thread (compile time so that the prologue's staging
+// arrays stay in VGPRs); twiddle and kinetic tables have at most EPT*NT entries.
+// PHT: read the FP64 V phase table (16 B/point) instead of V in storage
+// precision plus an FP64 sincos per point; the table wins when the pass is
+// latency-bound (one packet), V+sincos when it is HBM-bound (large batches).
+template <typename S, int NT, bool INV, bool FWD, int EPT, bool PHT>
 __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
 {
     using V = double2;
     using VS = V2<S>;
     extern __shared__ double smem_d[];
+    const int cnt = a.cnt;
     V *x = reinterpret_cast<V *>(smem_d);
-    const int cnt = a.axx.nl * a.np0;
-    double *red = reinterpret_cast<double *>(x + cnt);
+    V *ph = x + cnt;      // exp(+i cst2 V) as (cos, sin)
+    V *p0 = ph + cnt;     // psi0
+    V *twl = p0 + cnt;        // twiddles (+1 dummy slot)
+    V *kl = twl + a.ntw + 1;  // kinetic phases (+1 dummy slot)
+    double *red = reinterpret_cast<double *>(kl + a.np0 + 1);
     const int pk = blockIdx.y, blk = blockIdx.x;
     const long long po = (long long)pk * a.n + (long long)blk * cnt;
     VS *gx = reinterpret_cast<VS *>(a.x) + po;
@@ -53,36 +76,76 @@ __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
     const VS *__restrict__ g0 = reinterpret_cast<const VS *>(a.psi0) + po;
     const double b = a.b;
 
-    for (int i = threadIdx.x; i < cnt; i += NT) {
-        VS t = gx[i];
-        x[i] = {(double)t.x, (double)t.y};
-    }
-    __syncthreads();
+    // Prologue: one round of global loads.
     const long long g = INV ? *a.dstep : 0;  // the step this pass completes
+    VS st[EPT];
+    S vv[EPT];
+    V wv[EPT];
+    VS q[EPT];
+    V tr[EPT], kr[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int i = threadIdx.x + k * NT;
+        st[k] = gx[i];
+        if (PHT)
+            wv[k] = a.vph[po + i];
+        else
+            vv[k] = gv[i];
+        if (INV) q[k] = g0[i];
+    }
+    // Clamped indices and unconditional loads keep the staging arrays in
+    // VGPRs (conditionally written arrays are demoted to scratch).
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int j = threadIdx.x + k * NT;
+        tr[k] = a.tw[j < a.ntw ? j : 0];
+        if (FWD) kr[k] = a.kt[j < a.np0 ? j : 0];
+    }
+    // Out-of-range lanes write the dummy slot, so every load is used and
+    // the compiler cannot sink it into a branch (which would serialise it).
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int j = threadIdx.x + k * NT;
+        twl[j < a.ntw ? j : a.ntw] = tr[k];
+        if (FWD) kl[j < a.np0 ? j : a.np0] = kr[k];
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int i = threadIdx.x + k * NT;
+        x[i] = widen<S>(st[k]);
+        if (PHT) {
+            ph[i] = wv[k];
+        } else {
+            double sn, cs;
+            sincos(a.cst2 * (double)vv[k], &sn, &cs);
+            ph[i] = {cs, sn};
+        }
+        if (INV) p0[i] = widen<S>(q[k]);
+    }
     const bool samp = INV && (g % a.lc) == 0;
+    __syncthreads();
+
     if (INV)
-        for (int s = a.axx.nst - 1; s >= 1; --s) dit_stage<double, NT, -1>(x, a.axx, s, a.tw);
+        for (int s = a.ax.nst - 1; s >= 1; --s) dit_stage<double, NT, -1>(x, a.ax, s, twl);
     double cr = 0, ci = 0, nr = 0;
-    fused_outer<double, NT, INV, FWD>(x, a.axx, a.tw, [&](int i, V &v) {
-        double sn, cs;
-        sincos(a.cst2 * (double)gv[i], &sn, &cs);
-        const V w = {cs, sn};
+    fused_outer<double, NT, INV, FWD>(x, a.ax, twl, [&](int i, V &v) {
+        const V w = ph[i];
         if (INV) {
             v = cmulc(v, w);
             v.x *= b;
             v.y *= b;
             if (samp) {
-                VS p = g0[i];
-                cr += (double)p.x * v.x + (double)p.y * v.y;
-                ci += (double)p.x * v.y - (double)p.y * v.x;
+                const V p = p0[i];
+                cr += p.x * v.x + p.y * v.y;
+                ci += p.x * v.y - p.y * v.x;
                 nr += v.x * v.x + v.y * v.y;
             }
         }
         if (FWD) v = cmulc(v, w);
     });
     if (FWD) {
-        for (int s = 1; s < a.axx.nst; ++s) dif_stage<double, NT, 1>(x, a.axx, s, a.tw);
-        for (int i = threadIdx.x; i < cnt; i += NT) x[i] = cmulc(x[i], a.kx[i & (a.np0 - 1)]);
+        for (int s = 1; s < a.ax.nst; ++s) dif_stage<double, NT, 1>(x, a.ax, s, twl);
+        for (int i = threadIdx.x; i < cnt; i += NT) x[i] = cmulc(x[i], kl[i & (a.np0 - 1)]);
         __syncthreads();
     }
     for (int i = threadIdx.x; i < cnt; i += NT) gx[i] = {(S)x[i].x, (S)x[i].y};
@@ -98,29 +161,49 @@ __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
     }
 }
 
-template <typename S, int NT>
+// Forward FFT, kinetic phase and inverse FFT along an axis of stride S and
+// length L, on a tile of 2^ltile contiguous elements.
+template <typename S, int NT, int EPT>
 __global__ __launch_bounds__(NT) void pass_strided(PassArgs a)
 {
     using V = double2;
     using VS = V2<S>;
     extern __shared__ double smem_d[];
+    const int cnt = a.cnt;
     V *x = reinterpret_cast<V *>(smem_d);
+    V *twl = x + cnt;         // twiddles (+1 dummy slot)
+    V *kl = twl + a.ntw + 1;  // kinetic phases (+1 dummy slot)
     const int tile = 1 << a.ltile;
-    const int cnt = tile * a.L;
     const int pk = blockIdx.y, t = blockIdx.x;
     const int spt = a.S >> a.ltile;  // tiles across the lower axes
     const long long base =
         (long long)pk * a.n + (long long)(t / spt) * a.S * a.L + (long long)(t % spt) * tile;
     VS *gx = reinterpret_cast<VS *>(a.x) + base;
 
-    for (int i = threadIdx.x; i < cnt; i += NT) {
-        VS v = gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))];
-        x[i] = {(double)v.x, (double)v.y};
+    VS st[EPT];
+    V tr[EPT], kr[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int i = threadIdx.x + k * NT;
+        st[k] = gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))];
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int j = threadIdx.x + k * NT;
+        tr[k] = a.tw[j < a.ntw ? j : 0];
+        kr[k] = a.kt[j < a.L ? j : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int j = threadIdx.x + k * NT;
+        x[j] = widen<S>(st[k]);
+        twl[j < a.ntw ? j : a.ntw] = tr[k];
+        kl[j < a.L ? j : a.L] = kr[k];
     }
     __syncthreads();
-    for (int s = 0; s < a.axs.nst - 1; ++s) dif_stage<double, NT, 1>(x, a.axs, s, a.tw);
-    fused_inner<double, NT>(x, a.axs, [&](int i, V &v) { v = cmulc(v, a.ks[i >> a.ltile]); });
-    for (int s = a.axs.nst - 2; s >= 0; --s) dit_stage<double, NT, -1>(x, a.axs, s, a.tw);
+    for (int s = 0; s < a.ax.nst - 1; ++s) dif_stage<double, NT, 1>(x, a.ax, s, twl);
+    fused_inner<double, NT>(x, a.ax, [&](int i, V &v) { v = cmulc(v, kl[i >> a.ltile]); });
+    for (int s = a.ax.nst - 2; s >= 0; --s) dit_stage<double, NT, -1>(x, a.ax, s, twl);
     for (int i = threadIdx.x; i < cnt; i += NT)
         gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))] = {(S)x[i].x, (S)x[i].y};
     if (a.last && t == 0 && pk == 0 && threadIdx.x == 0) *a.dstep += 1;
@@ -151,24 +234,28 @@ __global__ void reduce_parts(const double *part, double *cbuf, int npk, int xblo
 
 namespace {
 
-void axis_from_plan(DevAxis &d, const FftPlan &p, int es, int nl, int lmod, int la, int lb, int tw0)
+void axis_from_plan(DevAxis &d, const FftPlan &p, int es, int nl, int lmod, int la, int lb)
 {
     d.n = p.n;
     d.ln = ilog2(p.n);
     d.nst = p.nst;
-    d.es = es;
-    d.nl = nl;
-    d.lnl = ilog2(nl);
-    d.lmod = lmod;
-    d.la = la;
-    d.lb = lb;
-    d.tw0 = tw0;
+    set_line_geometry(d, es, nl, lmod, la, lb);
+    d.tw0 = 0;
     for (int s = 0; s < p.nst; ++s) {
         d.rad[s] = p.rad[s];
         d.lm[s] = p.lm[s];
         d.lL[s] = p.lL[s];
         d.off[s] = p.off[s];
     }
+}
+
+// Geometry for the current shape.
+void apply_shape(SpoPassPlan &p)
+{
+    const PassShape &sh = p.shape;
+    axis_from_plan(p.axx, p.fpl[0], 1, sh.rows, sh.rows, p.np[0], 0);
+    for (int a = 1; a < p.ndim; ++a) axis_from_plan(p.axs[a], p.fpl[a], sh.tile, sh.tile, sh.tile, 1, 0);
+    p.xblocks = p.n / (p.np[0] * sh.rows);
 }
 
 template <typename K>
@@ -183,41 +270,6 @@ hipError_t launch(K kern, dim3 grid, int nt, size_t lds, hipStream_t s, const Pa
     return hipGetLastError();
 }
 
-template <typename R>
-hipError_t rows(const SpoPassPlan &p, const PassArgs &a, bool inv, bool fwd, hipStream_t s)
-{
-    const int nt = 64 * p.rows_per_wg;
-    size_t lds = sizeof(double2) * p.rows_per_wg * p.np[0] + sizeof(double) * 3 * (nt / 64);
-    dim3 grid(p.xblocks, p.npk);
-#define ESPEC_ROWS(NT_)                                                                  \
-    if (nt == NT_) {                                                                     \
-        if (inv && fwd) return launch(pass_rows<R, NT_, true, true>, grid, NT_, lds, s, a); \
-        if (inv) return launch(pass_rows<R, NT_, true, false>, grid, NT_, lds, s, a);     \
-        return launch(pass_rows<R, NT_, false, true>, grid, NT_, lds, s, a);              \
-    }
-    ESPEC_ROWS(64)
-    ESPEC_ROWS(128)
-    ESPEC_ROWS(256)
-#undef ESPEC_ROWS
-    return hipErrorInvalidValue;
-}
-
-template <typename R>
-hipError_t strided(const SpoPassPlan &p, PassArgs a, int axis, bool last, hipStream_t s)
-{
-    a.axs = p.axs[axis];
-    a.ks = static_cast<const double2 *>(p.kax) + p.kax_off[axis];
-    a.S = 1;
-    for (int k = 0; k < axis; ++k) a.S *= p.np[k];
-    a.L = p.np[axis];
-    a.ltile = ilog2(p.tile);
-    a.tiles = p.n / (p.tile * a.L);
-    a.last = last ? 1 : 0;
-    size_t lds = sizeof(double2) * p.tile * a.L;
-    return launch(pass_strided<R, 256>, dim3(a.tiles, p.npk), 256, lds, s, a);
-}
-
-template <typename R>
 PassArgs base_args(const SpoPassPlan &p)
 {
     PassArgs a{};
@@ -225,12 +277,10 @@ PassArgs base_args(const SpoPassPlan &p)
     a.npk = p.npk;
     a.np0 = p.np[0];
     a.xblocks = p.xblocks;
-    a.axx = p.axx;
-    a.tw = static_cast<const double2 *>(p.tw);
-    a.kx = static_cast<const double2 *>(p.kax) + p.kax_off[0];
     a.x = p.x;
     a.psi0 = p.psi0;
     a.vpot = p.vpot;
+    a.vph = static_cast<const double2 *>(p.vph);
     a.cst2 = p.cst2;
     a.part = p.part;
     a.dstep = p.dstep;
@@ -240,19 +290,110 @@ PassArgs base_args(const SpoPassPlan &p)
     return a;
 }
 
+size_t rows_lds(const SpoPassPlan &p, const PassShape &sh)
+{
+    const int cnt = sh.rows * p.np[0];
+    return sizeof(double2) * (3 * (size_t)cnt + p.ntw[0] + p.np[0] + 2) +
+           sizeof(double) * 3 * (sh.rows_nt / 64);
+}
+
+size_t strided_lds(const SpoPassPlan &p, int axis, const PassShape &sh)
+{
+    return sizeof(double2) * ((size_t)sh.tile * p.np[axis] + p.ntw[axis] + p.np[axis] + 2);
+}
+
+template <typename R>
+hipError_t rows(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, double b_override = 0)
+{
+    PassArgs a = base_args(p);
+    if (b_override > 0) a.b = b_override;
+    a.ax = p.axx;
+    a.cnt = p.shape.rows * p.np[0];
+    const int nt = p.shape.rows_nt;
+    a.ept = a.cnt / nt;
+    a.tw = static_cast<const double2 *>(p.tw) + p.tw_off[0];
+    a.ntw = p.ntw[0];
+    a.kt = static_cast<const double2 *>(p.kax) + p.kax_off[0];
+    const size_t lds = rows_lds(p, p.shape);
+    const bool pht = p.vph != nullptr;
+    dim3 grid(p.xblocks, p.npk);
+#define ESPEC_ROWS2(NT_, E_, P_)                                                                    \
+    if (nt == NT_ && a.ept == E_ && pht == P_) {                                                    \
+        if (inv && fwd) return launch(pass_rows<R, NT_, true, true, E_, P_>, grid, NT_, lds, s, a); \
+        if (inv) return launch(pass_rows<R, NT_, true, false, E_, P_>, grid, NT_, lds, s, a);       \
+        return launch(pass_rows<R, NT_, false, true, E_, P_>, grid, NT_, lds, s, a);                \
+    }
+#define ESPEC_ROWS(NT_, E_) ESPEC_ROWS2(NT_, E_, true) ESPEC_ROWS2(NT_, E_, false)
+    ESPEC_ROWS(64, 1)
+    ESPEC_ROWS(64, 2)
+    ESPEC_ROWS(64, 4)
+    ESPEC_ROWS(64, 8)
+    ESPEC_ROWS(64, 16)
+    ESPEC_ROWS(256, 1)
+    ESPEC_ROWS(256, 2)
+    ESPEC_ROWS(256, 4)
+    ESPEC_ROWS(256, 8)
+    ESPEC_ROWS(256, 16)
+#undef ESPEC_ROWS
+#undef ESPEC_ROWS2
+    return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t strided(const SpoPassPlan &p, int axis, bool last, hipStream_t s)
+{
+    PassArgs a = base_args(p);
+    a.ax = p.axs[axis];
+    a.tw = static_cast<const double2 *>(p.tw) + p.tw_off[axis];
+    a.ntw = p.ntw[axis];
+    a.kt = static_cast<const double2 *>(p.kax) + p.kax_off[axis];
+    a.S = 1;
+    for (int k = 0; k < axis; ++k) a.S *= p.np[k];
+    a.L = p.np[axis];
+    a.ltile = ilog2(p.shape.tile);
+    a.cnt = p.shape.tile * a.L;
+    const int nt = p.shape.tile_nt;
+    a.ept = a.cnt / nt;
+    a.last = last ? 1 : 0;
+    const size_t lds = strided_lds(p, axis, p.shape);
+    dim3 grid(p.n / (p.shape.tile * a.L), p.npk);
+#define ESPEC_STRIDED(NT_, E_) \
+    if (nt == NT_ && a.ept == E_) return launch(pass_strided<R, NT_, E_>, grid, NT_, lds, s, a);
+    ESPEC_STRIDED(64, 1)
+    ESPEC_STRIDED(64, 2)
+    ESPEC_STRIDED(64, 4)
+    ESPEC_STRIDED(64, 8)
+    ESPEC_STRIDED(64, 16)
+    ESPEC_STRIDED(256, 1)
+    ESPEC_STRIDED(256, 2)
+    ESPEC_STRIDED(256, 4)
+    ESPEC_STRIDED(256, 8)
+    ESPEC_STRIDED(256, 16)
+#undef ESPEC_STRIDED
+    return hipErrorInvalidValue;
+}
+
 template <typename R>
 hipError_t one_step(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, int *launches)
 {
-    PassArgs a = base_args<R>(p);
-    hipError_t e = rows<R>(p, a, inv, fwd, s);
+    hipError_t e = rows<R>(p, inv, fwd, s);
     ++*launches;
     if (e != hipSuccess || !fwd) return e;
     for (int ax = 1; ax < p.ndim; ++ax) {
-        e = strided<R>(p, a, ax, ax == p.ndim - 1, s);
+        e = strided<R>(p, ax, ax == p.ndim - 1, s);
         ++*launches;
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+void drop_graph(SpoPassPlan &p)
+{
+    if (p.gexec) (void)hipGraphExecDestroy(p.gexec);
+    if (p.graph) (void)hipGraphDestroy(p.graph);
+    p.gexec = nullptr;
+    p.graph = nullptr;
+    p.graph_steps = 0;
 }
 
 template <typename R>
@@ -305,19 +446,203 @@ int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *l
     return ESPEC_OK;
 }
 
+// ---------------------------------------------------------------- tuner
+
+bool nt_ok(int nt, int cnt)
+{
+    if (nt != 64 && nt != 256) return false;
+    const int e = cnt / nt;
+    return cnt % nt == 0 && (e == 1 || e == 2 || e == 4 || e == 8 || e == 16);
+}
+
+bool rows_ok(const SpoPassPlan &p, const PassShape &s)
+{
+    const int cnt = s.rows * p.np[0];
+    if (!nt_ok(s.rows_nt, cnt)) return false;
+    if (p.ntw[0] > kMaxEpt * s.rows_nt || p.np[0] > kMaxEpt * s.rows_nt) return false;
+    return (p.n / p.np[0]) % s.rows == 0 && cnt % s.rows_nt == 0 && cnt / s.rows_nt >= 1 &&
+           cnt / s.rows_nt <= kMaxEpt && rows_lds(p, s) <= 160 * 1024;
+}
+
+bool tile_ok(const SpoPassPlan &p, const PassShape &s)
+{
+    for (int a = 1; a < p.ndim; ++a) {
+        int stride = 1;
+        for (int k = 0; k < a; ++k) stride *= p.np[k];
+        const int cnt = s.tile * p.np[a];
+        if (!nt_ok(s.tile_nt, cnt)) return false;
+        if (p.ntw[a] > kMaxEpt * s.tile_nt || p.np[a] > kMaxEpt * s.tile_nt) return false;
+        if (s.tile > stride || cnt % s.tile_nt || cnt / s.tile_nt < 1 || cnt / s.tile_nt > kMaxEpt)
+            return false;
+        if (sizeof(double2) * ((size_t)cnt + p.ntw[a] + p.np[a] + 2) > 160 * 1024) return false;
+    }
+    return true;
+}
+
+using TuneKey = std::tuple<int, int, int, int, int, int>;
+std::mutex g_tune_mu;
+std::map<TuneKey, PassShape> g_tuned;
+
+// Mean duration (ms) of `reps` launches issued by `fn` between two events.
+template <class F>
+int time_launches(hipStream_t s, int reps, F fn, float *ms)
+{
+    hipEvent_t e0, e1;
+    ESPEC_HIP_TRY(hipEventCreate(&e0));
+    ESPEC_HIP_TRY(hipEventCreate(&e1));
+    ESPEC_HIP_TRY(fn());  // warm-up (code object load, LDS attribute)
+    ESPEC_HIP_TRY(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k) ESPEC_HIP_TRY(fn());
+    ESPEC_HIP_TRY(hipEventRecord(e1, s));
+    ESPEC_HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    ESPEC_HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return ESPEC_OK;
+}
+
+// Try the candidate launch shapes of the row pass and of the strided
+// passes, keep the fastest of each (timed with HIP events on the plan's
+// stream), cache the choice per grid and batch size, and restore psi.
+template <typename R>
+int tune_typed(SpoPassPlan &p, hipStream_t s)
+{
+    const TuneKey key{p.ndim, p.np[0], p.np[1], p.np[2], p.npk, p.mixed};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            p.shape = it->second;
+            apply_shape(p);
+            return ESPEC_OK;
+        }
+    }
+    const size_t bytes = (size_t)p.n * p.npk * sizeof(V2<R>);
+    const int reps = 6;
+    PassShape best = p.shape;
+    float best_ms = 1e30f;
+    const int rc_[][2] = {{1, 64}, {2, 64}, {4, 64}, {1, 256}, {2, 256}, {4, 256}, {8, 256}};
+    for (auto &c : rc_) {
+        PassShape sh = p.shape;
+        sh.rows = c[0];
+        sh.rows_nt = c[1];
+        if (!rows_ok(p, sh)) continue;
+        p.shape = sh;
+        apply_shape(p);
+        ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
+        float ms = 0;
+        // B = 1/np0 keeps the magnitude of psi over repeated row passes.
+        int rc = time_launches(s, reps, [&]() { return rows<R>(p, true, true, s, 1.0 / p.np[0]); }, &ms);
+        if (rc) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best.rows = sh.rows;
+            best.rows_nt = sh.rows_nt;
+        }
+    }
+    p.shape.rows = best.rows;
+    p.shape.rows_nt = best.rows_nt;
+    if (p.ndim > 1) {
+        best_ms = 1e30f;
+        const int tc_[][2] = {{1, 64}, {2, 64}, {4, 64}, {1, 256}, {2, 256}, {4, 256},
+                              {8, 256}, {16, 256}};
+        for (auto &c : tc_) {
+            PassShape sh = p.shape;
+            sh.tile = c[0];
+            sh.tile_nt = c[1];
+            if (!tile_ok(p, sh)) continue;
+            p.shape = sh;
+            apply_shape(p);
+            ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
+            float ms = 0;
+            int rc = time_launches(s, reps, [&]() { return strided<R>(p, 1, false, s); }, &ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.tile = sh.tile;
+                best.tile_nt = sh.tile_nt;
+            }
+        }
+        p.shape.tile = best.tile;
+        p.shape.tile_nt = best.tile_nt;
+    }
+    apply_shape(p);
+    ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
+    ESPEC_HIP_TRY(hipStreamSynchronize(s));
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tuned[key] = p.shape;
+    return ESPEC_OK;
+}
+
+// Per-kernel timing without graphs: events around every row pass and every
+// strided pass of steps 2..nsteps.  Algorithmic bytes per launch: the row
+// pass reads and writes psi and reads V, plus psi0 at sample steps; a
+// strided pass reads and writes psi.
+template <typename R>
+int profile_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, double *ms, double *bytes)
+{
+    std::vector<hipEvent_t> ev;
+    auto mark = [&]() -> hipError_t {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        ev.push_back(e);
+        return hipEventRecord(e, s);
+    };
+    int dummy = 0;
+    hipLaunchKernelGGL(set_step, dim3(1), dim3(1), 0, s, p.dstep, step0);
+    ESPEC_HIP_TRY(hipGetLastError());
+    ESPEC_HIP_TRY(one_step<R>(p, false, true, s, &dummy));
+    std::vector<int> kind;
+    for (int k = 1; k < nsteps; ++k) {
+        ESPEC_HIP_TRY(mark());
+        ESPEC_HIP_TRY(rows<R>(p, true, true, s));
+        ESPEC_HIP_TRY(mark());
+        kind.push_back(0);
+        for (int ax = 1; ax < p.ndim; ++ax) {
+            ESPEC_HIP_TRY(mark());
+            ESPEC_HIP_TRY(strided<R>(p, ax, ax == p.ndim - 1, s));
+            ESPEC_HIP_TRY(mark());
+            kind.push_back(1);
+        }
+    }
+    ESPEC_HIP_TRY(one_step<R>(p, true, false, s, &dummy));
+    ESPEC_HIP_TRY(hipStreamSynchronize(s));
+    double sum[2] = {0, 0};
+    int cnt[2] = {0, 0};
+    for (size_t i = 0; i < kind.size(); ++i) {
+        float t = 0;
+        ESPEC_HIP_TRY(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+        sum[kind[i]] += t;
+        cnt[kind[i]]++;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    const double vs = sizeof(V2<R>), rs = sizeof(R);
+    const double samp = 1.0 / p.lc;  // share of row passes that read psi0
+    ms[0] = cnt[0] ? sum[0] / cnt[0] : 0;
+    ms[1] = cnt[1] ? sum[1] / cnt[1] : 0;
+    bytes[0] = (double)p.n * p.npk * (2 * vs + rs + samp * vs);
+    bytes[1] = (double)p.n * p.npk * (2 * vs);
+    return ESPEC_OK;
+}
+
 }  // namespace
 
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   double *cbuf, int maxsamp)
+                   const void *vph, double *cbuf, int maxsamp, int tune, hipStream_t s)
 {
+    p.vph = vph;
     p.cst2 = cst2;
     p.ndim = ndim;
     p.n = 1;
     for (int a = 0; a < ndim; ++a) {
         p.np[a] = np[a];
         p.n *= np[a];
+        p.fpl[a] = fp[a];
     }
     p.npk = npk;
     p.mixed = mixed;
@@ -327,23 +652,22 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
     p.vpot = vpot;
     p.cbuf = cbuf;
     p.b = 1.0 / p.n;
-    p.tile = mixed ? 16 : 8;  // 128-byte rows of the strided tiles
-    if (p.tile > np[0]) p.tile = np[0];  // a tile never spans past one x row
-    p.rows_per_wg = 1;
-    p.xblocks = p.n / (p.np[0] * p.rows_per_wg);
+    // Default shape (the tuner may replace it): one x line per 64-thread
+    // workgroup; strided tiles of 128-byte rows.
+    p.shape.rows = 1;
+    p.shape.rows_nt = 64;
+    p.shape.tile = std::min(mixed ? 16 : 8, np[0]);
+    p.shape.tile_nt = 256;
     // Tables: twiddles per axis and per-axis kinetic phases (cos, sin) of
     // cst1 * T_a(k_a) stored at the digit-reversed position of k_a.
     std::vector<double> tw, kax;
     for (int a = 0; a < ndim; ++a) {
-        int tw0 = (int)(tw.size() / 2);
+        p.tw_off[a] = (int)(tw.size() / 2);
+        p.ntw[a] = (int)fp[a].twc.size();
         for (size_t i = 0; i < fp[a].twc.size(); ++i) {
             tw.push_back(fp[a].twc[i]);
             tw.push_back(fp[a].tws[i]);
         }
-        if (a == 0)
-            axis_from_plan(p.axx, fp[0], 1, p.rows_per_wg, p.rows_per_wg, p.np[0], 0, tw0);
-        else
-            axis_from_plan(p.axs[a], fp[a], p.tile, p.tile, p.tile, 1, 0, tw0);
         p.kax_off[a] = (int)(kax.size() / 2);
         size_t k0 = kax.size();
         kax.resize(k0 + 2 * (size_t)np[a]);
@@ -354,10 +678,40 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
             kax[k0 + 2 * pos + 1] = std::sin(th);
         }
     }
+    if (!rows_ok(p, p.shape)) {
+        bool found = false;
+        for (int r : {1, 2, 4, 8, 16})
+            for (int nt : {64, 256}) {
+                PassShape sh = p.shape;
+                sh.rows = r;
+                sh.rows_nt = nt;
+                if (!found && rows_ok(p, sh)) {
+                    p.shape = sh;
+                    found = true;
+                }
+            }
+        if (!found) return ESPEC_ERR_UNSUPPORTED;
+    }
+    if (ndim > 1 && !tile_ok(p, p.shape)) {
+        for (int t : {16, 8, 4, 2, 1})
+            for (int nt : {256, 64}) {
+                PassShape sh = p.shape;
+                sh.tile = t;
+                sh.tile_nt = nt;
+                if (tile_ok(p, sh)) {
+                    p.shape = sh;
+                    goto shaped;
+                }
+            }
+        return ESPEC_ERR_UNSUPPORTED;
+    }
+shaped:
+    apply_shape(p);
+    const int max_xblocks = p.n / p.np[0];  // rows = 1
     DevBuf btw, bk, bpart, bstep;
     ESPEC_HIP_TRY(btw.upload(tw.data(), sizeof(double) * tw.size()));
     ESPEC_HIP_TRY(bk.upload(kax.data(), sizeof(double) * kax.size()));
-    ESPEC_HIP_TRY(bpart.alloc(sizeof(double) * 3 * (size_t)maxsamp * npk * p.xblocks));
+    ESPEC_HIP_TRY(bpart.alloc(sizeof(double) * 3 * (size_t)maxsamp * npk * max_xblocks));
     ESPEC_HIP_TRY(hipMemset(bpart.p, 0, bpart.bytes));
     ESPEC_HIP_TRY(bstep.alloc(sizeof(long long)));
     p.tw = btw.p;
@@ -368,11 +722,26 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
     bpart.p = nullptr;
     p.dstep = static_cast<long long *>(bstep.p);
     bstep.p = nullptr;
+    // A defined step counter before any row pass reads it (the tuner's
+    // row passes sample into slot 0).
+    hipLaunchKernelGGL(set_step, dim3(1), dim3(1), 0, s, p.dstep, 0LL);
+    ESPEC_HIP_TRY(hipGetLastError());
+    if (tune) return mixed ? tune_typed<float>(p, s) : tune_typed<double>(p, s);
     return ESPEC_OK;
+}
+
+int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, double *ms,
+                     double *bytes)
+{
+    p.lc = lc;
+    if (nsteps < 2) return ESPEC_ERR_ARG;
+    return p.mixed ? profile_typed<float>(p, nsteps, step0, s, ms, bytes)
+                   : profile_typed<double>(p, nsteps, step0, s, ms, bytes);
 }
 
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, int *launches)
 {
+    if (lc != p.lc) drop_graph(p);  // the captured kernels carry lc
     p.lc = lc;
     *launches = 0;
     if (nsteps <= 0) return ESPEC_OK;
@@ -382,8 +751,7 @@ int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_
 
 void spo_pass_release(SpoPassPlan &p)
 {
-    if (p.gexec) (void)hipGraphExecDestroy(p.gexec);
-    if (p.graph) (void)hipGraphDestroy(p.graph);
+    drop_graph(p);
     if (p.tw) (void)hipFree(p.tw);
     if (p.kax) (void)hipFree(p.kax);
     if (p.part) (void)hipFree(p.part);

@@ -1,5 +1,5 @@
 // Multi-pass split-operator propagation for grids that do not fit in one
-// workgroup's LDS (256^2 complex64, 128^3): one pass per axis per step.
+// workgroup's LDS (256^2, 128^3): one pass per axis per step.
 #pragma once
 #include <vector>
 
@@ -7,20 +7,28 @@
 
 namespace espec {
 
+// Launch shape of the passes (chosen by the autotuner).
+struct PassShape {
+    int rows = 1, rows_nt = 64;    // x lines per row-pass workgroup, its threads
+    int tile = 16, tile_nt = 256;  // contiguous elements per strided tile, its threads
+};
+
 struct SpoPassPlan {
     int ndim = 0, np[3] = {1, 1, 1}, n = 0, npk = 0, mixed = 0, maxsamp = 0;
-    int rows_per_wg = 1;   // x-lines per workgroup of the row pass
-    int tile = 16;         // contiguous elements per strided-pass tile
+    PassShape shape;
     int xblocks = 0;       // row-pass workgroups per packet
-    DevAxis axx{};         // LDS geometry of a row-pass block
+    DevAxis axx{};         // LDS geometry of a row-pass block (twiddles at 0)
     DevAxis axs[3]{};      // LDS geometry of strided-pass tiles (axes 1, 2)
+    FftPlan fpl[3];        // host copies, to rebuild geometry when the shape changes
     void *x = nullptr;
     const void *psi0 = nullptr, *vpot = nullptr;  // storage precision
+    const void *vph = nullptr;  // FP64 V phase table, or null for V + sincos
     double *cbuf = nullptr;
-    void *tw = nullptr;    // double2 twiddles, all axes
+    void *tw = nullptr;    // double2 twiddles, axis a at tw_off[a], ntw[a] entries
+    int tw_off[3] = {}, ntw[3] = {};
     void *kax = nullptr;   // double2 per-axis kinetic phases in digit-reversed order
     int kax_off[3] = {};
-    double *part = nullptr;   // [maxsamp][npk][xblocks][3] row-pass partial sums
+    double *part = nullptr;      // [maxsamp][npk][max xblocks][3] row-pass partial sums
     long long *dstep = nullptr;  // steps completed, advanced on the device
     double b = 1, cst2 = 0;
     int lc = 1;
@@ -32,9 +40,11 @@ struct SpoPassPlan {
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   double *cbuf, int maxsamp);
+                   const void *vph, double *cbuf, int maxsamp, int tune, hipStream_t s);
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s,
                  int *launches);
+int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, double *ms,
+                     double *bytes);
 void spo_pass_release(SpoPassPlan &p);
 
 }  // namespace espec

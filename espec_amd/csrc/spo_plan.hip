@@ -18,6 +18,7 @@ struct espec_spo_plan {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf x, psi0, vph, kin, tw, perm, cbuf;
+    DevBuf vst;  // multi-pass: the potential in storage precision
     SpoLdsArgs la{};
     SpoPassPlan pp{};  // multi-pass plan for grids that do not fit in LDS
     float last_ms = 0;
@@ -39,12 +40,7 @@ void fill_axis(DevAxis &d, const FftPlan &p, int es, int nl, int lmod, int la, i
     d.n = p.n;
     d.ln = ilog2(p.n);
     d.nst = p.nst;
-    d.es = es;
-    d.nl = nl;
-    d.lnl = ilog2(nl);
-    d.lmod = lmod;
-    d.la = la;
-    d.lb = lb;
+    set_line_geometry(d, es, nl, lmod, la, lb);
     d.tw0 = tw0;
     for (int s = 0; s < p.nst; ++s) {
         d.rad[s] = p.rad[s];
@@ -103,11 +99,23 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         // kernels form exp(-i cst2 V) in FP64.
         std::vector<S> vs((size_t)n * npk);
         for (size_t i = 0; i < vs.size(); ++i) vs[i] = (S)vpot[i];
-        ESPEC_HIP_TRY(P.vph.upload(vs.data(), sizeof(S) * vs.size()));
+        ESPEC_HIP_TRY(P.vst.upload(vs.data(), sizeof(S) * vs.size()));
+        // FP64 phase table while it stays small enough to live in the MALL
+        // (then the pass is latency-bound and a table load beats sincos).
+        const bool table = (size_t)n * npk * sizeof(double2) <= (size_t)64 << 20;
+        if (table) {
+            std::vector<double> vph((size_t)2 * n * npk);
+            for (int p = 0; p < npk; ++p) {
+                std::vector<double> t = phase_table<double>(vpot + (size_t)p * n, n, cst2);
+                std::memcpy(vph.data() + (size_t)2 * n * p, t.data(), sizeof(double) * 2 * n);
+            }
+            ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(double) * vph.size()));
+        }
         std::vector<std::vector<double>> taxes;
         for (int a = 0; a < P.ndim; ++a) taxes.push_back(kinetic_axis(P.np[a], sh[a], sm[a], false));
         return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, cst2, P.x.p,
-                              P.psi0.p, P.vph.p, P.cbuf.as<double>(), P.maxsamp);
+                              P.psi0.p, P.vst.p, table ? P.vph.p : nullptr, P.cbuf.as<double>(),
+                              P.maxsamp, 1, P.stream);
     }
     // Kinetic phase: natural order for the reference kernel; for the fused
     // kernel, position of frequency k is sum_a perm_a[k_a] * stride_a.
@@ -205,6 +213,13 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
     const size_t lds_cap = 160 * 1024 - 1024;
     P->lds = spo_lds_bytes(P->n, P->ref, false, P->nt) <= lds_cap;
     P->la.vph_lds = (!P->ref && spo_lds_bytes(P->n, false, true, P->nt) <= lds_cap) ? 1 : 0;
+    {
+        int ntw = 0;
+        for (int a = 0; a < ndim; ++a) ntw += (int)make_fft_plan(np[a], P->ref).twc.size();
+        P->la.ntw = ntw;
+        P->la.tab_lds = (P->la.vph_lds &&
+                         spo_lds_bytes(P->n, false, true, P->nt, true, ntw) <= lds_cap) ? 1 : 0;
+    }
     if (!P->lds && P->ref) return ESPEC_ERR_UNSUPPORTED;
     ESPEC_HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     ESPEC_HIP_TRY(hipEventCreate(&P->ev0));
@@ -326,6 +341,55 @@ int espec_spo_plan_destroy(espec_spo_plan *P)
         (void)hipStreamSynchronize(P->stream);
         delete P;
     }
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_reset(espec_spo_plan *P)
+{
+    if (!P) return ESPEC_ERR_ARG;
+    ESPEC_HIP_TRY(hipMemcpyAsync(P->x.p, P->psi0.p, P->x.bytes, hipMemcpyDeviceToDevice, P->stream));
+    ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+    P->steps_done = 0;
+    return ESPEC_OK;
+}
+
+int espec_spo_plan_profile(espec_spo_plan *P, int nsteps, double *ms, double *bytes, int maxkinds,
+                           int *nkinds)
+{
+    if (!P || !ms || !bytes || !nkinds || maxkinds < 1 || nsteps < 2) return ESPEC_ERR_ARG;
+    double m[2] = {0, 0}, b[2] = {0, 0};
+    int kinds = 1;
+    if (P->lds) {
+        hipEvent_t e0, e1;
+        ESPEC_HIP_TRY(hipEventCreate(&e0));
+        ESPEC_HIP_TRY(hipEventCreate(&e1));
+        P->la.nsteps = nsteps;
+        P->la.step0 = P->steps_done;
+        ESPEC_HIP_TRY(hipEventRecord(e0, P->stream));
+        ESPEC_HIP_TRY(launch_spo_lds(P->la, P->mixed, P->ref, P->nt, P->stream));
+        ESPEC_HIP_TRY(hipEventRecord(e1, P->stream));
+        ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+        float t = 0;
+        ESPEC_HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        P->steps_done += nsteps;
+        // One launch reads psi and writes it back once, reads the V phase
+        // table once and psi0 at every sample step.
+        const double vs = P->mixed ? 8.0 : 16.0;
+        m[0] = t;
+        b[0] = (double)P->n * P->npk * (2 * vs + 16.0 + vs * ((double)nsteps / P->lc));
+    } else {
+        int rc = spo_pass_profile(P->pp, nsteps, P->steps_done, P->lc, P->stream, m, b);
+        if (rc) return rc;
+        P->steps_done += nsteps;
+        kinds = P->ndim > 1 ? 2 : 1;
+    }
+    for (int k = 0; k < kinds && k < maxkinds; ++k) {
+        ms[k] = m[k];
+        bytes[k] = b[k];
+    }
+    *nkinds = kinds < maxkinds ? kinds : maxkinds;
     return ESPEC_OK;
 }
 

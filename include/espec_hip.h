@@ -186,6 +186,21 @@ int espec_spo_plan_get(espec_spo_plan *plan, int pk, double *cr, double *ci,
 int espec_spo_plan_last_timing(espec_spo_plan *plan, double *ms,
                                int *launches);
 int espec_spo_plan_destroy(espec_spo_plan *plan);
+/* Restart a plan from its psi0 (device copy) at step 0, keeping its
+ * tables and captured graphs. */
+int espec_spo_plan_reset(espec_spo_plan *plan);
+/* Per-kernel timing: run nsteps steps with every kernel launched on its
+ * own (no graph) between HIP events on the plan's stream.  ms[k] receives
+ * the mean duration of kernel kind k and bytes[k] the algorithmic HBM
+ * bytes one launch of that kind moves (state read+write, V, psi0 at
+ * sample steps): k = 0 the LDS-resident step kernel or the row pass,
+ * k = 1 the strided (y, z) passes; nkinds receives the number of kinds. */
+int espec_spo_plan_profile(espec_spo_plan *plan, int nsteps, double *ms,
+                           double *bytes, int maxkinds, int *nkinds);
+
+/* Streaming copy kernel over nbytes, iters times: mean ms per launch and
+ * bytes moved per launch (read + write), for bandwidth calibration. */
+int espec_bw_probe(size_t nbytes, int iters, double *ms, double *bytes);
 
 #ifdef __cplusplus
 }
