@@ -61,6 +61,12 @@ def _load():
     L.espec_spo_plan_reset.argtypes = [vp]
     L.espec_spo_plan_profile.argtypes = [vp, ip, d, d, ip, P(ip)]
     L.espec_bw_probe.argtypes = [C.c_size_t, ip, P(dp), P(dp)]
+    L.espec_plnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, d, vp, ip, P(ip), ip]
+    L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
+    L.espec_sil_plan_run.argtypes = [vp, ip]
+    L.espec_sil_plan_sync.argtypes = [vp]
+    L.espec_sil_plan_get.argtypes = [vp, ip, vp, vp, vp, ip, P(ip), vp, vp]
+    L.espec_sil_plan_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -124,6 +130,23 @@ def pspofft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT):
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
 
 
+def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT):
+    """PLNZ (plnz.f:1-140, step sil.f:70-181): returns dict(cr, ci, norm, u, v).
+
+    shm is the FD kinetic prefactor per axis (espec.f:630); mp the Krylov
+    order; abstol the DSTEVX tolerance (<= 0: QL, > 0: bisection)."""
+    L = _load()
+    maxs = nsteps(ti, tf, dt) // lc + 1
+    cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
+    u, v = _f64(u1).copy(), _f64(v1).copy()
+    ns = C.c_int(0)
+    rc = L.espec_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0), u, v,
+                      _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags)
+    _check(rc, "PLNZ")
+    k = ns.value
+    return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
+
+
 def digt(np_, sh, sm, flags=EXACT):
     """DIGT (dkeft.f:204): kinetic grid p^2/2m and B = 1/N."""
     L = _load()
@@ -133,50 +156,91 @@ def digt(np_, sh, sm, flags=EXACT):
     return tc, b.value
 
 
-class SpoPlan:
-    """Device-resident batch of split-operator packets on one grid (no
-    reference counterpart: replaces the job farm of tools&scrips/parallel.x)."""
+class _Plan:
+    """Shared handle logic of the device-resident plans (prefix selects the
+    espec_<prefix>_plan_* entry points)."""
 
-    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=MIXED):
-        L = self._L = _load()
+    _prefix = ""
+
+    def _fn(self, name):
+        return getattr(self._L, f"espec_{self._prefix}_plan_{name}")
+
+    def _setup(self, np_, vpot, u, v, maxsamp):
+        self._L = _load()
         vpot = _f64(vpot)
         n = int(np.prod(np_))
         self.n, self.np = n, list(np_)
         self.npk = vpot.size // n
         self.maxsamp = maxsamp
         self._h = C.c_void_p()
-        self._u = _f64(u)
-        self._v = _f64(v) if v is not None else None
-        rc = L.espec_spo_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, dt, lc,
-                                     _v3(sh), _v3(sm), vpot.ravel(), self._u.ravel(),
-                                     self._v.ctypes.data if self._v is not None else None,
-                                     maxsamp, flags)
-        _check(rc, "spo_plan_create")
+        self._vpot = vpot.ravel()
+        self._u = _f64(u).ravel()
+        self._v = _f64(v).ravel() if v is not None else None
+        return self._v.ctypes.data if self._v is not None else None
 
     def run(self, nsteps):
-        _check(self._L.espec_spo_plan_run(self._h, nsteps), "spo_plan_run")
+        _check(self._fn("run")(self._h, nsteps), f"{self._prefix}_plan_run")
 
     def sync(self):
-        _check(self._L.espec_spo_plan_sync(self._h), "spo_plan_sync")
-
-    def timing(self):
-        ms, nl = C.c_double(), C.c_int()
-        _check(self._L.espec_spo_plan_last_timing(self._h, C.byref(ms), C.byref(nl)), "timing")
-        return ms.value, nl.value
+        _check(self._fn("sync")(self._h), f"{self._prefix}_plan_sync")
 
     def get(self, pk=0, psi=True):
         cr, ci, nr = np.zeros(self.maxsamp), np.zeros(self.maxsamp), np.zeros(self.maxsamp)
         u, v = np.zeros(self.n), np.zeros(self.n)
         ns = C.c_int()
-        _check(self._L.espec_spo_plan_get(self._h, pk, cr.ctypes.data, ci.ctypes.data,
-                                          nr.ctypes.data, self.maxsamp, C.byref(ns),
-                                          u.ctypes.data if psi else None,
-                                          v.ctypes.data if psi else None), "spo_plan_get")
+        _check(self._fn("get")(self._h, pk, cr.ctypes.data, ci.ctypes.data, nr.ctypes.data,
+                               self.maxsamp, C.byref(ns), u.ctypes.data if psi else None,
+                               v.ctypes.data if psi else None), f"{self._prefix}_plan_get")
         k = ns.value
         out = dict(cr=cr[:k], ci=ci[:k], norm=nr[:k])
         if psi:
             out.update(u=u, v=v)
         return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._fn("destroy")(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SilPlan(_Plan):
+    """Device-resident batch of short-iterative Lanczos packets on one grid
+    (PLNZ's time loop, plnz.f:109-131, for npk packets at once)."""
+
+    _prefix = "sil"
+
+    def __init__(self, np_, mp, abstol, dt, lc, shm, vpot, u, v=None, maxsamp=1,
+                 flags=FP64 | EXACT):
+        vp = self._setup(np_, vpot, u, v, maxsamp)
+        rc = self._L.espec_sil_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, mp,
+                                           abstol, dt, lc, _v3(shm, 0.0), self._vpot, self._u,
+                                           vp, maxsamp, flags)
+        _check(rc, "sil_plan_create")
+
+
+class SpoPlan(_Plan):
+    """Device-resident batch of split-operator packets on one grid (no
+    reference counterpart: replaces the job farm of tools&scrips/parallel.x)."""
+
+    _prefix = "spo"
+
+    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=MIXED):
+        vp = self._setup(np_, vpot, u, v, maxsamp)
+        rc = self._L.espec_spo_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, dt,
+                                           lc, _v3(sh), _v3(sm), self._vpot, self._u, vp,
+                                           maxsamp, flags)
+        _check(rc, "spo_plan_create")
+
+    def timing(self):
+        ms, nl = C.c_double(), C.c_int()
+        _check(self._L.espec_spo_plan_last_timing(self._h, C.byref(ms), C.byref(nl)), "timing")
+        return ms.value, nl.value
 
     def reset(self):
         """Restart from psi0 at step 0 (tables and captured graphs kept)."""
@@ -190,14 +254,3 @@ class SpoPlan:
         _check(self._L.espec_spo_plan_profile(self._h, nsteps, ms, by, 4, C.byref(k)),
                "spo_plan_profile")
         return [(ms[i], by[i]) for i in range(k.value)]
-
-    def close(self):
-        if self._h:
-            self._L.espec_spo_plan_destroy(self._h)
-            self._h = C.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass

@@ -198,6 +198,28 @@ int espec_spo_plan_reset(espec_spo_plan *plan);
 int espec_spo_plan_profile(espec_spo_plan *plan, int nsteps, double *ms,
                            double *bytes, int maxkinds, int *nkinds);
 
+/* ---------------------------------------------------------------------
+ * Batched short-iterative Lanczos plan: npk packets on one grid (PLNZ's
+ * time loop, plnz.f:109-131, without the per-step host round trip).
+ * vpot is [npk][n]; u, v [npk][n] (v may be NULL for a real start).
+ * Each step runs the two real recurrences of sil.f:70-181 of every packet
+ * side by side; C(t) = <psi0|psi(t)> and |psi|^2 every lc steps.
+ * ------------------------------------------------------------------- */
+typedef struct espec_sil_plan espec_sil_plan;
+int espec_sil_plan_create(espec_sil_plan **plan, int ndim, const int *np,
+                          int npk, int mp, double abstol, double dt, int lc,
+                          const double *shm, const double *vpot,
+                          const double *u, const double *v, int maxsamp,
+                          int flags);
+int espec_sil_plan_run(espec_sil_plan *plan, int nsteps);
+int espec_sil_plan_sync(espec_sil_plan *plan);
+/* Returns ESPEC_ERR_DIAG if a tridiagonal eigensolve failed to converge
+ * (the reference stops with 'Diagonalization error', sil.f:120). */
+int espec_sil_plan_get(espec_sil_plan *plan, int pk, double *cr, double *ci,
+                       double *nrm, int maxsamp, int *nsamp, double *u,
+                       double *v);
+int espec_sil_plan_destroy(espec_sil_plan *plan);
+
 /* Streaming copy kernel over nbytes, iters times: mean ms per launch and
  * bytes moved per launch (read + write), for bandwidth calibration. */
 int espec_bw_probe(size_t nbytes, int iters, double *ms, double *bytes);
