@@ -67,6 +67,22 @@ def _load():
     L.espec_sil_plan_sync.argtypes = [vp]
     L.espec_sil_plan_get.argtypes = [vp, ip, vp, vp, vp, ip, P(ip), vp, vp]
     L.espec_sil_plan_destroy.argtypes = [vp]
+    L.espec_psodfft.argtypes = prop
+    L.espec_sod_plan_create.argtypes = L.espec_spo_plan_create.argtypes
+    L.espec_sod_plan_run.argtypes = [vp, ip]
+    L.espec_sod_plan_sync.argtypes = [vp]
+    L.espec_sod_plan_get.argtypes = L.espec_spo_plan_get.argtypes
+    L.espec_sod_plan_destroy.argtypes = [vp]
+    L.espec_fft.argtypes = [d, d, ip, ip, ip, ip, ip]
+    L.espec_au.argtypes = [ip, i, d, d, d, d]
+    L.espec_lnz.argtypes = [ip, i, ip, ip, d, d, d, d, d, vp]
+    L.espec_dstevx.argtypes = [ip, d, d, dp, d, vp]
+    L.espec_lanczs.argtypes = [ip, i, ip, ip, ip, d, d, d, d]
+    L.espec_fcorrel.argtypes = [ip, d, d, P(dp)]
+    L.espec_assemble_corr.argtypes = [ip, d, d, d, d]
+    L.espec_hwhm.argtypes = [ip, dp, dp, dp, dp, d, d]
+    L.espec_spectrumtd.argtypes = [ip, ip, dp, dp, dp, dp, d, d, d, d, d, ip]
+    L.espec_lc_rule.argtypes = [dp, P(dp), dp, ip, ip, P(ip)]
     _lib = L
     return L
 
@@ -147,6 +163,107 @@ def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT)
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
 
 
+def psodfft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT):
+    """PSODFFT (psodfft.f:1-200, step sodfft.f): returns dict(cr, ci, norm, u, v).
+
+    Sample 0 is C(TI); the two half-strength start-up steps count as the
+    first step, then one sample every lc leapfrog steps."""
+    L = _load()
+    nst = 1 + nsteps(ti + dt, tf, dt)
+    maxs = nst // lc + 2
+    cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
+    u, v = _f64(u1).copy(), _f64(v1).copy()
+    ns = C.c_int(0)
+    rc = L.espec_psodfft(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v,
+                         _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags)
+    _check(rc, "PSODFFT")
+    k = ns.value
+    return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
+
+
+def fft(a, b, ntot, n, nspan, isn, flags=REF):
+    """FFT(A,B,NTOT,N,NSPAN,ISN) (fft3d.f:1) on the device: returns (a, b)."""
+    a, b = _f64(a).copy(), _f64(b).copy()
+    _check(_load().espec_fft(a, b, ntot, n, nspan, isn, flags), "FFT")
+    return a, b
+
+
+def au(np_, shm, vpot, u):
+    """AU (au.f:3): H u with the 4th-order FD kinetic operator."""
+    hu = np.zeros(int(np.prod(np_)))
+    _check(_load().espec_au(len(np_), _np3(np_), _v3(shm, 0.0), _f64(vpot), _f64(u), hu), "AU")
+    return hu
+
+
+def lnz(np_, m, nreort, shm, vpot, start):
+    """LNZ (lnz.f:4): (al[m], bt[m], basis[nreort][n])."""
+    n = int(np.prod(np_))
+    al, bt = np.zeros(m), np.zeros(m)
+    basis = np.zeros(nreort * n)
+    _check(_load().espec_lnz(len(np_), _np3(np_), m, nreort, _v3(shm, 0.0), _f64(vpot),
+                             _f64(start), al, bt, basis.ctypes.data), "LNZ")
+    return al, bt, basis.reshape(nreort, n)
+
+
+def dstevx(d, e, abstol=0.0):
+    """DSTEVX('V','A') (dstevx.f:5): (w, z) with eigenvectors as columns of z."""
+    m = len(d)
+    w, z = np.zeros(m), np.zeros(m * m)
+    _check(_load().espec_dstevx(m, _f64(d), _f64(np.append(_f64(e), 0.0)), abstol, w,
+                                z.ctypes.data), "DSTEVX")
+    return w, z.reshape(m, m).T
+
+
+def lanczs(np_, il, iu, nreort, shm, vpot):
+    """LANCZS (lanczs.f:1): (eigvl[iu-il+1], eigvc[iu-il+1][n])."""
+    n = int(np.prod(np_))
+    ns = iu - il + 1
+    ev, vc = np.zeros(ns), np.zeros(ns * n)
+    _check(_load().espec_lanczs(len(np_), _np3(np_), il, iu, nreort, _v3(shm, 0.0), _f64(vpot),
+                                ev, vc), "LANCZS")
+    return ev, vc.reshape(ns, n)
+
+
+def fcorrel(u1, u2):
+    """FCORREL (fcorrel.f:1): sum u1 u2."""
+    out = C.c_double()
+    u1, u2 = _f64(u1), _f64(u2)
+    _check(_load().espec_fcorrel(len(u1), u1, u2, C.byref(out)), "FCORREL")
+    return out.value
+
+
+def assemble_corr(mf, cr, ci):
+    """C(t) on t = -TF + (i-1) DTF from 2^(mf-1) samples (espec.f:1546-1566)."""
+    n = 1 << mf
+    u, v = np.zeros(n), np.zeros(n)
+    _check(_load().espec_assemble_corr(mf, _f64(cr), _f64(ci), u, v), "assemble_corr")
+    return u, v
+
+
+def hwhm(mf, dt, tf, tal, de, u, v):
+    """HWHM (hwhm.f:1): returns (u, v)."""
+    u, v = _f64(u).copy(), _f64(v).copy()
+    _check(_load().espec_hwhm(mf, dt, tf, tal, de, u, v), "HWHM")
+    return u, v
+
+
+def spectrumtd(iwin, mf, dt, tf, wp, de, u, v, flags=REF):
+    """SPECTRUMTD (spectrumtd.f:1): (E, W, FFTi) in print order."""
+    n = 1 << mf
+    u, v = _f64(u).copy(), _f64(v).copy()
+    e, w, wi = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(_load().espec_spectrumtd(iwin, mf, dt, tf, wp, de, u, v, e, w, wi, flags),
+           "SPECTRUMTD")
+    return e, w, wi
+
+
+def lc_rule(ti, tf, dt, mf, change=True):
+    """LC and the adjusted TF (espec.f:1324-1353)."""
+    t, lc = C.c_double(tf), C.c_int()
+    _check(_load().espec_lc_rule(ti, C.byref(t), dt, mf, int(change), C.byref(lc)), "lc_rule")
+    return lc.value, t.value
+
+
 def digt(np_, sh, sm, flags=EXACT):
     """DIGT (dkeft.f:204): kinetic grid p^2/2m and B = 1/N."""
     L = _load()
@@ -222,6 +339,21 @@ class SilPlan(_Plan):
                                            abstol, dt, lc, _v3(shm, 0.0), self._vpot, self._u,
                                            vp, maxsamp, flags)
         _check(rc, "sil_plan_create")
+
+
+class SodPlan(_Plan):
+    """Device-resident batch of second-order-differencing packets on one
+    grid (PSODFFT's time loop for npk packets at once); the first run step
+    is the two half-strength start-up steps."""
+
+    _prefix = "sod"
+
+    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=FP64 | EXACT):
+        vp = self._setup(np_, vpot, u, v, maxsamp)
+        rc = self._L.espec_sod_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, dt,
+                                           lc, _v3(sh), _v3(sm), self._vpot, self._u, vp,
+                                           maxsamp, flags)
+        _check(rc, "sod_plan_create")
 
 
 class SpoPlan(_Plan):

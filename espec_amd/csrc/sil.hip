@@ -147,22 +147,31 @@ __device__ double tridiag_bisect_one(int m, const double *d, const double *e2, d
 
 // ------------------------------------------------------------ kernels
 
+constexpr int kChunk = 32;            // Gram coefficients summed per block pass
+constexpr int kPart = kChunk + 2;     // partial-sum slots per block
+constexpr int kMaxReort = 512;        // widest Gram-Schmidt window (LMTREORT = 300, param.h:9)
+
 struct LnzArgs {
     int ndim, n, np[3], st[3];
     double shm[3];
-    const double *vpot;   // [npk][n] (or one shared table when vshared)
-    int vshared;
-    int nvec, m, nb;      // independent vectors, Krylov order, blocks per vector
-    void *basis;          // [nvec][m][n] storage precision
+    const double *vpot;   // potential of recurrence v: vpot + (v >> vshift) * n
+    int vshift;
+    int nvec, m, nb;      // independent recurrences, Krylov order, blocks per recurrence
+    int ncols, nchunk;    // stored basis columns (NREORT), Gram chunks of kChunk
+    void *basis;          // [nvec][ncols][n] storage precision
+    void *vring;          // [2][nvec][n] v_i and v_{i-1}
     void *w0, *w1;        // [nvec][n] ping-pong residuals
     double *al, *bt;      // [nvec][m]
     double *beta0;        // [nvec] norm of the start vector
-    double *rc;           // [nvec][m] Gram coefficients of the current iteration
-    double *part;         // [nvec][nb][kMaxKrylov + 2] partial sums
-    unsigned *ticket;     // [nvec]
-    int *dead;            // [nvec] start vector was zero (lnz.f:86)
-    int it;               // current iteration i
-    int nreort;           // vectors in the Gram-Schmidt pass: min(i+1, nreort)
+    double *rc;           // [nvec][ncols] Gram coefficients of the current iteration
+    double *part;         // [nvec][nchunk][nb][kPart] partial sums
+    unsigned *ticket;     // [nvec][nchunk]
+    int *dead;            // [nvec] start vector was zero (lnz.f:85-86)
+    int it;               // current iteration i (0-based)
+    int nreort;           // Gram window: columns j < min(i+1, nreort)
+    const double *acoef;  // [nacc][m] replay coefficients (LANCZS), or null
+    double *acc;          // [nacc][n] replay accumulators
+    int nacc;
 };
 
 // H u at point i (au1d.f/au2di.f/au3d.f): per axis the 4th-order stencil
@@ -187,34 +196,36 @@ __device__ __forceinline__ double apply_h(const LnzArgs &a, const S *u, const do
     return acc;
 }
 
-// Sum k values per thread over the block into red[0..k) (thread 0 valid).
+// Sum k values per thread over the block (thread 0 receives the sums).
 template <int NT>
 __device__ void block_sum_k(double *v, int k, double *red)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int j = 0; j < k; ++j) {
         double x = wave_sum(v[j]);
-        if (lane == 0) red[w * (kMaxKrylov + 2) + j] = x;
+        if (lane == 0) red[w * kPart + j] = x;
     }
     __syncthreads();
     if (threadIdx.x == 0)
         for (int j = 0; j < k; ++j) {
             double s = 0;
-            for (int q = 0; q < NT / 64; ++q) s += red[q * (kMaxKrylov + 2) + j];
+            for (int q = 0; q < NT / 64; ++q) s += red[q * kPart + j];
             v[j] = s;
         }
 }
 
-// Publish k block partials; returns true in the last block to arrive, in
-// which tot[0..k) then holds the deterministic (block-ordered) totals.
-__device__ bool last_block(const LnzArgs &a, int vec, const double *v, int k, double *tot)
+// Publish k block partials of (recurrence vec, chunk c); returns true in
+// the last block to arrive, whose thread 0 then holds in tot[0..k) the
+// totals summed in block order (deterministic).  No block waits.
+__device__ bool last_block(const LnzArgs &a, int vec, int c, const double *v, int k, double *tot)
 {
     __shared__ bool last;
-    double *p = a.part + ((long long)vec * a.nb + blockIdx.x) * (kMaxKrylov + 2);
+    const long long slot = (long long)vec * a.nchunk + c;
+    double *p = a.part + (slot * a.nb + blockIdx.x) * kPart;
     if (threadIdx.x == 0) {
         for (int j = 0; j < k; ++j) p[j] = v[j];
         __threadfence();
-        unsigned t = atomicAdd(&a.ticket[vec], 1u);
+        unsigned t = atomicAdd(&a.ticket[slot], 1u);
         last = (t == (unsigned)a.nb - 1);
     }
     __syncthreads();
@@ -223,20 +234,20 @@ __device__ bool last_block(const LnzArgs &a, int vec, const double *v, int k, do
         __threadfence();
         for (int j = 0; j < k; ++j) tot[j] = 0;
         for (int b = 0; b < a.nb; ++b) {
-            const double *q = a.part + ((long long)vec * a.nb + b) * (kMaxKrylov + 2);
+            const double *q = a.part + (slot * a.nb + b) * kPart;
             for (int j = 0; j < k; ++j) tot[j] += q[j];
         }
-        a.ticket[vec] = 0;
+        a.ticket[slot] = 0;
     }
     return true;
 }
 
 // Start: w = start vector of each recurrence (Re psi for even vec, Im psi
-// for odd vec, or an explicit start), beta0 = |w|.
+// for odd vec, or an explicit start), beta0 = |w| (lnz.f:75-86).
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_init(LnzArgs a, const V2<S> *x, const S *start)
 {
-    __shared__ double red[(NT / 64) * (kMaxKrylov + 2)];
+    __shared__ double red[(NT / 64) * kPart];
     const int vec = blockIdx.y;
     S *w = reinterpret_cast<S *>(a.w0) + (long long)vec * a.n;
     double acc = 0;
@@ -254,84 +265,98 @@ __global__ __launch_bounds__(NT) void lnz_init(LnzArgs a, const V2<S> *x, const 
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
     double tot[1];
-    if (last_block(a, vec, v, 1, tot) && threadIdx.x == 0) {
+    if (last_block(a, vec, 0, v, 1, tot) && threadIdx.x == 0) {
         a.beta0[vec] = sqrt(tot[0]);
         a.dead[vec] = tot[0] == 0.0;
     }
 }
 
+// lnz.f:89-108: v_i = w / beta, w = H v_i - beta v_{i-1}, alpha_i = <v_i, w>.
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
 {
-    __shared__ double red[(NT / 64) * (kMaxKrylov + 2)];
+    __shared__ double red[(NT / 64) * kPart];
     const int vec = blockIdx.y, i = a.it;
     const long long vo = (long long)vec * a.n;
     const S *win = reinterpret_cast<const S *>((i & 1) ? a.w1 : a.w0) + vo;
     S *wout = reinterpret_cast<S *>((i & 1) ? a.w0 : a.w1) + vo;
-    S *B = reinterpret_cast<S *>(a.basis) + (long long)vec * a.m * a.n;
-    const double *vp = a.vpot + (a.vshared ? 0 : (long long)(vec >> 1) * a.n);
+    S *vcur = reinterpret_cast<S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
+    const S *vprev = reinterpret_cast<const S *>(a.vring) + ((long long)((i + 1) & 1) * a.nvec + vec) * a.n;
+    S *B = reinterpret_cast<S *>(a.basis) + (long long)vec * a.ncols * a.n;
+    const double *vp = a.vpot + ((long long)(vec >> a.vshift)) * a.n;
     // A zero start vector leaves the basis at zero (lnz.f:85-86); the
     // recurrence is then carried as zeros so nothing reads a stale beta.
     const bool dead = a.dead[vec];
     const double b0 = dead ? 1.0 : (i == 0 ? a.beta0[vec] : a.bt[vec * a.m + i - 1]);
     const double rb = 1.0 / b0;
+    const bool store = i < a.ncols;
     double acc = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
         double vi = 0, hw = 0;
         if (!dead) {
             vi = (double)win[p] / b0;
             hw = apply_h<S>(a, win, vp, p) * rb;
-            if (i > 0) hw -= b0 * (double)B[(long long)(i - 1) * a.n + p];
+            if (i > 0) hw -= b0 * (double)vprev[p];
         }
-        B[(long long)i * a.n + p] = (S)vi;
+        vcur[p] = (S)vi;
+        if (store) B[(long long)i * a.n + p] = (S)vi;
         wout[p] = (S)hw;
         acc += vi * hw;
+        for (int k = 0; k < a.nacc; ++k) a.acc[(long long)k * a.n + p] += a.acoef[k * a.m + i] * vi;
     }
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
     double tot[1];
-    if (last_block(a, vec, v, 1, tot) && threadIdx.x == 0) a.al[vec * a.m + i] = tot[0];
+    if (last_block(a, vec, 0, v, 1, tot) && threadIdx.x == 0) a.al[vec * a.m + i] = tot[0];
 }
 
+// lnz.f:110-112 and reort.f:54-67: r_j = <v_j, w - alpha v_i> for the
+// columns of the Gram window, kChunk columns per grid.z slice (the residual
+// itself is updated by lnz_update, so every slice reads the same w).
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
 {
-    __shared__ double red[(NT / 64) * (kMaxKrylov + 2)];
-    const int vec = blockIdx.y, i = a.it;
+    __shared__ double red[(NT / 64) * kPart];
+    const int vec = blockIdx.y, i = a.it, c = blockIdx.z;
     const int nj = min(i + 1, a.nreort);
+    const int j0 = c * kChunk;
+    if (j0 >= nj) return;
+    const int jn = min(kChunk, nj - j0);
     const long long vo = (long long)vec * a.n;
-    S *w = reinterpret_cast<S *>((i & 1) ? a.w0 : a.w1) + vo;
-    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.m * a.n;
+    const S *w = reinterpret_cast<const S *>((i & 1) ? a.w0 : a.w1) + vo;
+    const S *vcur = reinterpret_cast<const S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
+    const S *B = reinterpret_cast<const S *>(a.basis) + ((long long)vec * a.ncols + j0) * a.n;
     const double al = a.al[vec * a.m + i];
-    double r[kMaxKrylov];
-    for (int j = 0; j < nj; ++j) r[j] = 0;
+    double r[kChunk];
+    for (int j = 0; j < kChunk; ++j) r[j] = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
-        const double wv = (double)w[p] - al * (double)B[(long long)i * a.n + p];
-        w[p] = (S)wv;
-        const double ws = (double)(S)wv;  // the stored value is what REORT reads
-        for (int j = 0; j < nj; ++j) r[j] += ws * (double)B[(long long)j * a.n + p];
+        const double ws = (double)(S)((double)w[p] - al * (double)vcur[p]);
+        for (int j = 0; j < jn; ++j) r[j] += ws * (double)B[(long long)j * a.n + p];
     }
-    block_sum_k<NT>(r, nj, red);
-    double tot[kMaxKrylov];
-    if (last_block(a, vec, r, nj, tot) && threadIdx.x == 0)
-        for (int j = 0; j < nj; ++j) a.rc[vec * a.m + j] = tot[j];
+    block_sum_k<NT>(r, jn, red);
+    double tot[kChunk];
+    if (last_block(a, vec, c, r, jn, tot) && threadIdx.x == 0)
+        for (int j = 0; j < jn; ++j) a.rc[(long long)vec * a.ncols + j0 + j] = tot[j];
 }
 
+// w <- (w - alpha v_i) - sum_j r_j v_j (reort.f:69-78), beta_i = |w|.
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
 {
-    __shared__ double red[(NT / 64) * (kMaxKrylov + 2)];
-    __shared__ double rs[kMaxKrylov];
+    __shared__ double red[(NT / 64) * kPart];
+    __shared__ double rs[kMaxReort];
     const int vec = blockIdx.y, i = a.it;
     const int nj = min(i + 1, a.nreort);
     const long long vo = (long long)vec * a.n;
     S *w = reinterpret_cast<S *>((i & 1) ? a.w0 : a.w1) + vo;
-    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.m * a.n;
-    if (threadIdx.x < nj) rs[threadIdx.x] = a.rc[vec * a.m + threadIdx.x];
+    const S *vcur = reinterpret_cast<const S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
+    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.ncols * a.n;
+    const double al = a.al[vec * a.m + i];
+    for (int j = threadIdx.x; j < nj; j += NT) rs[j] = a.rc[(long long)vec * a.ncols + j];
     __syncthreads();
     double acc = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
-        double wv = (double)w[p];
+        double wv = (double)(S)((double)w[p] - al * (double)vcur[p]);
         for (int j = 0; j < nj; ++j) wv -= rs[j] * (double)B[(long long)j * a.n + p];
         w[p] = (S)wv;
         const double ws = (double)(S)wv;
@@ -340,7 +365,7 @@ __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
     double tot[1];
-    if (last_block(a, vec, v, 1, tot) && threadIdx.x == 0) a.bt[vec * a.m + i] = sqrt(tot[0]);
+    if (last_block(a, vec, 0, v, 1, tot) && threadIdx.x == 0) a.bt[vec * a.m + i] = sqrt(tot[0]);
 }
 
 struct SilArgs {
@@ -353,20 +378,21 @@ struct SilArgs {
     int *info;            // [1] eigensolve failure flag
 };
 
-// Projection c_j = <v_j, start>, then in the last block the tridiagonal
-// eigensolve and the coefficients of sil.f:87-117 / 135-166.
+// Projection c_j = <v_j, start> (DGEMV 'T', sil.f:86-87), then in the last
+// block the tridiagonal eigensolve (sil.f:82) and the coefficients of
+// sil.f:88-117 / 136-166.
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
 {
-    __shared__ double red[(NT / 64) * (kMaxKrylov + 2)];
+    __shared__ double red[(NT / 64) * kPart];
     __shared__ double d[kMaxKrylov], e[kMaxKrylov], z[kMaxKrylov * kMaxKrylov], e2[kMaxKrylov];
     __shared__ double bnd[4];
     const LnzArgs &a = s.l;
     const int vec = blockIdx.y, m = a.m;
-    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * m * a.n;
+    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.ncols * a.n;
     const V2<S> *x0 = reinterpret_cast<const V2<S> *>(s.x0) + (long long)(vec >> 1) * a.n;
     double c[kMaxKrylov];
-    for (int j = 0; j < m; ++j) c[j] = 0;
+    for (int j = 0; j < kMaxKrylov; ++j) c[j] = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
         V2<S> v = x0[p];
         const double u = (vec & 1) ? (double)v.y : (double)v.x;
@@ -374,7 +400,7 @@ __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
     }
     block_sum_k<NT>(c, m, red);
     double tot[kMaxKrylov];
-    if (!last_block(a, vec, c, m, tot)) return;
+    if (!last_block(a, vec, 0, c, m, tot)) return;
     double *coef = s.coef + (long long)vec * 2 * m;
     if (a.dead[vec]) {
         if (threadIdx.x < 2 * m) coef[threadIdx.x] = 0;
@@ -443,9 +469,9 @@ __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void sil_combine(SilArgs s, V2<S> *x, const V2<S> *psi0,
                                                   double *cbuf, int maxsamp, int lc,
-                                                  long long *dstep)
+                                                  const long long *dstep)
 {
-    __shared__ double red[(NT / 64) * (kMaxKrylov + 2)];
+    __shared__ double red[(NT / 64) * kPart];
     __shared__ double cf[4 * kMaxKrylov];
     const LnzArgs &a = s.l;
     const int pk = blockIdx.y, m = a.m;
@@ -455,8 +481,8 @@ __global__ __launch_bounds__(NT) void sil_combine(SilArgs s, V2<S> *x, const V2<
         cf[2 * m + threadIdx.x] = s.coef[(long long)vi * 2 * m + threadIdx.x];
     }
     __syncthreads();
-    const S *Br = reinterpret_cast<const S *>(a.basis) + (long long)vr * m * a.n;
-    const S *Bi = reinterpret_cast<const S *>(a.basis) + (long long)vi * m * a.n;
+    const S *Br = reinterpret_cast<const S *>(a.basis) + (long long)vr * a.ncols * a.n;
+    const S *Bi = reinterpret_cast<const S *>(a.basis) + (long long)vi * a.ncols * a.n;
     const long long g = *dstep + 1;  // the step this kernel completes
     const bool samp = (g % lc) == 0;
     double acc[3] = {0, 0, 0};
@@ -483,11 +509,9 @@ __global__ __launch_bounds__(NT) void sil_combine(SilArgs s, V2<S> *x, const V2<
             acc[2] += ou * ou + ov * ov;
         }
     }
-    // Sample reduction (ticket slot of the packet's first vector) and the
-    // step counter, advanced once by the last block of packet 0.
     block_sum_k<NT>(acc, 3, red);
     double tot[3];
-    if (last_block(a, vr, acc, 3, tot) && threadIdx.x == 0) {
+    if (last_block(a, vr, 0, acc, 3, tot) && threadIdx.x == 0) {
         const long long nc = g / lc;
         if (samp && nc < maxsamp) {
             double *c = cbuf + ((long long)pk * maxsamp + nc) * 3;
@@ -501,7 +525,178 @@ __global__ __launch_bounds__(NT) void sil_combine(SilArgs s, V2<S> *x, const V2<
 __global__ void sil_step_done(long long *dstep) { *dstep += 1; }
 __global__ void sil_set_step(long long *dstep, long long v) { *dstep = v; }
 
-// -------------------------------------------------------------- plan
+// hu = H u (AU) for one vector.
+__global__ __launch_bounds__(256) void au_kernel(LnzArgs a, const double *u, double *hu)
+{
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < a.n; p += gridDim.x * 256)
+        hu[p] = apply_h<double>(a, u, a.vpot, p);
+}
+
+// FCORREL: sum u1 u2 over n, block-ordered two-level sum.
+__global__ __launch_bounds__(256) void dot_kernel(LnzArgs a, const double *u1, const double *u2,
+                                                  double *out)
+{
+    __shared__ double red[4 * kPart];
+    double acc = 0;
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < a.n; p += a.nb * 256) acc += u1[p] * u2[p];
+    double v[1] = {acc};
+    block_sum_k<256>(v, 1, red);
+    double tot[1];
+    if (last_block(a, 0, 0, v, 1, tot) && threadIdx.x == 0) *out = tot[0];
+}
+
+// DSTEVX('V','A') for one tridiagonal: QL vectors, QL or bisection values.
+__global__ __launch_bounds__(64) void dstevx_kernel(int m, double *d, double *e, double *e2,
+                                                    double abstol, double *w, double *z,
+                                                    int *info)
+{
+    __shared__ double bnd[4];
+    if (threadIdx.x == 0) {
+        if (abstol > 0) {
+            double pm, gl, gu, at;
+            tridiag_bounds(m, d, e, abstol, e2, &pm, &gl, &gu, &at);
+            bnd[0] = pm;
+            bnd[1] = gl;
+            bnd[2] = gu;
+            bnd[3] = at;
+        }
+    }
+    __syncthreads();
+    if (abstol > 0)
+        for (int k = threadIdx.x; k < m; k += 64)
+            w[k] = tridiag_bisect_one(m, d, e2, bnd[0], bnd[1], bnd[2], bnd[3], k);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *info = tridiag_ql(m, d, e, z);
+        if (abstol <= 0)
+            for (int k = 0; k < m; ++k) w[k] = d[k];
+    }
+}
+
+// LANCZS eigenvectors of the n x n Lanczos tridiagonal for the selected
+// indices: eigenvalue k by bisection, vector by inverse iteration with a
+// partially pivoted LU of T - lambda I (the method of dstein.f), scaled to
+// unit norm with its largest component positive (dstein.f), written as the
+// replay coefficients coef[k][J].  One thread per selected eigenvalue.
+__global__ __launch_bounds__(64) void lanczs_select(int n, const double *d, const double *e,
+                                                    double *e2, int il, int ns, double *eigvl,
+                                                    double *coef, double *work, int *info)
+{
+    __shared__ double bnd[4];
+    if (threadIdx.x == 0) {
+        double pm, gl, gu, at;
+        tridiag_bounds(n, d, e, 0.0, e2, &pm, &gl, &gu, &at);
+        bnd[0] = pm;
+        bnd[1] = gl;
+        bnd[2] = gu;
+        bnd[3] = at;
+    }
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k >= ns) return;
+    const double lam = tridiag_bisect_one(n, d, e2, bnd[0], bnd[1], bnd[2], bnd[3], il - 1 + k);
+    eigvl[k] = lam;
+    double *u0 = work + (size_t)k * 6 * n, *u1 = u0 + n, *u2 = u1 + n, *lm = u2 + n, *x = lm + n,
+           *pv = x + n;
+    const double eps = 2.2204460492503131e-16;
+    double tnorm = 0;
+    for (int j = 0; j < n; ++j)
+        tnorm = fmax(tnorm, fabs(d[j]) + (j > 0 ? fabs(e[j - 1]) : 0) + (j < n - 1 ? fabs(e[j]) : 0));
+    const double tiny = eps * fmax(tnorm, 1e-300);
+    // LU of T - lam I with partial pivoting: row k of U is (u0, u1, u2) at
+    // columns k, k+1, k+2.
+    double a0 = d[0] - lam, a1 = n > 1 ? e[0] : 0, a2 = 0;
+    for (int j = 0; j < n - 1; ++j) {
+        const double b0 = e[j], b1 = d[j + 1] - lam, b2 = j + 1 < n - 1 ? e[j + 1] : 0;
+        if (fabs(a0) >= fabs(b0)) {
+            const double l = a0 != 0 ? b0 / a0 : 0;
+            u0[j] = a0 != 0 ? a0 : tiny;
+            u1[j] = a1;
+            u2[j] = a2;
+            lm[j] = l;
+            pv[j] = 0;
+            a0 = b1 - l * a1;
+            a1 = b2 - l * a2;
+        } else {
+            const double l = a0 / b0;
+            u0[j] = b0;
+            u1[j] = b1;
+            u2[j] = b2;
+            lm[j] = l;
+            pv[j] = 1;
+            const double na0 = a1 - l * b1, na1 = a2 - l * b2;
+            a0 = na0;
+            a1 = na1;
+        }
+        a2 = 0;
+    }
+    u0[n - 1] = a0 != 0 ? a0 : tiny;
+    for (int j = 0; j < n; ++j) x[j] = 1.0 + 0.5 * sin(1.2345 * j + 0.5 * k);
+    for (int itr = 0; itr < 5; ++itr) {
+        for (int j = 0; j < n - 1; ++j) {  // apply P and L^-1
+            if (pv[j] != 0) {
+                const double t = x[j];
+                x[j] = x[j + 1];
+                x[j + 1] = t;
+            }
+            x[j + 1] -= lm[j] * x[j];
+        }
+        for (int j = n - 1; j >= 0; --j) {  // U^-1
+            double t = x[j];
+            if (j + 1 < n) t -= u1[j] * x[j + 1];
+            if (j + 2 < n) t -= u2[j] * x[j + 2];
+            x[j] = t / u0[j];
+        }
+        double nr = 0;
+        for (int j = 0; j < n; ++j) nr += x[j] * x[j];
+        nr = sqrt(nr);
+        if (!(nr > 0) || !isfinite(nr)) {
+            atomicExch(info, k + 1);
+            return;
+        }
+        for (int j = 0; j < n; ++j) x[j] /= nr;
+    }
+    int jm = 0;
+    for (int j = 1; j < n; ++j)
+        if (fabs(x[j]) > fabs(x[jm])) jm = j;
+    const double sg = x[jm] < 0 ? -1.0 : 1.0;
+    for (int j = 0; j < n; ++j) coef[(size_t)k * n + j] = sg * x[j];
+}
+
+// Working storage of nvec Lanczos recurrences with ncols stored columns.
+struct LnzBufs {
+    DevBuf basis, vring, w0, w1, al, bt, beta0, rc, part, ticket, dead;
+    hipError_t alloc(LnzArgs &l, size_t ss)
+    {
+        const size_t nv = l.nvec, n = l.n;
+        hipError_t e;
+        if ((e = basis.alloc(ss * nv * l.ncols * n))) return e;
+        if ((e = hipMemset(basis.p, 0, basis.bytes))) return e;
+        if ((e = vring.alloc(ss * 2 * nv * n))) return e;
+        if ((e = w0.alloc(ss * nv * n))) return e;
+        if ((e = w1.alloc(ss * nv * n))) return e;
+        if ((e = al.alloc(sizeof(double) * nv * l.m))) return e;
+        if ((e = bt.alloc(sizeof(double) * nv * l.m))) return e;
+        if ((e = beta0.alloc(sizeof(double) * nv))) return e;
+        if ((e = rc.alloc(sizeof(double) * nv * l.ncols))) return e;
+        if ((e = part.alloc(sizeof(double) * nv * l.nchunk * l.nb * kPart))) return e;
+        if ((e = ticket.alloc(sizeof(unsigned) * nv * l.nchunk))) return e;
+        if ((e = hipMemset(ticket.p, 0, ticket.bytes))) return e;
+        if ((e = dead.alloc(sizeof(int) * nv))) return e;
+        l.basis = basis.p;
+        l.vring = vring.p;
+        l.w0 = w0.p;
+        l.w1 = w1.p;
+        l.al = al.as<double>();
+        l.bt = bt.as<double>();
+        l.beta0 = beta0.as<double>();
+        l.rc = rc.as<double>();
+        l.part = part.as<double>();
+        l.ticket = ticket.as<unsigned>();
+        l.dead = dead.as<int>();
+        return hipSuccess;
+    }
+};
 
 }  // namespace espec
 
@@ -512,10 +707,10 @@ struct espec_sil_plan {
     bool mixed = false, ref = false;
     double cst = 0, abstol = 0;
     long long steps_done = 0;
-    int nb = 0, nt = 256;
+    int nb = 0;
     hipStream_t stream = nullptr;
-    DevBuf x, psi0, vpot, basis, w0, w1, al, bt, beta0, rc, part, ticket, dead, coef, info, cbuf,
-        dstep;
+    DevBuf x, psi0, vpot, coef, info, cbuf, dstep;
+    LnzBufs lb;
     SilArgs sa{};
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -529,21 +724,52 @@ struct espec_sil_plan {
 
 namespace {
 
-template <typename S>
-hipError_t sil_step(espec_sil_plan &P, hipStream_t s)
+// Grid geometry of an FD operator on an ndim grid (x fastest).
+bool set_grid(LnzArgs &l, int ndim, const int *np, const double *shm)
 {
-    LnzArgs &l = P.sa.l;
-    const int nv = 2 * P.npk;
-    dim3 grid(P.nb, nv);
-    hipLaunchKernelGGL((lnz_init<S, 256>), grid, dim3(256), 0, s, l, (const V2<S> *)P.x.p,
-                       (const S *)nullptr);
-    for (int i = 0; i < P.m; ++i) {
+    if (ndim < 1 || ndim > 3 || !np || !shm) return false;
+    l.ndim = ndim;
+    long long n = 1;
+    int st = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (a < ndim && np[a] < 4) return false;  // 5-point stencils (au1d.f)
+        l.np[a] = a < ndim ? np[a] : 1;
+        l.st[a] = st;
+        st *= l.np[a];
+        n *= l.np[a];
+        l.shm[a] = a < ndim ? shm[a] : 0.0;
+    }
+    if (n > (1LL << 30)) return false;
+    l.n = (int)n;
+    return true;
+}
+
+int blocks_for(int n) { return std::max(1, std::min(256, (n + 255) / 256)); }
+
+// The m iterations of LNZ for every recurrence of l, on stream s.
+template <typename S>
+hipError_t lnz_iterations(const LnzArgs &l, hipStream_t s)
+{
+    dim3 grid(l.nb, l.nvec), gridp(l.nb, l.nvec, l.nchunk);
+    for (int i = 0; i < l.m; ++i) {
         LnzArgs li = l;
         li.it = i;
         hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
-        hipLaunchKernelGGL((lnz_project<S, 256>), grid, dim3(256), 0, s, li);
+        hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
         hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
     }
+    return hipGetLastError();
+}
+
+template <typename S>
+hipError_t sil_step(espec_sil_plan &P, hipStream_t s)
+{
+    const LnzArgs &l = P.sa.l;
+    dim3 grid(P.nb, l.nvec);
+    hipLaunchKernelGGL((lnz_init<S, 256>), grid, dim3(256), 0, s, l, (const V2<S> *)P.x.p,
+                       (const S *)nullptr);
+    hipError_t e = lnz_iterations<S>(l, s);
+    if (e != hipSuccess) return e;
     SilArgs sa = P.sa;
     sa.x0 = P.x.p;
     hipLaunchKernelGGL((sil_coeffs<S, 256>), grid, dim3(256), 0, s, sa);
@@ -586,7 +812,7 @@ int sil_run_typed(espec_sil_plan &P, int nsteps)
 
 }  // namespace
 
-// <psi0|psi> for sample 0 (from spo_lds.hip).
+// <psi0|psi> for sample 0 (spo_lds.hip).
 namespace espec {
 hipError_t launch_spo_sample(const void *x, const void *psi0, int n, int npk, bool mixed,
                              double *cbuf, int maxsamp, int idx, hipStream_t s);
@@ -600,20 +826,15 @@ int espec_sil_plan_create(espec_sil_plan **out, int ndim, const int *np, int npk
                           double abstol, double dt, int lc, const double *shm, const double *vpot,
                           const double *u, const double *v, int maxsamp, int flags)
 {
-    if (!out || ndim < 1 || ndim > 3 || !np || npk < 1 || mp < 2 || mp > kMaxKrylov || lc < 1 ||
-        !shm || !vpot || !u || maxsamp < 1)
+    if (!out || npk < 1 || mp < 2 || mp > kMaxKrylov || lc < 1 || !vpot || !u || maxsamp < 1)
         return ESPEC_ERR_ARG;
     *out = nullptr;
     std::unique_ptr<espec_sil_plan> P(new espec_sil_plan);
+    LnzArgs &l = P->sa.l;
+    if (!set_grid(l, ndim, np, shm)) return ESPEC_ERR_ARG;
     P->ndim = ndim;
-    P->n = 1;
-    for (int a = 0; a < ndim; ++a) {
-        if (np[a] < 1) return ESPEC_ERR_ARG;
-        P->np[a] = np[a];
-        P->n *= np[a];
-    }
-    for (int a = 0; a < ndim; ++a)
-        if (np[a] < 4) return ESPEC_ERR_ARG;  // the 5-point stencils need 4 points (au1d.f)
+    for (int a = 0; a < ndim; ++a) P->np[a] = np[a];
+    P->n = l.n;
     P->npk = npk;
     P->m = mp;
     P->lc = lc;
@@ -623,70 +844,37 @@ int espec_sil_plan_create(espec_sil_plan **out, int ndim, const int *np, int npk
     P->cst = dt / kShfs;
     P->abstol = abstol;
     const int n = P->n, nv = 2 * npk;
-    P->nb = std::max(1, std::min(256, (n + 255) / 256));
+    P->nb = blocks_for(n);
     ESPEC_HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     const size_t ss = P->mixed ? sizeof(float) : sizeof(double);
     if (P->mixed) {
-        std::vector<float> xs((size_t)2 * n * npk);
-        for (size_t i = 0; i < (size_t)n * npk; ++i) {
-            xs[2 * i] = (float)u[i];
-            xs[2 * i + 1] = v ? (float)v[i] : 0.f;
-        }
+        std::vector<float> xs = interleave<float>(u, v, (size_t)n * npk);
         ESPEC_HIP_TRY(P->x.upload(xs.data(), sizeof(float) * xs.size()));
         ESPEC_HIP_TRY(P->psi0.upload(xs.data(), sizeof(float) * xs.size()));
     } else {
-        std::vector<double> xs((size_t)2 * n * npk);
-        for (size_t i = 0; i < (size_t)n * npk; ++i) {
-            xs[2 * i] = u[i];
-            xs[2 * i + 1] = v ? v[i] : 0.0;
-        }
+        std::vector<double> xs = interleave<double>(u, v, (size_t)n * npk);
         ESPEC_HIP_TRY(P->x.upload(xs.data(), sizeof(double) * xs.size()));
         ESPEC_HIP_TRY(P->psi0.upload(xs.data(), sizeof(double) * xs.size()));
     }
     ESPEC_HIP_TRY(P->vpot.upload(vpot, sizeof(double) * (size_t)n * npk));
-    ESPEC_HIP_TRY(P->basis.alloc(ss * (size_t)nv * mp * n));
-    ESPEC_HIP_TRY(P->w0.alloc(ss * (size_t)nv * n));
-    ESPEC_HIP_TRY(P->w1.alloc(ss * (size_t)nv * n));
-    ESPEC_HIP_TRY(P->al.alloc(sizeof(double) * nv * mp));
-    ESPEC_HIP_TRY(P->bt.alloc(sizeof(double) * nv * mp));
-    ESPEC_HIP_TRY(P->beta0.alloc(sizeof(double) * nv));
-    ESPEC_HIP_TRY(P->rc.alloc(sizeof(double) * nv * mp));
-    ESPEC_HIP_TRY(P->part.alloc(sizeof(double) * (size_t)nv * P->nb * (kMaxKrylov + 2)));
-    ESPEC_HIP_TRY(P->ticket.alloc(sizeof(unsigned) * nv));
-    ESPEC_HIP_TRY(hipMemset(P->ticket.p, 0, P->ticket.bytes));
-    ESPEC_HIP_TRY(P->dead.alloc(sizeof(int) * nv));
+    l.vpot = P->vpot.as<double>();
+    l.vshift = 1;
+    l.nvec = nv;
+    l.m = mp;
+    l.nb = P->nb;
+    l.ncols = mp;
+    l.nchunk = 1;
+    l.nreort = mp;
+    l.acoef = nullptr;
+    l.acc = nullptr;
+    l.nacc = 0;
+    ESPEC_HIP_TRY(P->lb.alloc(l, ss));
     ESPEC_HIP_TRY(P->coef.alloc(sizeof(double) * nv * 2 * mp));
     ESPEC_HIP_TRY(P->info.alloc(sizeof(int)));
     ESPEC_HIP_TRY(hipMemset(P->info.p, 0, sizeof(int)));
     ESPEC_HIP_TRY(P->cbuf.alloc(sizeof(double) * 3 * (size_t)maxsamp * npk));
     ESPEC_HIP_TRY(hipMemset(P->cbuf.p, 0, P->cbuf.bytes));
     ESPEC_HIP_TRY(P->dstep.alloc(sizeof(long long)));
-    LnzArgs &l = P->sa.l;
-    l.ndim = ndim;
-    l.n = n;
-    int st = 1;
-    for (int a = 0; a < 3; ++a) {
-        l.np[a] = a < ndim ? np[a] : 1;
-        l.st[a] = st;
-        st *= l.np[a];
-        l.shm[a] = a < ndim ? shm[a] : 0.0;
-    }
-    l.vpot = P->vpot.as<double>();
-    l.vshared = 0;
-    l.nvec = nv;
-    l.m = mp;
-    l.nb = P->nb;
-    l.basis = P->basis.p;
-    l.w0 = P->w0.p;
-    l.w1 = P->w1.p;
-    l.al = P->al.as<double>();
-    l.bt = P->bt.as<double>();
-    l.beta0 = P->beta0.as<double>();
-    l.rc = P->rc.as<double>();
-    l.part = P->part.as<double>();
-    l.ticket = P->ticket.as<unsigned>();
-    l.dead = P->dead.as<int>();
-    l.nreort = mp;
     P->sa.cst = P->cst;
     P->sa.abstol = abstol;
     P->sa.ref = P->ref ? 1 : 0;
@@ -779,6 +967,174 @@ int espec_plnz(int ndim, const int *np, int mp, double abstol, double dt, double
     if (!rc) rc = espec_sil_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u, v);
     espec_sil_plan_destroy(P);
     return rc;
+}
+
+int espec_au(int ndim, const int *np, const double *shm, const double *vpot, const double *u,
+             double *hu)
+{
+    LnzArgs l{};
+    if (!vpot || !u || !hu || !set_grid(l, ndim, np, shm)) return ESPEC_ERR_ARG;
+    const size_t bytes = sizeof(double) * l.n;
+    DevBuf dv, du, dh;
+    ESPEC_HIP_TRY(dv.upload(vpot, bytes));
+    ESPEC_HIP_TRY(du.upload(u, bytes));
+    ESPEC_HIP_TRY(dh.alloc(bytes));
+    l.vpot = dv.as<double>();
+    hipLaunchKernelGGL(au_kernel, dim3(std::min(1024, (l.n + 255) / 256)), dim3(256), 0, nullptr,
+                       l, du.as<double>(), dh.as<double>());
+    ESPEC_HIP_TRY(hipGetLastError());
+    ESPEC_HIP_TRY(hipMemcpy(hu, dh.p, bytes, hipMemcpyDeviceToHost));
+    return ESPEC_OK;
+}
+
+int espec_fcorrel(int n, const double *u1, const double *u2, double *out)
+{
+    if (n < 0 || !u1 || !u2 || !out) return ESPEC_ERR_ARG;
+    if (n == 0) {
+        *out = 0;
+        return ESPEC_OK;
+    }
+    LnzArgs l{};
+    l.n = n;
+    l.nb = blocks_for(n);
+    l.nchunk = 1;
+    DevBuf a, b, part, ticket, res;
+    ESPEC_HIP_TRY(a.upload(u1, sizeof(double) * n));
+    ESPEC_HIP_TRY(b.upload(u2, sizeof(double) * n));
+    ESPEC_HIP_TRY(part.alloc(sizeof(double) * l.nb * kPart));
+    ESPEC_HIP_TRY(ticket.alloc(sizeof(unsigned)));
+    ESPEC_HIP_TRY(hipMemset(ticket.p, 0, sizeof(unsigned)));
+    ESPEC_HIP_TRY(res.alloc(sizeof(double)));
+    l.part = part.as<double>();
+    l.ticket = ticket.as<unsigned>();
+    hipLaunchKernelGGL(dot_kernel, dim3(l.nb), dim3(256), 0, nullptr, l, a.as<double>(),
+                       b.as<double>(), res.as<double>());
+    ESPEC_HIP_TRY(hipGetLastError());
+    ESPEC_HIP_TRY(hipMemcpy(out, res.p, sizeof(double), hipMemcpyDeviceToHost));
+    return ESPEC_OK;
+}
+
+int espec_lnz(int ndim, const int *np, int m, int nreort, const double *shm, const double *vpot,
+              const double *start, double *al, double *bt, double *basis)
+{
+    LnzArgs l{};
+    if (m < 1 || nreort < 1 || nreort > kMaxReort || !vpot || !start || !al || !bt ||
+        !set_grid(l, ndim, np, shm))
+        return ESPEC_ERR_ARG;
+    DevBuf dv, ds;
+    ESPEC_HIP_TRY(dv.upload(vpot, sizeof(double) * l.n));
+    ESPEC_HIP_TRY(ds.upload(start, sizeof(double) * l.n));
+    l.vpot = dv.as<double>();
+    l.vshift = 0;
+    l.nvec = 1;
+    l.m = m;
+    l.nb = blocks_for(l.n);
+    l.ncols = nreort;
+    l.nchunk = (nreort + kChunk - 1) / kChunk;
+    l.nreort = nreort;
+    LnzBufs bufs;
+    ESPEC_HIP_TRY(bufs.alloc(l, sizeof(double)));
+    hipLaunchKernelGGL((lnz_init<double, 256>), dim3(l.nb, 1), dim3(256), 0, nullptr, l,
+                       (const double2 *)nullptr, ds.as<double>());
+    ESPEC_HIP_TRY(hipGetLastError());
+    int dead = 0;
+    ESPEC_HIP_TRY(hipMemcpy(&dead, bufs.dead.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (!dead) ESPEC_HIP_TRY(lnz_iterations<double>(l, nullptr));
+    if (!dead) {  // a zero start leaves AL, BT untouched (lnz.f:86)
+        ESPEC_HIP_TRY(hipMemcpy(al, bufs.al.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+        ESPEC_HIP_TRY(hipMemcpy(bt, bufs.bt.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    }
+    if (basis)
+        ESPEC_HIP_TRY(hipMemcpy(basis, bufs.basis.p, sizeof(double) * (size_t)nreort * l.n,
+                                hipMemcpyDeviceToHost));
+    return ESPEC_OK;
+}
+
+int espec_dstevx(int m, const double *d, const double *e, double abstol, double *w, double *z)
+{
+    if (m < 1 || m > 4096 || !d || !w || (m > 1 && !e)) return ESPEC_ERR_ARG;
+    std::vector<double> ee(m, 0.0);
+    for (int i = 0; i + 1 < m; ++i) ee[i] = e[i];
+    DevBuf dd, de, de2, dw, dz, di;
+    ESPEC_HIP_TRY(dd.upload(d, sizeof(double) * m));
+    ESPEC_HIP_TRY(de.upload(ee.data(), sizeof(double) * m));
+    ESPEC_HIP_TRY(de2.alloc(sizeof(double) * m));
+    ESPEC_HIP_TRY(dw.alloc(sizeof(double) * m));
+    ESPEC_HIP_TRY(dz.alloc(sizeof(double) * (size_t)m * m));
+    ESPEC_HIP_TRY(di.alloc(sizeof(int)));
+    hipLaunchKernelGGL(dstevx_kernel, dim3(1), dim3(64), 0, nullptr, m, dd.as<double>(),
+                       de.as<double>(), de2.as<double>(), abstol, dw.as<double>(),
+                       dz.as<double>(), di.as<int>());
+    ESPEC_HIP_TRY(hipGetLastError());
+    int info = 0;
+    ESPEC_HIP_TRY(hipMemcpy(&info, di.p, sizeof(int), hipMemcpyDeviceToHost));
+    ESPEC_HIP_TRY(hipMemcpy(w, dw.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    if (z) ESPEC_HIP_TRY(hipMemcpy(z, dz.p, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost));
+    return info ? ESPEC_ERR_DIAG : ESPEC_OK;
+}
+
+int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort, const double *shm,
+                 const double *vpot, double *eigvl, double *eigvc)
+{
+    LnzArgs l{};
+    if (nreort < 1 || nreort > kMaxReort || !vpot || !eigvl || !eigvc ||
+        !set_grid(l, ndim, np, shm))
+        return ESPEC_ERR_ARG;
+    const int n = l.n, ns = iu - il + 1;
+    if (il < 1 || iu > n || ns < 1 || ns > 64) return ESPEC_ERR_ARG;
+    if (nreort > n) nreort = n;
+    // RAN start vector, NSEED = 1 (lanczs.f:98-101, rans.f:32-39).
+    std::vector<double> start(n);
+    long long seed = 1;
+    for (int i = 0; i < n; ++i) {
+        seed = (seed * 1029 + 221591) % 1048576;
+        start[i] = (double)seed / 1048576.0;
+    }
+    DevBuf dv, ds, dacc, dcoef, de2, dw, dwork, dinfo;
+    ESPEC_HIP_TRY(dv.upload(vpot, sizeof(double) * n));
+    ESPEC_HIP_TRY(ds.upload(start.data(), sizeof(double) * n));
+    l.vpot = dv.as<double>();
+    l.vshift = 0;
+    l.nvec = 1;
+    l.m = n;
+    l.nb = blocks_for(n);
+    l.ncols = nreort;
+    l.nchunk = (nreort + kChunk - 1) / kChunk;
+    l.nreort = nreort;
+    LnzBufs bufs;
+    ESPEC_HIP_TRY(bufs.alloc(l, sizeof(double)));
+    // 1) n-step recurrence for the tridiagonal (lanczs.f:102-104).
+    hipLaunchKernelGGL((lnz_init<double, 256>), dim3(l.nb, 1), dim3(256), 0, nullptr, l,
+                       (const double2 *)nullptr, ds.as<double>());
+    ESPEC_HIP_TRY(lnz_iterations<double>(l, nullptr));
+    // 2) selected eigenpairs of the tridiagonal (lanczs.f:105-108).
+    ESPEC_HIP_TRY(dcoef.alloc(sizeof(double) * (size_t)ns * n));
+    ESPEC_HIP_TRY(de2.alloc(sizeof(double) * n));
+    ESPEC_HIP_TRY(dw.alloc(sizeof(double) * ns));
+    ESPEC_HIP_TRY(dwork.alloc(sizeof(double) * (size_t)ns * 6 * n));
+    ESPEC_HIP_TRY(dinfo.alloc(sizeof(int)));
+    ESPEC_HIP_TRY(hipMemset(dinfo.p, 0, sizeof(int)));
+    hipLaunchKernelGGL(lanczs_select, dim3(1), dim3(64), 0, nullptr, n, l.al, l.bt,
+                       de2.as<double>(), il, ns, dw.as<double>(), dcoef.as<double>(),
+                       dwork.as<double>(), dinfo.as<int>());
+    ESPEC_HIP_TRY(hipGetLastError());
+    // 3) replay the same recurrence, accumulating sum_J q_k[J] v_J for every
+    //    selected k in one pass (lanczs.f:109-176 replays once per k).
+    ESPEC_HIP_TRY(dacc.alloc(sizeof(double) * (size_t)ns * n));
+    ESPEC_HIP_TRY(hipMemset(dacc.p, 0, dacc.bytes));
+    ESPEC_HIP_TRY(hipMemset(bufs.basis.p, 0, bufs.basis.bytes));
+    LnzArgs r = l;
+    r.acoef = dcoef.as<double>();
+    r.acc = dacc.as<double>();
+    r.nacc = ns;
+    hipLaunchKernelGGL((lnz_init<double, 256>), dim3(r.nb, 1), dim3(256), 0, nullptr, r,
+                       (const double2 *)nullptr, ds.as<double>());
+    ESPEC_HIP_TRY(lnz_iterations<double>(r, nullptr));
+    int info = 0;
+    ESPEC_HIP_TRY(hipMemcpy(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost));
+    ESPEC_HIP_TRY(hipMemcpy(eigvl, dw.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    ESPEC_HIP_TRY(hipMemcpy(eigvc, dacc.p, sizeof(double) * (size_t)ns * n, hipMemcpyDeviceToHost));
+    return info ? ESPEC_ERR_DIAG : ESPEC_OK;
 }
 
 }  // extern "C"

@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "api_common.hpp"
+#include "axis.hpp"
 #include "spo_lds.hpp"
 #include "spo_pass.hpp"
 
@@ -35,33 +36,6 @@ struct espec_spo_plan {
 
 namespace {
 
-void fill_axis(DevAxis &d, const FftPlan &p, int es, int nl, int lmod, int la, int lb, int tw0)
-{
-    d.n = p.n;
-    d.ln = ilog2(p.n);
-    d.nst = p.nst;
-    set_line_geometry(d, es, nl, lmod, la, lb);
-    d.tw0 = tw0;
-    for (int s = 0; s < p.nst; ++s) {
-        d.rad[s] = p.rad[s];
-        d.lm[s] = p.lm[s];
-        d.lL[s] = p.lL[s];
-        d.off[s] = p.off[s];
-    }
-}
-
-// Line geometry of axis a of an ndim grid (x fastest).
-void axis_geometry(int ndim, const int *np, int a, int &es, int &nl, int &lmod, int &la, int &lb)
-{
-    int n = ntot(ndim, np);
-    es = 1;
-    for (int k = 0; k < a; ++k) es *= np[k];
-    nl = n / np[a];
-    lmod = es;          // lines inside one slab of the lower axes
-    la = 1;
-    lb = es * np[a];    // jump over the transformed axis
-}
-
 // S is the storage precision of psi (float: ESPEC_MIXED); every table the
 // kernels multiply by (twiddles, V and T phases) is FP64.
 template <typename S>
@@ -73,18 +47,7 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
     std::vector<FftPlan> fp;
     std::vector<double> tw;
     std::vector<int> perm;
-    for (int a = 0; a < P.ndim; ++a) {
-        fp.push_back(make_fft_plan(P.np[a], P.ref));
-        int es, nl, lmod, la, lb;
-        axis_geometry(P.ndim, P.np, a, es, nl, lmod, la, lb);
-        fill_axis(P.la.ax[a], fp[a], es, nl, lmod, la, lb, (int)(tw.size() / 2));
-        for (size_t i = 0; i < fp[a].twc.size(); ++i) {
-            tw.push_back(fp[a].twc[i]);
-            tw.push_back(fp[a].tws[i]);
-        }
-        P.la.perm_off[a] = (int)perm.size();
-        perm.insert(perm.end(), fp[a].perm.begin(), fp[a].perm.end());
-    }
+    axis_tables(P.ndim, P.np, P.ref, P.la.ax, fp, tw, perm, P.la.perm_off);
     std::vector<S> xs((size_t)2 * n * npk);
     for (int p = 0; p < npk; ++p) {
         std::vector<S> t = interleave<S>(u + (size_t)p * n, v ? v + (size_t)p * n : nullptr, n);
@@ -120,20 +83,7 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
     // Kinetic phase: natural order for the reference kernel; for the fused
     // kernel, position of frequency k is sum_a perm_a[k_a] * stride_a.
     std::vector<double> tc = kinetic_grid(P.ndim, P.np, sh, sm, P.ref);
-    std::vector<double> tcs(n);
-    if (P.ref) {
-        tcs = tc;
-    } else {
-        for (int k = 0; k < n; ++k) {
-            int r = k, pos = 0, stride = 1;
-            for (int a = 0; a < P.ndim; ++a) {
-                pos += fp[a].perm[r % P.np[a]] * stride;
-                r /= P.np[a];
-                stride *= P.np[a];
-            }
-            tcs[pos] = tc[k];
-        }
-    }
+    std::vector<double> tcs = P.ref ? tc : scramble(P.ndim, P.np, fp, tc);
     std::vector<double> kin = phase_table<double>(tcs.data(), n, cst1);
     std::vector<double> vph((size_t)2 * n * npk);
     for (int p = 0; p < npk; ++p) {
@@ -142,7 +92,7 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
     }
     ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(double) * vph.size()));
     ESPEC_HIP_TRY(P.kin.upload(kin.data(), sizeof(double) * kin.size()));
-    ESPEC_HIP_TRY(P.tw.upload(tw.data(), sizeof(double) * (tw.size() ? tw.size() : 2)));
+    ESPEC_HIP_TRY(P.tw.upload(tw.data(), sizeof(double) * tw.size()));
     ESPEC_HIP_TRY(P.perm.upload(perm.data(), sizeof(int) * perm.size()));
     return ESPEC_OK;
 }

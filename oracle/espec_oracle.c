@@ -553,7 +553,7 @@ double orc_eigenerg(int ndim, const int *np, const double *shm,
 /* reort.f:54-80: one classical Gram-Schmidt pass. */
 void orc_reort(int n, int mcols, const double *basis, double *w)
 {
-    double rv[64];
+    double rvs[64], *rv = mcols > 64 ? malloc(sizeof(double) * mcols) : rvs;
     for (int j = 0; j < mcols; ++j) {
         double s = 0;
         for (int i = 0; i < n; ++i) s = s + w[i] * basis[(size_t)j * n + i];
@@ -561,6 +561,7 @@ void orc_reort(int n, int mcols, const double *basis, double *w)
     }
     for (int j = 0; j < mcols; ++j)
         for (int i = 0; i < n; ++i) w[i] = w[i] - rv[j] * basis[(size_t)j * n + i];
+    if (rv != rvs) free(rv);
 }
 
 /* lnz.f:65-119. */
@@ -765,6 +766,7 @@ int orc_plnz_step(int ndim, const int *np, int mp, double abstol, double cst,
                   int mode)
 {
     int n = ntot_of(ndim, np), info;
+    if (mp < 1 || mp > 64) return -2;
     double *basis = malloc(sizeof(double) * (size_t)n * mp);
     double *wv = malloc(sizeof(double) * 2 * n), *wu = wv + n;
     double al[64], bt[64], lam[64], q[64 * 64], c[64], u2[64], ur[64], wvs[64], work[64];

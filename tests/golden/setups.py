@@ -91,10 +91,11 @@ def test_case(t):
         ps = load("eigpot_test3_psi")
         pv = load("eigpot_test3_pot")
         n = len(ps)
-        xi_a, xf_a = pv[0, 0], pv[-1, 0]
-        x, sh = grid(n, xi_a, xf_a)
+        # The .FILE grid is in bohr already ('R1_i/a.u. = 0.151178E+01',
+        # results.test4): SH = (XF-XI)/(NP-1) without the angstrom factor.
+        sh = (pv[-1, 0] - pv[0, 0]) / (n - 1.0)
         shm = 1.0 / (24.0 * m * sh * sh)
-        psi = ps[:, 1] / np.sqrt(np.sum(ps[:, 1] ** 2))
+        psi = ps[:, 1].copy()  # used as read (.GETR); its norm prints as 1.0000002
         return dict(np_=[n], sh=[sh], sm=[m], shm=[shm], vpot=pv[:, 1].copy(), u=psi,
                     v=np.zeros(n), dt=2e-3, ti=0.0, tf=250.0, mf=12, method="PSIL", mp=10,
                     abstol=1e-6)
