@@ -52,16 +52,27 @@ def timed_run(E, c, flags, steps, warmup):
     return t1 - t0, res, plan
 
 
-def pmc_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if any."""
+def pmc_traffic(name, kernel):
+    """HBM bytes per dispatch of `kernel` (template instantiation name) from
+    the committed rocprofv3 --pmc summary profiles/pmc_<name>.json, if it
+    holds that instantiation (the autotuner may pick another shape)."""
     path = os.path.join(ROOT, "profiles", f"pmc_{name}.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
-    except (OSError, ValueError):
+            k = json.load(f)["kernels"].get(kernel)
+        return None if k is None else k["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
         return None
+
+
+def kernel_names(shape, np0, np1, storage="float"):
+    """Instantiation names of the two pass kernels for a launch shape."""
+    ept_r = shape["rows"] * np0 // shape["rows_nt"]
+    ept_s = shape["tile"] * np1 // shape["tile_nt"]
+    return (f"pass_rows<{storage}, {shape['rows_nt']}, true, true, {ept_r}, true>",
+            f"pass_strided<{storage}, {shape['tile_nt']}, {ept_s}>")
 
 
 def main():
@@ -93,16 +104,21 @@ def main():
     sec_64, res_64, plan_64 = timed_run(E, c, E.FP64 | E.EXACT, K, W)
     log(f"[bench] fp64: {sec_64 * 1e3:.2f} ms host")
 
-    # Per-kernel durations (HIP events on the plan's stream, no graph).
+    # Per-kernel durations (timing-only HIP events on the plan's stream
+    # around every launch of a 200-step run, no graph).
     plan_mix.reset()
     prof = plan_mix.profile(args.profile_steps)
     plan_mix.reset()
-    names = ["pass_rows (x pass: V phases, C(t), x FFT pair)", "pass_strided (y pass: y FFT pair)"]
-    kinds = [dict(kernel=names[i] if len(prof) > 1 else "spo_lds_fast", ms=m, bytes=b)
+    shape = plan_mix.shape()
+    inst = kernel_names(shape, c.np[0], c.np[1])
+    descr = ["x pass: inverse x, V phases, C(t), forward x, Kx",
+             "y pass: forward y, Ky, inverse y"]
+    kinds = [dict(kernel=inst[i] if len(prof) > 1 else "spo_lds_fast",
+                  role=descr[i] if len(prof) > 1 else "whole step in LDS", ms=m, bytes=b)
              for i, (m, b) in enumerate(prof)]
     dom = max(kinds, key=lambda k: k["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3)
-    log(f"[bench] kernels: " + ", ".join(f"{k['kernel'].split()[0]} {k['ms'] * 1e3:.2f} us" for k in kinds))
+    log(f"[bench] kernels: " + ", ".join(f"{k['kernel']} {k['ms'] * 1e3:.2f} us" for k in kinds))
 
     ms_bw, by_bw = E.bw_probe(1 << 30, 10)
     copy_bw = by_bw / (ms_bw * 1e-3)
@@ -124,7 +140,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "256x256 coupled-Morse split-operator FFT (2-D), mixed path",
                    "grid": c.np, "dt_au": 1.0, "lc": 1, "steps": K,
-                   "launches_per_run": launches},
+                   "launches_per_run": launches, "launch_shape": plan_mix.shape()},
         "fp64": {"value": npts * K / sec_64, "ms_per_step": sec_64 * 1e3 / K,
                  "mixed_over_fp64": sec_64 / sec_mix},
         "kernels": kinds,
@@ -136,8 +152,9 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": pmc_traffic("config2_mixed"),
+            "traffic": pmc_traffic("config2_mixed", dom["kernel"]),
         },
+        "timed_region_us_per_launch": ev_ms_mix * 1e3 / max(launches, 1),
     }
 
     if not args.no_cpu:

@@ -60,6 +60,7 @@ def _load():
     L.espec_spo_plan_destroy.argtypes = [vp]
     L.espec_spo_plan_reset.argtypes = [vp]
     L.espec_spo_plan_profile.argtypes = [vp, ip, d, d, ip, P(ip)]
+    L.espec_spo_plan_shape.argtypes = [vp, i]
     L.espec_bw_probe.argtypes = [C.c_size_t, ip, P(dp), P(dp)]
     L.espec_plnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, d, vp, ip, P(ip), ip]
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
@@ -77,12 +78,15 @@ def _load():
     L.espec_au.argtypes = [ip, i, d, d, d, d]
     L.espec_lnz.argtypes = [ip, i, ip, ip, d, d, d, d, d, vp]
     L.espec_dstevx.argtypes = [ip, d, d, dp, d, vp]
-    L.espec_lanczs.argtypes = [ip, i, ip, ip, ip, d, d, d, d]
+    L.espec_lanczs.argtypes = [ip, i, ip, ip, ip, dp, d, d, d, d]
     L.espec_fcorrel.argtypes = [ip, d, d, P(dp)]
     L.espec_assemble_corr.argtypes = [ip, d, d, d, d]
     L.espec_hwhm.argtypes = [ip, dp, dp, dp, dp, d, d]
     L.espec_spectrumtd.argtypes = [ip, ip, dp, dp, dp, dp, d, d, d, d, d, ip]
     L.espec_lc_rule.argtypes = [dp, P(dp), dp, ip, ip, P(ip)]
+    L.espec_ecnorm.argtypes = [ip, d, P(dp)]
+    L.espec_efft4.argtypes = [ip, i, dp, d, d, d, d, P(dp), ip]
+    L.espec_eigenerg.argtypes = [ip, i, d, d, d, P(dp)]
     _lib = L
     return L
 
@@ -214,13 +218,13 @@ def dstevx(d, e, abstol=0.0):
     return w, z.reshape(m, m).T
 
 
-def lanczs(np_, il, iu, nreort, shm, vpot):
+def lanczs(np_, il, iu, nreort, shm, vpot, abstol=0.0):
     """LANCZS (lanczs.f:1): (eigvl[iu-il+1], eigvc[iu-il+1][n])."""
     n = int(np.prod(np_))
     ns = iu - il + 1
     ev, vc = np.zeros(ns), np.zeros(ns * n)
-    _check(_load().espec_lanczs(len(np_), _np3(np_), il, iu, nreort, _v3(shm, 0.0), _f64(vpot),
-                                ev, vc), "LANCZS")
+    _check(_load().espec_lanczs(len(np_), _np3(np_), il, iu, nreort, abstol, _v3(shm, 0.0),
+                                _f64(vpot), ev, vc), "LANCZS")
     return ev, vc.reshape(ns, n)
 
 
@@ -229,6 +233,30 @@ def fcorrel(u1, u2):
     out = C.c_double()
     u1, u2 = _f64(u1), _f64(u2)
     _check(_load().espec_fcorrel(len(u1), u1, u2, C.byref(out)), "FCORREL")
+    return out.value
+
+
+def ecnorm(u):
+    """ECNORM (ecnorm.f:1): sqrt(sum u^2)."""
+    out = C.c_double()
+    u = _f64(u)
+    _check(_load().espec_ecnorm(len(u), u, C.byref(out)), "ECNORM")
+    return out.value
+
+
+def efft4(np_, b, u, v, tc, vpot, flags=REF):
+    """EFFT4 (eigenergfft.f:1): <psi|H|psi> with the FFT kinetic term."""
+    out = C.c_double()
+    _check(_load().espec_efft4(len(np_), _np3(np_), b, _f64(u), _f64(v), _f64(tc), _f64(vpot),
+                               C.byref(out), flags), "EFFT4")
+    return out.value
+
+
+def eigenerg(np_, shm, u, vpot):
+    """EIGENERG (eigenerg.f:1): <u|H u> with the FD operator."""
+    out = C.c_double()
+    _check(_load().espec_eigenerg(len(np_), _np3(np_), _v3(shm, 0.0), _f64(u), _f64(vpot),
+                                  C.byref(out)), "EIGENERG")
     return out.value
 
 
@@ -373,6 +401,13 @@ class SpoPlan(_Plan):
         ms, nl = C.c_double(), C.c_int()
         _check(self._L.espec_spo_plan_last_timing(self._h, C.byref(ms), C.byref(nl)), "timing")
         return ms.value, nl.value
+
+    def shape(self):
+        """Launch shape: dict(lds, rows, rows_nt, tile, tile_nt, spread, xmap)."""
+        a = np.zeros(7, dtype=np.int32)
+        _check(self._L.espec_spo_plan_shape(self._h, a), "spo_plan_shape")
+        return dict(zip(("lds", "rows", "rows_nt", "tile", "tile_nt", "spread", "xmap"),
+                        a.tolist()))
 
     def reset(self):
         """Restart from psi0 at step 0 (tables and captured graphs kept)."""

@@ -574,18 +574,20 @@ __global__ __launch_bounds__(64) void dstevx_kernel(int m, double *d, double *e,
 }
 
 // LANCZS eigenvectors of the n x n Lanczos tridiagonal for the selected
-// indices: eigenvalue k by bisection, vector by inverse iteration with a
+// indices: eigenvalue k by bisection (to ABSTOL when > 0, as dstebz.f in
+// DSTEVX('V','I'), lanczs.f:90; else to rounding), vector by inverse iteration with a
 // partially pivoted LU of T - lambda I (the method of dstein.f), scaled to
 // unit norm with its largest component positive (dstein.f), written as the
 // replay coefficients coef[k][J].  One thread per selected eigenvalue.
 __global__ __launch_bounds__(64) void lanczs_select(int n, const double *d, const double *e,
-                                                    double *e2, int il, int ns, double *eigvl,
-                                                    double *coef, double *work, int *info)
+                                                    double *e2, int il, int ns, double abstol,
+                                                    double *eigvl, double *coef, double *work,
+                                                    int *info)
 {
     __shared__ double bnd[4];
     if (threadIdx.x == 0) {
         double pm, gl, gu, at;
-        tridiag_bounds(n, d, e, 0.0, e2, &pm, &gl, &gu, &at);
+        tridiag_bounds(n, d, e, abstol, e2, &pm, &gl, &gu, &at);
         bnd[0] = pm;
         bnd[1] = gl;
         bnd[2] = gu;
@@ -1073,8 +1075,8 @@ int espec_dstevx(int m, const double *d, const double *e, double abstol, double 
     return info ? ESPEC_ERR_DIAG : ESPEC_OK;
 }
 
-int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort, const double *shm,
-                 const double *vpot, double *eigvl, double *eigvc)
+int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort, double abstol,
+                 const double *shm, const double *vpot, double *eigvl, double *eigvc)
 {
     LnzArgs l{};
     if (nreort < 1 || nreort > kMaxReort || !vpot || !eigvl || !eigvc ||
@@ -1115,7 +1117,7 @@ int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort, const doub
     ESPEC_HIP_TRY(dinfo.alloc(sizeof(int)));
     ESPEC_HIP_TRY(hipMemset(dinfo.p, 0, sizeof(int)));
     hipLaunchKernelGGL(lanczs_select, dim3(1), dim3(64), 0, nullptr, n, l.al, l.bt,
-                       de2.as<double>(), il, ns, dw.as<double>(), dcoef.as<double>(),
+                       de2.as<double>(), il, ns, abstol, dw.as<double>(), dcoef.as<double>(),
                        dwork.as<double>(), dinfo.as<int>());
     ESPEC_HIP_TRY(hipGetLastError());
     // 3) replay the same recurrence, accumulating sum_J q_k[J] v_J for every

@@ -192,6 +192,53 @@ hipError_t run_fftnt(double2 *x, int n, int m, const double2 *tw, hipStream_t s)
     return hipGetLastError();
 }
 
+// FFT(A,B,NTOT,N,NSPAN,ISN) on device arrays re/im (natural order in and out).
+hipError_t run_line_fft(double *re, double *im, int ntot, int n, int nspan, int isn, bool ref,
+                        hipStream_t s)
+{
+    if (n == 1) return hipSuccess;
+    FftPlan p = make_fft_plan(n, ref);
+    LineArgs la{};
+    la.ax.n = n;
+    la.ax.ln = ilog2(n);
+    la.ax.nst = p.nst;
+    set_line_geometry(la.ax, 1, 1, 1, 0, 0);
+    la.ax.tw0 = 0;
+    for (int st = 0; st < p.nst; ++st) {
+        la.ax.rad[st] = p.rad[st];
+        la.ax.lm[st] = p.lm[st];
+        la.ax.lL[st] = p.lL[st];
+        la.ax.off[st] = p.off[st];
+    }
+    la.stride = nspan / n;
+    la.nspan = nspan;
+    la.nlines = ntot / n;
+    std::vector<double> tw;
+    for (size_t i = 0; i < p.twc.size(); ++i) {
+        tw.push_back(p.twc[i]);
+        tw.push_back(p.tws[i]);
+    }
+    if (tw.empty()) tw.assign(2, 0.0);
+    DevBuf dtw, dperm;
+    hipError_t e = dtw.upload(tw.data(), sizeof(double) * tw.size());
+    if (e == hipSuccess) e = dperm.upload(p.perm.data(), sizeof(int) * n);
+    if (e != hipSuccess) return e;
+    la.tw = dtw.as<double2>();
+    la.perm = dperm.as<int>();
+    const size_t lds = sizeof(double2) * 2 * n;
+    auto kern = isn > 0 ? fft_lines<1> : fft_lines<-1>;
+    if (lds > 64 * 1024) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(std::min(la.nlines, 4096)), dim3(256), lds, s, la, re, im);
+    e = hipGetLastError();
+    // the tables are freed on return: finish before that
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
 std::vector<double> fftnt_twiddles(int n, bool ref)
 {
     const double twopi = ref ? (double)(float)kTwoPi : kTwoPi;  // cfft.f:35
@@ -320,45 +367,11 @@ int espec_fft(double *a, double *b, int ntot, int n, int nspan, int isn, int fla
         return ESPEC_ERR_ARG;
     if (!is_pow2(n) || n > kLdsLine) return ESPEC_ERR_UNSUPPORTED;
     if (n == 1) return ESPEC_OK;
-    const bool ref = (flags & ESPEC_REF) != 0;
-    FftPlan p = make_fft_plan(n, ref);
-    LineArgs la{};
-    la.ax.n = n;
-    la.ax.ln = ilog2(n);
-    la.ax.nst = p.nst;
-    set_line_geometry(la.ax, 1, 1, 1, 0, 0);
-    la.ax.tw0 = 0;
-    for (int s = 0; s < p.nst; ++s) {
-        la.ax.rad[s] = p.rad[s];
-        la.ax.lm[s] = p.lm[s];
-        la.ax.lL[s] = p.lL[s];
-        la.ax.off[s] = p.off[s];
-    }
-    la.stride = nspan / n;
-    la.nspan = nspan;
-    la.nlines = ntot / n;
-    std::vector<double> tw;
-    for (size_t i = 0; i < p.twc.size(); ++i) {
-        tw.push_back(p.twc[i]);
-        tw.push_back(p.tws[i]);
-    }
-    if (tw.empty()) tw.assign(2, 0.0);
-    DevBuf da, db, dtw, dperm;
+    DevBuf da, db;
     ESPEC_HIP_TRY(da.upload(a, sizeof(double) * ntot));
     ESPEC_HIP_TRY(db.upload(b, sizeof(double) * ntot));
-    ESPEC_HIP_TRY(dtw.upload(tw.data(), sizeof(double) * tw.size()));
-    ESPEC_HIP_TRY(dperm.upload(p.perm.data(), sizeof(int) * n));
-    la.tw = dtw.as<double2>();
-    la.perm = dperm.as<int>();
-    const size_t lds = sizeof(double2) * 2 * n;
-    const int grid = std::min(la.nlines, 4096);
-    auto kern = isn > 0 ? fft_lines<1> : fft_lines<-1>;
-    if (lds > 64 * 1024)
-        ESPEC_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, nullptr, la, da.as<double>(),
-                       db.as<double>());
-    ESPEC_HIP_TRY(hipGetLastError());
+    ESPEC_HIP_TRY(run_line_fft(da.as<double>(), db.as<double>(), ntot, n, nspan, isn,
+                               (flags & ESPEC_REF) != 0, nullptr));
     ESPEC_HIP_TRY(hipMemcpy(a, da.p, sizeof(double) * ntot, hipMemcpyDeviceToHost));
     ESPEC_HIP_TRY(hipMemcpy(b, db.p, sizeof(double) * ntot, hipMemcpyDeviceToHost));
     return ESPEC_OK;

@@ -11,6 +11,8 @@ namespace espec {
 struct PassShape {
     int rows = 1, rows_nt = 64;    // x lines per row-pass workgroup, its threads
     int tile = 16, tile_nt = 256;  // contiguous elements per strided tile, its threads
+    int spread = 1;                // active workgroups at linear ids 0, spread, 2 spread, ...
+    int xmap = 1;                  // contiguous strided tiles per XCD
 };
 
 struct SpoPassPlan {

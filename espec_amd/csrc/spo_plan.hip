@@ -303,6 +303,20 @@ int espec_spo_plan_reset(espec_spo_plan *P)
     return ESPEC_OK;
 }
 
+int espec_spo_plan_shape(espec_spo_plan *P, int *shape)
+{
+    if (!P || !shape) return ESPEC_ERR_ARG;
+    const PassShape &s = P->pp.shape;
+    shape[0] = P->lds ? 1 : 0;
+    shape[1] = s.rows;
+    shape[2] = s.rows_nt;
+    shape[3] = s.tile;
+    shape[4] = s.tile_nt;
+    shape[5] = s.spread;
+    shape[6] = s.xmap;
+    return ESPEC_OK;
+}
+
 int espec_spo_plan_profile(espec_spo_plan *P, int nsteps, double *ms, double *bytes, int maxkinds,
                            int *nkinds)
 {

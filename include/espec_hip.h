@@ -16,8 +16,11 @@
  *
  * Flags select the arithmetic:
  *   ESPEC_FP64   complex128 state, FP64 everything;
- *   ESPEC_MIXED  complex64 state and FFT/stencil arithmetic, FP64 norm,
- *                C(t), Lanczos alpha/beta and Gram coefficients;
+ *   ESPEC_MIXED  complex64 state in HBM (psi, psi0, Krylov basis), rounded
+ *                once per launch or pass; every kernel computes in FP64
+ *                (FFT stages, phases, stencils, norm, C(t), Lanczos
+ *                alpha/beta and Gram coefficients): FP32 twiddle and phase
+ *                arithmetic accumulates coherently and misses 1e-6 in C(t);
  *   ESPEC_REF    the reference's exact semantics (f32-rounded 2*pi in the
  *                FFT twiddles, the accumulated kinetic grid, the
  *                non-unitary kinetic and Lanczos recombination updates)
@@ -132,13 +135,27 @@ int espec_dstevx(int m, const double *d, const double *e, double abstol,
 /* Replaces LANCZS (lanczs.f:1-3): eigenpairs il..iu (1-based) of the FD
  * Hamiltonian from the RAN start vector (rans.f:3) with an n-step Lanczos
  * and partial reorthogonalisation against nreort vectors; eigenvectors
- * rebuilt by replaying the recurrence.  eigvc is n x (iu-il+1). */
+ * rebuilt by replaying the recurrence once for all il..iu (the reference
+ * replays it once per eigenvector).  abstol > 0: eigenvalues are bisection
+ * midpoints (DSTEVX('V','I'), dstebz.f).  eigvc is n x (iu-il+1). */
 int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort,
-                 const double *shm, const double *vpot, double *eigvl,
-                 double *eigvc);
+                 double abstol, const double *shm, const double *vpot,
+                 double *eigvl, double *eigvc);
 
 /* Replaces FCORREL(N,U1,U2) (fcorrel.f:1): sum u1*u2 in FP64 on device. */
 int espec_fcorrel(int n, const double *u1, const double *u2, double *out);
+/* Replaces ECNORM(N,US) (ecnorm.f:1): sqrt(sum u^2). */
+int espec_ecnorm(int n, const double *u, double *out);
+/* Replaces EFFT4(NDIM,N,NP,B,U,V,WU,WV,TC,VPOT) (eigenergfft.f:1): <psi|V|psi>
+ * + <psi|F^-1 TC F|psi> B on the half-swapped copy, FFTs of fft3d.f's
+ * definition (ESPEC_REF: f32 twiddles).  tc as filled by DIGT. */
+int espec_efft4(int ndim, const int *np, double b, const double *u,
+                const double *v, const double *tc, const double *vpot,
+                double *e, int flags);
+/* Replaces EIGENERG(DIM,N,NP,SHM,U1,VPOT,WORK,VAR) (eigenerg.f:1): <u|H u>
+ * with the FD operator of AU. */
+int espec_eigenerg(int ndim, const int *np, const double *shm, const double *u,
+                   const double *vpot, double *e);
 
 /* ---------------------------------------------------------------------
  * Correlation -> spectrum (espec.f:1546-1594).
@@ -219,6 +236,30 @@ int espec_sil_plan_get(espec_sil_plan *plan, int pk, double *cr, double *ci,
                        double *nrm, int maxsamp, int *nsamp, double *u,
                        double *v);
 int espec_sil_plan_destroy(espec_sil_plan *plan);
+
+/* Launch shape chosen for the plan, shape[7]: 1 if LDS-resident, rows per
+ * row-pass workgroup, its threads, strided tile width, its threads, XCD
+ * spread, XCD-contiguous tile map (no reference counterpart; for reports). */
+int espec_spo_plan_shape(espec_spo_plan *plan, int *shape);
+
+/* ---------------------------------------------------------------------
+ * Batched second-order-differencing plans: npk packets on one grid
+ * (PSODFFT's time loop, psodfft.f:109-200).  The first step of the first
+ * run is the two half-strength start-up steps; grids above 2048 points run
+ * one pass per axis (ESPEC_EXACT only).
+ * ------------------------------------------------------------------- */
+typedef struct espec_sod_plan espec_sod_plan;
+int espec_sod_plan_create(espec_sod_plan **plan, int ndim, const int *np,
+                          int npk, double dt, int lc, const double *sh,
+                          const double *sm, const double *vpot,
+                          const double *u, const double *v, int maxsamp,
+                          int flags);
+int espec_sod_plan_run(espec_sod_plan *plan, int nsteps);
+int espec_sod_plan_sync(espec_sod_plan *plan);
+int espec_sod_plan_get(espec_sod_plan *plan, int pk, double *cr, double *ci,
+                       double *nrm, int maxsamp, int *nsamp, double *u,
+                       double *v);
+int espec_sod_plan_destroy(espec_sod_plan *plan);
 
 /* Streaming copy kernel over nbytes, iters times: mean ms per launch and
  * bytes moved per launch (read + write), for bandwidth calibration. */

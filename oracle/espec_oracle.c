@@ -851,10 +851,13 @@ double orc_ran(int *nseed)
     return 1.0 * *nseed / 1048576;
 }
 
-/* lanczs.f:76-183. */
+/* lanczs.f:76-183.  Eigenvalues: DSTEVX('V','I') (lanczs.f:90) takes the
+ * bisection path (dstebz.f) with ABSTOL > 0, whose values are interval
+ * midpoints; with ABSTOL <= 0 the QL values.  Vectors: QL's, which the
+ * inverse iteration of dstein.f reproduces at either value. */
 int orc_lanczs(int ndim, const int *np, int il, int iu, int nreort,
-               const double *shm, const double *vpot, double *eigvl,
-               double *eigvc)
+               double abstol, const double *shm, const double *vpot,
+               double *eigvl, double *eigvc)
 {
     int n = ntot_of(ndim, np), ns = iu - il + 1;
     double *al = malloc(sizeof(double) * 2 * n), *bt = al + n;
@@ -867,6 +870,12 @@ int orc_lanczs(int ndim, const int *np, int il, int iu, int nreort,
     orc_lnz(ndim, np, n, nreort, shm, vpot, start, al, bt, basis);
     int info = orc_tridiag_eig(n, al, bt, lam, q);
     if (info) goto out;
+    if (abstol > 0) {
+        double *wb = malloc(sizeof(double) * n);
+        orc_tridiag_bisect(n, al, bt, abstol, wb);
+        for (int k = il - 1; k < iu; ++k) lam[k] = wb[k];
+        free(wb);
+    }
     for (int k = 0; k < ns; ++k) {
         double *qk = q + (size_t)(il - 1 + k) * n;
         /* dstein.f normalises each eigenvector so that its largest

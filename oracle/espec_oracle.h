@@ -118,10 +118,11 @@ double orc_ran(int *nseed);
  * from the RAN start vector with an n-step Lanczos and partial
  * reorthogonalisation against the first nreort vectors; eigenvectors are
  * rebuilt by replaying the recurrence (lanczs.f:104-177).  eigvc is
- * n x (iu-il+1).  Uses the QL eigensolver (ABSTOL <= 0 semantics). */
+ * n x (iu-il+1).  abstol > 0: eigenvalues are dstebz.f bisection
+ * midpoints (DSTEVX('V','I'), lanczs.f:90); vectors from QL. */
 int orc_lanczs(int ndim, const int *np, int il, int iu, int nreort,
-               const double *shm, const double *vpot, double *eigvl,
-               double *eigvc);
+               double abstol, const double *shm, const double *vpot,
+               double *eigvl, double *eigvc);
 
 /* espec.f:1324-1353: LC and the *CHANGE final-time rule.  Returns LC and
  * writes the (possibly changed) final time. */

@@ -83,7 +83,7 @@ class Oracle:
         L.orc_plnz.restype = _ip
         L.orc_ran.argtypes = [C.POINTER(_ip)]
         L.orc_ran.restype = _dp
-        L.orc_lanczs.argtypes = [_ip, _i, _ip, _ip, _ip, _d, _d, _d, _d]
+        L.orc_lanczs.argtypes = [_ip, _i, _ip, _ip, _ip, _dp, _d, _d, _d, _d]
         L.orc_lanczs.restype = _ip
         L.orc_lc_rule.argtypes = [_dp, C.POINTER(_dp), _dp, _ip, _ip]
         L.orc_lc_rule.restype = _ip
@@ -212,11 +212,11 @@ class Oracle:
         out = [self.lib.orc_ran(C.byref(s)) for _ in range(count)]
         return np.array(out), s.value
 
-    def lanczs(self, np_, il, iu, nreort, shm, vpot):
+    def lanczs(self, np_, il, iu, nreort, shm, vpot, abstol=0.0):
         n = int(np.prod(np_))
         ns = iu - il + 1
         ev, vc = np.zeros(ns), np.zeros(ns * n)
-        info = self.lib.orc_lanczs(len(np_), _np3(np_), il, iu, nreort, _v3(shm, 0.0),
+        info = self.lib.orc_lanczs(len(np_), _np3(np_), il, iu, nreort, abstol, _v3(shm, 0.0),
                                    _f64(vpot), ev, vc)
         return info, ev, vc.reshape(ns, n)
 

@@ -103,17 +103,20 @@ def test_dstevx(gpu, orc, abstol, m):
         assert np.abs(z[:, j] - s * z2[:, j]).max() <= 1e-12
 
 
+@pytest.mark.parametrize("abstol", [0.0, 1e-6])
 @pytest.mark.parametrize("n,nreort", [(201, 50), (201, 201), (64, 64)])
-def test_lanczs(gpu, orc, n, nreort):
+def test_lanczs(gpu, orc, n, nreort, abstol):
     m = 7.49 * 1836.152667539
     x = np.linspace(0.5, 2.8, n) / 0.5291772083
     h = x[1] - x[0]
     v = 0.5 * 0.1 * (x - 2.2) ** 2
     shm = [1.0 / (24 * m * h * h)]
-    print(f"case: lanczs n={n} nreort={nreort}", flush=True)
-    ev, vc = E.lanczs([n], 1, 4, nreort, shm, v)
-    info, ev2, vc2 = orc.lanczs([n], 1, 4, nreort, shm, v)
-    assert np.abs(ev - ev2).max() <= 1e-12
+    print(f"case: lanczs n={n} nreort={nreort} abstol={abstol}", flush=True)
+    ev, vc = E.lanczs([n], 1, 4, nreort, shm, v, abstol)
+    info, ev2, vc2 = orc.lanczs([n], 1, 4, nreort, shm, v, abstol)
+    # with ABSTOL > 0 a midpoint may sit in the neighbouring interval when
+    # the two tridiagonals differ in the last bit
+    assert np.abs(ev - ev2).max() <= max(1e-12, abstol)
     for k in range(4):
         assert np.abs(vc[k] - vc2[k]).max() <= 1e-9
 
@@ -172,3 +175,29 @@ def test_spectrum_chain(gpu, orc, t, mf, iwin, wp, hw, de, ade, src, mode):
                                          (0.0, 150.0, 5e-5, 14), (0.0, 250.0, 2e-3, 12)])
 def test_lc_rule(gpu, orc, ti, tf, dt, mf):
     assert E.lc_rule(ti, tf, dt, mf) == orc.lc_rule(ti, tf, dt, mf)
+
+
+@pytest.mark.parametrize("np_", [[256], [32, 16], [8, 8, 4]])
+@pytest.mark.parametrize("mode", ["REF", "EXACT"])
+def test_efft4(gpu, orc, np_, mode):
+    import oracle
+    rng = np.random.default_rng(len(np_))
+    n = int(np.prod(np_))
+    u, v, vp = rng.standard_normal(n), rng.standard_normal(n), rng.uniform(0, 0.1, n)
+    sh, sm = [0.05, 0.07, 0.09][: len(np_)], [1836.0, 918.0, 500.0][: len(np_)]
+    tc, b = orc.digt(np_, sh, sm, getattr(oracle, mode))
+    print(f"case: efft4 {np_} {mode}", flush=True)
+    e = E.efft4(np_, b, u, v, tc, vp, getattr(E, mode))
+    eo = orc.efft4(np_, u, v, tc, vp, getattr(oracle, mode))
+    assert abs(e - eo) <= 1e-12 * abs(eo)
+
+
+@pytest.mark.parametrize("np_", [[201], [16, 12], [6, 7, 8]])
+def test_eigenerg_ecnorm(gpu, orc, np_):
+    rng = np.random.default_rng(len(np_) + 10)
+    n = int(np.prod(np_))
+    u, vp = rng.standard_normal(n), rng.uniform(0, 0.1, n)
+    shm = [0.3, 0.5, 0.7][: len(np_)]
+    print(f"case: eigenerg {np_}", flush=True)
+    assert abs(E.eigenerg(np_, shm, u, vp) - orc.eigenerg(np_, shm, u, vp)) <= 1e-12 * n
+    assert abs(E.ecnorm(u) - np.sqrt(orc.fcorrel(u, u))) <= 1e-13 * np.sqrt(n)

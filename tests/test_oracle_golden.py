@@ -104,3 +104,28 @@ def test_test8_input_is_printed_correlation():
     """The .ONLYSPEC input file and the printed correlation are the same
     numbers (getcorr.f reads what espec.f:1605-1612 prints back)."""
     assert np.array_equal(S.load("correl8_corr"), S.load("test8_corr"))
+
+
+# LANCZS eigenvalues printed by the reference ('state has been calculating',
+# lanczs.f:1031-1032, 8 significant digits) with ABSTOL = 1e-6: the
+# bisection leaves each value anywhere in its final interval, so the bar is
+# ABSTOL / 2 plus the last printed digit.
+LANCZS_GOLD = {
+    14: ("ohs", 201, [0.44281536E-02, 0.13282522E-01, 0.22133939E-01, 0.30979451E-01,
+                      0.39816698E-01, 0.48643908E-01, 0.57458719E-01]),
+    5: ("morse", 256, [0.44125657E-02, 0.13143364E-01, 0.21748722E-01, 0.30226180E-01,
+                       0.38576558E-01, 0.46799856E-01, 0.54894435E-01, 0.62860294E-01]),
+}
+
+
+@pytest.mark.parametrize("t", sorted(LANCZS_GOLD))
+def test_lanczs_eigenvalues(orc, t):
+    kind, n, gold = LANCZS_GOLD[t]
+    m = 7.49 * S.AMU
+    x, sh = S.grid(n, 0.5, 2.8)
+    shm = 1.0 / (24.0 * m * sh * sh)
+    v0 = S.ohs(x, 1.146, 1943.79, m) if kind == "ohs" else S.morse_cm(x, 1.146, 1943.79, 13.71, m)
+    nreort = min(n, 300)  # NREORT = LMTREORT (espec.f:151, param.h:9)
+    info, ev, _ = orc.lanczs([n], 1, len(gold), nreort, [shm], v0, 1e-6)
+    assert info == 0
+    assert np.abs(ev - np.array(gold)).max() <= 6e-7
