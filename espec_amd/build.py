@@ -74,8 +74,15 @@ def build_fortran(verbose=False):
     os.makedirs(moddir, exist_ok=True)
     tmp = FLIB + ".tmp"
     _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", moddir, "-o", tmp] + srcs
-         + [f"-L{HERE}", "-lespec_hip", f"-Wl,-rpath,{HERE}"], verbose)
+         + [f"-L{HERE}", "-lespec_hip", "-Wl,-rpath,$ORIGIN"], verbose)
     os.replace(tmp, FLIB)
+    # The test host program that calls the drop-ins as eSPec's main program does.
+    drv = os.path.join(ROOT, "tests", "fortran", "dropin_driver.f90")
+    if os.path.exists(drv):
+        exe = os.path.join(ROOT, "tests", "fortran", "dropin_driver.x")
+        _run([FLANG, "-O2", "-module-dir", moddir, "-o", exe + ".tmp", drv, f"-L{HERE}",
+              "-lespec_f", "-lespec_hip", f"-Wl,-rpath,$ORIGIN/../../espec_amd"], verbose)
+        os.replace(exe + ".tmp", exe)
     return FLIB
 
 

@@ -632,6 +632,31 @@ int espec_sod_plan_get(espec_sod_plan *P, int pk, double *cr, double *ci, double
     return ESPEC_OK;
 }
 
+int espec_sod_plan_get_prev(espec_sod_plan *P, int pk, double *u, double *v)
+{
+    if (!P || pk < 0 || pk >= P->npk || !u || !v) return ESPEC_ERR_ARG;
+    ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+    const size_t n = P->n;
+    if (P->mixed) {
+        std::vector<float> h(2 * n);
+        ESPEC_HIP_TRY(hipMemcpy(h.data(), P->prev.as<float>() + 2 * n * pk, sizeof(float) * 2 * n,
+                                hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            u[i] = h[2 * i];
+            v[i] = h[2 * i + 1];
+        }
+    } else {
+        std::vector<double> h(2 * n);
+        ESPEC_HIP_TRY(hipMemcpy(h.data(), P->prev.as<double>() + 2 * n * pk,
+                                sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            u[i] = h[2 * i];
+            v[i] = h[2 * i + 1];
+        }
+    }
+    return ESPEC_OK;
+}
+
 int espec_sod_plan_destroy(espec_sod_plan *P)
 {
     if (P) {

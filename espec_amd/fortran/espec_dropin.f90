@@ -1,0 +1,540 @@
+! Drop-in replacements for eSPec's hot-path subroutines.  Each keeps the
+! name, argument list and meaning of the reference routine it replaces
+! (cited per routine) so that linking libespec_f.so in place of those
+! object files moves the work to the GPU; the arithmetic runs in
+! libespec_hip.so through the ISO_C_BINDING interfaces of espec_capi.
+!
+! Differences a caller can see:
+!   - no routine STOPs: failures come back in INFO (>= 2, espec_strerror)
+!     or, for routines without INFO, as a message on stdout;
+!   - absorbing boundaries (ABSORB = .TRUE.) and the coupled-surface DIM
+!     variants return INFO = 6 (not on the device);
+!   - ESPEC_MODE in the environment selects the arithmetic (REF: the
+!     reference's own rounding, the default; EXACT; MIXED).
+
+! pspofft.f:1-150
+subroutine PSPOFFT(ABSORB, DIM, PRTCRL, TPABSOR, INFO, LC, MP, MPI, MPR, MXDCT, N, DT, TI, TF, &
+                   TOL, NP, SHM, SH, SM, U1, V1, XI, XF, TC, VPOT, VABC, LNZVC, WRK1, WRK2, VAR)
+  use iso_c_binding
+  use espec_capi
+  implicit none
+  logical :: ABSORB
+  character(len=*) :: DIM, PRTCRL, TPABSOR
+  integer :: INFO, LC, MP, MPI, MPR, MXDCT, N
+  real(8) :: DT, TI, TF, TOL
+  integer :: NP(*)
+  real(8) :: SHM(*), SH(*), SM(*), U1(*), V1(*), XI(*), XF(*), TC(*), VPOT(*), VABC(*)
+  real(8) :: LNZVC(MXDCT, *), WRK1(*), WRK2(*), VAR(*)
+  type(c_ptr) :: plan
+  integer :: ndim, flags, nst, maxs, ns, rc, k, i, step, chunk
+  real(8) :: b, t, cr, ci, ct, et, e0, fnr, fni, fnt
+  real(8), allocatable :: u0(:), v0(:), crs(:), cis(:), nrs(:)
+  logical :: every, prt
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  if (ndim == 0 .or. ABSORB) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  flags = mode_flags()
+  rc = espec_digt(ndim, NP, SH, SM, TC, b, flags)
+  nst = trip(TI + DT, TF + DT, DT)
+  maxs = nst / LC + 1
+  allocate(u0(N), v0(N), crs(maxs), cis(maxs), nrs(maxs))
+  u0 = U1(1:N)
+  v0 = V1(1:N)
+  rc = espec_spo_plan_create(plan, ndim, NP, 1, DT, LC, SH, SM, VPOT, U1, V1, maxs, flags)
+  if (rc /= 0) then
+    INFO = rc
+    return
+  end if
+  every = starts(PRTCRL, '.YES')
+  prt = every .or. starts(PRTCRL, '.PARTIAL')
+  if (prt) then
+    t = TI
+    call corr(N, u0, v0, U1, V1, cr, ci, fnr, fni)
+    ct = sqrt(cr**2 + ci**2)
+    rc = espec_efft4(ndim, NP, b, U1, V1, TC, VPOT, et, flags)
+    e0 = et
+    fnt = sqrt(fnr**2 + fni**2)
+    write(*,*)
+    write(*,*) 'Partial auto-correlation function:'
+    write(*,*) '=================================='
+    write(*,1011) '*t/fs*', '*E/a.u.*', '*Cr(t)*', '*Ci(t)*', '*|C(t)|*', '*|C|*'
+    write(*,1021) t, et, cr, ci, ct, fnt
+    chunk = LC
+    if (every) chunk = 1
+    step = 0
+    do while (step < nst)
+      k = min(chunk, nst - step)
+      rc = espec_spo_plan_run(plan, k)
+      if (rc /= 0) exit
+      do i = 1, k
+        t = t + DT
+      end do
+      step = step + k
+      if (every .or. mod(step, LC) == 0) then
+        rc = espec_spo_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U1, V1)
+        call corr(N, u0, v0, U1, V1, cr, ci, fnr, fni)
+        ct = sqrt(cr**2 + ci**2)
+        rc = espec_efft4(ndim, NP, b, U1, V1, TC, VPOT, et, flags)
+        fnt = sqrt(fnr**2 + fni**2)
+        if (abs((et - e0) / e0) * 1.0d2 > TOL) INFO = 1
+        write(*,1021) t, et, cr, ci, ct, fnt
+      end if
+    end do
+  else
+    rc = espec_spo_plan_run(plan, nst)
+  end if
+  if (rc == 0) rc = espec_spo_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U1, V1)
+  if (rc /= 0) then
+    INFO = rc
+  else
+    do k = 1, ns
+      LNZVC(k, MPR) = crs(k)
+      LNZVC(k, MPI) = cis(k)
+    end do
+  end if
+  rc = espec_spo_plan_destroy(plan)
+  deallocate(u0, v0, crs, cis, nrs)
+1011 format(/,7X,A6,7X,A8,8X,A7,7X,A7,6X,A8,6X,A5)
+1021 format(1X,F14.8,3X,E12.6,2(4X,F10.7),4X,F10.8,4X,F8.6)
+end subroutine PSPOFFT
+
+! psodfft.f:1-211: psi(t-dt) returns in U1, V1 and psi(t) in U2, V2.
+subroutine PSODFFT(ABSORB, DIM, PRTCRL, TPABSOR, INFO, LC, MP, MPI, MPR, MXDCT, N, DT, TI, TF, &
+                   TOL, NP, SHM, SH, SM, U1, U2, V1, V2, XI, XF, TC, VPOT, VABC, WRKR, WRKI, VAR, &
+                   LNZVC)
+  use iso_c_binding
+  use espec_capi
+  implicit none
+  logical :: ABSORB
+  character(len=*) :: DIM, PRTCRL, TPABSOR
+  integer :: INFO, LC, MP, MPI, MPR, MXDCT, N
+  real(8) :: DT, TI, TF, TOL
+  integer :: NP(*)
+  real(8) :: SHM(*), SH(*), SM(*), U1(*), U2(*), V1(*), V2(*), XI(*), XF(*), TC(*), VPOT(*)
+  real(8) :: VABC(*), WRKR(*), WRKI(*), VAR(*), LNZVC(MXDCT, *)
+  type(c_ptr) :: plan
+  integer :: ndim, flags, nst, maxs, ns, rc, k, i, step, chunk
+  real(8) :: b, t, cr, ci, ct, et, e0, fnr, fni, fnt, f0
+  real(8), allocatable :: u0(:), v0(:), crs(:), cis(:), nrs(:)
+  logical :: every, prt
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  if (ndim == 0 .or. ABSORB) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  flags = mode_flags()
+  rc = espec_digt(ndim, NP, SH, SM, TC, b, flags)
+  ! the start-up (two half steps) is step 1; then DO T=TI+2DT,TF+DT,DT
+  nst = 1 + trip(TI + 2 * DT, TF + DT, DT)
+  maxs = nst / LC + 1
+  allocate(u0(N), v0(N), crs(maxs), cis(maxs), nrs(maxs))
+  u0 = U1(1:N)
+  v0 = V1(1:N)
+  U2(1:N) = u0
+  V2(1:N) = v0
+  rc = espec_sod_plan_create(plan, ndim, NP, 1, DT, LC, SH, SM, VPOT, U1, V1, maxs, flags)
+  if (rc /= 0) then
+    INFO = rc
+    return
+  end if
+  every = starts(PRTCRL, '.YES')
+  prt = every .or. starts(PRTCRL, '.PARTIAL')
+  call corr(N, u0, v0, U2, V2, cr, ci, fnr, fni)
+  rc = espec_efft4(ndim, NP, b, U2, V2, TC, VPOT, et, flags)
+  e0 = et
+  f0 = sqrt(fnr**2 + fni**2)
+  if (prt) then
+    t = TI
+    ct = sqrt(cr**2 + ci**2)
+    write(*,*)
+    write(*,*) 'Partial auto-correlation function:'
+    write(*,*) '=================================='
+    write(*,1011) '*t/fs*', '*E/a.u.*', '*Cr(t)*', '*Ci(t)*', '*|C(t)|*', '*|C|*'
+    write(*,1021) t, et, cr, ci, ct, f0
+    chunk = LC
+    if (every) chunk = 1
+    step = 0
+    do while (step < nst)
+      k = min(chunk, nst - step)
+      rc = espec_sod_plan_run(plan, k)
+      if (rc /= 0) exit
+      if (step == 0) then
+        t = TI + DT
+        do i = 2, k
+          t = t + DT
+        end do
+      else
+        do i = 1, k
+          t = t + DT
+        end do
+      end if
+      step = step + k
+      if (every .or. mod(step, LC) == 0) then
+        rc = espec_sod_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U2, V2)
+        call corr(N, u0, v0, U2, V2, cr, ci, fnr, fni)
+        ct = sqrt(cr**2 + ci**2)
+        rc = espec_efft4(ndim, NP, b, U2, V2, TC, VPOT, et, flags)
+        fnt = sqrt(fnr**2 + fni**2)
+        if (step > 1) then  ! psodfft.f:178-183 checks inside the leapfrog loop
+          if (abs((et - e0) / e0) * 1.0d2 > TOL .or. abs((fnt - f0) / f0) * 1.0d2 > TOL) INFO = 1
+        end if
+        write(*,1021) t, et, cr, ci, ct, fnt
+      end if
+    end do
+  else
+    rc = espec_sod_plan_run(plan, nst)
+  end if
+  if (rc == 0) rc = espec_sod_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U2, V2)
+  if (rc == 0) rc = espec_sod_plan_get_prev(plan, 0, U1, V1)
+  if (rc /= 0) then
+    INFO = rc
+  else
+    do k = 1, ns
+      LNZVC(k, MPR) = crs(k)
+      LNZVC(k, MPI) = cis(k)
+    end do
+    ! psodfft.f:194-204: final drift check
+    call corr(N, u0, v0, U2, V2, cr, ci, fnr, fni)
+    rc = espec_efft4(ndim, NP, b, U2, V2, TC, VPOT, et, flags)
+    fnt = sqrt(fnr**2 + fni**2)
+    if (abs((et - e0) / e0) * 1.0d2 > TOL .or. abs((fnt - f0) / f0) * 1.0d2 > TOL) INFO = 1
+  end if
+  rc = espec_sod_plan_destroy(plan)
+  deallocate(u0, v0, crs, cis, nrs)
+1011 format(/,7X,A6,7X,A8,8X,A7,7X,A7,6X,A8,6X,A5)
+1021 format(1X,F14.8,3X,E12.6,2(4X,F10.7),4X,F10.8,4X,F8.6)
+end subroutine PSODFFT
+
+! plnz.f:1-156
+subroutine PLNZ(ABSORB, CHANGE, DIM, PRTCRL, TPABSOR, INFO, LC, LMTREORT, MP, MPI, MPR, MXDCT, &
+                N, ABSTOL, DT, TI, TF, TOL, NP, IWORK, SHM, VPOT, VABC, U, V, XI, XF, WU, WV, WP, &
+                WORK, VAR, LNZVC)
+  use iso_c_binding
+  use espec_capi
+  implicit none
+  logical :: ABSORB, CHANGE
+  character(len=*) :: DIM, PRTCRL, TPABSOR
+  integer :: INFO, LC, LMTREORT, MP, MPI, MPR, MXDCT, N
+  real(8) :: ABSTOL, DT, TI, TF, TOL
+  integer :: NP(*), IWORK(*)
+  real(8) :: SHM(*), VPOT(*), VABC(*), U(*), V(*), XI(*), XF(*), WU(*), WV(*), WP(*), WORK(*)
+  real(8) :: VAR(*), LNZVC(MXDCT, *)
+  type(c_ptr) :: plan
+  integer :: ndim, flags, nst, maxs, ns, rc, k, i, step, chunk
+  real(8) :: t, cr, ci, ct, er, ei, et, e0, fnr, fni, fnt, f0
+  real(8), allocatable :: u0(:), v0(:), crs(:), cis(:), nrs(:)
+  logical :: every, prt
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  if (ndim == 0 .or. ABSORB) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  flags = mode_flags()
+  nst = trip(TI + DT, TF + DT, DT)
+  maxs = nst / LC + 1
+  allocate(u0(N), v0(N), crs(maxs), cis(maxs), nrs(maxs))
+  u0 = U(1:N)
+  v0 = V(1:N)
+  rc = espec_sil_plan_create(plan, ndim, NP, 1, MP, ABSTOL, DT, LC, SHM, VPOT, U, V, maxs, flags)
+  if (rc /= 0) then
+    INFO = rc
+    return
+  end if
+  every = starts(PRTCRL, '.YES')
+  prt = every .or. starts(PRTCRL, '.PARTIAL')
+  call corr(N, u0, v0, U, V, cr, ci, fnr, fni)
+  rc = espec_eigenerg(ndim, NP, SHM, U, VPOT, er)
+  rc = espec_eigenerg(ndim, NP, SHM, V, VPOT, ei)
+  et = er + ei
+  e0 = et
+  f0 = fnr
+  if (prt) then
+    t = TI
+    ct = cr**2 + ci**2
+    write(*,*)
+    write(*,*) 'Partial auto-correlation function:'
+    write(*,*) '=================================='
+    write(*,1011) '*t*', '*Energy*', '*Cr(t)*', '*Ci(t)*', '*|C(t)|^2*', '*|C|*'
+    write(*,1021) t, et, cr, ci, ct, f0
+    chunk = LC
+    if (every) chunk = 1
+    step = 0
+    do while (step < nst)
+      k = min(chunk, nst - step)
+      rc = espec_sil_plan_run(plan, k)
+      if (rc /= 0) exit
+      do i = 1, k
+        t = t + DT
+      end do
+      step = step + k
+      if (every .or. mod(step, LC) == 0) then
+        rc = espec_sil_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U, V)
+        if (rc /= 0) exit
+        call corr(N, u0, v0, U, V, cr, ci, fnr, fni)
+        ct = cr**2 + ci**2
+        rc = espec_eigenerg(ndim, NP, SHM, U, VPOT, er)
+        rc = espec_eigenerg(ndim, NP, SHM, V, VPOT, ei)
+        et = er + ei
+        fnt = fnr
+        if (abs((et - e0) / e0) * 1.0d2 > TOL .or. abs((fnt - f0) / f0) * 1.0d2 > TOL) INFO = 1
+        write(*,1021) t, et, cr, ci, ct, fnt
+      end if
+    end do
+  else
+    rc = espec_sil_plan_run(plan, nst)
+  end if
+  if (rc == 0) rc = espec_sil_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U, V)
+  if (rc /= 0) then
+    if (rc == 5) write(*,'(A)') '<<<>>> Diagonalization error <<<>>>'
+    INFO = rc
+  else
+    do k = 1, ns
+      LNZVC(k, MPR) = crs(k)
+      LNZVC(k, MPI) = cis(k)
+    end do
+    rc = espec_eigenerg(ndim, NP, SHM, U, VPOT, er)
+    rc = espec_eigenerg(ndim, NP, SHM, V, VPOT, ei)
+    et = er + ei
+    call corr(N, u0, v0, U, V, cr, ci, fnr, fni)
+    if (abs((et - e0) / e0) * 1.0d2 > TOL .or. abs((fnr - f0) / f0) * 1.0d2 > TOL) INFO = 1
+  end if
+  rc = espec_sil_plan_destroy(plan)
+  deallocate(u0, v0, crs, cis, nrs)
+1011 format(/,7X,A6,7X,A8,8X,A7,7X,A7,4X,A10,6X,A5)
+1021 format(1X,F14.8,3X,E12.6,2(4X,F10.7),4X,F10.8,4X,F8.6)
+end subroutine PLNZ
+
+! lanczs.f:1-190: eigenvectors into EIGVC(:,1..IU-IL+1) and LNZVC(:,1..IU-IL+1).
+subroutine LANCZS(CHANGE, DIM, IL, IU, INFO, NREORT, LMTREORT, N, NSEED, MXDCT, ABSTOL, IWK, &
+                  NP, AL, BT, EIGVL, SHM, V0, VI, VPOT, WK1, WK2, VAR, LNZVC, EIGVC)
+  use espec_capi
+  implicit none
+  logical :: CHANGE
+  character(len=*) :: DIM
+  integer :: IL, IU, INFO, NREORT, LMTREORT, N, NSEED, MXDCT
+  real(8) :: ABSTOL
+  integer :: IWK(*), NP(*)
+  real(8) :: AL(*), BT(*), EIGVL(*), SHM(*), V0(*), VI(*), VPOT(*), WK1(*), WK2(*), VAR(*)
+  real(8) :: LNZVC(MXDCT, *), EIGVC(MXDCT, *)
+  integer :: ndim, ns, k, nc, rc
+  character(len=2) :: chnc, chaux
+  real(8), allocatable :: vc(:)
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  if (ndim == 0) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  if (NREORT > LMTREORT) then  ! lanczs.f:68-74
+    write(*,*) 'Error NREORT > LMTREORT'
+    write(*,*) 'NREORT=', NREORT, '  LMTREORT=', LMTREORT
+    NREORT = LMTREORT
+    write(*,*) 'Changing NREORT to correct value!'
+    write(*,*) 'NREORT=', NREORT, '  LMTREORT=', LMTREORT
+  end if
+  NSEED = 1
+  ns = IU - IL + 1
+  allocate(vc(N * ns))
+  rc = espec_lanczs(ndim, NP, IL, IU, NREORT, ABSTOL, SHM, VPOT, EIGVL, vc)
+  if (rc /= 0) INFO = rc
+  nc = IL - 2
+  do k = 1, ns
+    nc = nc + 1
+    write(chnc, '(I2)') nc
+    if (chnc(2:2) == '1') then
+      chaux = 'st'
+    else if (chnc(2:2) == '2') then
+      chaux = 'nd'
+    else if (chnc(2:2) == '3') then
+      chaux = 'rd'
+    else
+      chaux = 'th'
+    end if
+    if (nc < 10 .and. INFO == 0) then
+      write(*,1031) nc, chaux, EIGVL(k)
+    else if (INFO == 0) then
+      write(*,1032) nc, chaux, EIGVL(k)
+    else
+      write(*,1033) nc, chaux
+    end if
+    EIGVC(1:N, k) = vc((k - 1) * N + 1:k * N)
+    LNZVC(1:N, k) = vc((k - 1) * N + 1:k * N)
+  end do
+  deallocate(vc)
+1031 format(4X,I1,A2,1X,'state has been calculating =',E16.8,'/a.u..')
+1032 format(3X,I2,A2,1X,'state has been calculating =',E16.8,'/a.u..')
+1033 format(7X,I2,A2,1X,'state hasn`t been calculating correctly!')
+end subroutine LANCZS
+
+! fft3d.f:1
+subroutine FFT(A, B, NTOT, N, NSPAN, ISN)
+  use espec_capi
+  implicit none
+  integer :: NTOT, N, NSPAN, ISN
+  real(8) :: A(*), B(*)
+  integer :: rc
+  rc = espec_fft(A, B, NTOT, N, NSPAN, ISN, mode_flags())
+  if (rc /= 0) write(*,'(A,I3)') '<<<>>> FFT on the device failed, code ', rc
+end subroutine FFT
+
+! dkeft.f:1-144 (the 1-D grid, and per-axis sums for 2-D/3-D)
+subroutine DIGT(DIM, NDIM, N, B, B1, B2, NP, SH, SM, TC)
+  use espec_capi
+  implicit none
+  character(len=*) :: DIM
+  integer :: NDIM, N
+  real(8) :: B, B1, B2
+  integer :: NP(*)
+  real(8) :: SH(*), SM(*), TC(*)
+  integer :: rc
+  NDIM = dim_of(DIM)
+  if (NDIM == 0) return
+  rc = espec_digt(NDIM, NP, SH, SM, TC, B, mode_flags())
+  B1 = SH(1)
+  B2 = 1.0d0 / (NP(1) * SH(1))
+end subroutine DIGT
+
+! au.f:3-84
+subroutine AU(DIM, N, NP, SHM, VPOT, U, WORK, VAR)
+  use espec_capi
+  implicit none
+  character(len=*) :: DIM
+  integer :: N, NP(*)
+  real(8) :: SHM(*), VPOT(*), U(*), WORK(*), VAR(*)
+  integer :: rc
+  rc = espec_au(dim_of(DIM), NP, SHM, VPOT, U, WORK)
+  if (rc /= 0) write(*,'(A,I3)') '<<<>>> AU on the device failed, code ', rc
+end subroutine AU
+
+! au1d.f:1
+subroutine AU_1D(N, SHM, VPOT, US, AUS)
+  use espec_capi
+  implicit none
+  integer :: N
+  real(8) :: SHM(*), VPOT(*), US(*), AUS(*)
+  integer :: rc, np(1)
+  np(1) = N
+  rc = espec_au(1, np, SHM, VPOT, US, AUS)
+end subroutine AU_1D
+
+! au2di.f:1
+subroutine AU_2D(NP, SHM, VPOT, US, AUS)
+  use espec_capi
+  implicit none
+  integer :: NP(*)
+  real(8) :: SHM(*), VPOT(*), US(*), AUS(*)
+  integer :: rc
+  rc = espec_au(2, NP, SHM, VPOT, US, AUS)
+end subroutine AU_2D
+
+! au3d.f:1
+subroutine AU_3D(NP, SHM, VPOT, US, AUS)
+  use espec_capi
+  implicit none
+  integer :: NP(*)
+  real(8) :: SHM(*), VPOT(*), US(*), AUS(*)
+  integer :: rc
+  rc = espec_au(3, NP, SHM, VPOT, US, AUS)
+end subroutine AU_3D
+
+! fcorrel.f:1-68
+real(8) function FCORREL(N, U1, U2)
+  use espec_capi
+  implicit none
+  integer :: N
+  real(8) :: U1(*), U2(*)
+  integer :: rc
+  rc = espec_fcorrel(N, U1, U2, FCORREL)
+end function FCORREL
+
+! ecnorm.f:1-68
+real(8) function ECNORM(N, US)
+  use espec_capi
+  implicit none
+  integer :: N
+  real(8) :: US(*)
+  integer :: rc
+  rc = espec_ecnorm(N, US, ECNORM)
+end function ECNORM
+
+! eigenergfft.f:1-103
+real(8) function EFFT4(NDIM, N, NP, B, U, V, WU, WV, TC, VPOT)
+  use espec_capi
+  implicit none
+  integer :: NDIM, N, NP(*)
+  real(8) :: B, U(*), V(*), WU(*), WV(*), TC(*), VPOT(*)
+  integer :: rc
+  rc = espec_efft4(NDIM, NP, B, U, V, TC, VPOT, EFFT4, mode_flags())
+end function EFFT4
+
+! eigenerg.f:1-70
+real(8) function EIGENERG(DIM, N, NP, SH, U1, VPOT, WORK, VAR)
+  use espec_capi
+  implicit none
+  character(len=*) :: DIM
+  integer :: N, NP(*)
+  real(8) :: SH(*), U1(*), VPOT(*), WORK(*), VAR(*)
+  integer :: rc
+  rc = espec_eigenerg(dim_of(DIM), NP, SH, U1, VPOT, EIGENERG)
+end function EIGENERG
+
+! rans.f:3-42
+real(8) function RAN(NSEED)
+  implicit none
+  integer :: NSEED
+  NSEED = int(mod(int(NSEED, 8) * 1029_8 + 221591_8, 1048576_8))
+  RAN = dble(NSEED) / 1048576.0d0
+end function RAN
+
+! hwhm.f:1-92
+subroutine HWHM(M, DT, TF, TAL, DE, U, V)
+  use espec_capi
+  implicit none
+  integer :: M
+  real(8) :: DT, TF, TAL, DE, U(*), V(*)
+  integer :: rc
+  rc = espec_hwhm(M, DT, TF, TAL, DE, U, V)
+end subroutine HWHM
+
+! spectrumtd.f:1-116, including its printed table.
+subroutine SPECTRUMTD(TPWIND, M, DT, TF, WP, DE, U, V)
+  use espec_capi
+  implicit none
+  character(len=*) :: TPWIND
+  integer :: M
+  real(8) :: DT, TF, WP, DE, U(*), V(*)
+  integer :: iwin, rc, n, i
+  real(8), allocatable :: e(:), w(:), wi(:)
+  if (starts(TPWIND, '.SG')) then
+    iwin = 1
+  else if (starts(TPWIND, '.NONE') .or. starts(TPWIND, '.NULL')) then
+    iwin = 0
+  else
+    write(*,'(A)') '<<<>>> Windowing function error <<<>>>'
+    return
+  end if
+  n = 2**M
+  allocate(e(n), w(n), wi(n))
+  rc = espec_spectrumtd(iwin, M, DT, TF, WP, DE, U, V, e, w, wi, mode_flags())
+  if (rc == 0) then
+    write(*,*)
+    write(*,*) 'Spectrum:'
+    write(*,*) '========='
+    write(*,1011)
+    do i = 1, n
+      write(*,1021) e(i), w(i), w(i), wi(i)
+    end do
+  end if
+  deallocate(e, w, wi)
+1011 format(5X,'*E (eV)*',11X,'*W*',11X,'*FFTr*',10X,'*FFTi*')
+1021 format(1X,4(F12.7,4X))
+end subroutine SPECTRUMTD

@@ -259,6 +259,9 @@ int espec_sod_plan_sync(espec_sod_plan *plan);
 int espec_sod_plan_get(espec_sod_plan *plan, int pk, double *cr, double *ci,
                        double *nrm, int maxsamp, int *nsamp, double *u,
                        double *v);
+/* psi(t-dt) of packet pk (PSODFFT returns it in U1, V1; psodfft.f:200). */
+int espec_sod_plan_get_prev(espec_sod_plan *plan, int pk, double *u,
+                            double *v);
 int espec_sod_plan_destroy(espec_sod_plan *plan);
 
 /* Streaming copy kernel over nbytes, iters times: mean ms per launch and

@@ -58,3 +58,38 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_fortran_dropin_exports_reference_names():
+    """libespec_f.so exports the reference's subroutine symbols."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "espec_amd", "libespec_f.so"))
+    for name in ("pspofft_", "psodfft_", "plnz_", "lanczs_", "fft_", "digt_", "au_", "au_1d_",
+                 "au_2d_", "au_3d_", "fcorrel_", "ecnorm_", "efft4_", "eigenerg_", "ran_",
+                 "hwhm_", "spectrumtd_"):
+        assert hasattr(lib, name), name
+
+
+def test_fortran_ran_is_the_reference_lcg(orc):
+    """RAN through the Fortran ABI (by-reference INTEGER), no device needed."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "espec_amd", "libespec_f.so"))
+    lib.ran_.restype = ctypes.c_double
+    s = ctypes.c_int(1)
+    got = [lib.ran_(ctypes.byref(s)) for _ in range(50)]
+    want, seed = orc.ran(1, 50)
+    assert np.array_equal(got, want) and s.value == seed
+
+
+def test_fortran_driver_reads_case_and_reports_without_device():
+    """The host program parses a full test input; with no GPU visible the
+    drop-in returns INFO = 3 (HIP error) instead of stopping."""
+    import espec_amd as E
+    if E.device_count() > 0:
+        pytest.skip("a device is visible")
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import setups
+    from test_dropin_gpu import DRIVER, driver_input
+    c = setups.test_case(12)
+    r = subprocess.run([DRIVER], input=driver_input(c), capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr[-500:]
+    assert "INFO    3" in r.stdout
