@@ -116,7 +116,12 @@ def main():
     kinds = [dict(kernel=inst[i] if len(prof) > 1 else "spo_lds_fast",
                   role=descr[i] if len(prof) > 1 else "whole step in LDS", ms=m, bytes=b)
              for i, (m, b) in enumerate(prof)]
-    dom = max(kinds, key=lambda k: k["ms"])
+    # The row pass carries every byte of the per-grid-point-step model
+    # (state read + write, V, psi0/LC: 2*8 + 4 + 8/LC for complex64); the
+    # strided pass only adds the multi-pass round trip, which the model
+    # does not count.  The roofline kernel is therefore the row pass (or
+    # the single LDS-resident step kernel).
+    dom = kinds[0]
     achieved = dom["bytes"] / (dom["ms"] * 1e-3)
     log(f"[bench] kernels: " + ", ".join(f"{k['kernel']} {k['ms'] * 1e3:.2f} us" for k in kinds))
 
@@ -155,6 +160,10 @@ def main():
             "traffic": pmc_traffic("config2_mixed", dom["kernel"]),
         },
         "timed_region_us_per_launch": ev_ms_mix * 1e3 / max(launches, 1),
+        # whole-step view: gp-steps/s x (2*8 + 4 + 8/LC) bytes against the peak
+        "step_roofline": {"bytes_per_gp_step": 28.0,
+                          "achieved_GBps": npts * K / sec_mix * 28.0 / 1e9,
+                          "frac": npts * K / sec_mix * 28.0 / HBM_PEAK},
     }
 
     if not args.no_cpu:
