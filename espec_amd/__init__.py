@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libespec_hip.so")
 
 FP64, MIXED = 0x0, 0x1
 EXACT, REF = 0x0, 0x2
+PER_PACKET_MASS = 0x4
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 AMU = 1836.152667539    # electron masses per amu (espec.f:614)
 A0A = 0.5291772083      # angstrom per bohr (espec.f:67)
@@ -391,9 +392,16 @@ class SpoPlan(_Plan):
     _prefix = "spo"
 
     def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=MIXED):
+        """sm: masses per axis, or an [npk][ndim] array of per-packet masses."""
         vp = self._setup(np_, vpot, u, v, maxsamp)
+        sm = np.asarray(sm, dtype=np.float64)
+        if sm.ndim == 2:
+            flags |= PER_PACKET_MASS
+            self._sm = np.ascontiguousarray(sm).ravel()
+        else:
+            self._sm = _v3(sm)
         rc = self._L.espec_spo_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, dt,
-                                           lc, _v3(sh), _v3(sm), self._vpot, self._u, vp,
+                                           lc, _v3(sh), self._sm, self._vpot, self._u, vp,
                                            maxsamp, flags)
         _check(rc, "spo_plan_create")
 

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
     VS *gx = reinterpret_cast<VS *>(a.x) + po;
     const VS *__restrict__ g0 = reinterpret_cast<const VS *>(a.psi0) + po;
     const V *__restrict__ gvg = reinterpret_cast<const V *>(a.vph) + po;
-    const V *__restrict__ gkg = reinterpret_cast<const V *>(a.kin);
+    const V *__restrict__ gkg = reinterpret_cast<const V *>(a.kin) + (a.kin_pp ? po : 0);
     const V *__restrict__ twg = reinterpret_cast<const V *>(a.tw);
     const int n = a.n;
     const double b = a.b;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(NT) void spo_lds_ref(SpoLdsArgs a)
     V *gx = reinterpret_cast<V *>(a.x) + po;
     const V *__restrict__ g0 = reinterpret_cast<const V *>(a.psi0) + po;
     const V *__restrict__ gv = reinterpret_cast<const V *>(a.vph) + po;
-    const V *__restrict__ gk = reinterpret_cast<const V *>(a.kin);
+    const V *__restrict__ gk = reinterpret_cast<const V *>(a.kin) + (a.kin_pp ? po : 0);
     const V *__restrict__ tw = reinterpret_cast<const V *>(a.tw);
     const int n = a.n, nh = n / 2;
     const double b = a.b;

@@ -13,7 +13,9 @@ struct SpoLdsArgs {
     void *x;               // [npk][n] state (complex64 mixed, complex128 FP64)
     const void *psi0;      // [npk][n] initial state for C(t), same precision
     const void *vph;       // [npk][n] double2 (cos, sin) of cst2 V, one table per packet
-    const void *kin;       // [n] double2 (cos, sin) of cst1 T, shared (digit-reversed for the fast kernel)
+    const void *kin;       // double2 (cos, sin) of cst1 T (digit-reversed for the fast kernel):
+                           // [n] shared, or [npk][n] when kin_pp
+    int kin_pp;            // per-packet kinetic tables
     double b;              // 1/N
     int nsteps, lc, maxsamp;
     long long step0;       // steps done before this launch

@@ -45,6 +45,7 @@ struct PassArgs {
     int lc, maxsamp;
     int S, L, ltile, last;
     int spread;            // only workgroups with blockIdx.x % spread == 0 work (XCD placement)
+    int ktps;              // kinetic table stride between packets (0: shared)
     int xmap;              // strided pass: give each XCD a contiguous range of tiles
 };
 
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
     for (int k = 0; k < EPT; ++k) {
         const int j = threadIdx.x + k * NT;
         tr[k] = a.tw[j < a.ntw ? j : 0];
-        if (FWD) kr[k] = a.kt[j < a.np0 ? j : 0];
+        if (FWD) kr[k] = a.kt[(long long)pk * a.ktps + (j < a.np0 ? j : 0)];
     }
     // Out-of-range lanes write the dummy slot, so every load is used and
     // the compiler cannot sink it into a branch (which would serialise it).
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(NT) void pass_strided(PassArgs a)
     for (int k = 0; k < EPT; ++k) {
         const int j = threadIdx.x + k * NT;
         tr[k] = a.tw[j < a.ntw ? j : 0];
-        kr[k] = a.kt[j < a.L ? j : 0];
+        kr[k] = a.kt[(long long)pk * a.ktps + (j < a.L ? j : 0)];
     }
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
@@ -316,6 +317,7 @@ PassArgs base_args(const SpoPassPlan &p)
     a.maxsamp = p.maxsamp;
     a.spread = p.shape.spread;
     a.xmap = 0;
+    a.ktps = p.kax_pstride;
     return a;
 }
 
@@ -709,6 +711,9 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
                    const void *vph, double *cbuf, int maxsamp, int tune, hipStream_t s)
 {
+    // taxes holds ndim kinetic axes for one shared set of masses, or
+    // ndim per packet (packet-major) when the masses differ per packet.
+    const int nset = (int)taxes.size() / ndim;
     p.vph = vph;
     p.cst2 = cst2;
     p.ndim = ndim;
@@ -735,6 +740,7 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
     // Tables: twiddles per axis and per-axis kinetic phases (cos, sin) of
     // cst1 * T_a(k_a) stored at the digit-reversed position of k_a.
     std::vector<double> tw, kax;
+    int pstride = 0;
     for (int a = 0; a < ndim; ++a) {
         p.tw_off[a] = (int)(tw.size() / 2);
         p.ntw[a] = (int)fp[a].twc.size();
@@ -742,16 +748,21 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
             tw.push_back(fp[a].twc[i]);
             tw.push_back(fp[a].tws[i]);
         }
-        p.kax_off[a] = (int)(kax.size() / 2);
-        size_t k0 = kax.size();
-        kax.resize(k0 + 2 * (size_t)np[a]);
-        for (int k = 0; k < np[a]; ++k) {
-            double th = cst1 * taxes[a][k];
-            int pos = fp[a].perm[k];
-            kax[k0 + 2 * pos] = std::cos(th);
-            kax[k0 + 2 * pos + 1] = std::sin(th);
-        }
+        p.kax_off[a] = pstride;
+        pstride += np[a];
     }
+    p.kax_pstride = nset > 1 ? pstride : 0;
+    kax.resize(2 * (size_t)pstride * nset);
+    for (int q = 0; q < nset; ++q)
+        for (int a = 0; a < ndim; ++a) {
+            const size_t k0 = 2 * ((size_t)q * pstride + p.kax_off[a]);
+            for (int k = 0; k < np[a]; ++k) {
+                const double th = cst1 * taxes[(size_t)q * ndim + a][k];
+                const int pos = fp[a].perm[k];
+                kax[k0 + 2 * pos] = std::cos(th);
+                kax[k0 + 2 * pos + 1] = std::sin(th);
+            }
+        }
     if (!rows_ok(p, p.shape)) {
         bool found = false;
         for (int r : {1, 2, 4, 8, 16})

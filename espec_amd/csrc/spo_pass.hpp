@@ -29,7 +29,8 @@ struct SpoPassPlan {
     void *tw = nullptr;    // double2 twiddles, axis a at tw_off[a], ntw[a] entries
     int tw_off[3] = {}, ntw[3] = {};
     void *kax = nullptr;   // double2 per-axis kinetic phases in digit-reversed order
-    int kax_off[3] = {};
+    int kax_off[3] = {};   // offset of axis a within one packet's set
+    int kax_pstride = 0;   // entries between packets' sets (0: one shared set)
     double *part = nullptr;      // [maxsamp][npk][max xblocks][3] row-pass partial sums
     long long *dstep = nullptr;  // steps completed, advanced on the device
     double b = 1, cst2 = 0;

@@ -12,6 +12,7 @@ using namespace espec;
 
 struct espec_spo_plan {
     int ndim = 1, np[3] = {1, 1, 1}, n = 0, npk = 0, lc = 1, maxsamp = 0, flags = 0;
+    bool per_packet_mass = false;
     int nt = 64;
     bool mixed = false, ref = false, lds = true;
     double dt = 0, b = 1;
@@ -75,16 +76,26 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
             ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(double) * vph.size()));
         }
         std::vector<std::vector<double>> taxes;
-        for (int a = 0; a < P.ndim; ++a) taxes.push_back(kinetic_axis(P.np[a], sh[a], sm[a], false));
+        const int nset = P.per_packet_mass ? npk : 1;
+        for (int q = 0; q < nset; ++q)
+            for (int a = 0; a < P.ndim; ++a)
+                taxes.push_back(kinetic_axis(P.np[a], sh[a], sm[(size_t)q * P.ndim + a], false));
         return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, cst2, P.x.p,
                               P.psi0.p, P.vst.p, table ? P.vph.p : nullptr, P.cbuf.as<double>(),
                               P.maxsamp, 1, P.stream);
     }
     // Kinetic phase: natural order for the reference kernel; for the fused
     // kernel, position of frequency k is sum_a perm_a[k_a] * stride_a.
-    std::vector<double> tc = kinetic_grid(P.ndim, P.np, sh, sm, P.ref);
-    std::vector<double> tcs = P.ref ? tc : scramble(P.ndim, P.np, fp, tc);
-    std::vector<double> kin = phase_table<double>(tcs.data(), n, cst1);
+    const int nset = P.per_packet_mass ? npk : 1;
+    std::vector<double> kin;
+    for (int q = 0; q < nset; ++q) {
+        std::vector<double> tc =
+            kinetic_grid(P.ndim, P.np, sh, sm + (size_t)q * P.ndim, P.ref);
+        std::vector<double> tcs = P.ref ? tc : scramble(P.ndim, P.np, fp, tc);
+        std::vector<double> k = phase_table<double>(tcs.data(), n, cst1);
+        kin.insert(kin.end(), k.begin(), k.end());
+    }
+    P.la.kin_pp = nset > 1 ? 1 : 0;
     std::vector<double> vph((size_t)2 * n * npk);
     for (int p = 0; p < npk; ++p) {
         std::vector<double> t = phase_table<double>(vpot + (size_t)p * n, n, cst2);
@@ -155,6 +166,7 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
     P->flags = flags;
     P->mixed = (flags & ESPEC_MIXED) != 0;
     P->ref = (flags & ESPEC_REF) != 0;
+    P->per_packet_mass = (flags & ESPEC_PER_PACKET_MASS) != 0;
     if (P->ref && P->mixed) return ESPEC_ERR_UNSUPPORTED;
     P->b = 1.0 / P->n;
     P->nt = P->n <= 1024 ? 64 : 256;

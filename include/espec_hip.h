@@ -38,6 +38,9 @@ extern "C" {
 #define ESPEC_MIXED 0x1
 #define ESPEC_EXACT 0x0
 #define ESPEC_REF   0x2
+/* Plans only: sm is [npk][ndim], one set of masses per packet (the
+ * kinetic phases become per-packet tables). */
+#define ESPEC_PER_PACKET_MASS 0x4
 
 enum {
     ESPEC_OK = 0,
