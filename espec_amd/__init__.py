@@ -62,6 +62,7 @@ def _load():
     L.espec_spo_plan_reset.argtypes = [vp]
     L.espec_spo_plan_profile.argtypes = [vp, ip, d, d, ip, P(ip)]
     L.espec_spo_plan_shape.argtypes = [vp, i]
+    L.espec_spo_plan_set_shape.argtypes = [vp, i]
     L.espec_bw_probe.argtypes = [C.c_size_t, ip, P(dp), P(dp)]
     L.espec_plnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, d, vp, ip, P(ip), ip]
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
@@ -385,6 +386,10 @@ class SodPlan(_Plan):
         _check(rc, "sod_plan_create")
 
 
+_SHAPE_KEYS = ("lds", "rows", "rows_nt", "tile", "tile_nt", "spread", "xmap", "grid", "rows_pen",
+               "tile_pen", "pen_pf")
+
+
 class SpoPlan(_Plan):
     """Device-resident batch of split-operator packets on one grid (no
     reference counterpart: replaces the job farm of tools&scrips/parallel.x)."""
@@ -411,11 +416,19 @@ class SpoPlan(_Plan):
         return ms.value, nl.value
 
     def shape(self):
-        """Launch shape: dict(lds, rows, rows_nt, tile, tile_nt, spread, xmap)."""
-        a = np.zeros(7, dtype=np.int32)
+        """Launch shape: dict(lds, rows, rows_nt, tile, tile_nt, spread, xmap, grid,
+        rows_pen, tile_pen, pen_pf)."""
+        a = np.zeros(len(_SHAPE_KEYS), dtype=np.int32)
         _check(self._L.espec_spo_plan_shape(self._h, a), "spo_plan_shape")
-        return dict(zip(("lds", "rows", "rows_nt", "tile", "tile_nt", "spread", "xmap"),
-                        a.tolist()))
+        return dict(zip(_SHAPE_KEYS, a.tolist()))
+
+    def set_shape(self, **kw):
+        """Pin the multi-pass launch shape: the current shape with the given
+        fields replaced (keys as returned by shape())."""
+        sh = self.shape()
+        sh.update(kw)
+        a = np.array([sh[k] for k in _SHAPE_KEYS], dtype=np.int32)
+        _check(self._L.espec_spo_plan_set_shape(self._h, a), "spo_plan_set_shape")
 
     def reset(self):
         """Restart from psi0 at step 0 (tables and captured graphs kept)."""

@@ -36,6 +36,14 @@ struct Bfly {
     int base, step, j, e0;
 };
 
+// LDS position of element i.  With PAD, one slot is skipped after every 16
+// elements: butterflies whose inputs are 16 or 64 elements apart (the late
+// stages, where lanes step from block to block) then fall in different
+// banks instead of 4-way conflicts on the same 256-byte bank row.
+template <bool PAD> __device__ __forceinline__ int lx(int i) { return PAD ? i + (i >> 4) : i; }
+// LDS slots to reserve for n padded elements.
+constexpr int lds_slots(int n, bool pad) { return pad ? n + (n >> 4) + 1 : n; }
+
 // Threads walk butterflies of one line first when the line is contiguous
 // (es == 1) and across lines first otherwise, so that neighbouring lanes
 // touch neighbouring LDS words in both cases.
@@ -59,7 +67,7 @@ __device__ __forceinline__ Bfly bfly_at(const DevAxis &ax, int s, int g, int lnb
 }
 
 // One forward DIF stage: butterfly, then twiddle W^{q j} on output q.
-template <typename R, int NT, int SIGN>
+template <typename R, int NT, int SIGN, bool PAD = false>
 __device__ void dif_stage(V2<R> *x, const DevAxis &ax, int s, const V2<R> *__restrict__ tw)
 {
     using V = V2<R>;
@@ -70,24 +78,24 @@ __device__ void dif_stage(V2<R> *x, const DevAxis &ax, int s, const V2<R> *__res
     for (int g = threadIdx.x; g < total; g += NT) {
         Bfly b = bfly_at(ax, s, g, lnb);
         if (r == 4) {
-            V a0 = x[b.base], a1 = x[b.base + b.step], a2 = x[b.base + 2 * b.step],
-              a3 = x[b.base + 3 * b.step];
+            V a0 = x[lx<PAD>(b.base)], a1 = x[lx<PAD>(b.base + b.step)], a2 = x[lx<PAD>(b.base + 2 * b.step)],
+              a3 = x[lx<PAD>(b.base + 3 * b.step)];
             bfly4<SIGN>(a0, a1, a2, a3);
             if (b.j) {
                 a1 = ctw<SIGN>(a1, t[b.j]);
                 a2 = ctw<SIGN>(a2, t[m + b.j]);
                 a3 = ctw<SIGN>(a3, t[2 * m + b.j]);
             }
-            x[b.base] = a0;
-            x[b.base + b.step] = a1;
-            x[b.base + 2 * b.step] = a2;
-            x[b.base + 3 * b.step] = a3;
+            x[lx<PAD>(b.base)] = a0;
+            x[lx<PAD>(b.base + b.step)] = a1;
+            x[lx<PAD>(b.base + 2 * b.step)] = a2;
+            x[lx<PAD>(b.base + 3 * b.step)] = a3;
         } else {
-            V a0 = x[b.base], a1 = x[b.base + b.step];
+            V a0 = x[lx<PAD>(b.base)], a1 = x[lx<PAD>(b.base + b.step)];
             bfly2(a0, a1);
             if (b.j) a1 = ctw<SIGN>(a1, t[b.j]);
-            x[b.base] = a0;
-            x[b.base + b.step] = a1;
+            x[lx<PAD>(b.base)] = a0;
+            x[lx<PAD>(b.base + b.step)] = a1;
         }
     }
     __syncthreads();
@@ -95,7 +103,7 @@ __device__ void dif_stage(V2<R> *x, const DevAxis &ax, int s, const V2<R> *__res
 
 // Inverse of a DIF stage of the opposite sign: twiddle first, then the
 // radix-r DFT with sign SIGN (unnormalised).
-template <typename R, int NT, int SIGN>
+template <typename R, int NT, int SIGN, bool PAD = false>
 __device__ void dit_stage(V2<R> *x, const DevAxis &ax, int s, const V2<R> *__restrict__ tw)
 {
     using V = V2<R>;
@@ -106,24 +114,24 @@ __device__ void dit_stage(V2<R> *x, const DevAxis &ax, int s, const V2<R> *__res
     for (int g = threadIdx.x; g < total; g += NT) {
         Bfly b = bfly_at(ax, s, g, lnb);
         if (r == 4) {
-            V a0 = x[b.base], a1 = x[b.base + b.step], a2 = x[b.base + 2 * b.step],
-              a3 = x[b.base + 3 * b.step];
+            V a0 = x[lx<PAD>(b.base)], a1 = x[lx<PAD>(b.base + b.step)], a2 = x[lx<PAD>(b.base + 2 * b.step)],
+              a3 = x[lx<PAD>(b.base + 3 * b.step)];
             if (b.j) {
                 a1 = ctw<SIGN>(a1, t[b.j]);
                 a2 = ctw<SIGN>(a2, t[m + b.j]);
                 a3 = ctw<SIGN>(a3, t[2 * m + b.j]);
             }
             bfly4<SIGN>(a0, a1, a2, a3);
-            x[b.base] = a0;
-            x[b.base + b.step] = a1;
-            x[b.base + 2 * b.step] = a2;
-            x[b.base + 3 * b.step] = a3;
+            x[lx<PAD>(b.base)] = a0;
+            x[lx<PAD>(b.base + b.step)] = a1;
+            x[lx<PAD>(b.base + 2 * b.step)] = a2;
+            x[lx<PAD>(b.base + 3 * b.step)] = a3;
         } else {
-            V a0 = x[b.base], a1 = x[b.base + b.step];
+            V a0 = x[lx<PAD>(b.base)], a1 = x[lx<PAD>(b.base + b.step)];
             if (b.j) a1 = ctw<SIGN>(a1, t[b.j]);
             bfly2(a0, a1);
-            x[b.base] = a0;
-            x[b.base + b.step] = a1;
+            x[lx<PAD>(b.base)] = a0;
+            x[lx<PAD>(b.base + b.step)] = a1;
         }
     }
     __syncthreads();
@@ -133,7 +141,7 @@ __device__ void dit_stage(V2<R> *x, const DevAxis &ax, int s, const V2<R> *__res
 // optionally the inverse stage (SIGN -1) that completes the previous
 // step, then op(index, value) on each natural-order element, then
 // optionally the forward stage (SIGN +1) that starts the next step.
-template <typename R, int NT, bool INV, bool FWD, class Op>
+template <typename R, int NT, bool INV, bool FWD, bool PAD = false, class Op>
 __device__ void fused_outer(V2<R> *x, const DevAxis &ax, const V2<R> *__restrict__ tw, Op op)
 {
     using V = V2<R>;
@@ -145,7 +153,7 @@ __device__ void fused_outer(V2<R> *x, const DevAxis &ax, const V2<R> *__restrict
         Bfly b = bfly_at(ax, s, g, lnb);
         if (r == 4) {
             int i0 = b.base, i1 = i0 + b.step, i2 = i1 + b.step, i3 = i2 + b.step;
-            V a0 = x[i0], a1 = x[i1], a2 = x[i2], a3 = x[i3];
+            V a0 = x[lx<PAD>(i0)], a1 = x[lx<PAD>(i1)], a2 = x[lx<PAD>(i2)], a3 = x[lx<PAD>(i3)];
             V w1, w2, w3;
             if (b.j) {
                 w1 = t[b.j];
@@ -172,13 +180,13 @@ __device__ void fused_outer(V2<R> *x, const DevAxis &ax, const V2<R> *__restrict
                     a3 = ctw<1>(a3, w3);
                 }
             }
-            x[i0] = a0;
-            x[i1] = a1;
-            x[i2] = a2;
-            x[i3] = a3;
+            x[lx<PAD>(i0)] = a0;
+            x[lx<PAD>(i1)] = a1;
+            x[lx<PAD>(i2)] = a2;
+            x[lx<PAD>(i3)] = a3;
         } else {
             int i0 = b.base, i1 = i0 + b.step;
-            V a0 = x[i0], a1 = x[i1];
+            V a0 = x[lx<PAD>(i0)], a1 = x[lx<PAD>(i1)];
             V w1;
             if (b.j) w1 = t[b.j];
             if (INV) {
@@ -191,8 +199,8 @@ __device__ void fused_outer(V2<R> *x, const DevAxis &ax, const V2<R> *__restrict
                 bfly2(a0, a1);
                 if (b.j) a1 = ctw<1>(a1, w1);
             }
-            x[i0] = a0;
-            x[i1] = a1;
+            x[lx<PAD>(i0)] = a0;
+            x[lx<PAD>(i1)] = a1;
         }
     }
     __syncthreads();
@@ -200,7 +208,7 @@ __device__ void fused_outer(V2<R> *x, const DevAxis &ax, const V2<R> *__restrict
 
 // Last stage (m = 1, no twiddles) of the last axis fused with the
 // momentum-space multiply and the first inverse stage.
-template <typename R, int NT, class Op>
+template <typename R, int NT, bool PAD = false, class Op>
 __device__ void fused_inner(V2<R> *x, const DevAxis &ax, Op op)
 {
     using V = V2<R>;
@@ -211,26 +219,26 @@ __device__ void fused_inner(V2<R> *x, const DevAxis &ax, Op op)
         Bfly b = bfly_at(ax, s, g, lnb);
         if (r == 4) {
             int i0 = b.base, i1 = i0 + b.step, i2 = i1 + b.step, i3 = i2 + b.step;
-            V a0 = x[i0], a1 = x[i1], a2 = x[i2], a3 = x[i3];
+            V a0 = x[lx<PAD>(i0)], a1 = x[lx<PAD>(i1)], a2 = x[lx<PAD>(i2)], a3 = x[lx<PAD>(i3)];
             bfly4<1>(a0, a1, a2, a3);
             op(i0, a0);
             op(i1, a1);
             op(i2, a2);
             op(i3, a3);
             bfly4<-1>(a0, a1, a2, a3);
-            x[i0] = a0;
-            x[i1] = a1;
-            x[i2] = a2;
-            x[i3] = a3;
+            x[lx<PAD>(i0)] = a0;
+            x[lx<PAD>(i1)] = a1;
+            x[lx<PAD>(i2)] = a2;
+            x[lx<PAD>(i3)] = a3;
         } else {
             int i0 = b.base, i1 = i0 + b.step;
-            V a0 = x[i0], a1 = x[i1];
+            V a0 = x[lx<PAD>(i0)], a1 = x[lx<PAD>(i1)];
             bfly2(a0, a1);
             op(i0, a0);
             op(i1, a1);
             bfly2(a0, a1);
-            x[i0] = a0;
-            x[i1] = a1;
+            x[lx<PAD>(i0)] = a0;
+            x[lx<PAD>(i1)] = a1;
         }
     }
     __syncthreads();
@@ -238,7 +246,7 @@ __device__ void fused_inner(V2<R> *x, const DevAxis &ax, Op op)
 
 // Gather along an axis: y[line + k es] = x[line + perm[k] es] (undoes the
 // digit reversal of a full DIF pass on that axis).
-template <typename R, int NT>
+template <typename R, int NT, bool PAD = false>
 __device__ void permute_axis(const V2<R> *x, V2<R> *y, const DevAxis &ax, const int *__restrict__ perm)
 {
     const int total = ax.nl << ax.ln;
@@ -252,7 +260,7 @@ __device__ void permute_axis(const V2<R> *x, V2<R> *y, const DevAxis &ax, const 
             k = g >> ax.lnl;
         }
         int lb = line_base(ax, l);
-        y[lb + (k << ax.les)] = x[lb + (perm[k] << ax.les)];
+        y[lx<PAD>(lb + (k << ax.les))] = x[lx<PAD>(lb + (perm[k] << ax.les))];
     }
     __syncthreads();
 }

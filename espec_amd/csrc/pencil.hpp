@@ -1,0 +1,223 @@
+// Register-resident line FFTs ("pencils") of length N = 16 M, M = 4, 8 or
+// 16: M lanes of one wavefront hold a line, 16 elements per lane, and a
+// transform is two rounds of in-register DFTs with one LDS exchange between
+// them (one wave-local round trip instead of one per radix-4 stage).
+//
+// Index split (s = +1 forward, -1 inverse; W = exp(2 pi i / N)):
+//   n = M n1 + n2,  k = k1 + 16 k2,   n1, k1 in [0,16),  n2, k2 in [0,M)
+//   X[k1 + 16 k2] = sum_n2 e^{s 2pi i n2 k2 / M} W^{s n2 k1}
+//                         sum_n1 x[M n1 + n2] e^{s 2pi i n1 k1 / 16}
+// Natural layout: lane j holds x[M r + j] in register r.
+// Spectral layout: lane j holds, in register r = q M + k2 (q in [0,16/M)),
+// X[k] with k = (16/M) j + q + 16 k2; it is stored at line position
+// p = r M + j, so that every register is a coalesced row of M lanes in
+// both layouts.  spectral_index() gives k for a position p.
+#pragma once
+#include "common.hpp"
+#include "fft_lds.hpp"
+
+namespace espec {
+namespace pencil {
+
+// Frequency index held at spectral-layout position p of a line of 16 M.
+inline int spectral_index(int p, int m)
+{
+    const int r = p / m, j = p % m, q = r / m, k2 = r % m;
+    return j * (16 / m) + q + 16 * k2;
+}
+
+// a * exp(SIGN i pi e / 8) for compile-time e.
+template <int SIGN, int E>
+__device__ __forceinline__ double2 rot16(double2 a)
+{
+    constexpr int e = ((SIGN > 0 ? E : -E) % 16 + 16) % 16;
+    constexpr double c8 = 0.92387953251128675613, s8 = 0.38268343236508977173,
+                     r2 = 0.70710678118654752440;
+    constexpr double cs[16] = {1, c8, r2, s8, 0, -s8, -r2, -c8, -1, -c8, -r2, -s8, 0, s8, r2, c8};
+    constexpr double sn[16] = {0, s8, r2, c8, 1, c8, r2, s8, 0, -s8, -r2, -c8, -1, -c8, -r2, -s8};
+    if constexpr (e == 0) return a;
+    else if constexpr (e == 4) return {-a.y, a.x};
+    else if constexpr (e == 8) return {-a.x, -a.y};
+    else if constexpr (e == 12) return {a.y, -a.x};
+    else return {a.x * cs[e] - a.y * sn[e], a.x * sn[e] + a.y * cs[e]};
+}
+
+// In-register DFT of length L in {4, 8, 16} on v[O], v[O+1], ..., natural
+// order in and out, sign SIGN, unnormalised.
+template <int SIGN, int O>
+__device__ __forceinline__ void dft4(double2 *v)
+{
+    bfly4<SIGN>(v[O], v[O + 1], v[O + 2], v[O + 3]);
+}
+
+template <int SIGN, int O>
+__device__ __forceinline__ void dft8(double2 *v)
+{
+    // n = 2 n1 + n2: DFT4 over n1, twiddle e^{s 2pi i n2 k1 / 8}, DFT2 over n2.
+    bfly4<SIGN>(v[O + 0], v[O + 2], v[O + 4], v[O + 6]);
+    bfly4<SIGN>(v[O + 1], v[O + 3], v[O + 5], v[O + 7]);
+    v[O + 3] = rot16<SIGN, 2>(v[O + 3]);
+    v[O + 5] = rot16<SIGN, 4>(v[O + 5]);
+    v[O + 7] = rot16<SIGN, 6>(v[O + 7]);
+    bfly2(v[O + 0], v[O + 1]);
+    bfly2(v[O + 2], v[O + 3]);
+    bfly2(v[O + 4], v[O + 5]);
+    bfly2(v[O + 6], v[O + 7]);
+    // position 2 k1 + k2 holds X[k1 + 4 k2]
+    double2 t[8];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 2; ++k2) t[k1 + 4 * k2] = v[O + 2 * k1 + k2];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[O + i] = t[i];
+}
+
+template <int SIGN, int O>
+__device__ __forceinline__ void dft16(double2 *v)
+{
+    // n = 4 n1 + n2: DFT4 over n1, twiddle e^{s 2pi i n2 k1 / 16}, DFT4 over n2.
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) bfly4<SIGN>(v[O + n2], v[O + 4 + n2], v[O + 8 + n2], v[O + 12 + n2]);
+    v[O + 5] = rot16<SIGN, 1>(v[O + 5]);
+    v[O + 6] = rot16<SIGN, 2>(v[O + 6]);
+    v[O + 7] = rot16<SIGN, 3>(v[O + 7]);
+    v[O + 9] = rot16<SIGN, 2>(v[O + 9]);
+    v[O + 10] = rot16<SIGN, 4>(v[O + 10]);
+    v[O + 11] = rot16<SIGN, 6>(v[O + 11]);
+    v[O + 13] = rot16<SIGN, 3>(v[O + 13]);
+    v[O + 14] = rot16<SIGN, 6>(v[O + 14]);
+    v[O + 15] = rot16<SIGN, 9>(v[O + 15]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) bfly4<SIGN>(v[O + 4 * k1], v[O + 4 * k1 + 1], v[O + 4 * k1 + 2], v[O + 4 * k1 + 3]);
+    // position 4 k1 + k2 holds X[k1 + 4 k2]
+    double2 t[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = v[O + 4 * k1 + k2];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[O + i] = t[i];
+}
+
+template <int L, int SIGN, int O>
+__device__ __forceinline__ void dft(double2 *v)
+{
+    static_assert(L == 4 || L == 8 || L == 16, "pencil DFT length");
+    if constexpr (L == 4) dft4<SIGN, O>(v);
+    else if constexpr (L == 8) dft8<SIGN, O>(v);
+    else dft16<SIGN, O>(v);
+}
+
+// v[O + e] *= W^{SIGN k e} for e in [0, CNT), W = exp(2 pi i / N), from two
+// table entries (W^k, W^{4k}; 4k < N) and at most three products each.
+template <int SIGN, int CNT, int O>
+__device__ __forceinline__ void twiddle_powers(double2 *v, const double2 *__restrict__ tw, int k)
+{
+    const double2 w1 = tw[k];
+    const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+    double2 w4 = {1, 0}, w8 = {1, 0}, w12 = {1, 0};
+    if constexpr (CNT > 4) {
+        w4 = tw[4 * k];
+        w8 = cmul(w4, w4);
+        w12 = cmul(w8, w4);
+    }
+#pragma unroll
+    for (int e = 1; e < CNT; ++e) {
+        const double2 lo = (e & 3) == 1 ? w1 : (e & 3) == 2 ? w2 : w3;
+        const double2 hi = (e >> 2) == 1 ? w4 : (e >> 2) == 2 ? w8 : w12;
+        const double2 w = (e & 3) == 0 ? hi : (e >> 2) == 0 ? lo : cmul(hi, lo);
+        v[O + e] = ctw<SIGN>(v[O + e], w);
+    }
+}
+
+// sin and cos of th for |th| < kSincosMax: Cody-Waite reduction by pi/2
+// in three FMA steps (pi/2 split as P1 + P2 + P3, each rounded to double),
+// then the Taylor series of sin to r^15 and of cos to r^16 on |r| <= pi/4
+// (truncation below 5e-17), and the quadrant.  Cheaper in registers than a
+// general sincos, whose large-argument reduction the callers never need.
+constexpr double kSincosMax = 1.0e6;
+__device__ __forceinline__ void sincos_reduced(double th, double *sn, double *cs)
+{
+    constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17,
+                     P3 = -1.4973849048591698e-33, TWO_OVER_PI = 0.6366197723675814;
+    const double k = rint(th * TWO_OVER_PI);
+    double r = fma(-k, P1, th);
+    r = fma(-k, P2, r);
+    r = fma(-k, P3, r);
+    const double r2 = r * r;
+    // 1/n! for the odd (sin) and even (cos) terms
+    double ps = 1.0 / 1307674368000.0;                       // 1/15!
+    ps = fma(ps, r2, -1.0 / 6227020800.0);                   // 1/13!
+    ps = fma(ps, r2, 1.0 / 39916800.0);                      // 1/11!
+    ps = fma(ps, r2, -1.0 / 362880.0);                       // 1/9!
+    ps = fma(ps, r2, 1.0 / 5040.0);                          // 1/7!
+    ps = fma(ps, r2, -1.0 / 120.0);                          // 1/5!
+    ps = fma(ps, r2, 1.0 / 6.0);                             // 1/3!
+    const double s = fma(-ps * r2, r, r);                    // r - r^3/3! + ...
+    double pc = 1.0 / 20922789888000.0;                      // 1/16!
+    pc = fma(pc, r2, -1.0 / 87178291200.0);                  // 1/14!
+    pc = fma(pc, r2, 1.0 / 479001600.0);                     // 1/12!
+    pc = fma(pc, r2, -1.0 / 3628800.0);                      // 1/10!
+    pc = fma(pc, r2, 1.0 / 40320.0);                         // 1/8!
+    pc = fma(pc, r2, -1.0 / 720.0);                          // 1/6!
+    pc = fma(pc, r2, 1.0 / 24.0);                            // 1/4!
+    pc = fma(pc, r2, -0.5);                                  // 1/2!
+    const double c = fma(pc, r2, 1.0);
+    const int q = (int)(long long)k & 3;
+    *sn = q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+    *cs = q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+}
+
+// LDS slot of exchange element (k1, n2) of a line: row k1 of M entries,
+// one pad slot per 16 so that column reads (stride M) hit distinct banks.
+template <int M> __device__ __forceinline__ int xslot(int k1, int n2)
+{
+    const int i = k1 * M + n2;
+    return i + (i >> 4);
+}
+// Exchange slots per line.
+template <int M> constexpr int xslots() { return 17 * M; }
+
+// Natural -> spectral, sign SIGN.  xl: this line's exchange slots.  The
+// caller orders LDS between consecutive exchanges of the same slots.
+template <int M, int SIGN>
+__device__ __forceinline__ void forward(double2 (&v)[16], int j, double2 *xl, const double2 *__restrict__ tw)
+{
+    constexpr int Q = 16 / M;
+    dft16<SIGN, 0>(v);                         // v[k1] = sum_n1 x[M n1 + j] ...
+    twiddle_powers<SIGN, 16, 0>(v, tw, j);     // * W^{s j k1}
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) xl[xslot<M>(k1, j)] = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int n2 = 0; n2 < M; ++n2) v[q * M + n2] = xl[xslot<M>(j * Q + q, n2)];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) dft<M, SIGN, 0>(v + q * M);
+}
+
+// Spectral -> natural, sign SIGN (exact inverse of forward<M, -SIGN> up to
+// the factor N).
+template <int M, int SIGN>
+__device__ __forceinline__ void inverse(double2 (&v)[16], int j, double2 *xl, const double2 *__restrict__ tw)
+{
+    constexpr int Q = 16 / M;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        dft<M, SIGN, 0>(v + q * M);
+        twiddle_powers<SIGN, M, 0>(v + q * M, tw, j * Q + q);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int n2 = 0; n2 < M; ++n2) xl[xslot<M>(j * Q + q, n2)] = v[q * M + n2];
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = xl[xslot<M>(k1, j)];
+    dft16<SIGN, 0>(v);
+}
+
+}  // namespace pencil
+}  // namespace espec

@@ -32,8 +32,8 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
     using VS = V2<S>;
     extern __shared__ double smem_d[];
     // LDS: psi | V phases (vph_lds) | kinetic phases, twiddles (tab_lds) | reduction
-    V *x = reinterpret_cast<V *>(smem_d);
-    V *vl = x + a.n;
+    V *x = reinterpret_cast<V *>(smem_d);  // padded (lx<true>)
+    V *vl = x + lds_slots(a.n, true);
     V *kl = vl + (a.vph_lds ? a.n : 0);
     V *twl = kl + (a.tab_lds ? a.n : 0);
     double *red = reinterpret_cast<double *>(twl + (a.tab_lds ? a.ntw : 0));
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
 
     for (int i = threadIdx.x; i < n; i += NT) {
         VS t = gx[i];
-        x[i] = {(double)t.x, (double)t.y};
+        x[lx<true>(i)] = {(double)t.x, (double)t.y};
         if (a.vph_lds) vl[i] = gvg[i];
         if (a.tab_lds) kl[i] = gkg[i];
     }
@@ -62,20 +62,20 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
     const V *gk = a.tab_lds ? kl : gkg;
     const V *tw = a.tab_lds ? twl : twg;
 
-    fused_outer<double, NT, false, true>(x, ax0, tw, [&](int i, V &v) { v = cmulc(v, gv[i]); });
+    fused_outer<double, NT, false, true, true>(x, ax0, tw, [&](int i, V &v) { v = cmulc(v, gv[i]); });
     for (int st = 0; st < a.nsteps; ++st) {
         for (int d = 0; d < a.ndim; ++d) {
             const DevAxis &ax = a.ax[d];
             const int s0 = d == 0 ? 1 : 0;
             const int s1 = d == a.ndim - 1 ? ax.nst - 1 : ax.nst;
-            for (int s = s0; s < s1; ++s) dif_stage<double, NT, 1>(x, ax, s, tw);
+            for (int s = s0; s < s1; ++s) dif_stage<double, NT, 1, true>(x, ax, s, tw);
         }
-        fused_inner<double, NT>(x, axl, [&](int i, V &v) { v = cmulc(v, gk[i]); });
+        fused_inner<double, NT, true>(x, axl, [&](int i, V &v) { v = cmulc(v, gk[i]); });
         for (int d = a.ndim - 1; d >= 0; --d) {
             const DevAxis &ax = a.ax[d];
             const int s0 = d == 0 ? 1 : 0;
             const int s1 = d == a.ndim - 1 ? ax.nst - 1 : ax.nst;
-            for (int s = s1 - 1; s >= s0; --s) dit_stage<double, NT, -1>(x, ax, s, tw);
+            for (int s = s1 - 1; s >= s0; --s) dit_stage<double, NT, -1, true>(x, ax, s, tw);
         }
         const long long g = a.step0 + st + 1;
         const bool samp = (g % a.lc) == 0;
@@ -94,12 +94,12 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
             return w;
         };
         if (st + 1 < a.nsteps)
-            fused_outer<double, NT, true, true>(x, ax0, tw, [&](int i, V &v) {
+            fused_outer<double, NT, true, true, true>(x, ax0, tw, [&](int i, V &v) {
                 V w = post(i, v);
                 v = cmulc(v, w);
             });
         else
-            fused_outer<double, NT, true, false>(x, ax0, tw, [&](int i, V &v) { post(i, v); });
+            fused_outer<double, NT, true, false, true>(x, ax0, tw, [&](int i, V &v) { post(i, v); });
         if (samp) {
             block_sum3<NT>(cr, ci, nr, red);
             const long long nc = g / a.lc;
@@ -111,7 +111,10 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
             }
         }
     }
-    for (int i = threadIdx.x; i < n; i += NT) gx[i] = {(S)x[i].x, (S)x[i].y};
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const V t = x[lx<true>(i)];
+        gx[i] = {(S)t.x, (S)t.y};
+    }
 }
 
 template <int NT>
@@ -219,9 +222,11 @@ __global__ __launch_bounds__(NT) void spo_sample(const V2<R> *x, const V2<R> *ps
 
 size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt, bool tab_lds, int ntw)
 {
-    size_t v = (ref || vph_lds) ? 2 : 1;
-    if (!ref && tab_lds) v += 1;
-    return sizeof(double2) * (n * v + (!ref && tab_lds ? ntw : 0)) + sizeof(double) * 3 * (nt / 64);
+    // fast kernel: padded state (lds_slots), V phases, kinetic phases + twiddles
+    if (ref) return sizeof(double2) * 2 * (size_t)n + sizeof(double) * 3 * (nt / 64);
+    size_t v = lds_slots(n, true) + (vph_lds ? (size_t)n : 0);
+    if (tab_lds) v += (size_t)n + ntw;
+    return sizeof(double2) * v + sizeof(double) * 3 * (nt / 64);
 }
 
 template <typename K>

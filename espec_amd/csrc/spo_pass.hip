@@ -23,6 +23,7 @@
 
 #include "api_common.hpp"
 #include "fft_lds.hpp"
+#include "pencil.hpp"
 #include "spo_pass.hpp"
 
 namespace espec {
@@ -47,6 +48,8 @@ struct PassArgs {
     int spread;            // only workgroups with blockIdx.x % spread == 0 work (XCD placement)
     int ktps;              // kinetic table stride between packets (0: shared)
     int xmap;              // strided pass: give each XCD a contiguous range of tiles
+    const double2 *twr;    // pencil kernels: exp(2 pi i m / L), m in [0, L)
+    const double2 *kreg;   // pencil kernels: kinetic phases in spectral-layout order
 };
 
 template <typename S>
@@ -61,6 +64,59 @@ __device__ __forceinline__ double2 widen(V2<S> v)
 // PHT: read the FP64 V phase table (16 B/point) instead of V in storage
 // precision plus an FP64 sincos per point; the table wins when the pass is
 // latency-bound (one packet), V+sincos when it is HBM-bound (large batches).
+//
+// Work items (row blocks over all packets) are taken in a grid-stride loop:
+// with fewer workgroups than items, each workgroup issues the global loads
+// of its next item into registers before it transforms the current one
+// in LDS, so HBM latency overlaps the FFT.
+// Copies a table of cnt double2 entries to LDS (slot cnt takes the
+// clamped lanes).  Clamped indices and unconditional stores keep the
+// staging in VGPRs and every load used, so no load is sunk into a branch.
+template <int NT, int EPT>
+__device__ __forceinline__ void stage_table(double2 *dst, const double2 *src, int cnt)
+{
+    double tr[EPT], ti[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int j = threadIdx.x + k * NT;
+        const double2 v = src[j < cnt ? j : 0];
+        tr[k] = v.x;
+        ti[k] = v.y;
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int j = threadIdx.x + k * NT;
+        dst[j < cnt ? j : cnt] = {tr[k], ti[k]};
+    }
+}
+
+// Register staging of one row block; loop-carried arrays are scalar
+// (arrays of double2 carried across the loop are not promoted to VGPRs).
+template <typename S, int NT, int EPT, bool PHT, bool INV>
+__device__ __forceinline__ void rows_fetch(const PassArgs &a, int w, V2<S> (&st)[EPT],
+                                           S (&vv)[EPT], double (&wc)[EPT], double (&ws)[EPT],
+                                           V2<S> (&q)[EPT])
+{
+    const int pk = w / a.xblocks, blk = w - pk * a.xblocks;
+    const long long po = (long long)pk * a.n + (long long)blk * a.cnt;
+    const V2<S> *gx = reinterpret_cast<const V2<S> *>(a.x) + po;
+    const S *gv = reinterpret_cast<const S *>(a.vpot) + po;
+    const V2<S> *g0 = reinterpret_cast<const V2<S> *>(a.psi0) + po;
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int i = threadIdx.x + k * NT;
+        st[k] = gx[i];
+        if (PHT) {
+            const double2 t = a.vph[po + i];
+            wc[k] = t.x;
+            ws[k] = t.y;
+        } else {
+            vv[k] = gv[i];
+        }
+        if (INV) q[k] = g0[i];
+    }
+}
+
 template <typename S, int NT, bool INV, bool FWD, int EPT, bool PHT>
 __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
 {
@@ -69,7 +125,7 @@ __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
     extern __shared__ double smem_d[];
     const int cnt = a.cnt;
     V *x = reinterpret_cast<V *>(smem_d);
-    V *ph = x + cnt;      // exp(+i cst2 V) as (cos, sin)
+    V *ph = x + lds_slots(cnt, true);  // exp(+i cst2 V) as (cos, sin)
     V *p0 = ph + cnt;     // psi0
     V *twl = p0 + cnt;        // twiddles (+1 dummy slot)
     V *kl = twl + a.ntw + 1;  // kinetic phases (+1 dummy slot)
@@ -78,100 +134,110 @@ __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
     // only every spread-th one active confines a small grid to 8/spread
     // XCDs so that its passes hand the state over inside one L2.
     if (a.spread > 1 && blockIdx.x % a.spread) return;
-    const int pk = blockIdx.y, blk = blockIdx.x / a.spread;
-    const long long po = (long long)pk * a.n + (long long)blk * cnt;
-    VS *gx = reinterpret_cast<VS *>(a.x) + po;
-    const S *__restrict__ gv = reinterpret_cast<const S *>(a.vpot) + po;
-    const VS *__restrict__ g0 = reinterpret_cast<const VS *>(a.psi0) + po;
+    const int nact = gridDim.x / a.spread, total = a.xblocks * a.npk;
+    int w = blockIdx.x / a.spread;
+    if (w >= total) return;
     const double b = a.b;
-
-    // Prologue: one round of global loads.
     const long long g = INV ? *a.dstep : 0;  // the step this pass completes
+    const bool samp = INV && (g % a.lc) == 0;
+
+    stage_table<NT, EPT>(twl, a.tw, a.ntw);  // twiddles once per workgroup
     VS st[EPT];
     S vv[EPT];
-    V wv[EPT];
+    double wc[EPT], ws[EPT];
     VS q[EPT];
-    V tr[EPT], kr[EPT];
+    rows_fetch<S, NT, EPT, PHT, INV>(a, w, st, vv, wc, ws, q);
+    int kpk = -1;  // packet whose kinetic phases are in LDS
+    for (;;) {
+        const int pk = w / a.xblocks, blk = w - pk * a.xblocks;
+        const long long po = (long long)pk * a.n + (long long)blk * cnt;
+        VS *gx = reinterpret_cast<VS *>(a.x) + po;
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int i = threadIdx.x + k * NT;
-        st[k] = gx[i];
-        if (PHT)
-            wv[k] = a.vph[po + i];
-        else
-            vv[k] = gv[i];
-        if (INV) q[k] = g0[i];
-    }
-    // Clamped indices and unconditional loads keep the staging arrays in
-    // VGPRs (conditionally written arrays are demoted to scratch).
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int j = threadIdx.x + k * NT;
-        tr[k] = a.tw[j < a.ntw ? j : 0];
-        if (FWD) kr[k] = a.kt[(long long)pk * a.ktps + (j < a.np0 ? j : 0)];
-    }
-    // Out-of-range lanes write the dummy slot, so every load is used and
-    // the compiler cannot sink it into a branch (which would serialise it).
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int j = threadIdx.x + k * NT;
-        twl[j < a.ntw ? j : a.ntw] = tr[k];
-        if (FWD) kl[j < a.np0 ? j : a.np0] = kr[k];
-    }
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int i = threadIdx.x + k * NT;
-        x[i] = widen<S>(st[k]);
-        if (PHT) {
-            ph[i] = wv[k];
-        } else {
-            double sn, cs;
-            sincos(a.cst2 * (double)vv[k], &sn, &cs);
-            ph[i] = {cs, sn};
+        for (int k = 0; k < EPT; ++k) {
+            const int i = threadIdx.x + k * NT;
+            x[lx<true>(i)] = widen<S>(st[k]);
+            if (PHT) {
+                ph[i] = {wc[k], ws[k]};
+            } else {
+                double sn, cs;
+                sincos(a.cst2 * (double)vv[k], &sn, &cs);
+                ph[i] = {cs, sn};
+            }
+            if (INV) p0[i] = widen<S>(q[k]);
         }
-        if (INV) p0[i] = widen<S>(q[k]);
-    }
-    const bool samp = INV && (g % a.lc) == 0;
-    __syncthreads();
+        if (FWD && pk != kpk) {  // kinetic phases of a new packet
+            stage_table<NT, EPT>(kl, a.kt + (long long)pk * a.ktps, a.np0);
+            kpk = pk;
+        }
+        __syncthreads();
+        const int wn = w + nact;
+        if (wn < total) rows_fetch<S, NT, EPT, PHT, INV>(a, wn, st, vv, wc, ws, q);
 
-    if (INV)
-        for (int s = a.ax.nst - 1; s >= 1; --s) dit_stage<double, NT, -1>(x, a.ax, s, twl);
-    double cr = 0, ci = 0, nr = 0;
-    fused_outer<double, NT, INV, FWD>(x, a.ax, twl, [&](int i, V &v) {
-        const V w = ph[i];
-        if (INV) {
-            v = cmulc(v, w);
-            v.x *= b;
-            v.y *= b;
-            if (samp) {
-                const V p = p0[i];
-                cr += p.x * v.x + p.y * v.y;
-                ci += p.x * v.y - p.y * v.x;
-                nr += v.x * v.x + v.y * v.y;
+        if (INV)
+            for (int s = a.ax.nst - 1; s >= 1; --s) dit_stage<double, NT, -1, true>(x, a.ax, s, twl);
+        double cr = 0, ci = 0, nr = 0;
+        fused_outer<double, NT, INV, FWD, true>(x, a.ax, twl, [&](int i, V &v) {
+            const V wph = ph[i];
+            if (INV) {
+                v = cmulc(v, wph);
+                v.x *= b;
+                v.y *= b;
+                if (samp) {
+                    const V p = p0[i];
+                    cr += p.x * v.x + p.y * v.y;
+                    ci += p.x * v.y - p.y * v.x;
+                    nr += v.x * v.x + v.y * v.y;
+                }
+            }
+            if (FWD) v = cmulc(v, wph);
+        });
+        if (FWD) {
+            for (int s = 1; s < a.ax.nst; ++s) dif_stage<double, NT, 1, true>(x, a.ax, s, twl);
+            for (int i = threadIdx.x; i < cnt; i += NT) {
+                const int li = lx<true>(i);
+                x[li] = cmulc(x[li], kl[i & (a.np0 - 1)]);
+            }
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < cnt; i += NT) {
+            const V t = x[lx<true>(i)];
+            gx[i] = {(S)t.x, (S)t.y};
+        }
+        if (samp) {
+            block_sum3<NT>(cr, ci, nr, red);
+            const long long nc = g / a.lc;
+            if (threadIdx.x == 0 && nc < a.maxsamp) {
+                double *c = a.part + (((long long)nc * a.npk + pk) * a.xblocks + blk) * 3;
+                c[0] = cr;
+                c[1] = ci;
+                c[2] = nr;
             }
         }
-        if (FWD) v = cmulc(v, w);
-    });
-    if (FWD) {
-        for (int s = 1; s < a.ax.nst; ++s) dif_stage<double, NT, 1>(x, a.ax, s, twl);
-        for (int i = threadIdx.x; i < cnt; i += NT) x[i] = cmulc(x[i], kl[i & (a.np0 - 1)]);
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < cnt; i += NT) gx[i] = {(S)x[i].x, (S)x[i].y};
-    if (samp) {
-        block_sum3<NT>(cr, ci, nr, red);
-        const long long nc = g / a.lc;
-        if (threadIdx.x == 0 && nc < a.maxsamp) {
-            double *c = a.part + (((long long)nc * a.npk + pk) * a.xblocks + blk) * 3;
-            c[0] = cr;
-            c[1] = ci;
-            c[2] = nr;
-        }
+        if (wn >= total) break;
+        w = wn;
+        __syncthreads();  // LDS is rewritten by the next item
     }
 }
 
 // Forward FFT, kinetic phase and inverse FFT along an axis of stride S and
-// length L, on a tile of 2^ltile contiguous elements.
+// length L, on tiles of 2^ltile contiguous elements (grid-stride over the
+// tiles of all packets, next tile prefetched as in pass_rows).
+template <typename S, int NT, int EPT>
+__device__ __forceinline__ long long strided_base(const PassArgs &a, int w, int ntl, int *pk_out)
+{
+    const int pk = w / ntl;
+    int t = w - pk * ntl;
+    // Neighbouring tiles share cache lines (a tile is 2^ltile contiguous
+    // elements); dealt round robin they land on different XCDs, whose L2s
+    // then each fetch and write back the same lines.  With xmap, XCD
+    // (t mod 8) takes the contiguous tile range [x n/8, (x+1) n/8).
+    if (a.xmap) t = (t & 7) * (ntl >> 3) + (t >> 3);
+    const int spt = a.S >> a.ltile;  // tiles across the lower axes
+    *pk_out = pk;
+    return (long long)pk * a.n + (long long)(t / spt) * a.S * a.L +
+           (long long)(t % spt) * (1 << a.ltile);
+}
+
 template <typename S, int NT, int EPT>
 __global__ __launch_bounds__(NT) void pass_strided(PassArgs a)
 {
@@ -180,52 +246,240 @@ __global__ __launch_bounds__(NT) void pass_strided(PassArgs a)
     extern __shared__ double smem_d[];
     const int cnt = a.cnt;
     V *x = reinterpret_cast<V *>(smem_d);
-    V *twl = x + cnt;         // twiddles (+1 dummy slot)
+    V *twl = x + lds_slots(cnt, true);  // twiddles (+1 dummy slot)
     V *kl = twl + a.ntw + 1;  // kinetic phases (+1 dummy slot)
     const int tile = 1 << a.ltile;
     if (a.spread > 1 && blockIdx.x % a.spread) return;
-    const int pk = blockIdx.y;
-    int t = blockIdx.x / a.spread;
-    // Neighbouring tiles share cache lines (a tile is 2^ltile contiguous
-    // elements); dealt round robin they land on different XCDs, whose L2s
-    // then each fetch and write back the same lines.  With xmap, XCD
-    // (t mod 8) takes the contiguous tile range [x n/8, (x+1) n/8).
-    if (a.xmap) {
-        const int ntl = gridDim.x / a.spread;
-        t = (t & 7) * (ntl >> 3) + (t >> 3);
-    }
-    const int spt = a.S >> a.ltile;  // tiles across the lower axes
-    const long long base =
-        (long long)pk * a.n + (long long)(t / spt) * a.S * a.L + (long long)(t % spt) * tile;
-    VS *gx = reinterpret_cast<VS *>(a.x) + base;
-
+    const int ntl = a.n / (tile * a.L);  // tiles per packet
+    const int nact = gridDim.x / a.spread, total = ntl * a.npk;
+    int w = blockIdx.x / a.spread;
+    if (w >= total) return;
+    stage_table<NT, EPT>(twl, a.tw, a.ntw);
     VS st[EPT];
-    V tr[EPT], kr[EPT];
+    int pk;
+    long long base = strided_base<S, NT, EPT>(a, w, ntl, &pk);
+    auto fetch = [&](long long bs) {
+        const VS *gx = reinterpret_cast<const VS *>(a.x) + bs;
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int i = threadIdx.x + k * NT;
-        st[k] = gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))];
-    }
+        for (int k = 0; k < EPT; ++k) {
+            const int i = threadIdx.x + k * NT;
+            st[k] = gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))];
+        }
+    };
+    fetch(base);
+    int kpk = -1;
+    for (;;) {
+        VS *gx = reinterpret_cast<VS *>(a.x) + base;
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int j = threadIdx.x + k * NT;
-        tr[k] = a.tw[j < a.ntw ? j : 0];
-        kr[k] = a.kt[(long long)pk * a.ktps + (j < a.L ? j : 0)];
+        for (int k = 0; k < EPT; ++k) {
+            const int j = threadIdx.x + k * NT;
+            x[lx<true>(j)] = widen<S>(st[k]);
+        }
+        if (pk != kpk) {  // kinetic phases of a new packet
+            stage_table<NT, EPT>(kl, a.kt + (long long)pk * a.ktps, a.L);
+            kpk = pk;
+        }
+        __syncthreads();
+        const int wn = w + nact;
+        int pkn = pk;
+        long long basen = base;
+        if (wn < total) {
+            basen = strided_base<S, NT, EPT>(a, wn, ntl, &pkn);
+            fetch(basen);
+        }
+        for (int s = 0; s < a.ax.nst - 1; ++s) dif_stage<double, NT, 1, true>(x, a.ax, s, twl);
+        fused_inner<double, NT, true>(x, a.ax, [&](int i, V &v) { v = cmulc(v, kl[i >> a.ltile]); });
+        for (int s = a.ax.nst - 2; s >= 0; --s) dit_stage<double, NT, -1, true>(x, a.ax, s, twl);
+        for (int i = threadIdx.x; i < cnt; i += NT) {
+            const V t = x[lx<true>(i)];
+            gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))] = {(S)t.x, (S)t.y};
+        }
+        if (a.last && w == 0 && threadIdx.x == 0) *a.dstep += 1;
+        if (wn >= total) break;
+        w = wn;
+        pk = pkn;
+        base = basen;
+        __syncthreads();
     }
+}
+
+// ------------------------------------------------------------ pencils
+// The same two passes on register pencils (pencil.hpp) for axis lengths
+// L = 16 M, M in {4, 8, 16}.  A workgroup is one wavefront holding C = 64/M
+// lines; the state between passes keeps x in the spectral layout of the
+// pencil (a permutation of k that makes every register a coalesced row).
+// Work items are taken in a grid-stride loop; with single-precision
+// storage the next item's state and V are loaded while the current one is
+// transformed.
+
+// Loads of one item: 16 elements per lane at po + r * stride.
+template <typename S, bool V>
+__device__ __forceinline__ void pen_fetch(const PassArgs &a, long long po, long long stride,
+                                          S (&sr)[16], S (&si)[16], S (&sv)[16])
+{
+    const V2<S> *gx = reinterpret_cast<const V2<S> *>(a.x) + po;
+    const S *gv = reinterpret_cast<const S *>(a.vpot) + po;
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int j = threadIdx.x + k * NT;
-        x[j] = widen<S>(st[k]);
-        twl[j < a.ntw ? j : a.ntw] = tr[k];
-        kl[j < a.L ? j : a.L] = kr[k];
+    for (int r = 0; r < 16; ++r) {
+        const V2<S> t = gx[r * stride];
+        sr[r] = t.x;
+        si[r] = t.y;
+        if (V) sv[r] = gv[r * stride];
     }
-    __syncthreads();
-    for (int s = 0; s < a.ax.nst - 1; ++s) dif_stage<double, NT, 1>(x, a.ax, s, twl);
-    fused_inner<double, NT>(x, a.ax, [&](int i, V &v) { v = cmulc(v, kl[i >> a.ltile]); });
-    for (int s = a.ax.nst - 2; s >= 0; --s) dit_stage<double, NT, -1>(x, a.ax, s, twl);
-    for (int i = threadIdx.x; i < cnt; i += NT)
-        gx[(long long)(i >> a.ltile) * a.S + (i & (tile - 1))] = {(S)x[i].x, (S)x[i].y};
-    if (a.last && t == 0 && pk == 0 && threadIdx.x == 0) *a.dstep += 1;
+}
+
+// PF: load the next item's state and V while transforming the current one.
+// Two waves per SIMD are requested: the exchange slots (17 KiB per wave)
+// cap residency near that anyway.
+template <typename S, int M, bool INV, bool FWD, bool PHT, bool PF>
+__global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
+{
+    constexpr int C = 64 / M, L = 16 * M;
+    __shared__ double2 xs[C * pencil::xslots<M>()];
+    const int j = threadIdx.x % M, l = threadIdx.x / M;
+    double2 *xl = xs + l * pencil::xslots<M>();
+    const int total = a.xblocks * a.npk;
+    int w = blockIdx.x;
+    if (w >= total) return;
+    const long long g = INV ? *a.dstep : 0;
+    const bool samp = INV && (g % a.lc) == 0;
+    const double b = a.b;
+    auto origin = [&](int wi, int &pk, int &blk) {
+        pk = wi / a.xblocks;
+        blk = wi - pk * a.xblocks;
+        return (long long)pk * a.n + (long long)(blk * C + l) * L + j;
+    };
+    S sr[16], si[16], sv[16];
+    int pk, blk;
+    long long po = origin(w, pk, blk);
+    if (PF) pen_fetch<S, !PHT>(a, po, M, sr, si, sv);
+    for (;;) {
+        if (!PF) pen_fetch<S, !PHT>(a, po, M, sr, si, sv);
+        double2 v[16];
+        S cv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            v[r] = {(double)sr[r], (double)si[r]};
+            cv[r] = sv[r];
+        }
+        const int wn = w + gridDim.x;
+        int pkn = pk, blkn = blk;
+        const long long pon = wn < total ? origin(wn, pkn, blkn) : po;
+        if (PF && wn < total) pen_fetch<S, !PHT>(a, pon, M, sr, si, sv);
+        // psi0 for the sample, in flight during the inverse transform
+        S q0r[16], q0i[16];
+        if (samp) {
+            const V2<S> *g0 = reinterpret_cast<const V2<S> *>(a.psi0) + po;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const V2<S> t = g0[r * M];
+                q0r[r] = t.x;
+                q0i[r] = t.y;
+            }
+        }
+        if (INV) pencil::inverse<M, -1>(v, j, xl, a.twr);
+        double cr = 0, ci = 0, nr = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            double2 ph;
+            if (PHT) {
+                ph = a.vph[po + r * M];
+            } else {
+                double sn, cs;
+                pencil::sincos_reduced(a.cst2 * (double)cv[r], &sn, &cs);
+                ph = {cs, sn};
+            }
+            if (INV) {
+                v[r] = cmulc(v[r], ph);
+                v[r].x *= b;
+                v[r].y *= b;
+                if (samp) {
+                    const double px = q0r[r], py = q0i[r];
+                    cr += px * v[r].x + py * v[r].y;
+                    ci += px * v[r].y - py * v[r].x;
+                    nr += v[r].x * v[r].x + v[r].y * v[r].y;
+                }
+            }
+            if (FWD) v[r] = cmulc(v[r], ph);
+        }
+        if (FWD) {
+            pencil::forward<M, 1>(v, j, xl, a.twr);
+            const double2 *kt = a.kreg + (long long)pk * a.ktps + j;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = cmulc(v[r], kt[r * M]);
+        }
+        V2<S> *gx = reinterpret_cast<V2<S> *>(a.x) + po;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) gx[r * M] = {(S)v[r].x, (S)v[r].y};
+        if (samp) {
+            cr = wave_sum(cr);
+            ci = wave_sum(ci);
+            nr = wave_sum(nr);
+            const long long nc = g / a.lc;
+            if (threadIdx.x == 0 && nc < a.maxsamp) {
+                double *c = a.part + (((long long)nc * a.npk + pk) * a.xblocks + blk) * 3;
+                c[0] = cr;
+                c[1] = ci;
+                c[2] = nr;
+            }
+        }
+        if (wn >= total) break;
+        w = wn;
+        pk = pkn;
+        blk = blkn;
+        po = pon;
+        __syncthreads();  // exchange slots are rewritten by the next item
+    }
+}
+
+// Strided pass on pencils: lane = c + C j, line c of the item's C
+// consecutive columns, lane j within the line.
+template <typename S, int M, bool PF>
+__global__ __launch_bounds__(64, 2) void pass_strided_pen(PassArgs a)
+{
+    constexpr int C = 64 / M;
+    __shared__ double2 xs[C * pencil::xslots<M>()];
+    const int c = threadIdx.x % C, j = threadIdx.x / C;
+    double2 *xl = xs + c * pencil::xslots<M>();
+    const int groups = a.n / (a.L * C);  // column groups per packet
+    const int total = groups * a.npk;
+    int w = blockIdx.x;
+    if (w >= total) return;
+    const long long stride = (long long)M * a.S;
+    auto origin = [&](int wi, int &pk) {
+        pk = wi / groups;
+        const int col = (wi - pk * groups) * C + c;
+        return (long long)pk * a.n + (col % a.S) + (long long)(col / a.S) * a.S * a.L +
+               (long long)j * a.S;
+    };
+    S sr[16], si[16], sv[16];
+    int pk;
+    long long po = origin(w, pk);
+    if (PF) pen_fetch<S, false>(a, po, stride, sr, si, sv);
+    for (;;) {
+        if (!PF) pen_fetch<S, false>(a, po, stride, sr, si, sv);
+        double2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = {(double)sr[r], (double)si[r]};
+        const int wn = w + gridDim.x;
+        int pkn = pk;
+        const long long pon = wn < total ? origin(wn, pkn) : po;
+        if (PF && wn < total) pen_fetch<S, false>(a, pon, stride, sr, si, sv);
+        pencil::forward<M, 1>(v, j, xl, a.twr);
+        const double2 *kt = a.kreg + (long long)pk * a.ktps + j;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = cmulc(v[r], kt[r * M]);
+        pencil::inverse<M, -1>(v, j, xl, a.twr);
+        V2<S> *gx = reinterpret_cast<V2<S> *>(a.x) + po;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) gx[r * stride] = {(S)v[r].x, (S)v[r].y};
+        if (a.last && w == 0 && threadIdx.x == 0) *a.dstep += 1;
+        if (wn >= total) break;
+        w = wn;
+        pk = pkn;
+        po = pon;
+        __syncthreads();
+    }
 }
 
 __global__ void set_step(long long *d, long long v) { *d = v; }
@@ -324,13 +578,14 @@ PassArgs base_args(const SpoPassPlan &p)
 size_t rows_lds(const SpoPassPlan &p, const PassShape &sh)
 {
     const int cnt = sh.rows * p.np[0];
-    return sizeof(double2) * (3 * (size_t)cnt + p.ntw[0] + p.np[0] + 2) +
+    return sizeof(double2) * (lds_slots(cnt, true) + 2 * (size_t)cnt + p.ntw[0] + p.np[0] + 2) +
            sizeof(double) * 3 * (sh.rows_nt / 64);
 }
 
 size_t strided_lds(const SpoPassPlan &p, int axis, const PassShape &sh)
 {
-    return sizeof(double2) * ((size_t)sh.tile * p.np[axis] + p.ntw[axis] + p.np[axis] + 2);
+    return sizeof(double2) *
+           (lds_slots(sh.tile * p.np[axis], true) + p.ntw[axis] + p.np[axis] + 2);
 }
 
 template <typename R>
@@ -345,9 +600,31 @@ hipError_t rows(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, double 
     a.tw = static_cast<const double2 *>(p.tw) + p.tw_off[0];
     a.ntw = p.ntw[0];
     a.kt = static_cast<const double2 *>(p.kax) + p.kax_off[0];
-    const size_t lds = rows_lds(p, p.shape);
     const bool pht = p.vph != nullptr;
-    dim3 grid(p.xblocks * p.shape.spread, p.npk);
+    const int total = p.xblocks * p.npk;
+    const int act = p.shape.grid > 0 ? std::min(total, p.shape.grid) : total;
+    if (p.shape.rows_pen) {
+        a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[0];
+        a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[0];
+        const dim3 pg(act);
+        const bool pf = p.shape.pen_pf != 0;
+#define ESPEC_PEN_ROWS2(M_, P_)                                                                    \
+    if (p.np[0] == 16 * M_ && pht == P_) {                                                         \
+        if (inv && fwd && pf) return launch(pass_rows_pen<R, M_, true, true, P_, true>, pg, 64, 0, s, a); \
+        if (inv && fwd) return launch(pass_rows_pen<R, M_, true, true, P_, false>, pg, 64, 0, s, a); \
+        if (inv) return launch(pass_rows_pen<R, M_, true, false, P_, false>, pg, 64, 0, s, a);      \
+        return launch(pass_rows_pen<R, M_, false, true, P_, false>, pg, 64, 0, s, a);               \
+    }
+#define ESPEC_PEN_ROWS(M_) ESPEC_PEN_ROWS2(M_, true) ESPEC_PEN_ROWS2(M_, false)
+        ESPEC_PEN_ROWS(4)
+        ESPEC_PEN_ROWS(8)
+        ESPEC_PEN_ROWS(16)
+#undef ESPEC_PEN_ROWS
+#undef ESPEC_PEN_ROWS2
+        return hipErrorInvalidValue;
+    }
+    const size_t lds = rows_lds(p, p.shape);
+    dim3 grid(act * p.shape.spread);
 #define ESPEC_ROWS2(NT_, E_, P_)                                                                    \
     if (nt == NT_ && a.ept == E_ && pht == P_) {                                                    \
         if (inv && fwd) return launch(pass_rows<R, NT_, true, true, E_, P_>, grid, NT_, lds, s, a); \
@@ -386,10 +663,24 @@ hipError_t strided(const SpoPassPlan &p, int axis, bool last, hipStream_t s)
     const int nt = p.shape.tile_nt;
     a.ept = a.cnt / nt;
     a.last = last ? 1 : 0;
+    if (p.shape.tile_pen) {
+        a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[axis];
+        a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[axis];
+        const int total = p.n / (a.L * (64 / (a.L / 16))) * p.npk;
+        const int act = p.shape.grid > 0 ? std::min(total, p.shape.grid) : total;
+        const dim3 pg(act);
+        const bool pf = p.shape.pen_pf != 0;
+        if (a.L == 64) return launch(pf ? pass_strided_pen<R, 4, true> : pass_strided_pen<R, 4, false>, pg, 64, 0, s, a);
+        if (a.L == 128) return launch(pf ? pass_strided_pen<R, 8, true> : pass_strided_pen<R, 8, false>, pg, 64, 0, s, a);
+        if (a.L == 256) return launch(pf ? pass_strided_pen<R, 16, true> : pass_strided_pen<R, 16, false>, pg, 64, 0, s, a);
+        return hipErrorInvalidValue;
+    }
     const int ntiles = p.n / (p.shape.tile * a.L);
     a.xmap = (p.shape.xmap && p.shape.spread == 1 && ntiles % 8 == 0) ? 1 : 0;
     const size_t lds = strided_lds(p, axis, p.shape);
-    dim3 grid(p.n / (p.shape.tile * a.L) * p.shape.spread, p.npk);
+    const int total = ntiles * p.npk;
+    const int act = p.shape.grid > 0 ? std::min(total, p.shape.grid) : total;
+    dim3 grid(act * p.shape.spread);
 #define ESPEC_STRIDED(NT_, E_) \
     if (nt == NT_ && a.ept == E_) return launch(pass_strided<R, NT_, E_>, grid, NT_, lds, s, a);
     ESPEC_STRIDED(64, 1)
@@ -488,8 +779,14 @@ bool nt_ok(int nt, int cnt)
     return cnt % nt == 0 && (e == 1 || e == 2 || e == 4 || e == 8 || e == 16);
 }
 
+bool pen_len(int l) { return l == 64 || l == 128 || l == 256; }
+
 bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 {
+    if (s.rows_pen)
+        return pen_len(p.np[0]) && s.rows == 1024 / p.np[0] && s.rows_nt == 64 &&
+               (p.n / p.np[0]) % s.rows == 0 &&
+               (p.vph != nullptr || p.phase_max < pencil::kSincosMax);
     const int cnt = s.rows * p.np[0];
     if (!nt_ok(s.rows_nt, cnt)) return false;
     if (p.ntw[0] > kMaxEpt * s.rows_nt || p.np[0] > kMaxEpt * s.rows_nt) return false;
@@ -499,6 +796,14 @@ bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 
 bool tile_ok(const SpoPassPlan &p, const PassShape &s)
 {
+    if (s.tile_pen) {
+        for (int a = 1; a < p.ndim; ++a) {
+            int stride = 1;
+            for (int k = 0; k < a; ++k) stride *= p.np[k];
+            if (!pen_len(p.np[a]) || stride % (1024 / p.np[a])) return false;
+        }
+        return true;
+    }
     for (int a = 1; a < p.ndim; ++a) {
         int stride = 1;
         for (int k = 0; k < a; ++k) stride *= p.np[k];
@@ -507,7 +812,8 @@ bool tile_ok(const SpoPassPlan &p, const PassShape &s)
         if (p.ntw[a] > kMaxEpt * s.tile_nt || p.np[a] > kMaxEpt * s.tile_nt) return false;
         if (s.tile > stride || cnt % s.tile_nt || cnt / s.tile_nt < 1 || cnt / s.tile_nt > kMaxEpt)
             return false;
-        if (sizeof(double2) * ((size_t)cnt + p.ntw[a] + p.np[a] + 2) > 160 * 1024) return false;
+        if (sizeof(double2) * (lds_slots(cnt, true) + p.ntw[a] + p.np[a] + 2) > 160 * 1024)
+            return false;
     }
     return true;
 }
@@ -565,11 +871,14 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
     const int reps = 16;
     PassShape best = p.shape;
     float best_ms = 1e30f;
-    const int rc_[][2] = {{1, 64}, {2, 64}, {4, 64}, {1, 256}, {2, 256}, {4, 256}, {8, 256}};
+    // {rows, threads, pencil}
+    const int rc_[][3] = {{1, 64, 0},  {2, 64, 0},  {4, 64, 0},  {1, 256, 0},
+                          {2, 256, 0}, {4, 256, 0}, {8, 256, 0}, {1024 / p.np[0], 64, 1}};
     for (auto &c : rc_) {
         PassShape sh = p.shape;
         sh.rows = c[0];
         sh.rows_nt = c[1];
+        sh.rows_pen = c[2];
         if (!rows_ok(p, sh)) continue;
         p.shape = sh;
         apply_shape(p);
@@ -582,18 +891,22 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
             best_ms = ms;
             best.rows = sh.rows;
             best.rows_nt = sh.rows_nt;
+            best.rows_pen = sh.rows_pen;
         }
     }
     p.shape.rows = best.rows;
     p.shape.rows_nt = best.rows_nt;
+    p.shape.rows_pen = best.rows_pen;
     if (p.ndim > 1) {
         best_ms = 1e30f;
-        const int tc_[][2] = {{1, 64}, {2, 64}, {4, 64}, {1, 256}, {2, 256}, {4, 256},
-                              {8, 256}, {16, 256}};
+        // {tile, threads, pencil}
+        const int tc_[][3] = {{1, 64, 0},  {2, 64, 0},  {4, 64, 0},  {1, 256, 0},  {2, 256, 0},
+                              {4, 256, 0}, {8, 256, 0}, {16, 256, 0}, {p.shape.tile, p.shape.tile_nt, 1}};
         for (auto &c : tc_) {
             PassShape sh = p.shape;
             sh.tile = c[0];
             sh.tile_nt = c[1];
+            sh.tile_pen = c[2];
             if (!tile_ok(p, sh)) continue;
             p.shape = sh;
             apply_shape(p);
@@ -605,10 +918,12 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
                 best_ms = ms;
                 best.tile = sh.tile;
                 best.tile_nt = sh.tile_nt;
+                best.tile_pen = sh.tile_pen;
             }
         }
         p.shape.tile = best.tile;
         p.shape.tile_nt = best.tile_nt;
+        p.shape.tile_pen = best.tile_pen;
     }
     // XCD placement: time whole steps, since what it changes is where the
     // state lives between one pass and the next.
@@ -635,6 +950,41 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
             }
         p.shape.spread = best_spread;
         p.shape.xmap = best_xmap;
+    }
+    // Workgroups per pass: one per work item, or a grid-stride loop over
+    // fewer, each prefetching its next item (pays off at large batches,
+    // where short workgroups leave HBM latency exposed).  Multiples of 8
+    // keep a workgroup on one XCD for all of its items.
+    {
+        int items = p.xblocks * p.npk;
+        for (int a = 1; a < p.ndim; ++a) {
+            const int per = p.shape.tile_pen ? 1024 / p.np[a] : p.shape.tile;
+            items = std::max(items, p.n / (per * p.np[a]) * p.npk);
+        }
+        best_ms = 1e30f;
+        int best_grid = 0;
+        int best_pf = 0;
+        const bool pen = p.shape.rows_pen || p.shape.tile_pen;
+        for (int pf : {0, 1})
+        for (int gs : {0, 4096, 2048, 1024, 512}) {
+            if (gs && (items <= gs || p.shape.spread > 1)) continue;
+            if (pf && !pen) continue;
+            p.shape.grid = gs;
+            p.shape.pen_pf = pf;
+            apply_shape(p);
+            ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
+            float ms = 0;
+            int dummy = 0;
+            int rc = time_launches(s, reps, [&]() { return one_step<R>(p, true, true, s, &dummy); }, &ms);
+            if (rc) return rc;
+            if (clearly_faster(ms, best_ms)) {
+                best_ms = ms;
+                best_grid = gs;
+                best_pf = pf;
+            }
+        }
+        p.shape.grid = best_grid;
+        p.shape.pen_pf = best_pf;
     }
     apply_shape(p);
     ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
@@ -709,8 +1059,10 @@ int profile_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, do
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   const void *vph, double *cbuf, int maxsamp, int tune, hipStream_t s)
+                   const void *vph, double *cbuf, int maxsamp, double phase_max, int tune,
+                   hipStream_t s)
 {
+    p.phase_max = phase_max;
     // taxes holds ndim kinetic axes for one shared set of masses, or
     // ndim per packet (packet-major) when the masses differ per packet.
     const int nset = (int)taxes.size() / ndim;
@@ -753,6 +1105,16 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
     }
     p.kax_pstride = nset > 1 ? pstride : 0;
     kax.resize(2 * (size_t)pstride * nset);
+    // Pencil tables: exact twiddles exp(2 pi i m / L), and the kinetic
+    // phases at the spectral-layout position of each frequency.
+    std::vector<double> twr, kreg(kax.size());
+    for (int a = 0; a < ndim; ++a) {
+        p.twr_off[a] = (int)(twr.size() / 2);
+        for (int m = 0; m < np[a]; ++m) {
+            twr.push_back(std::cos(kTwoPi * m / np[a]));
+            twr.push_back(std::sin(kTwoPi * m / np[a]));
+        }
+    }
     for (int q = 0; q < nset; ++q)
         for (int a = 0; a < ndim; ++a) {
             const size_t k0 = 2 * ((size_t)q * pstride + p.kax_off[a]);
@@ -762,6 +1124,13 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                 kax[k0 + 2 * pos] = std::cos(th);
                 kax[k0 + 2 * pos + 1] = std::sin(th);
             }
+            if (pen_len(np[a]))
+                for (int pos = 0; pos < np[a]; ++pos) {
+                    const double th =
+                        cst1 * taxes[(size_t)q * ndim + a][pencil::spectral_index(pos, np[a] / 16)];
+                    kreg[k0 + 2 * pos] = std::cos(th);
+                    kreg[k0 + 2 * pos + 1] = std::sin(th);
+                }
         }
     if (!rows_ok(p, p.shape)) {
         bool found = false;
@@ -793,9 +1162,11 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
 shaped:
     apply_shape(p);
     const int max_xblocks = p.n / p.np[0];  // rows = 1
-    DevBuf btw, bk, bpart, bstep;
+    DevBuf btw, bk, btwr, bkr, bpart, bstep;
     ESPEC_HIP_TRY(btw.upload(tw.data(), sizeof(double) * tw.size()));
     ESPEC_HIP_TRY(bk.upload(kax.data(), sizeof(double) * kax.size()));
+    ESPEC_HIP_TRY(btwr.upload(twr.data(), sizeof(double) * twr.size()));
+    ESPEC_HIP_TRY(bkr.upload(kreg.data(), sizeof(double) * kreg.size()));
     ESPEC_HIP_TRY(bpart.alloc(sizeof(double) * 3 * (size_t)maxsamp * npk * max_xblocks));
     ESPEC_HIP_TRY(hipMemset(bpart.p, 0, bpart.bytes));
     ESPEC_HIP_TRY(bstep.alloc(sizeof(long long)));
@@ -803,6 +1174,10 @@ shaped:
     btw.p = nullptr;
     p.kax = bk.p;
     bk.p = nullptr;
+    p.twr = btwr.p;
+    btwr.p = nullptr;
+    p.kreg = bkr.p;
+    bkr.p = nullptr;
     p.part = static_cast<double *>(bpart.p);
     bpart.p = nullptr;
     p.dstep = static_cast<long long *>(bstep.p);
@@ -834,11 +1209,25 @@ int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_
                    : run_typed<double>(p, nsteps, step0, s, launches);
 }
 
+int spo_pass_set_shape(SpoPassPlan &p, const PassShape &s)
+{
+    if (!rows_ok(p, s) || (p.ndim > 1 && !tile_ok(p, s))) return ESPEC_ERR_ARG;
+    if (s.spread < 1 || s.spread > 8 || (s.spread & (s.spread - 1)) || s.grid < 0 ||
+        (s.grid % 8) != 0)
+        return ESPEC_ERR_ARG;
+    drop_graph(p);  // the captured launches carry the old shape
+    p.shape = s;
+    apply_shape(p);
+    return ESPEC_OK;
+}
+
 void spo_pass_release(SpoPassPlan &p)
 {
     drop_graph(p);
     if (p.tw) (void)hipFree(p.tw);
     if (p.kax) (void)hipFree(p.kax);
+    if (p.twr) (void)hipFree(p.twr);
+    if (p.kreg) (void)hipFree(p.kreg);
     if (p.part) (void)hipFree(p.part);
     if (p.dstep) (void)hipFree(p.dstep);
     p = SpoPassPlan{};

@@ -13,6 +13,10 @@ struct PassShape {
     int tile = 16, tile_nt = 256;  // contiguous elements per strided tile, its threads
     int spread = 1;                // active workgroups at linear ids 0, spread, 2 spread, ...
     int xmap = 1;                  // contiguous strided tiles per XCD
+    int grid = 0;                  // workgroups per pass (0: one per work item)
+    int rows_pen = 0;              // row pass on register pencils (rows = 64 / (np0 / 16))
+    int tile_pen = 0;              // strided passes on register pencils
+    int pen_pf = 0;                // pencil passes prefetch their next work item
 };
 
 struct SpoPassPlan {
@@ -29,11 +33,15 @@ struct SpoPassPlan {
     void *tw = nullptr;    // double2 twiddles, axis a at tw_off[a], ntw[a] entries
     int tw_off[3] = {}, ntw[3] = {};
     void *kax = nullptr;   // double2 per-axis kinetic phases in digit-reversed order
+    void *twr = nullptr;   // double2 exp(2 pi i m / np[a]), axis a at twr_off[a] (pencils)
+    int twr_off[3] = {};
+    void *kreg = nullptr;  // as kax, in the pencils' spectral-layout order
     int kax_off[3] = {};   // offset of axis a within one packet's set
     int kax_pstride = 0;   // entries between packets' sets (0: one shared set)
     double *part = nullptr;      // [maxsamp][npk][max xblocks][3] row-pass partial sums
     long long *dstep = nullptr;  // steps completed, advanced on the device
     double b = 1, cst2 = 0;
+    double phase_max = 0;  // max |cst2 V| over the batch (range of the pencils' sincos)
     int lc = 1;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -43,11 +51,14 @@ struct SpoPassPlan {
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   const void *vph, double *cbuf, int maxsamp, int tune, hipStream_t s);
+                   const void *vph, double *cbuf, int maxsamp, double phase_max, int tune,
+                   hipStream_t s);
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s,
                  int *launches);
 int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, double *ms,
                      double *bytes);
 void spo_pass_release(SpoPassPlan &p);
+// Replace the tuned launch shape (ESPEC_ERR_ARG if the grid cannot use it).
+int spo_pass_set_shape(SpoPassPlan &p, const PassShape &s);
 
 }  // namespace espec

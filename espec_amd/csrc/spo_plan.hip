@@ -80,9 +80,11 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         for (int q = 0; q < nset; ++q)
             for (int a = 0; a < P.ndim; ++a)
                 taxes.push_back(kinetic_axis(P.np[a], sh[a], sm[(size_t)q * P.ndim + a], false));
+        double vmax = 0;
+        for (size_t i = 0; i < (size_t)n * npk; ++i) vmax = std::max(vmax, std::fabs(vpot[i]));
         return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, cst2, P.x.p,
                               P.psi0.p, P.vst.p, table ? P.vph.p : nullptr, P.cbuf.as<double>(),
-                              P.maxsamp, 1, P.stream);
+                              P.maxsamp, std::fabs(cst2) * vmax, 1, P.stream);
     }
     // Kinetic phase: natural order for the reference kernel; for the fused
     // kernel, position of frequency k is sum_a perm_a[k_a] * stride_a.
@@ -326,7 +328,29 @@ int espec_spo_plan_shape(espec_spo_plan *P, int *shape)
     shape[4] = s.tile_nt;
     shape[5] = s.spread;
     shape[6] = s.xmap;
+    shape[7] = s.grid;
+    shape[8] = s.rows_pen;
+    shape[9] = s.tile_pen;
+    shape[10] = s.pen_pf;
     return ESPEC_OK;
+}
+
+int espec_spo_plan_set_shape(espec_spo_plan *P, const int *shape)
+{
+    if (!P || !shape) return ESPEC_ERR_ARG;
+    if (P->lds) return ESPEC_ERR_UNSUPPORTED;  // one fixed kernel shape
+    PassShape s;
+    s.rows = shape[1];
+    s.rows_nt = shape[2];
+    s.tile = shape[3];
+    s.tile_nt = shape[4];
+    s.spread = shape[5];
+    s.xmap = shape[6];
+    s.grid = shape[7];
+    s.rows_pen = shape[8];
+    s.tile_pen = shape[9];
+    s.pen_pf = shape[10];
+    return spo_pass_set_shape(P->pp, s);
 }
 
 int espec_spo_plan_profile(espec_spo_plan *P, int nsteps, double *ms, double *bytes, int maxkinds,

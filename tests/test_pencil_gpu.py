@@ -1,0 +1,100 @@
+"""Multi-pass split-operator plans with pinned launch shapes: the stage
+kernels and the register-pencil kernels, with and without the grid-stride
+loop and prefetch, against the FP64 oracle (|dC| <= 1e-10 FP64, <= 1e-6
+mixed)."""
+import numpy as np
+import pytest
+
+import espec_amd as E
+from espec_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {
+    "stage": dict(rows_pen=0, tile_pen=0, grid=0),
+    "stage_loop": dict(rows_pen=0, tile_pen=0, grid=16),
+    "pen": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=0),
+    "pen_rows": dict(rows_pen=1, tile_pen=0, pen_pf=0, grid=0),
+    "pen_cols": dict(rows_pen=0, tile_pen=1, pen_pf=0, grid=0),
+    "pen_loop_pf": dict(rows_pen=1, tile_pen=1, pen_pf=1, grid=24),
+    "pen_loop": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=40),
+}
+
+
+def pinned(plan, name, np0):
+    kw = dict(SHAPES[name])
+    if kw.get("rows_pen"):
+        kw.update(rows=1024 // np0, rows_nt=64)
+    plan.set_shape(**kw)
+    got = plan.shape()
+    for k, v in kw.items():
+        assert got[k] == v, (k, got)
+
+
+def dC(g, o):
+    k = min(len(g["cr"]), len(o["cr"]))
+    return max(np.abs(g["cr"][:k] - o["cr"][:k]).max(), np.abs(g["ci"][:k] - o["ci"][:k]).max())
+
+
+@pytest.mark.parametrize("shape", list(SHAPES))
+@pytest.mark.parametrize("flags,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-6)])
+def test_config2_shapes(gpu, orc, shape, flags, tol):
+    from oracle import EXACT
+    c = configs.config2()
+    nst, lc = 40, 1
+    print(f"case: config2 shape={shape} flags={flags}", flush=True)
+    plan = E.SpoPlan(c.np, c.dt_fs, lc, c.sh, c.sm, c.vpot[None], c.u[None], c.v[None],
+                     maxsamp=nst + 1, flags=flags | E.EXACT)
+    pinned(plan, shape, c.np[0])
+    plan.run(nst)
+    g = plan.get(0)
+    o = orc.pspofft(c.np, c.dt_fs, 0.0, (nst - 1) * c.dt_fs, lc, c.sh, c.sm, c.u, c.v, c.vpot,
+                    EXACT)
+    assert dC(g, o) <= tol
+    plan.close()
+
+
+@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "stage"])
+def test_3d_shapes(gpu, orc, shape):
+    """64^3 (every axis a pencil length), two runs continuing one propagation."""
+    from oracle import EXACT
+    c = configs.config3(n=64)
+    nst, lc = 12, 2
+    print(f"case: 3d 64^3 shape={shape}", flush=True)
+    plan = E.SpoPlan(c.np, c.dt_fs, lc, c.sh, c.sm, c.vpot[None], c.u[None], c.v[None],
+                     maxsamp=nst // lc + 1, flags=E.FP64 | E.EXACT)
+    pinned(plan, shape, c.np[0])
+    plan.run(5)
+    plan.run(nst - 5)
+    g = plan.get(0)
+    o = orc.pspofft(c.np, c.dt_fs, 0.0, (nst - 1) * c.dt_fs, lc, c.sh, c.sm, c.u, c.v, c.vpot,
+                    EXACT)
+    assert dC(g, o) <= 1e-10
+
+
+@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "pen_loop"])
+@pytest.mark.parametrize("flags,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-6)])
+def test_batch_per_packet_masses(gpu, orc, shape, flags, tol):
+    """Five 128^2 packets with their own masses, potentials and states;
+    C(t) every third step."""
+    from oracle import EXACT
+    rng = np.random.default_rng(21)
+    pk = [configs.packet2d(128, rng.uniform(0.1, 0.2, 2), rng.uniform(0.8, 1.4, 2),
+                           rng.uniform(500, 2000, 2), rng.uniform(0.1, 0.5, 2),
+                           rng.uniform(0.1, 0.3, 2), 0.5, 30) for _ in range(5)]
+    nst, lc = 30, 3
+    n = 128 * 128
+    print(f"case: batch 5x128^2 shape={shape} flags={flags}", flush=True)
+    plan = E.SpoPlan([128, 128], pk[0].dt_fs, lc, pk[0].sh,
+                     np.array([p.sm for p in pk], dtype=np.float64),
+                     np.stack([p.vpot.reshape(n) for p in pk]),
+                     np.stack([p.u.reshape(n) for p in pk]),
+                     np.stack([p.v.reshape(n) for p in pk]),
+                     maxsamp=nst // lc + 1, flags=flags | E.EXACT)
+    pinned(plan, shape, 128)
+    plan.run(nst)
+    for k, p in enumerate(pk):
+        o = orc.pspofft(p.np, p.dt_fs, 0.0, (nst - 1) * p.dt_fs, lc, p.sh, p.sm, p.u, p.v,
+                        p.vpot, EXACT)
+        assert dC(plan.get(k), o) <= tol, k
+    plan.close()
