@@ -13,7 +13,12 @@ dominant kernel (HIP events on the plan's stream) and the FP64 C oracle
 timed on the host as the CPU baseline.  Inputs are resident in HBM before
 the timed region starts; only the propagation loop is timed.
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 100]
+Also reported, under "batched": the 2048-packet uneven batch (1-D and 2-D
+grids, per-packet potentials, states and masses; 1000 steps of dt = 0.5
+a.u.) on the mixed and FP64 paths, with its whole-job rate against the HBM
+peak at 28 B per grid-point-step and a C(t) check of sampled packets.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 100] [--batch-steps 1000]
 """
 import argparse
 import json
@@ -68,11 +73,73 @@ def pmc_traffic(name, kernel):
 
 
 def kernel_names(shape, np0, np1, storage="float"):
-    """Instantiation names of the two pass kernels for a launch shape."""
-    ept_r = shape["rows"] * np0 // shape["rows_nt"]
-    ept_s = shape["tile"] * np1 // shape["tile_nt"]
-    return (f"pass_rows<{storage}, {shape['rows_nt']}, true, true, {ept_r}, true>",
-            f"pass_strided<{storage}, {shape['tile_nt']}, {ept_s}>")
+    """Instantiation names (as rocprofv3 prints them, namespace and
+    arguments stripped) of the row-pass and strided-pass kernels that a
+    launch shape selects for steps 2..K."""
+    pht = "true" if shape.get("vph") else "false"
+    if shape["rows_pen"]:
+        rows = f"pass_rows_pen<{storage}, {np0 // 16}, true, true, {pht}>"
+    else:
+        ept_r = shape["rows"] * np0 // shape["rows_nt"]
+        rows = f"pass_rows<{storage}, {shape['rows_nt']}, true, true, {ept_r}, {pht}>"
+    if shape["tile_pen"]:
+        pf = "true" if shape["pen_pf"] else "false"
+        cols = f"pass_strided_pen<{storage}, {np1 // 16}, {pf}>"
+    else:
+        ept_s = shape["tile"] * np1 // shape["tile_nt"]
+        cols = f"pass_strided<{storage}, {shape['tile_nt']}, {ept_s}>"
+    return rows, cols
+
+
+def batched(E, configs, steps, check):
+    """The batched uneven-packet workload: 2048 seeded packets (1-D N in
+    {128..1024}, 2-D 64^2..256^2), each with its own potential, state and
+    masses, dt = 0.5 a.u., `steps` steps, C(t) every step.  Packets are
+    grouped by grid into device-resident plans on their own streams; the
+    timed region is one run of every group (inputs resident).  With
+    `check`, the C(t) of one packet per 2-D class and of one 1-D packet is
+    compared with the FP64 oracle over the whole run."""
+    from espec_amd.batch import SpoBatch
+    pk = configs.config4(count=2048, nsteps=steps, seed=4)
+    out = {"packets": len(pk), "steps": steps, "lc": 1}
+    for name, flags in (("mixed", E.MIXED | E.EXACT), ("fp64", E.FP64 | E.EXACT)):
+        b = SpoBatch(pk, flags=flags)
+        b.run(8)
+        b.sync()
+        b.reset()
+        sec = b.timed_run(steps)
+        gps = b.points * steps / sec
+        out[name] = {"seconds": sec, "value": gps, "unit": "grid-point-steps/s",
+                     "algorithmic_GBps": gps * 28.0 / 1e9 if name == "mixed" else gps * 56.0 / 1e9,
+                     "groups": [{"grid": g["np"], "packets": len(g["idx"]),
+                                 "shape": g["plan"].shape()} for g in b.groups]}
+        log(f"[bench] batched {name}: {sec:.3f} s, {gps:.3e} gp-steps/s")
+        if check and name == "mixed":
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            from oracle import Oracle, EXACT
+            O = Oracle()
+            picks = {}
+            for i, p in enumerate(pk):
+                key = tuple(p.np) if len(p.np) == 2 else ("1-D",)
+                picks.setdefault(key, i)
+            worst = {}
+            for key, i in picks.items():
+                p = pk[i]
+                o = O.pspofft(p.np, p.dt_fs, 0.0, (steps - 1) * p.dt_fs, 1, p.sh, p.sm, p.u, p.v,
+                              p.vpot, EXACT)
+                g = b.get(i)
+                n = min(len(o["cr"]), len(g["cr"]))
+                d = max(np.abs(o["cr"][:n] - g["cr"][:n]).max(),
+                        np.abs(o["ci"][:n] - g["ci"][:n]).max())
+                worst["x".join(map(str, p.np))] = float(d)
+            out["max_abs_dC_mixed_sample"] = worst
+            log(f"[bench] batched mixed |dC| sample: {worst}")
+        b.close()
+    m = out["mixed"]["value"]
+    out["step_roofline"] = {"bytes_per_gp_step": 28.0, "achieved_GBps": m * 28.0 / 1e9,
+                            "frac": m * 28.0 / HBM_PEAK}
+    out["mixed_over_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
 
 
 def main():
@@ -82,6 +149,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--batch-steps", type=int, default=1000,
+                    help="steps of the batched 2048-packet workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagation of one packet (or a batch of "
@@ -190,6 +259,8 @@ def main():
         log(f"[bench] CPU: {t_cpu:.1f} s; max|dC| mixed {dmix:.2e} fp64 {d64:.2e}")
     plan_mix.close()
     plan_64.close()
+    if args.batch_steps > 0:
+        out["batched"] = batched(E, configs, args.batch_steps, not args.no_cpu)
     print(json.dumps(out))
 
 

@@ -387,7 +387,7 @@ class SodPlan(_Plan):
 
 
 _SHAPE_KEYS = ("lds", "rows", "rows_nt", "tile", "tile_nt", "spread", "xmap", "grid", "rows_pen",
-               "tile_pen", "pen_pf")
+               "tile_pen", "pen_pf", "vph")
 
 
 class SpoPlan(_Plan):
@@ -417,7 +417,7 @@ class SpoPlan(_Plan):
 
     def shape(self):
         """Launch shape: dict(lds, rows, rows_nt, tile, tile_nt, spread, xmap, grid,
-        rows_pen, tile_pen, pen_pf)."""
+        rows_pen, tile_pen, pen_pf, vph)."""
         a = np.zeros(len(_SHAPE_KEYS), dtype=np.int32)
         _check(self._L.espec_spo_plan_shape(self._h, a), "spo_plan_shape")
         return dict(zip(_SHAPE_KEYS, a.tolist()))

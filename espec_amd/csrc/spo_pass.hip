@@ -328,10 +328,10 @@ __device__ __forceinline__ void pen_fetch(const PassArgs &a, long long po, long 
     }
 }
 
-// PF: load the next item's state and V while transforming the current one.
 // Two waves per SIMD are requested: the exchange slots (17 KiB per wave)
-// cap residency near that anyway.
-template <typename S, int M, bool INV, bool FWD, bool PHT, bool PF>
+// cap residency near that anyway.  (Prefetching the next item here costs
+// spills: the row pass holds psi0 and V beside the state.)
+template <typename S, int M, bool INV, bool FWD, bool PHT>
 __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
 {
     constexpr int C = 64 / M, L = 16 * M;
@@ -352,9 +352,8 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
     S sr[16], si[16], sv[16];
     int pk, blk;
     long long po = origin(w, pk, blk);
-    if (PF) pen_fetch<S, !PHT>(a, po, M, sr, si, sv);
     for (;;) {
-        if (!PF) pen_fetch<S, !PHT>(a, po, M, sr, si, sv);
+        pen_fetch<S, !PHT>(a, po, M, sr, si, sv);
         double2 v[16];
         S cv[16];
 #pragma unroll
@@ -365,7 +364,6 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
         const int wn = w + gridDim.x;
         int pkn = pk, blkn = blk;
         const long long pon = wn < total ? origin(wn, pkn, blkn) : po;
-        if (PF && wn < total) pen_fetch<S, !PHT>(a, pon, M, sr, si, sv);
         // psi0 for the sample, in flight during the inverse transform
         S q0r[16], q0i[16];
         if (samp) {
@@ -434,6 +432,7 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
 
 // Strided pass on pencils: lane = c + C j, line c of the item's C
 // consecutive columns, lane j within the line.
+// PF: load the next item's state while transforming the current one.
 template <typename S, int M, bool PF>
 __global__ __launch_bounds__(64, 2) void pass_strided_pen(PassArgs a)
 {
@@ -447,6 +446,9 @@ __global__ __launch_bounds__(64, 2) void pass_strided_pen(PassArgs a)
     if (w >= total) return;
     const long long stride = (long long)M * a.S;
     auto origin = [&](int wi, int &pk) {
+        // With xmap, XCD (wi mod 8) takes a contiguous range of column
+        // groups, so that the groups sharing a cache line meet in one L2.
+        if (a.xmap) wi = (wi & 7) * (total >> 3) + (wi >> 3);
         pk = wi / groups;
         const int col = (wi - pk * groups) * C + c;
         return (long long)pk * a.n + (col % a.S) + (long long)(col / a.S) * a.S * a.L +
@@ -607,13 +609,11 @@ hipError_t rows(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, double 
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[0];
         a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[0];
         const dim3 pg(act);
-        const bool pf = p.shape.pen_pf != 0;
-#define ESPEC_PEN_ROWS2(M_, P_)                                                                    \
-    if (p.np[0] == 16 * M_ && pht == P_) {                                                         \
-        if (inv && fwd && pf) return launch(pass_rows_pen<R, M_, true, true, P_, true>, pg, 64, 0, s, a); \
-        if (inv && fwd) return launch(pass_rows_pen<R, M_, true, true, P_, false>, pg, 64, 0, s, a); \
-        if (inv) return launch(pass_rows_pen<R, M_, true, false, P_, false>, pg, 64, 0, s, a);      \
-        return launch(pass_rows_pen<R, M_, false, true, P_, false>, pg, 64, 0, s, a);               \
+#define ESPEC_PEN_ROWS2(M_, P_)                                                               \
+    if (p.np[0] == 16 * M_ && pht == P_) {                                                    \
+        if (inv && fwd) return launch(pass_rows_pen<R, M_, true, true, P_>, pg, 64, 0, s, a); \
+        if (inv) return launch(pass_rows_pen<R, M_, true, false, P_>, pg, 64, 0, s, a);       \
+        return launch(pass_rows_pen<R, M_, false, true, P_>, pg, 64, 0, s, a);                \
     }
 #define ESPEC_PEN_ROWS(M_) ESPEC_PEN_ROWS2(M_, true) ESPEC_PEN_ROWS2(M_, false)
         ESPEC_PEN_ROWS(4)
@@ -668,8 +668,9 @@ hipError_t strided(const SpoPassPlan &p, int axis, bool last, hipStream_t s)
         a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[axis];
         const int total = p.n / (a.L * (64 / (a.L / 16))) * p.npk;
         const int act = p.shape.grid > 0 ? std::min(total, p.shape.grid) : total;
+        a.xmap = (p.shape.xmap && total % 8 == 0) ? 1 : 0;
         const dim3 pg(act);
-        const bool pf = p.shape.pen_pf != 0;
+        const bool pf = p.shape.pen_pf != 0;  // strided passes only
         if (a.L == 64) return launch(pf ? pass_strided_pen<R, 4, true> : pass_strided_pen<R, 4, false>, pg, 64, 0, s, a);
         if (a.L == 128) return launch(pf ? pass_strided_pen<R, 8, true> : pass_strided_pen<R, 8, false>, pg, 64, 0, s, a);
         if (a.L == 256) return launch(pf ? pass_strided_pen<R, 16, true> : pass_strided_pen<R, 16, false>, pg, 64, 0, s, a);
@@ -720,6 +721,28 @@ void drop_graph(SpoPassPlan &p)
     p.graph_steps = 0;
 }
 
+// Record kGraphSteps full steps (row pass + strided passes each) into a
+// graph; replays advance the device step counter themselves.  Capture
+// records without executing, so it costs no steps.
+constexpr int kGraphSteps = 32;
+
+template <typename R>
+int capture_typed(SpoPassPlan &p)
+{
+    hipStream_t cs;
+    ESPEC_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int dummy = 0;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    for (int k = 0; e == hipSuccess && k < kGraphSteps; ++k) e = one_step<R>(p, true, true, cs, &dummy);
+    hipError_t e2 = hipStreamEndCapture(cs, &p.graph);
+    (void)hipStreamDestroy(cs);
+    ESPEC_HIP_TRY(e);
+    ESPEC_HIP_TRY(e2);
+    ESPEC_HIP_TRY(hipGraphInstantiate(&p.gexec, p.graph, nullptr, nullptr, 0));
+    p.graph_steps = kGraphSteps;
+    return ESPEC_OK;
+}
+
 template <typename R>
 int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *launches)
 {
@@ -732,20 +755,10 @@ int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *l
     // same launch sequence repeats, so it is captured once into a graph of
     // graph_steps steps and replayed.
     int left = nsteps - 1;
-    const int gsteps = 32;
-    if (left >= gsteps) {
+    if (left >= kGraphSteps) {
         if (!p.gexec) {
-            hipStream_t cs;
-            ESPEC_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            int dummy = 0;
-            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-            for (int k = 0; e == hipSuccess && k < gsteps; ++k) e = one_step<R>(p, true, true, cs, &dummy);
-            hipError_t e2 = hipStreamEndCapture(cs, &p.graph);
-            (void)hipStreamDestroy(cs);
-            ESPEC_HIP_TRY(e);
-            ESPEC_HIP_TRY(e2);
-            ESPEC_HIP_TRY(hipGraphInstantiate(&p.gexec, p.graph, nullptr, nullptr, 0));
-            p.graph_steps = gsteps;
+            const int rc = capture_typed<R>(p);
+            if (rc) return rc;
         }
         while (left >= p.graph_steps) {
             ESPEC_HIP_TRY(hipGraphLaunch(p.gexec, s));
@@ -964,7 +977,7 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
         best_ms = 1e30f;
         int best_grid = 0;
         int best_pf = 0;
-        const bool pen = p.shape.rows_pen || p.shape.tile_pen;
+        const bool pen = p.shape.tile_pen;
         for (int pf : {0, 1})
         for (int gs : {0, 4096, 2048, 1024, 512}) {
             if (gs && (items <= gs || p.shape.spread > 1)) continue;
@@ -1197,6 +1210,14 @@ int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStr
     if (nsteps < 2) return ESPEC_ERR_ARG;
     return p.mixed ? profile_typed<float>(p, nsteps, step0, s, ms, bytes)
                    : profile_typed<double>(p, nsteps, step0, s, ms, bytes);
+}
+
+int spo_pass_prepare(SpoPassPlan &p, int lc)
+{
+    if (lc != p.lc) drop_graph(p);
+    p.lc = lc;
+    if (p.gexec) return ESPEC_OK;
+    return p.mixed ? capture_typed<float>(p) : capture_typed<double>(p);
 }
 
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, int *launches)

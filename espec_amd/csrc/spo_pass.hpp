@@ -16,7 +16,7 @@ struct PassShape {
     int grid = 0;                  // workgroups per pass (0: one per work item)
     int rows_pen = 0;              // row pass on register pencils (rows = 64 / (np0 / 16))
     int tile_pen = 0;              // strided passes on register pencils
-    int pen_pf = 0;                // pencil passes prefetch their next work item
+    int pen_pf = 0;                // pencil strided passes prefetch their next work item
 };
 
 struct SpoPassPlan {
@@ -60,5 +60,8 @@ int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStr
 void spo_pass_release(SpoPassPlan &p);
 // Replace the tuned launch shape (ESPEC_ERR_ARG if the grid cannot use it).
 int spo_pass_set_shape(SpoPassPlan &p, const PassShape &s);
+// Capture the per-step launch graph for sampling interval lc ahead of the
+// first run (a run captures it itself when needed).
+int spo_pass_prepare(SpoPassPlan &p, int lc);
 
 }  // namespace espec

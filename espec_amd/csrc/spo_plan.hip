@@ -191,6 +191,10 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
     int rc = P->mixed ? build_tables<float>(*P, sh, sm, vpot, u, v)
                       : build_tables<double>(*P, sh, sm, vpot, u, v);
     if (rc) return rc;
+    if (!P->lds) {
+        rc = spo_pass_prepare(P->pp, lc);
+        if (rc) return rc;
+    }
     SpoLdsArgs &a = P->la;
     a.ndim = ndim;
     a.n = P->n;
@@ -332,6 +336,7 @@ int espec_spo_plan_shape(espec_spo_plan *P, int *shape)
     shape[8] = s.rows_pen;
     shape[9] = s.tile_pen;
     shape[10] = s.pen_pf;
+    shape[11] = P->lds ? 0 : (P->pp.vph != nullptr);
     return ESPEC_OK;
 }
 
@@ -350,7 +355,8 @@ int espec_spo_plan_set_shape(espec_spo_plan *P, const int *shape)
     s.rows_pen = shape[8];
     s.tile_pen = shape[9];
     s.pen_pf = shape[10];
-    return spo_pass_set_shape(P->pp, s);
+    const int rc = spo_pass_set_shape(P->pp, s);
+    return rc ? rc : spo_pass_prepare(P->pp, P->lc);
 }
 
 int espec_spo_plan_profile(espec_spo_plan *P, int nsteps, double *ms, double *bytes, int maxkinds,

@@ -240,15 +240,16 @@ int espec_sil_plan_get(espec_sil_plan *plan, int pk, double *cr, double *ci,
                        double *v);
 int espec_sil_plan_destroy(espec_sil_plan *plan);
 
-/* Launch shape chosen for the plan, shape[11]: 1 if LDS-resident, rows per
+/* Launch shape chosen for the plan, shape[12]: 1 if LDS-resident, rows per
  * row-pass workgroup, its threads, strided tile width, its threads, XCD
  * spread, XCD-contiguous tile map, workgroups per pass (0: one per row block
  * or tile; else a grid-stride loop), register-pencil row pass, register-pencil
- * strided passes, pencil prefetch of the next work item (no reference
+ * strided passes, prefetch of the next work item in the pencil strided
+ * passes, and 1 if the row pass reads an FP64 V phase table (no reference
  * counterpart; for reports). */
 int espec_spo_plan_shape(espec_spo_plan *plan, int *shape);
-/* Pin a multi-pass plan's launch shape (shape[11] as above; shape[0] is
- * ignored) in place of the autotuner's choice, e.g. to test one kernel
+/* Pin a multi-pass plan's launch shape (shape[12] as above; shape[0] and
+ * shape[11] are ignored) in place of the autotuner's choice, e.g. to test one kernel
  * family.  ESPEC_ERR_ARG if the grid cannot use that shape,
  * ESPEC_ERR_UNSUPPORTED for an LDS-resident plan (no reference counterpart). */
 int espec_spo_plan_set_shape(espec_spo_plan *plan, const int *shape);

@@ -13,11 +13,11 @@ pytestmark = pytest.mark.gpu
 SHAPES = {
     "stage": dict(rows_pen=0, tile_pen=0, grid=0),
     "stage_loop": dict(rows_pen=0, tile_pen=0, grid=16),
-    "pen": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=0),
+    "pen": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=0, xmap=0),
     "pen_rows": dict(rows_pen=1, tile_pen=0, pen_pf=0, grid=0),
-    "pen_cols": dict(rows_pen=0, tile_pen=1, pen_pf=0, grid=0),
-    "pen_loop_pf": dict(rows_pen=1, tile_pen=1, pen_pf=1, grid=24),
-    "pen_loop": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=40),
+    "pen_cols": dict(rows_pen=0, tile_pen=1, pen_pf=0, grid=0, xmap=1),
+    "pen_loop_pf": dict(rows_pen=1, tile_pen=1, pen_pf=1, grid=24, xmap=1),
+    "pen_loop": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=40, xmap=0),
 }
 
 
