@@ -353,7 +353,11 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
     int pk, blk;
     long long po = origin(w, pk, blk);
     for (;;) {
-        pen_fetch<S, !PHT>(a, po, M, sr, si, sv);
+        // Single precision: V and psi0 are loaded with the state and are in
+        // flight during the inverse transform.  Double precision: after it,
+        // since holding them across the transform spills.
+        constexpr bool early = sizeof(S) == 4;
+        pen_fetch<S, !PHT && early>(a, po, M, sr, si, sv);
         double2 v[16];
         S cv[16];
 #pragma unroll
@@ -364,9 +368,8 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
         const int wn = w + gridDim.x;
         int pkn = pk, blkn = blk;
         const long long pon = wn < total ? origin(wn, pkn, blkn) : po;
-        // psi0 for the sample, in flight during the inverse transform
         S q0r[16], q0i[16];
-        if (samp) {
+        auto load_psi0 = [&]() {
             const V2<S> *g0 = reinterpret_cast<const V2<S> *>(a.psi0) + po;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
@@ -374,8 +377,15 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
                 q0r[r] = t.x;
                 q0i[r] = t.y;
             }
-        }
+        };
+        if (samp && early) load_psi0();
         if (INV) pencil::inverse<M, -1>(v, j, xl, a.twr);
+        if (samp && !early) load_psi0();
+        if (!PHT && !early) {
+            const S *gv = reinterpret_cast<const S *>(a.vpot) + po;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) cv[r] = gv[r * M];
+        }
         double cr = 0, ci = 0, nr = 0;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {

@@ -65,8 +65,12 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         for (size_t i = 0; i < vs.size(); ++i) vs[i] = (S)vpot[i];
         ESPEC_HIP_TRY(P.vst.upload(vs.data(), sizeof(S) * vs.size()));
         // FP64 phase table while it stays small enough to live in the MALL
-        // (then the pass is latency-bound and a table load beats sincos).
-        const bool table = (size_t)n * npk * sizeof(double2) <= (size_t)64 << 20;
+        // (then the pass is latency-bound and a table load beats sincos),
+        // and always with double-precision storage: there the row pass
+        // cannot hold V, psi0 and the state across the transform without
+        // spilling, and the table removes V and the sincos from it.
+        const bool table =
+            !P.mixed || (size_t)n * npk * sizeof(double2) <= (size_t)64 << 20;
         if (table) {
             std::vector<double> vph((size_t)2 * n * npk);
             for (int p = 0; p < npk; ++p) {
