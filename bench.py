@@ -17,8 +17,12 @@ Also reported, under "batched": the 2048-packet uneven batch (1-D and 2-D
 grids, per-packet potentials, states and masses; 1000 steps of dt = 0.5
 a.u.) on the mixed and FP64 paths, with its whole-job rate against the HBM
 peak at 28 B per grid-point-step and a C(t) check of sampled packets.
+Under "krylov_controller": the learned Krylov-order/step controller
+(trained on the device) against fixed-order short-iterative Lanczos on the
+256x256 surface, wall time and |dC| against the order-20 run.
 
     python bench.py [--gpus 1] [--steps 2000] [--warmup 100] [--batch-steps 1000]
+                    [--krylov-steps 2000]
 """
 import argparse
 import json
@@ -151,6 +155,74 @@ def batched(E, configs, steps, check):
     return out
 
 
+def krylov_controller(E, configs, steps):
+    """The learned Krylov-order/step controller on short-iterative Lanczos
+    propagation of the 256x256 surface (FD Hamiltonian, dt = 1 a.u., C(t)
+    every step, FP64).  The network is trained on the device from traces
+    of 32 surfaces varied within +-20% around it (states advanced at orders
+    12, 14, 16 and 21, 400 steps each), then `steps` adaptive steps
+    (budget 1e-10 per dt, fallback order 12) are timed against the fixed
+    orders 10, 11 and 12; |dC| is measured against the order-20 run."""
+    from espec_amd import krylov
+    tol = 1e-10
+    t0 = time.perf_counter()
+    fam = krylov.family(count=32, n=256, nsteps=0, seed=5, groups=4)
+    traces = [krylov.trace_packets([p for p in fam if p.meta["group"] == g], 400, mprop=mp)
+              for g, mp in zip(range(4), (12, 14, 16, 21))]
+    x, t = krylov.dataset(traces, tol)
+    params, hist = krylov.train(x, t, epochs=40, lr=3e-3, seed=3)
+    t_train = time.perf_counter() - t0
+    log(f"[bench] controller: {len(x)} samples, traces + training {t_train:.1f} s, "
+        f"loss {hist[0]:.2e} -> {hist[-1]:.2e}")
+    c = configs.config2(nsteps=steps)
+    shm = krylov.shm_of(c)
+
+    def plan(m):
+        return E.SilPlan(c.np, m, 0.0, c.dt_fs, 1, shm, c.vpot, c.u, c.v, maxsamp=steps + 1)
+
+    ref = plan(20)
+    ref.run(steps)
+    g20 = ref.get(0)
+    ref.close()
+
+    def dC(g):
+        return float(max(np.abs(g["cr"] - g20["cr"]).max(), np.abs(g["ci"] - g20["ci"]).max()))
+
+    out = {"workload": "256x256 coupled-Morse SIL, FD Hamiltonian, dt 1 a.u., FP64",
+           "steps": steps, "budget_per_dt": tol, "train_samples": int(len(x)),
+           "train_seconds": t_train, "fixed": {}}
+    for m in (10, 11, 12):
+        w = plan(m)
+        w.run(16)
+        w.sync()
+        w.close()
+        f = plan(m)
+        f.sync()
+        t1 = time.perf_counter()
+        f.run(steps)
+        f.sync()
+        out["fixed"][str(m)] = {"seconds": time.perf_counter() - t1, "max_abs_dC": dC(f.get(0))}
+        f.close()
+    a = plan(21)
+    a.adaptive(8, params, tol=tol, m_fix=12)
+    a.close()
+    a = plan(21)
+    a.sync()
+    t1 = time.perf_counter()
+    st = a.adaptive(steps, params, tol=tol, m_fix=12)
+    ta = time.perf_counter() - t1
+    out["adaptive"] = {"seconds": ta, "max_abs_dC": dC(a.get(0)), "stats": st}
+    a.close()
+    ok = [m for m, v in out["fixed"].items() if v["max_abs_dC"] <= 1e-6]
+    base = min(ok, key=int) if ok else None
+    out["cheapest_fixed_within_1e-6"] = base
+    out["speedup_vs_that_fixed"] = (out["fixed"][base]["seconds"] / ta) if base else None
+    log(f"[bench] controller: adaptive {ta:.2f} s |dC| {out['adaptive']['max_abs_dC']:.1e}; "
+        f"fixed " + ", ".join(f"m={m} {v['seconds']:.2f} s |dC| {v['max_abs_dC']:.1e}"
+                              for m, v in out["fixed"].items()))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +232,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=1000,
                     help="steps of the batched 2048-packet workload (0 skips it)")
+    ap.add_argument("--krylov-steps", type=int, default=2000,
+                    help="steps of the learned Krylov controller comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagation of one packet (or a batch of "
@@ -271,6 +345,8 @@ def main():
     plan_64.close()
     if args.batch_steps > 0:
         out["batched"] = batched(E, configs, args.batch_steps, not args.no_cpu)
+    if args.krylov_steps > 0:
+        out["krylov_controller"] = krylov_controller(E, configs, args.krylov_steps)
     print(json.dumps(out))
 
 

@@ -63,6 +63,14 @@ def _load():
     L.espec_spo_plan_profile.argtypes = [vp, ip, d, d, ip, P(ip)]
     L.espec_spo_plan_shape.argtypes = [vp, i]
     L.espec_spo_plan_set_shape.argtypes = [vp, i]
+    f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    fp_or_none = C.c_void_p
+    L.espec_sil_plan_adaptive.argtypes = [vp, fp_or_none, ip, dp, ip, P(C.c_longlong)]
+    L.espec_sil_plan_trace.argtypes = [vp, ip, ip, f, d]
+    L.espec_sil_plan_adaptive_log.argtypes = [vp, C.c_void_p, ip, P(ip)]
+    L.espec_mlp_infer.argtypes = [f, f, ip, f]
+    L.espec_mlp_grad.argtypes = [f, f, f, f, P(C.c_float)]
+    L.espec_mlp_train.argtypes = [f, f, f, ip, ip, C.c_float, C.c_uint, f]
     L.espec_bw_probe.argtypes = [C.c_size_t, ip, P(dp), P(dp)]
     L.espec_plnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, d, vp, ip, P(ip), ip]
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
@@ -369,6 +377,46 @@ class SilPlan(_Plan):
                                            abstol, dt, lc, _v3(shm, 0.0), self._vpot, self._u,
                                            vp, maxsamp, flags)
         _check(rc, "sil_plan_create")
+        self.mp = mp
+
+    def adaptive(self, ndt, params=None, tol=1e-10, m_fix=10):
+        """Advance ndt base steps with the learned controller choosing the
+        Krylov order and step scale (params: its 4866 FP32 parameters), or
+        with the fixed-order step every dt when params is None (no reference
+        counterpart).  Returns the run statistics."""
+        st = (C.c_longlong * 7)()
+        pp = None
+        if params is not None:
+            self._params = np.ascontiguousarray(params, dtype=np.float32)
+            assert self._params.size == MLP_NPAR
+            pp = self._params.ctypes.data
+        _check(self._L.espec_sil_plan_adaptive(self._h, pp, ndt, tol, m_fix, st),
+               "sil_plan_adaptive")
+        keys = ("steps", "scale_half", "scale_1", "scale_2", "fallbacks", "krylov_sum", "waits")
+        return dict(zip(keys, list(st)))
+
+    def adaptive_log(self, cap=None):
+        """Record the next adaptive run's attempts (cap records), or, with
+        cap None, stop and return the records [n][5] = (base step, order,
+        scale, estimate, budget)."""
+        n = C.c_int()
+        if cap is not None:
+            self._log = np.zeros((cap, 5))
+            _check(self._L.espec_sil_plan_adaptive_log(self._h, self._log.ctypes.data, cap,
+                                                       C.byref(n)), "adaptive_log")
+            return None
+        _check(self._L.espec_sil_plan_adaptive_log(self._h, None, 0, C.byref(n)), "adaptive_log")
+        return self._log[:n.value].copy()
+
+    def trace(self, nsteps, mprop=None):
+        """nsteps steps of mp (>= 20) iterations advancing with order mprop
+        (default mp): controller inputs [nsteps][npk][8] and error estimates
+        [nsteps][2 npk][17][3] of orders 4..20 at dt/2, dt, 2 dt."""
+        feat = np.zeros((nsteps, self.npk, 8), dtype=np.float32)
+        lab = np.zeros((nsteps, 2 * self.npk, 17, 3), dtype=np.float64)
+        _check(self._L.espec_sil_plan_trace(self._h, nsteps, mprop or self.mp, feat, lab),
+               "sil_plan_trace")
+        return feat, lab
 
 
 class SodPlan(_Plan):
@@ -384,6 +432,42 @@ class SodPlan(_Plan):
                                            lc, _v3(sh), _v3(sm), self._vpot, self._u, vp,
                                            maxsamp, flags)
         _check(rc, "sod_plan_create")
+
+
+MLP_NPAR = 4866  # controller network parameters: W1[64][8] b1 W2[64][64] b2 W3[2][64] b3
+
+
+def mlp_infer(params, feat):
+    """Controller network outputs [n][2] for inputs [n][8] (bfloat16 on the device)."""
+    L = _load()
+    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros((feat.shape[0], 2), dtype=np.float32)
+    _check(L.espec_mlp_infer(np.ascontiguousarray(params, dtype=np.float32), feat,
+                             feat.shape[0], out), "mlp_infer")
+    return out
+
+
+def mlp_grad(params, x, t):
+    """Gradient and loss of one 64-row minibatch on the device."""
+    L = _load()
+    g = np.zeros(MLP_NPAR, dtype=np.float32)
+    loss = C.c_float()
+    _check(L.espec_mlp_grad(np.ascontiguousarray(params, dtype=np.float32),
+                            np.ascontiguousarray(x, dtype=np.float32).reshape(64, 8),
+                            np.ascontiguousarray(t, dtype=np.float32).reshape(64, 2), g,
+                            C.byref(loss)), "mlp_grad")
+    return g, loss.value
+
+
+def mlp_train(params, x, t, epochs, lr=1e-3, seed=1):
+    """Adam training on the device; returns (new params, per-epoch loss)."""
+    L = _load()
+    p = np.array(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 8)
+    t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 2)
+    loss = np.zeros(epochs, dtype=np.float32)
+    _check(L.espec_mlp_train(p, x, t, x.shape[0], epochs, lr, seed, loss), "mlp_train")
+    return p, loss
 
 
 _SHAPE_KEYS = ("lds", "rows", "rows_nt", "tile", "tile_nt", "spread", "xmap", "grid", "rows_pen",

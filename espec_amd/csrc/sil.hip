@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "api_common.hpp"
+#include "mlp.hpp"
 #include "sil.hpp"
 
 namespace espec {
@@ -92,6 +93,79 @@ __device__ int tridiag_ql(int m, double *d, double *e, double *z)
                 t = z[i * m + r];
                 z[i * m + r] = z[k * m + r];
                 z[k * m + r] = t;
+            }
+        }
+    }
+    return 0;
+}
+
+// The same QL run by one whole wavefront, d, e and z in LDS: every lane
+// computes the rotation sequence (identical scalars and uniform control
+// flow, each lane writing the same d and e values), and lane k applies
+// each rotation to row k of the eigenvector columns it couples.  The
+// rotations dominate the scalar version's time (O(m^2) per sweep).
+__device__ int tridiag_ql_wave(int m, double *d, double *e, double *z)
+{
+    const int k = threadIdx.x & 63;
+    for (int i = k; i < m * m; i += 64) z[i] = (i % (m + 1)) == 0 ? 1.0 : 0.0;
+    e[m - 1] = 0;
+    for (int l = 0; l < m; ++l) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < m - 1; ++mm) {
+                double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+                if (fabs(e[mm]) <= 2.2204460492503131e-16 * dd) break;
+            }
+            if (mm != l) {
+                if (++iter > 60) return l + 1;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = hypot(g, 1.0);
+                g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                double s = 1, c = 1, p = 0;
+                int i;
+                for (i = mm - 1; i >= l; --i) {
+                    double f = s * e[i], bb = c * e[i];
+                    r = hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0) {
+                        d[i + 1] -= p;
+                        e[mm] = 0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * bb;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - bb;
+                    if (k < m) {
+                        const double zf = z[(i + 1) * m + k];
+                        const double zi = z[i * m + k];
+                        z[(i + 1) * m + k] = s * zi + c * zf;
+                        z[i * m + k] = c * zi - s * zf;
+                    }
+                }
+                if (r == 0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[mm] = 0;
+            }
+        } while (mm != l);
+    }
+    for (int i = 0; i < m - 1; ++i) {
+        int q = i;
+        for (int j = i + 1; j < m; ++j)
+            if (d[j] < d[q]) q = j;
+        if (q != i) {
+            const double t = d[i];
+            const double dq = d[q];
+            d[q] = t;
+            d[i] = dq;
+            if (k < m) {
+                const double zt = z[i * m + k];
+                z[i * m + k] = z[q * m + k];
+                z[q * m + k] = zt;
             }
         }
     }
@@ -216,8 +290,12 @@ __device__ void block_sum_k(double *v, int k, double *red)
 
 // Publish k block partials of (recurrence vec, chunk c); returns true in
 // the last block to arrive, whose thread 0 then holds in tot[0..k) the
-// totals summed in block order (deterministic).  No block waits.
-__device__ bool last_block(const LnzArgs &a, int vec, int c, const double *v, int k, double *tot)
+// totals.  No block waits.  The last block sums the partials in parallel
+// (block b's go to thread b, nb <= NT) and then in the fixed order of
+// block_sum_k, so results are deterministic.
+template <int NT>
+__device__ bool last_block(const LnzArgs &a, int vec, int c, const double *v, int k, double *tot,
+                           double *red)
 {
     __shared__ bool last;
     const long long slot = (long long)vec * a.nchunk + c;
@@ -230,13 +308,14 @@ __device__ bool last_block(const LnzArgs &a, int vec, int c, const double *v, in
     }
     __syncthreads();
     if (!last) return false;
+    __threadfence();
+    double s[kPart];
+    const double *q = a.part + (slot * a.nb + threadIdx.x) * kPart;
+    const bool mine = (int)threadIdx.x < a.nb;
+    for (int j = 0; j < k; ++j) s[j] = mine ? q[j] : 0.0;
+    block_sum_k<NT>(s, k, red);
     if (threadIdx.x == 0) {
-        __threadfence();
-        for (int j = 0; j < k; ++j) tot[j] = 0;
-        for (int b = 0; b < a.nb; ++b) {
-            const double *q = a.part + (slot * a.nb + b) * kPart;
-            for (int j = 0; j < k; ++j) tot[j] += q[j];
-        }
+        for (int j = 0; j < k; ++j) tot[j] = s[j];
         a.ticket[slot] = 0;
     }
     return true;
@@ -265,7 +344,7 @@ __global__ __launch_bounds__(NT) void lnz_init(LnzArgs a, const V2<S> *x, const 
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
     double tot[1];
-    if (last_block(a, vec, 0, v, 1, tot) && threadIdx.x == 0) {
+    if (last_block<NT>(a, vec, 0, v, 1, tot, red) && threadIdx.x == 0) {
         a.beta0[vec] = sqrt(tot[0]);
         a.dead[vec] = tot[0] == 0.0;
     }
@@ -307,7 +386,7 @@ __global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
     double tot[1];
-    if (last_block(a, vec, 0, v, 1, tot) && threadIdx.x == 0) a.al[vec * a.m + i] = tot[0];
+    if (last_block<NT>(a, vec, 0, v, 1, tot, red) && threadIdx.x == 0) a.al[vec * a.m + i] = tot[0];
 }
 
 // lnz.f:110-112 and reort.f:54-67: r_j = <v_j, w - alpha v_i> for the
@@ -335,7 +414,7 @@ __global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
     }
     block_sum_k<NT>(r, jn, red);
     double tot[kChunk];
-    if (last_block(a, vec, c, r, jn, tot) && threadIdx.x == 0)
+    if (last_block<NT>(a, vec, c, r, jn, tot, red) && threadIdx.x == 0)
         for (int j = 0; j < jn; ++j) a.rc[(long long)vec * a.ncols + j0 + j] = tot[j];
 }
 
@@ -365,7 +444,7 @@ __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
     double tot[1];
-    if (last_block(a, vec, 0, v, 1, tot) && threadIdx.x == 0) a.bt[vec * a.m + i] = sqrt(tot[0]);
+    if (last_block<NT>(a, vec, 0, v, 1, tot, red) && threadIdx.x == 0) a.bt[vec * a.m + i] = sqrt(tot[0]);
 }
 
 struct SilArgs {
@@ -400,7 +479,7 @@ __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
     }
     block_sum_k<NT>(c, m, red);
     double tot[kMaxKrylov];
-    if (!last_block(a, vec, 0, c, m, tot)) return;
+    if (!last_block<NT>(a, vec, 0, c, m, tot, red)) return;
     double *coef = s.coef + (long long)vec * 2 * m;
     if (a.dead[vec]) {
         if (threadIdx.x < 2 * m) coef[threadIdx.x] = 0;
@@ -427,9 +506,9 @@ __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
     if (s.abstol > 0 && threadIdx.x < m)
         lam_b = tridiag_bisect_one(m, d, e2, bnd[0], bnd[1], bnd[2], bnd[3], threadIdx.x);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int bad = tridiag_ql(m, d, e, z);
-        if (bad) *s.info = bad;
+    if (threadIdx.x < 64) {
+        const int bad = tridiag_ql_wave(m, d, e, z);
+        if (bad && threadIdx.x == 0) *s.info = bad;
     }
     __syncthreads();
     if (s.abstol > 0 && threadIdx.x < m) d[threadIdx.x] = lam_b;
@@ -511,7 +590,7 @@ __global__ __launch_bounds__(NT) void sil_combine(SilArgs s, V2<S> *x, const V2<
     }
     block_sum_k<NT>(acc, 3, red);
     double tot[3];
-    if (last_block(a, vr, 0, acc, 3, tot) && threadIdx.x == 0) {
+    if (last_block<NT>(a, vr, 0, acc, 3, tot, red) && threadIdx.x == 0) {
         const long long nc = g / lc;
         if (samp && nc < maxsamp) {
             double *c = cbuf + ((long long)pk * maxsamp + nc) * 3;
@@ -542,7 +621,7 @@ __global__ __launch_bounds__(256) void dot_kernel(LnzArgs a, const double *u1, c
     double v[1] = {acc};
     block_sum_k<256>(v, 1, red);
     double tot[1];
-    if (last_block(a, 0, 0, v, 1, tot) && threadIdx.x == 0) *out = tot[0];
+    if (last_block<256>(a, 0, 0, v, 1, tot, red) && threadIdx.x == 0) *out = tot[0];
 }
 
 // DSTEVX('V','A') for one tridiagonal: QL vectors, QL or bisection values.
@@ -665,6 +744,295 @@ __global__ __launch_bounds__(64) void lanczs_select(int n, const double *d, cons
     for (int j = 0; j < n; ++j) coef[(size_t)k * n + j] = sg * x[j];
 }
 
+// ------------------------------------------------ adaptive Krylov steps
+// Steps whose Krylov order m and time step tau are chosen per step by the
+// controller network (mlp.hip), judged by the a-posteriori estimate of the
+// Krylov exponential's error
+//   err(m, tau) = beta0 beta_m |[exp(-i tau T_m)]_{m,1}|
+// (beta0 = |start|, beta_m = |w| after iteration m), with the fixed-order
+// step as fallback.  Exact arithmetic (no sil.f:98-99 coefficient reuse).
+// State updates go to a second buffer and are adopted only when the step
+// is accepted.  No reference counterpart.
+
+constexpr int kLabM0 = 4, kLabM = 17;  // label orders 4..20
+// The step runs one order beyond the network's: its error estimate judges
+// the network's order against the next one (the inputs come from the
+// previous step, and a rejected step costs its fallback).
+constexpr int kMarginM = 1;
+constexpr int kLabT = 3;               // label steps tau/2, tau, 2 tau
+
+struct CtlArgs {
+    LnzArgs l;          // l.m: stored Krylov columns (stride of al, bt)
+    const void *x0;     // [npk][n] state at the start of the step
+    int mact;           // Krylov order of this step (5 <= mact <= l.m)
+    int nsub;           // coefficient sets: tau (1), or tau and 2 tau (2)
+    double tau;         // time step of set 0, a.u.
+    double tfeat;       // base time step for the controller inputs, a.u.
+    double *coef;       // [nvec][2 sets][2][l.m]: cos part, sin part
+    double *est;        // [nvec] error estimate of the step
+    double *est4;       // [nvec] err(4, tfeat)
+    int *info;
+};
+
+// err(m, tau) from the eigenpairs (lam, z column-major, ld m) of T_m.
+__device__ double krylov_est(int m, const double *lam, const double *z, double bm, double b0,
+                             double tau)
+{
+    double sr = 0, si = 0;
+    for (int j = 0; j < m; ++j) {
+        const double w = z[j * m + m - 1] * z[j * m];
+        sr += w * cos(lam[j] * tau);
+        si -= w * sin(lam[j] * tau);
+    }
+    return b0 * bm * sqrt(sr * sr + si * si);
+}
+
+// As leading_ql, by one whole wavefront on LDS arrays.
+__device__ int leading_ql_wave(const LnzArgs &a, int vec, int m, double *d, double *e, double *z)
+{
+    for (int j = 0; j < m; ++j) {  // every lane stores the same values
+        d[j] = a.al[vec * a.m + j];
+        e[j] = j < m - 1 ? a.bt[vec * a.m + j] : 0;
+    }
+    return tridiag_ql_wave(m, d, e, z);
+}
+
+// Leading m x m tridiagonal of recurrence vec into (d, e); QL in place.
+__device__ int leading_ql(const LnzArgs &a, int vec, int m, double *d, double *e, double *z)
+{
+    for (int j = 0; j < m; ++j) {
+        d[j] = a.al[vec * a.m + j];
+        e[j] = j < m - 1 ? a.bt[vec * a.m + j] : 0;
+    }
+    return tridiag_ql(m, d, e, z);
+}
+
+// exp(-i t T) e1 in the Krylov basis from the eigenpairs (lam, z column-
+// major, ld m) of T: y[i] = sum_j z[j][i] z[j][0] exp(-i lam_j t).
+__device__ void krylov_e1(int m, const double *lam, const double *z, double t, double *yr,
+                          double *yi)
+{
+    for (int i = 0; i < m; ++i) {
+        double sr = 0, si = 0;
+        for (int j = 0; j < m; ++j) {
+            const double w = z[j * m + i] * z[j * m];
+            sr += w * cos(lam[j] * t);
+            si -= w * sin(lam[j] * t);
+        }
+        yr[i] = sr;
+        yi[i] = si;
+    }
+}
+
+// beta0 |y_m - y_M| for orders m < M (y_m padded with zeros): the error of
+// the order-m approximation measured against the order-M one.
+__device__ double krylov_diff(int m, const double *lm, const double *zm, int M, const double *lM,
+                              const double *zM, double b0, double t)
+{
+    double ar[kMaxKrylov], ai[kMaxKrylov], br[kMaxKrylov], bi[kMaxKrylov];
+    krylov_e1(m, lm, zm, t, ar, ai);
+    krylov_e1(M, lM, zM, t, br, bi);
+    double s = 0;
+    for (int i = 0; i < M; ++i) {
+        const double dr = br[i] - (i < m ? ar[i] : 0.0), di = bi[i] - (i < m ? ai[i] : 0.0);
+        s += dr * dr + di * di;
+    }
+    return b0 * sqrt(s);
+}
+
+// Projection onto the basis (as sil_coeffs), eigensolves of T_mact and
+// T_{mact-1} (two wavefronts), the coefficient sets of order mact, and the
+// step's error estimate: the larger of beta0 |y_{mact-1} - y_mact| at the
+// step's end and the residual estimate of order mact.  Also err(4, tfeat),
+// a controller input.
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void ctl_coeffs(CtlArgs c)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    __shared__ double d[2][kMaxKrylov], ee[2][kMaxKrylov], z[2][kMaxKrylov * kMaxKrylov];
+    __shared__ int bad;
+    const LnzArgs &a = c.l;
+    const int vec = blockIdx.y, m = c.mact, ms = a.m;
+    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.ncols * a.n;
+    const V2<S> *x0 = reinterpret_cast<const V2<S> *>(c.x0) + (long long)(vec >> 1) * a.n;
+    double cp[kMaxKrylov];
+    for (int j = 0; j < kMaxKrylov; ++j) cp[j] = 0;
+    for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
+        const V2<S> v = x0[p];
+        const double u = (vec & 1) ? (double)v.y : (double)v.x;
+        for (int j = 0; j < m; ++j) cp[j] += (double)B[(long long)j * a.n + p] * u;
+    }
+    block_sum_k<NT>(cp, m, red);
+    double tot[kMaxKrylov];
+    if (!last_block<NT>(a, vec, 0, cp, m, tot, red)) return;
+    double *coef = c.coef + (long long)vec * 4 * ms;
+    if (a.dead[vec]) {
+        for (int i = threadIdx.x; i < 4 * ms; i += NT) coef[i] = 0;
+        if (threadIdx.x == 0) {
+            c.est[vec] = 0;
+            c.est4[vec] = 0;
+        }
+        return;
+    }
+    const double b0 = a.beta0[vec];
+    if (threadIdx.x < 64) {  // wavefront 0: T_m; wavefront 1: T_{m-1}
+        const int r = leading_ql_wave(a, vec, m, d[0], ee[0], z[0]);
+        if (threadIdx.x == 0) bad = r;
+    } else if (threadIdx.x < 128) {
+        leading_ql_wave(a, vec, m - 1, d[1], ee[1], z[1]);
+    } else if (threadIdx.x == 128) {
+        double d4[4], e4[4], z4[16];
+        leading_ql(a, vec, 4, d4, e4, z4);
+        c.est4[vec] = krylov_est(4, d4, z4, a.bt[vec * ms + 3], b0, c.tfeat);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (bad) *c.info = bad;
+        const double *dd = d[0], *zz = z[0];
+        double u2[kMaxKrylov], pc[kMaxKrylov], ps[kMaxKrylov];
+        for (int j = 0; j < m; ++j) {  // Q^T c
+            double t = 0;
+            for (int i = 0; i < m; ++i) t += zz[j * m + i] * tot[i];
+            u2[j] = t;
+        }
+        for (int q = 0; q < c.nsub; ++q) {
+            const double tq = (q + 1) * c.tau;
+            for (int j = 0; j < m; ++j) {
+                const double cs = cos(dd[j] * tq), sn = sin(dd[j] * tq);
+                pc[j] = cs * u2[j];
+                ps[j] = ((vec & 1) ? sn : -sn) * u2[j];
+            }
+            double *cq = coef + q * 2 * ms;
+            for (int i = 0; i < m; ++i) {  // Q (...)
+                double tc = 0, ts = 0;
+                for (int j = 0; j < m; ++j) {
+                    tc += pc[j] * zz[j * m + i];
+                    ts += ps[j] * zz[j * m + i];
+                }
+                cq[i] = tc;
+                cq[ms + i] = ts;
+            }
+        }
+        const double tend = c.nsub * c.tau;
+        const double res = krylov_est(m, dd, zz, a.bt[vec * ms + m - 1], b0, tend);
+        const double dif = krylov_diff(m - 1, d[1], z[1], m, dd, zz, b0, tend);
+        c.est[vec] = fmax(res, dif);
+    }
+}
+
+// Rebuild psi at tau (and 2 tau) into xout; C(t) and |psi|^2 of each set
+// into samples g0 (and g0 + 1) when emit.
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void ctl_combine(CtlArgs c, V2<S> *xout, const V2<S> *psi0,
+                                                  double *cbuf, int maxsamp, long long g0,
+                                                  int emit)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    __shared__ double cf[2][4 * kMaxKrylov];
+    const LnzArgs &a = c.l;
+    const int pk = blockIdx.y, m = c.mact, ms = a.m, ns = c.nsub;
+    const int vr = 2 * pk, vi = 2 * pk + 1;
+    for (int q = 0; q < ns; ++q)
+        for (int i = threadIdx.x; i < 2 * m; i += NT) {
+            const int part = i / m, j = i % m;  // part 0: cos, 1: sin
+            cf[q][part * m + j] = c.coef[(long long)vr * 4 * ms + q * 2 * ms + part * ms + j];
+            cf[q][2 * m + part * m + j] = c.coef[(long long)vi * 4 * ms + q * 2 * ms + part * ms + j];
+        }
+    __syncthreads();
+    const S *Br = reinterpret_cast<const S *>(a.basis) + (long long)vr * a.ncols * a.n;
+    const S *Bi = reinterpret_cast<const S *>(a.basis) + (long long)vi * a.ncols * a.n;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
+        double u[2] = {0, 0}, wv[2] = {0, 0}, v[2] = {0, 0}, wu[2] = {0, 0};
+        for (int j = 0; j < m; ++j) {
+            const double br = (double)Br[(long long)j * a.n + p];
+            const double bi = (double)Bi[(long long)j * a.n + p];
+            for (int q = 0; q < 2; ++q) {
+                if (q >= ns) break;
+                u[q] += cf[q][j] * br;
+                wv[q] += cf[q][m + j] * br;
+                v[q] += cf[q][2 * m + j] * bi;
+                wu[q] += cf[q][3 * m + j] * bi;
+            }
+        }
+        const V2<S> q0 = psi0[(long long)pk * a.n + p];
+        for (int q = 0; q < 2; ++q) {
+            if (q >= ns) break;
+            V2<S> o;
+            o.x = (S)(u[q] + wu[q]);
+            o.y = (S)(wv[q] + v[q]);
+            if (q == ns - 1) xout[(long long)pk * a.n + p] = o;
+            const double ou = (double)o.x, ov = (double)o.y;
+            acc[3 * q] += (double)q0.x * ou + (double)q0.y * ov;
+            acc[3 * q + 1] += (double)q0.x * ov - (double)q0.y * ou;
+            acc[3 * q + 2] += ou * ou + ov * ov;
+        }
+    }
+    block_sum_k<NT>(acc, 3 * ns, red);
+    double tot[6];
+    if (last_block<NT>(a, vr, 0, acc, 3 * ns, tot, red) && threadIdx.x == 0 && emit)
+        for (int q = 0; q < ns; ++q) {
+            const long long g = g0 + q;
+            if (g < maxsamp) {
+                double *cb = cbuf + ((long long)pk * maxsamp + g) * 3;
+                cb[0] = tot[3 * q];
+                cb[1] = tot[3 * q + 1];
+                cb[2] = tot[3 * q + 2];
+            }
+        }
+}
+
+// Controller inputs of each packet from its dominant recurrence (the
+// larger of |Re psi|, |Im psi|): alpha_1, beta_1, alpha_2, beta_2 times the
+// base step; log10 of beta_4 and beta_5 times the base step (a rough
+// component of the state, amplified by H^j, dominates the Krylov vectors
+// from about the fourth on); log10 err(4, tfeat) / 10 and
+// log10 |1 - |psi|^2| / 16.
+__global__ void ctl_feat(CtlArgs c, float *feat, int npk)
+{
+    const int pk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pk >= npk) return;
+    const LnzArgs &a = c.l;
+    const double br = a.beta0[2 * pk], bi = a.beta0[2 * pk + 1];
+    const int v = bi > br ? 2 * pk + 1 : 2 * pk;
+    float *f = feat + (long long)pk * kMlpIn;
+    for (int j = 0; j < 2; ++j) {
+        f[2 * j] = (float)(a.al[v * a.m + j] * c.tfeat);
+        f[2 * j + 1] = (float)(a.bt[v * a.m + j] * c.tfeat);
+    }
+    f[4] = (float)log10(fmax(a.bt[v * a.m + 3] * c.tfeat, 1e-30));
+    f[5] = (float)log10(fmax(a.bt[v * a.m + 4] * c.tfeat, 1e-30));
+    f[6] = (float)(log10(c.est4[v] + 1e-300) / 10.0);
+    f[7] = (float)(log10(fabs(br * br + bi * bi - 1.0) + 1e-16) / 16.0);
+}
+
+// Training labels of a trace step (mp = l.m iterations): for each network
+// order k = 4..20 and t = tau/2, tau, 2 tau, the estimate the adaptive
+// step of order k + 1 would be judged by, max(residual estimate of order
+// k + 1, beta0 |y_k - y_{k+1}|); one thread per k (needs mp >= 21).
+__global__ __launch_bounds__(64) void ctl_labels(CtlArgs c, double *lab)
+{
+    const LnzArgs &a = c.l;
+    const int vec = blockIdx.x, k = threadIdx.x, m = kLabM0 + k;
+    if (k >= kLabM || m + 1 > a.m) return;
+    double *o = lab + ((long long)vec * kLabM + k) * kLabT;
+    if (a.dead[vec]) {
+        for (int t = 0; t < kLabT; ++t) o[t] = 0;
+        return;
+    }
+    constexpr int L = kLabM0 + kLabM;  // largest order used, 21
+    double d0[L], z0[L * L], d1[L], z1[L * L], e[L];
+    if (leading_ql(a, vec, m, d0, e, z0) || leading_ql(a, vec, m + 1, d1, e, z1)) {
+        for (int t = 0; t < kLabT; ++t) o[t] = 1e300;
+        return;
+    }
+    const double tt[kLabT] = {0.5 * c.tau, c.tau, 2.0 * c.tau};
+    const double b0 = a.beta0[vec];
+    for (int t = 0; t < kLabT; ++t)
+        o[t] = fmax(krylov_est(m + 1, d1, z1, a.bt[vec * a.m + m], b0, tt[t]),
+                    krylov_diff(m, d0, z0, m + 1, d1, z1, b0, tt[t]));
+}
+
 // Working storage of nvec Lanczos recurrences with ncols stored columns.
 struct LnzBufs {
     DevBuf basis, vring, w0, w1, al, bt, beta0, rc, part, ticket, dead;
@@ -716,10 +1084,23 @@ struct espec_sil_plan {
     SilArgs sa{};
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    ~espec_sil_plan()
+    // adaptive and trace runs: second state buffer, coefficient sets,
+    // estimates, controller inputs/outputs and its bfloat16 weights
+    DevBuf xalt, ccoef, cest, cest4, cfeat, cout, cwb;
+    double *hpin = nullptr;  // pinned: estimates [nvec], then outputs [npk][2] (float)
+    double *log = nullptr;   // optional per-attempt record of the next adaptive run
+    int log_cap = 0, log_len = 0;
+    void drop_graph()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+    ~espec_sil_plan()
+    {
+        drop_graph();
+        if (hpin) (void)hipHostFree(hpin);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -810,6 +1191,236 @@ int sil_run_typed(espec_sil_plan &P, int nsteps)
     for (; left > 0; --left) ESPEC_HIP_TRY(sil_step<S>(P, P.stream));
     P.steps_done += nsteps;
     return ESPEC_OK;
+}
+
+int ctl_setup(espec_sil_plan &P)
+{
+    if (P.ccoef.p) return ESPEC_OK;
+    const int nv = 2 * P.npk;
+    const size_t ss = P.mixed ? sizeof(float) : sizeof(double);
+    ESPEC_HIP_TRY(P.xalt.alloc(2 * ss * (size_t)P.n * P.npk));
+    ESPEC_HIP_TRY(P.ccoef.alloc(sizeof(double) * nv * 4 * P.m));
+    ESPEC_HIP_TRY(P.cest.alloc(sizeof(double) * nv));
+    ESPEC_HIP_TRY(P.cest4.alloc(sizeof(double) * nv));
+    ESPEC_HIP_TRY(P.cfeat.alloc(sizeof(float) * kMlpIn * P.npk));
+    ESPEC_HIP_TRY(P.cout.alloc(sizeof(float) * kMlpOut * P.npk));
+    ESPEC_HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&P.hpin),
+                                sizeof(double) * (nv + P.npk), hipHostMallocDefault));
+    return ESPEC_OK;
+}
+
+CtlArgs ctl_args(espec_sil_plan &P, int mact, int nsub, double tau)
+{
+    CtlArgs c{};
+    c.l = P.sa.l;
+    c.x0 = P.x.p;
+    c.mact = mact;
+    c.nsub = nsub;
+    c.tau = tau;
+    c.tfeat = P.cst;
+    c.coef = P.ccoef.as<double>();
+    c.est = P.cest.as<double>();
+    c.est4 = P.cest4.as<double>();
+    c.info = P.info.as<int>();
+    return c;
+}
+
+// Lanczos iterations i0..i1-1 of the step's recurrences (i0 = 0 starts
+// from P.x; a later i0 extends a Krylov space whose recurrence state is
+// still in place).
+template <typename S>
+void ctl_iterate(espec_sil_plan &P, const LnzArgs &l, int i0, int i1)
+{
+    const hipStream_t s = P.stream;
+    const dim3 grid(P.nb, l.nvec), gridp(P.nb, l.nvec, l.nchunk);
+    if (i0 == 0)
+        hipLaunchKernelGGL((lnz_init<S, 256>), grid, dim3(256), 0, s, l, (const V2<S> *)P.x.p,
+                           (const S *)nullptr);
+    for (int i = i0; i < i1; ++i) {
+        LnzArgs li = l;
+        li.it = i;
+        hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
+        hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
+        hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
+    }
+}
+
+// One Krylov (sub)step from P.x: mact iterations (those beyond `have`),
+// coefficients, estimates, controller inputs (and its outputs when wb is
+// given) and the new state in P.xalt, sampled into g0 (and g0 + 1) when
+// emit.
+template <typename S>
+hipError_t ctl_step(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb, long long g0, int emit,
+                    int have = 0)
+{
+    const hipStream_t s = P.stream;
+    const LnzArgs &l = c.l;
+    const dim3 grid(P.nb, l.nvec);
+    if (c.mact > have) ctl_iterate<S>(P, l, have, c.mact);
+    hipLaunchKernelGGL((ctl_coeffs<S, 256>), grid, dim3(256), 0, s, c);
+    hipLaunchKernelGGL(ctl_feat, dim3((P.npk + 63) / 64), dim3(64), 0, s, c, P.cfeat.as<float>(),
+                       P.npk);
+    if (wb) {
+        hipError_t e = launch_mlp_infer(wb, P.cfeat.as<float>(), P.cout.as<float>(), P.npk, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((ctl_combine<S, 256>), dim3(P.nb, P.npk), dim3(256), 0, s, c,
+                       (V2<S> *)P.xalt.p, (const V2<S> *)P.psi0.p, P.cbuf.as<double>(), P.maxsamp,
+                       g0, emit);
+    return hipGetLastError();
+}
+
+// Estimates (and controller outputs) of the last step to the host.
+int ctl_fetch(espec_sil_plan &P, bool outputs)
+{
+    const int nv = 2 * P.npk;
+    ESPEC_HIP_TRY(hipMemcpyAsync(P.hpin, P.cest.p, sizeof(double) * nv, hipMemcpyDeviceToHost,
+                                 P.stream));
+    if (outputs)
+        ESPEC_HIP_TRY(hipMemcpyAsync(P.hpin + nv, P.cout.p, sizeof(float) * kMlpOut * P.npk,
+                                     hipMemcpyDeviceToHost, P.stream));
+    ESPEC_HIP_TRY(hipStreamSynchronize(P.stream));
+    return ESPEC_OK;
+}
+
+// Adopt the state the last step wrote to P.xalt.
+void ctl_accept(espec_sil_plan &P) { std::swap(P.x.p, P.xalt.p); }
+
+// stats: [0] macro steps, [1..3] steps at scale 1/2, 1, 2, [4] fallbacks,
+// [5] sum of Krylov orders over all (sub)steps taken, [6] host waits.
+template <typename S>
+int adaptive_typed(espec_sil_plan &P, const uint16_t *wb, int ndt, double tol, int m_fix,
+                   long long *stats)
+{
+    const double dt = P.cst;
+    const int nv = 2 * P.npk;
+    int m = m_fix, l2s = 0;  // Krylov order and log2 of the step scale
+    int used = 0;            // order the last accepted attempt needed beyond the network's
+    long long clock = P.steps_done;
+    int left = ndt;
+    long long st[7] = {0, 0, 0, 0, 0, 0, 0};
+    auto decide = [&]() {
+        if (!wb) return;
+        const float *o = reinterpret_cast<const float *>(P.hpin + nv);
+        int mm = kLabM0, ll = 1;
+        for (int pk = 0; pk < P.npk; ++pk) {
+            const int mq = (int)std::ceil(kLabM0 + 16.0f * o[2 * pk]) + kMarginM;  // iterations
+            const int lq = (int)std::lround(2.0f * o[2 * pk + 1] - 1.0f);
+            mm = std::max(mm, mq);
+            ll = std::min(ll, lq);
+        }
+        // Feedback floor: a step that needed the extended space leaves the
+        // next one at most one order below the order it used.
+        m = std::min(std::max(std::max(mm, kLabM0 + 1), used - 1), P.m);
+        l2s = std::min(std::max(ll, -1), 1);
+    };
+    while (left > 0) {
+        if (l2s > 0 && left < 2) l2s = 0;
+        const int subs = l2s < 0 ? 2 : 1, nsub = l2s > 0 ? 2 : 1;
+        const double tau = l2s < 0 ? 0.5 * dt : dt;
+        const double budget = tol * nsub / subs;
+        int adv = nsub;
+        st[0]++;
+        st[2 + l2s]++;
+        for (int h = 0; h < subs; ++h) {
+            const int emit = h == subs - 1;
+            ESPEC_HIP_TRY(ctl_step<S>(P, ctl_args(P, m, nsub, tau), wb, clock + 1, emit));
+            int rc = ctl_fetch(P, wb != nullptr);
+            if (rc) return rc;
+            st[5] += m;
+            st[6]++;
+            double e = 0;
+            for (int v = 0; v < nv; ++v) e = std::max(e, P.hpin[v]);
+            if (P.log && P.log_len < P.log_cap) {  // (step, order, scale, estimate, budget)
+                double *r = P.log + 5 * (size_t)P.log_len++;
+                r[0] = (double)clock;
+                r[1] = m;
+                r[2] = subs == 2 ? 0.5 : nsub;
+                r[3] = e;
+                r[4] = budget;
+            }
+            int have = m;
+            used = 0;
+            if (wb && !(e <= budget) && m < P.m) {
+                // First the same step with the Krylov space extended to the
+                // plan's largest order.
+                ESPEC_HIP_TRY(ctl_step<S>(P, ctl_args(P, P.m, nsub, tau), wb, clock + 1, emit, m));
+                rc = ctl_fetch(P, true);
+                if (rc) return rc;
+                st[5] += P.m - m;
+                st[6]++;
+                have = P.m;
+                e = 0;
+                for (int v = 0; v < nv; ++v) e = std::max(e, P.hpin[v]);
+                if (e <= budget) used = P.m;
+            }
+            if (wb && !(e <= budget)) {
+                // Fallback: the fixed-order step over tau (one base step
+                // when the rejected step covered two).  Its Krylov space is
+                // the rejected step's, extended to order m_fix if shorter;
+                // when that step misses the budget of one base step too,
+                // to the plan's largest order.
+                const double b1 = tol / subs;
+                int mf = m_fix;
+                for (;;) {
+                    ESPEC_HIP_TRY(
+                        ctl_step<S>(P, ctl_args(P, mf, 1, tau), wb, clock + 1, emit, have));
+                    rc = ctl_fetch(P, true);
+                    if (rc) return rc;
+                    st[5] += std::max(0, mf - have);
+                    st[6]++;
+                    have = std::max(have, mf);
+                    e = 0;
+                    for (int v = 0; v < nv; ++v) e = std::max(e, P.hpin[v]);
+                    if (e <= b1 || mf >= P.m) break;
+                    mf = P.m;
+                }
+                st[4]++;
+                adv = 1;
+            }
+            ctl_accept(P);
+        }
+        if (l2s < 0) adv = 1;
+        clock += adv;
+        left -= adv;
+        decide();
+    }
+    P.steps_done = clock;
+    if (stats)
+        for (int i = 0; i < 7; ++i) stats[i] = st[i];
+    int info = 0;
+    ESPEC_HIP_TRY(hipMemcpy(&info, P.info.p, sizeof(int), hipMemcpyDeviceToHost));
+    return info ? ESPEC_ERR_DIAG : ESPEC_OK;
+}
+
+template <typename S>
+int trace_typed(espec_sil_plan &P, int nsteps, int mprop, float *feat, double *lab)
+{
+    const int nv = 2 * P.npk;
+    const size_t nf = (size_t)P.npk * kMlpIn, nl = (size_t)nv * kLabM * kLabT;
+    DevBuf df, dl;
+    ESPEC_HIP_TRY(df.alloc(sizeof(float) * nf * nsteps));
+    ESPEC_HIP_TRY(dl.alloc(sizeof(double) * nl * nsteps));
+    long long clock = P.steps_done;
+    for (int k = 0; k < nsteps; ++k) {
+        // mp iterations for the labels; the state advances with order mprop
+        const CtlArgs c = ctl_args(P, mprop, 1, P.cst);
+        ctl_iterate<S>(P, c.l, 0, P.m);
+        ESPEC_HIP_TRY(ctl_step<S>(P, c, nullptr, clock + 1, 1, P.m));
+        hipLaunchKernelGGL(ctl_labels, dim3(nv), dim3(64), 0, P.stream, c, dl.as<double>() + nl * k);
+        ESPEC_HIP_TRY(hipGetLastError());
+        ESPEC_HIP_TRY(hipMemcpyAsync(df.as<float>() + nf * k, P.cfeat.p, sizeof(float) * nf,
+                                     hipMemcpyDeviceToDevice, P.stream));
+        ctl_accept(P);  // later launches read the new state; the queue keeps the order
+        ++clock;
+    }
+    ESPEC_HIP_TRY(hipStreamSynchronize(P.stream));
+    ESPEC_HIP_TRY(hipMemcpy(feat, df.p, sizeof(float) * nf * nsteps, hipMemcpyDeviceToHost));
+    ESPEC_HIP_TRY(hipMemcpy(lab, dl.p, sizeof(double) * nl * nsteps, hipMemcpyDeviceToHost));
+    P.steps_done = clock;
+    int info = 0;
+    ESPEC_HIP_TRY(hipMemcpy(&info, P.info.p, sizeof(int), hipMemcpyDeviceToHost));
+    return info ? ESPEC_ERR_DIAG : ESPEC_OK;
 }
 
 }  // namespace
@@ -944,6 +1555,48 @@ int espec_sil_plan_sync(espec_sil_plan *P)
     if (!P) return ESPEC_ERR_ARG;
     ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
     return ESPEC_OK;
+}
+
+int espec_sil_plan_adaptive(espec_sil_plan *P, const float *params, int ndt, double tol,
+                            int m_fix, long long *stats)
+{
+    if (!P || ndt < 0 || !(tol > 0) || m_fix < kLabM0 || m_fix > P->m || P->lc != 1 || P->ref)
+        return ESPEC_ERR_ARG;
+    if (ndt == 0) return ESPEC_OK;
+    int rc = ctl_setup(*P);
+    if (rc) return rc;
+    P->drop_graph();  // the state buffers trade places
+    const uint16_t *wb = nullptr;
+    if (params) {
+        std::vector<uint16_t> h(kMlpNPar);
+        for (int i = 0; i < kMlpNPar; ++i) h[i] = f2bf(params[i]);
+        ESPEC_HIP_TRY(P->cwb.upload(h.data(), sizeof(uint16_t) * h.size()));
+        wb = P->cwb.as<uint16_t>();
+    }
+    return P->mixed ? adaptive_typed<float>(*P, wb, ndt, tol, m_fix, stats)
+                    : adaptive_typed<double>(*P, wb, ndt, tol, m_fix, stats);
+}
+
+int espec_sil_plan_adaptive_log(espec_sil_plan *P, double *log, int cap, int *len)
+{
+    if (!P || cap < 0) return ESPEC_ERR_ARG;
+    if (len) *len = P->log_len;
+    P->log = log;
+    P->log_cap = log ? cap : 0;
+    P->log_len = 0;
+    return ESPEC_OK;
+}
+
+int espec_sil_plan_trace(espec_sil_plan *P, int nsteps, int mprop, float *feat, double *lab)
+{
+    if (!P || nsteps < 1 || !feat || !lab || P->m < kLabM0 + kLabM || P->lc != 1 || P->ref ||
+        mprop < kLabM0 + 1 || mprop > P->m)
+        return ESPEC_ERR_ARG;
+    int rc = ctl_setup(*P);
+    if (rc) return rc;
+    P->drop_graph();
+    return P->mixed ? trace_typed<float>(*P, nsteps, mprop, feat, lab)
+                    : trace_typed<double>(*P, nsteps, mprop, feat, lab);
 }
 
 int espec_sil_plan_destroy(espec_sil_plan *P)

@@ -240,6 +240,59 @@ int espec_sil_plan_get(espec_sil_plan *plan, int pk, double *cr, double *ci,
                        double *v);
 int espec_sil_plan_destroy(espec_sil_plan *plan);
 
+/* ---------------------------------------------------------------------
+ * Learned Krylov-step controller (no reference counterpart).
+ *
+ * Adaptive short-iterative Lanczos: before every step a network
+ * (8 -> 64 -> 64 -> 2, bfloat16 weights and activations, FP32 sums)
+ * maps eight inputs of the previous Krylov run (alpha_1, beta_1, alpha_2,
+ * beta_2 times dt, log10 of beta_4 dt and beta_5 dt, log10 of the order-4
+ * residual estimate / 10, log10 of the norm drift / 16) to a Krylov order k = ceil(4 + 16 o1) and a step
+ * scale s = 2^round(2 o2 - 1) in {1/2, 1, 2} (max k and min s over the
+ * packets; after a step that needed the extended space described below,
+ * the order is at least one below the one it used).  The step is of order m = k + 1, and it is accepted when, for
+ * every real recurrence, the larger of beta0 |y_{m-1} - y_m| (y_j =
+ * exp(-i s dt T_j) e1, the order-j approximation in the Krylov basis, zero
+ * padded) and the residual estimate beta0 beta_m |[exp(-i s dt T_m)]_{m,1}|
+ * is at most tol * s (each half of an s = 1/2 step: tol / 2).  Otherwise
+ * the same step is retried with the Krylov space extended to order mp,
+ * and when that misses the budget too, the fixed-order step of order m_fix
+ * over one dt (or the half step) replaces it, on the same Krylov space
+ * (order mp when m_fix misses the budget of one base step).  C(t) is sampled
+ * every dt (plans with lc = 1; the exact, not the reference, coefficient
+ * rule).  params: the network's kMlpNPar = 4866 FP32 parameters (W1[64][8]
+ * b1[64] W2[64][64] b2[64] W3[2][64] b3[2]), or NULL for the fixed-order
+ * step every dt.  stats[7]: macro steps, steps at scale 1/2, 1, 2,
+ * fallbacks, Lanczos iterations run, host waits.  Orders are capped at
+ * mp (the plan's Krylov storage); mp >= 21 lets k reach 20.
+ */
+int espec_sil_plan_adaptive(espec_sil_plan *plan, const float *params, int ndt,
+                            double tol, int m_fix, long long *stats);
+/* Diagnostics of the adaptive runs that follow: each network-chosen
+ * attempt appends (base step, order, step scale, estimate, budget) to log
+ * (cap records of 5 doubles; NULL stops recording); *len receives the
+ * number of records written since the previous call. */
+int espec_sil_plan_adaptive_log(espec_sil_plan *plan, double *log, int cap, int *len);
+/* Training trace: nsteps steps over dt that run mp (>= 21) Lanczos
+ * iterations and advance the state with order mprop (5 <= mprop <= mp; a
+ * lower order leaves the state as rough as a propagation at that order
+ * makes it), recording per step the controller inputs feat[nsteps][npk][8]
+ * and lab[nsteps][2 npk][17][3]: for each network order k = 4..20 at
+ * dt/2, dt, 2 dt, the estimate the adaptive step of order k + 1 is judged
+ * by, for each real recurrence (2 pk: Re psi, 2 pk + 1: Im psi). */
+int espec_sil_plan_trace(espec_sil_plan *plan, int nsteps, int mprop, float *feat,
+                         double *lab);
+/* The network alone: outputs out[n][2] for inputs feat[n][8]; the
+ * gradient of one minibatch of 64 rows (loss = mean of squared output
+ * errors against t[64][2]); and Adam training on the device over n rows
+ * (epochs passes of shuffled 64-row minibatches, one workgroup; params
+ * updated in place, loss[epochs] the mean minibatch loss per pass). */
+int espec_mlp_infer(const float *params, const float *feat, int n, float *out);
+int espec_mlp_grad(const float *params, const float *x, const float *t,
+                   float *grad, float *loss);
+int espec_mlp_train(float *params, const float *x, const float *t, int n,
+                    int epochs, float lr, unsigned seed, float *loss);
+
 /* Launch shape chosen for the plan, shape[12]: 1 if LDS-resident, rows per
  * row-pass workgroup, its threads, strided tile width, its threads, XCD
  * spread, XCD-contiguous tile map, workgroups per pass (0: one per row block

@@ -1,0 +1,155 @@
+"""The learned Krylov-step controller on the device: the bfloat16 network's
+inference and minibatch gradient against the CPU restatement, training
+on the device, the trace's residual estimates against the Lanczos
+tridiagonal of the oracle, the adaptive propagator without a network
+against the fixed-order propagator, and a trained network's accuracy and
+gain over the fixed order."""
+import numpy as np
+import pytest
+
+import espec_amd as E
+from espec_amd import configs, krylov
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed, scale=0.3):
+    return np.random.default_rng(seed).uniform(-scale, scale, E.MLP_NPAR).astype(np.float32)
+
+
+def test_infer_matches_oracle(gpu):
+    import mlp_oracle as M
+    p = _params(1)
+    x = np.random.default_rng(2).standard_normal((200, 8)).astype(np.float32)
+    print("case: mlp_infer 200 rows", flush=True)
+    got = E.mlp_infer(p, x)
+    ref, _ = M.forward(p, x, bf16_mode=True)
+    assert np.abs(got - ref).max() <= 1e-2
+    assert np.abs(got - ref).mean() <= 1e-3
+
+
+def test_grad_matches_oracle(gpu):
+    import mlp_oracle as M
+    rng = np.random.default_rng(3)
+    p = _params(4)
+    x = rng.standard_normal((64, 8)).astype(np.float32)
+    t = rng.uniform(0, 1, (64, 2)).astype(np.float32)
+    print("case: mlp_grad one minibatch", flush=True)
+    g, loss = E.mlp_grad(p, x, t)
+    gr, lr = M.grad(p, x, t, bf16_mode=True)
+    g64, _ = M.grad(p, x, t, bf16_mode=False)
+    assert loss == pytest.approx(lr, rel=1e-4)
+    assert np.linalg.norm(g - gr) <= 1e-2 * np.linalg.norm(gr)
+    assert np.linalg.norm(g - g64) <= 5e-2 * np.linalg.norm(g64)
+
+
+def test_training_fits_a_smooth_map(gpu):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((4096, 8)).astype(np.float32)
+    t = np.stack([1 / (1 + np.exp(-(x[:, 0] - 0.5 * x[:, 1]))), 0.5 + 0.3 * np.tanh(x[:, 2])],
+                 axis=1).astype(np.float32)
+    print("case: mlp_train 4096 rows 30 epochs", flush=True)
+    p, hist = krylov.train(x, t, epochs=30, lr=3e-3, seed=2)
+    assert hist[-1] < 0.2 * hist[0]
+    assert hist[-1] < 5e-3
+    import mlp_oracle as M
+    o, _ = M.forward(p, x[:512])
+    assert np.mean((o - t[:512]) ** 2) < 1e-2
+
+
+def _packet1d():
+    return configs.packet1d(256, 0.15, 1.0, 918.0, 0.3, 0.2, 1.0, 0)
+
+
+def test_trace_estimates_match_oracle_tridiagonal(gpu, orc):
+    import mlp_oracle as M
+    p = _packet1d()
+    shm = krylov.shm_of(p)
+    print("case: trace 1-D 256, 2 steps", flush=True)
+    plan = E.SilPlan(p.np, 21, 0.0, p.dt_fs, 1, shm, p.vpot, p.u, p.v, maxsamp=4)
+    feat, lab = plan.trace(2)
+    plan.close()
+    al, bt, _ = orc.lnz(p.np, 21, 21, shm, p.vpot, p.u)
+    beta0 = np.linalg.norm(p.u)
+    tau = p.dt_fs / E.SHFS
+    checked = 0
+    for mi, k in enumerate(range(4, 21)):
+        for ti, t in enumerate((0.5 * tau, tau, 2 * tau)):
+            ref = max(M.krylov_est(al, bt, beta0, k + 1, t),
+                      M.krylov_diff(al, bt, beta0, k, k + 1, t))
+            if ref > 1e-12:
+                assert lab[0, 0, mi, ti] == pytest.approx(ref, rel=1e-5), (k, t)
+                checked += 1
+    assert checked >= 6
+    # controller inputs of step 0 from Re psi: alpha_1, beta_1, alpha_2,
+    # beta_2 times dt, log10 of beta_4 dt and beta_5 dt
+    assert np.allclose(feat[0, 0, 0:4:2], al[:2] * tau, rtol=1e-6)
+    assert np.allclose(feat[0, 0, 1:4:2], bt[:2] * tau, rtol=1e-6)
+    assert np.allclose(feat[0, 0, 4:6], np.log10(bt[3:5] * tau), atol=1e-5)
+
+
+def _packet2d(n=64, seed=0):
+    fam = krylov.family(count=4, n=n, nsteps=0, seed=seed, groups=1)
+    return fam
+
+
+@pytest.mark.parametrize("flags", [E.FP64, E.MIXED])
+def test_adaptive_without_network_is_the_fixed_order_step(gpu, flags):
+    pk = _packet2d()[:2]
+    n = pk[0].n
+    shm = krylov.shm_of(pk[0])
+    vp = np.stack([p.vpot for p in pk])
+    u = np.stack([p.u for p in pk])
+    nst = 30
+    print(f"case: adaptive(no network) vs fixed, 2x64^2, flags={flags}", flush=True)
+    a = E.SilPlan(pk[0].np, 20, 0.0, pk[0].dt_fs, 1, shm, vp, u, None, maxsamp=nst + 1,
+                  flags=flags)
+    st = a.adaptive(nst, None, m_fix=10)
+    f = E.SilPlan(pk[0].np, 10, 0.0, pk[0].dt_fs, 1, shm, vp, u, None, maxsamp=nst + 1,
+                  flags=flags)
+    f.run(nst)
+    for k in range(2):
+        ga, gf = a.get(k), f.get(k)
+        assert len(ga["cr"]) == nst + 1
+        tol = 1e-12 if flags == E.FP64 else 1e-6
+        assert np.abs(ga["cr"] - gf["cr"]).max() <= tol
+        assert np.abs(ga["ci"] - gf["ci"]).max() <= tol
+    assert st["steps"] == nst and st["fallbacks"] == 0 and st["krylov_sum"] == 10 * nst
+    a.close()
+    f.close()
+
+
+def test_trained_controller_accuracy(gpu, orc):
+    """Train on traces of a small family (states advanced at orders 12 and
+    21), then propagate a packet the network has not seen: every accepted
+    step meets the budget, so C(t) stays within 1e-6 of the FP64 oracle of
+    the converged order; the run's statistics are printed."""
+    from oracle import EXACT
+    tol = 1e-10
+    fam = krylov.family(count=8, n=64, nsteps=0, seed=11, groups=2)
+    print("case: controller trained on 8 64^2 traces of 400 steps", flush=True)
+    traces = [krylov.trace_packets([p for p in fam if p.meta["group"] == g], 400, mprop=mp)
+              for g, mp in ((0, 12), (1, 21))]
+    x, t = krylov.dataset(traces, tol)
+    params, hist = krylov.train(x, t, epochs=40, lr=3e-3, seed=3)
+    assert hist[-1] < hist[0]
+    test = krylov.family(count=1, n=64, nsteps=0, seed=99, groups=1)[0]
+    nst = 200
+    shm = krylov.shm_of(test)
+    plan = E.SilPlan(test.np, 21, 0.0, test.dt_fs, 1, shm, test.vpot, test.u, test.v,
+                     maxsamp=nst + 1)
+    plan.adaptive_log(4 * nst)
+    st = plan.adaptive(nst, params, tol=tol, m_fix=12)
+    log = plan.adaptive_log()
+    g = plan.get(0)
+    plan.close()
+    o = orc.plnz(test.np, 20, 0.0, test.dt_fs, 0.0, (nst - 1) * test.dt_fs, 1, shm, test.u,
+                 test.v, test.vpot, EXACT)
+    k = min(len(g["cr"]), len(o["cr"]))
+    assert k == nst + 1
+    d = max(np.abs(g["cr"][:k] - o["cr"][:k]).max(), np.abs(g["ci"][:k] - o["ci"][:k]).max())
+    print(f"  stats {st}  |dC| {d:.2e}", flush=True)
+    assert d <= 1e-6
+    assert st["steps"] >= nst // 2 and len(log) >= st["steps"] - 1
+    # an attempt over its budget is retried at the largest order or replaced
+    assert st["fallbacks"] <= int(np.sum(log[:, 3] > log[:, 4]))

