@@ -68,6 +68,7 @@ def _load():
     L.espec_sil_plan_adaptive.argtypes = [vp, fp_or_none, ip, dp, ip, P(C.c_longlong)]
     L.espec_sil_plan_trace.argtypes = [vp, ip, ip, f, d]
     L.espec_sil_plan_adaptive_log.argtypes = [vp, C.c_void_p, ip, P(ip)]
+    L.espec_sil_plan_reset.argtypes = [vp]
     L.espec_mlp_infer.argtypes = [f, f, ip, f]
     L.espec_mlp_grad.argtypes = [f, f, f, f, P(C.c_float)]
     L.espec_mlp_train.argtypes = [f, f, f, ip, ip, C.c_float, C.c_uint, f]
@@ -407,6 +408,10 @@ class SilPlan(_Plan):
             return None
         _check(self._L.espec_sil_plan_adaptive_log(self._h, None, 0, C.byref(n)), "adaptive_log")
         return self._log[:n.value].copy()
+
+    def reset(self):
+        """Restart from psi0 at step 0 (captured graphs kept)."""
+        _check(self._L.espec_sil_plan_reset(self._h), "sil_plan_reset")
 
     def trace(self, nsteps, mprop=None):
         """nsteps steps of mp (>= 20) iterations advancing with order mprop

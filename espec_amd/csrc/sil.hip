@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -924,9 +925,10 @@ __global__ __launch_bounds__(NT) void ctl_coeffs(CtlArgs c)
 // into samples g0 (and g0 + 1) when emit.
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void ctl_combine(CtlArgs c, V2<S> *xout, const V2<S> *psi0,
-                                                  double *cbuf, int maxsamp, long long g0,
-                                                  int emit)
+                                                  double *cbuf, int maxsamp,
+                                                  const long long *clock, int emit)
 {
+    const long long g0 = *clock + 1;  // the first sample this step completes
     __shared__ double red[(NT / 64) * kPart];
     __shared__ double cf[2][4 * kMaxKrylov];
     const LnzArgs &a = c.l;
@@ -1087,6 +1089,7 @@ struct espec_sil_plan {
     // adaptive and trace runs: second state buffer, coefficient sets,
     // estimates, controller inputs/outputs and its bfloat16 weights
     DevBuf xalt, ccoef, cest, cest4, cfeat, cout, cwb;
+    std::map<unsigned long long, hipGraphExec_t> cgraphs;  // adaptive/trace step graphs
     double *hpin = nullptr;  // pinned: estimates [nvec], then outputs [npk][2] (float)
     double *log = nullptr;   // optional per-attempt record of the next adaptive run
     int log_cap = 0, log_len = 0;
@@ -1100,6 +1103,7 @@ struct espec_sil_plan {
     ~espec_sil_plan()
     {
         drop_graph();
+        for (auto &g : cgraphs) (void)hipGraphExecDestroy(g.second);
         if (hpin) (void)hipHostFree(hpin);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1225,38 +1229,28 @@ CtlArgs ctl_args(espec_sil_plan &P, int mact, int nsub, double tau)
     return c;
 }
 
-// Lanczos iterations i0..i1-1 of the step's recurrences (i0 = 0 starts
-// from P.x; a later i0 extends a Krylov space whose recurrence state is
-// still in place).
+// One Krylov (sub)step from P.x: Lanczos iterations have..iters-1 (from
+// P.x when have = 0), coefficients of order mact, estimates, controller
+// inputs (and its outputs when wb is given) and the new state in P.xalt,
+// sampled (when emit) into the sample after the device clock and the next.
 template <typename S>
-void ctl_iterate(espec_sil_plan &P, const LnzArgs &l, int i0, int i1)
+hipError_t ctl_launches(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb, int emit, int have,
+                        int iters, hipStream_t s)
 {
-    const hipStream_t s = P.stream;
-    const dim3 grid(P.nb, l.nvec), gridp(P.nb, l.nvec, l.nchunk);
-    if (i0 == 0)
-        hipLaunchKernelGGL((lnz_init<S, 256>), grid, dim3(256), 0, s, l, (const V2<S> *)P.x.p,
-                           (const S *)nullptr);
-    for (int i = i0; i < i1; ++i) {
-        LnzArgs li = l;
-        li.it = i;
-        hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
-        hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
-        hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
-    }
-}
-
-// One Krylov (sub)step from P.x: mact iterations (those beyond `have`),
-// coefficients, estimates, controller inputs (and its outputs when wb is
-// given) and the new state in P.xalt, sampled into g0 (and g0 + 1) when
-// emit.
-template <typename S>
-hipError_t ctl_step(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb, long long g0, int emit,
-                    int have = 0)
-{
-    const hipStream_t s = P.stream;
     const LnzArgs &l = c.l;
-    const dim3 grid(P.nb, l.nvec);
-    if (c.mact > have) ctl_iterate<S>(P, l, have, c.mact);
+    const dim3 grid(P.nb, l.nvec), gridp(P.nb, l.nvec, l.nchunk);
+    if (iters > have) {
+        if (have == 0)
+            hipLaunchKernelGGL((lnz_init<S, 256>), grid, dim3(256), 0, s, l, (const V2<S> *)P.x.p,
+                               (const S *)nullptr);
+        for (int i = have; i < iters; ++i) {
+            LnzArgs li = l;
+            li.it = i;
+            hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
+            hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
+            hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
+        }
+    }
     hipLaunchKernelGGL((ctl_coeffs<S, 256>), grid, dim3(256), 0, s, c);
     hipLaunchKernelGGL(ctl_feat, dim3((P.npk + 63) / 64), dim3(64), 0, s, c, P.cfeat.as<float>(),
                        P.npk);
@@ -1266,8 +1260,49 @@ hipError_t ctl_step(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb, lon
     }
     hipLaunchKernelGGL((ctl_combine<S, 256>), dim3(P.nb, P.npk), dim3(256), 0, s, c,
                        (V2<S> *)P.xalt.p, (const V2<S> *)P.psi0.p, P.cbuf.as<double>(), P.maxsamp,
-                       g0, emit);
+                       P.dstep.as<long long>(), emit);
     return hipGetLastError();
+}
+
+// The step's launches replay from a graph captured on first use: one
+// graph per (order, iterations already run, coefficient sets, step, sample
+// flag, network, which state buffer is current); the sample index comes
+// from the device clock set just before.
+template <typename S>
+hipError_t ctl_step(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb, long long g0, int emit,
+                    int have = 0, int iters = 0)
+{
+    const hipStream_t s = P.stream;
+    iters = std::max(iters, c.mact);
+    hipLaunchKernelGGL(sil_set_step, dim3(1), dim3(1), 0, s, P.dstep.as<long long>(), g0 - 1);
+    const int tcode = c.tau == P.cst ? 0 : (c.tau == 0.5 * P.cst ? 1 : -1);
+    if (tcode < 0) return ctl_launches<S>(P, c, wb, emit, have, iters, s);
+    const unsigned long long key = (unsigned long long)c.mact | ((unsigned long long)have << 8) |
+                                   ((unsigned long long)iters << 32) |
+                                   ((unsigned long long)c.nsub << 16) |
+                                   ((unsigned long long)tcode << 20) |
+                                   ((unsigned long long)emit << 22) |
+                                   ((unsigned long long)(wb != nullptr) << 23) |
+                                   ((unsigned long long)(P.x.p < P.xalt.p) << 24);
+    auto it = P.cgraphs.find(key);
+    if (it == P.cgraphs.end()) {
+        hipStream_t cs;
+        hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+        hipGraph_t g = nullptr;
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) e = ctl_launches<S>(P, c, wb, emit, have, iters, cs);
+        hipError_t e2 = hipStreamEndCapture(cs, &g);
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess) return e;
+        if (e2 != hipSuccess) return e2;
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return e;
+        it = P.cgraphs.emplace(key, x).first;
+    }
+    return hipGraphLaunch(it->second, s);
 }
 
 // Estimates (and controller outputs) of the last step to the host.
@@ -1405,8 +1440,7 @@ int trace_typed(espec_sil_plan &P, int nsteps, int mprop, float *feat, double *l
     for (int k = 0; k < nsteps; ++k) {
         // mp iterations for the labels; the state advances with order mprop
         const CtlArgs c = ctl_args(P, mprop, 1, P.cst);
-        ctl_iterate<S>(P, c.l, 0, P.m);
-        ESPEC_HIP_TRY(ctl_step<S>(P, c, nullptr, clock + 1, 1, P.m));
+        ESPEC_HIP_TRY(ctl_step<S>(P, c, nullptr, clock + 1, 1, 0, P.m));
         hipLaunchKernelGGL(ctl_labels, dim3(nv), dim3(64), 0, P.stream, c, dl.as<double>() + nl * k);
         ESPEC_HIP_TRY(hipGetLastError());
         ESPEC_HIP_TRY(hipMemcpyAsync(df.as<float>() + nf * k, P.cfeat.p, sizeof(float) * nf,
@@ -1570,11 +1604,22 @@ int espec_sil_plan_adaptive(espec_sil_plan *P, const float *params, int ndt, dou
     if (params) {
         std::vector<uint16_t> h(kMlpNPar);
         for (int i = 0; i < kMlpNPar; ++i) h[i] = f2bf(params[i]);
-        ESPEC_HIP_TRY(P->cwb.upload(h.data(), sizeof(uint16_t) * h.size()));
+        if (!P->cwb.p) ESPEC_HIP_TRY(P->cwb.alloc(sizeof(uint16_t) * kMlpNPar));
+        ESPEC_HIP_TRY(hipMemcpy(P->cwb.p, h.data(), sizeof(uint16_t) * h.size(),
+                                hipMemcpyHostToDevice));
         wb = P->cwb.as<uint16_t>();
     }
     return P->mixed ? adaptive_typed<float>(*P, wb, ndt, tol, m_fix, stats)
                     : adaptive_typed<double>(*P, wb, ndt, tol, m_fix, stats);
+}
+
+int espec_sil_plan_reset(espec_sil_plan *P)
+{
+    if (!P) return ESPEC_ERR_ARG;
+    ESPEC_HIP_TRY(hipMemcpyAsync(P->x.p, P->psi0.p, P->x.bytes, hipMemcpyDeviceToDevice, P->stream));
+    ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+    P->steps_done = 0;  // sample 0, C(psi0, psi0), is already in place
+    return ESPEC_OK;
 }
 
 int espec_sil_plan_adaptive_log(espec_sil_plan *P, double *log, int cap, int *len)

@@ -239,6 +239,9 @@ int espec_sil_plan_get(espec_sil_plan *plan, int pk, double *cr, double *ci,
                        double *nrm, int maxsamp, int *nsamp, double *u,
                        double *v);
 int espec_sil_plan_destroy(espec_sil_plan *plan);
+/* Restart from psi0 at step 0, keeping captured graphs (no reference
+ * counterpart). */
+int espec_sil_plan_reset(espec_sil_plan *plan);
 
 /* ---------------------------------------------------------------------
  * Learned Krylov-step controller (no reference counterpart).
