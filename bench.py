@@ -17,12 +17,13 @@ Also reported, under "batched": the 2048-packet uneven batch (1-D and 2-D
 grids, per-packet potentials, states and masses; 1000 steps of dt = 0.5
 a.u.) on the mixed and FP64 paths, with its whole-job rate against the HBM
 peak at 28 B per grid-point-step and a C(t) check of sampled packets.
+Under "config3": the 128^3 surface, 1000 steps, mixed and FP64.
 Under "krylov_controller": the learned Krylov-order/step controller
 (trained on the device) against fixed-order short-iterative Lanczos on the
 256x256 surface, wall time and |dC| against the order-20 run.
 
     python bench.py [--gpus 1] [--steps 2000] [--warmup 100] [--batch-steps 1000]
-                    [--krylov-steps 2000]
+                    [--config3-steps 1000] [--krylov-steps 2000]
 """
 import argparse
 import json
@@ -155,6 +156,48 @@ def batched(E, configs, steps, check):
     return out
 
 
+def config3(E, configs, steps, check):
+    """The 128^3 three-mode surface (config 3 of the workload set): split-
+    operator FFT, dt = 1 a.u., C(t) every step, `steps` steps on the mixed
+    and FP64 paths, with inputs resident.  |dC| of the mixed run against
+    the FP64 run over all steps and, with `check`, of both against the FP64
+    oracle over the first 10 steps."""
+    c = configs.config3(nsteps=steps)
+    out = {"workload": "128^3 three-mode Morse split-operator FFT (3-D)", "steps": steps}
+    res = {}
+    for name, flags in (("mixed", E.MIXED | E.EXACT), ("fp64", E.FP64 | E.EXACT)):
+        plan = E.SpoPlan(c.np, c.dt_fs, 1, c.sh, c.sm, c.vpot, c.u, None, maxsamp=steps + 1,
+                         flags=flags)
+        plan.run(40)
+        plan.sync()
+        plan.reset()
+        t0 = time.perf_counter()
+        plan.run(steps)
+        plan.sync()
+        sec = time.perf_counter() - t0
+        res[name] = plan.get(0)
+        gps = c.n * steps / sec
+        bpp = 28.0 if name == "mixed" else 56.0
+        out[name] = {"seconds": sec, "value": gps, "unit": "grid-point-steps/s",
+                     "algorithmic_GBps": gps * bpp / 1e9, "frac_of_hbm_peak": gps * bpp / HBM_PEAK,
+                     "shape": plan.shape()}
+        log(f"[bench] config3 {name}: {sec:.3f} s, {gps:.3e} gp-steps/s")
+        plan.close()
+    out["max_abs_dC_mixed_vs_fp64"] = float(max(np.abs(res["mixed"]["cr"] - res["fp64"]["cr"]).max(),
+                                                np.abs(res["mixed"]["ci"] - res["fp64"]["ci"]).max()))
+    out["mixed_over_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle, EXACT
+        o = Oracle().pspofft(c.np, c.dt_fs, 0.0, 9 * c.dt_fs, 1, c.sh, c.sm, c.u, c.v, c.vpot,
+                             EXACT)
+        k = len(o["cr"])
+        out["max_abs_dC_vs_oracle_10_steps"] = {
+            n: float(max(np.abs(res[n]["cr"][:k] - o["cr"]).max(),
+                         np.abs(res[n]["ci"][:k] - o["ci"]).max())) for n in res}
+    return out
+
+
 def krylov_controller(E, configs, steps):
     """The learned Krylov-order/step controller on short-iterative Lanczos
     propagation of the 256x256 surface (FD Hamiltonian, dt = 1 a.u., C(t)
@@ -238,6 +281,8 @@ def main():
                     help="steps of the batched 2048-packet workload (0 skips it)")
     ap.add_argument("--krylov-steps", type=int, default=2000,
                     help="steps of the learned Krylov controller comparison (0 skips it)")
+    ap.add_argument("--config3-steps", type=int, default=1000,
+                    help="steps of the 128^3 workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagation of one packet (or a batch of "
@@ -349,6 +394,8 @@ def main():
     plan_64.close()
     if args.batch_steps > 0:
         out["batched"] = batched(E, configs, args.batch_steps, not args.no_cpu)
+    if args.config3_steps > 0:
+        out["config3"] = config3(E, configs, args.config3_steps, not args.no_cpu)
     if args.krylov_steps > 0:
         out["krylov_controller"] = krylov_controller(E, configs, args.krylov_steps)
     print(json.dumps(out))
