@@ -96,7 +96,9 @@ def kernel_names(shape, np0, np1, storage="float"):
     else:
         ept_r = shape["rows"] * np0 // shape["rows_nt"]
         rows = f"pass_rows<{storage}, {shape['rows_nt']}, true, true, {ept_r}, {pht}>"
-    if shape["tile_pen"]:
+    if shape["tile_pen"] == 2:
+        cols = f"pass_strided_tile<{storage}, {np1 // 16}>"
+    elif shape["tile_pen"]:
         pf = "true" if shape["pen_pf"] else "false"
         cols = f"pass_strided_pen<{storage}, {np1 // 16}, {pf}>"
     else:

@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libespec_hip.so")
 FP64, MIXED = 0x0, 0x1
 EXACT, REF = 0x0, 0x2
 PER_PACKET_MASS = 0x4
+MULTIPASS = 0x8
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 AMU = 1836.152667539    # electron masses per amu (espec.f:614)
 A0A = 0.5291772083      # angstrom per bohr (espec.f:67)

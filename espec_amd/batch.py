@@ -6,19 +6,44 @@ packets with their own potentials, initial states and masses.  run()
 enqueues every group on its own stream, so the groups execute side by
 side; sync() waits for all of them.  This replaces the reference's job
 farm (tools&scrips/parallel.x), which starts one espec.x per packet.
+
+A 2-D or 3-D group whose packets fit one workgroup's LDS can run either
+on the LDS-resident kernel (one workgroup per packet, every step in LDS)
+or on the multi-pass kernels (one pass per axis per step through HBM,
+many workgroups per packet); with kind="auto" both plans are built and
+timed over a few steps, and the faster one is kept.
 """
 import time
 from collections import OrderedDict
 
 import numpy as np
 
-from . import MIXED, EXACT, SpoPlan
+from . import MIXED, EXACT, MULTIPASS, SpoPlan
+
+
+def _step_time(plan, nsteps=40):
+    """Seconds per step of `plan` (best of two timed runs), then reset."""
+    plan.run(4)
+    plan.sync()
+    best = float("inf")
+    for _ in range(2):
+        t0 = time.perf_counter()
+        plan.run(nsteps)
+        plan.sync()
+        best = min(best, (time.perf_counter() - t0) / nsteps)
+    plan.reset()
+    return best
 
 
 class SpoBatch:
     """Split-operator propagation of a list of configs.Packet objects."""
 
-    def __init__(self, packets, flags=MIXED | EXACT, maxsamp=None, lc=1):
+    def __init__(self, packets, flags=MIXED | EXACT, maxsamp=None, lc=1, kind="auto"):
+        """kind: "auto" (time the LDS-resident and multi-pass plans where
+        both apply), "lds" (LDS-resident where the grid fits) or
+        "multipass"."""
+        if kind not in ("auto", "lds", "multipass"):
+            raise ValueError(f"kind must be auto, lds or multipass, not {kind!r}")
         self.packets = packets
         self.lc = lc
         groups = OrderedDict()
@@ -35,7 +60,18 @@ class SpoBatch:
             u = np.stack([packets[i].u for i in idx]).reshape(len(idx), n)
             v = np.stack([packets[i].v for i in idx]).reshape(len(idx), n)
             sm = np.array([packets[i].sm for i in idx], dtype=np.float64)
-            plan = SpoPlan(p0.np, p0.dt_fs, lc, p0.sh, sm, vp, u, v, maxsamp=ms, flags=flags)
+
+            def make(f):
+                return SpoPlan(p0.np, p0.dt_fs, lc, p0.sh, sm, vp, u, v, maxsamp=ms, flags=f)
+
+            plan = make(flags | (MULTIPASS if kind == "multipass" else 0))
+            if kind == "auto" and len(p0.np) > 1 and plan.shape()["lds"]:
+                alt = make(flags | MULTIPASS)
+                if _step_time(alt) < 0.97 * _step_time(plan):
+                    plan.close()
+                    plan = alt
+                else:
+                    alt.close()
             self.groups.append(dict(np=list(p0.np), n=n, idx=idx, plan=plan))
             for k, i in enumerate(idx):
                 self.where[i] = (len(self.groups) - 1, k)

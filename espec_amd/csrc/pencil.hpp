@@ -26,10 +26,66 @@ inline int spectral_index(int p, int m)
     return j * (16 / m) + q + 16 * k2;
 }
 
-// a * exp(SIGN i pi e / 8) for compile-time e.
-template <int SIGN, int E>
-__device__ __forceinline__ double2 rot16(double2 a)
+// Scalar type of a complex register type.
+template <typename V> struct ScalarOf;
+template <> struct ScalarOf<double2> { using T = double; };
+template <> struct ScalarOf<float2> { using T = float; };
+
+// Single-precision arithmetic with split constants.  A constant c is
+// carried as hi + lo (hi = float(c), lo = float(c - hi)), exact to about
+// 2^-48, so that a product with it has only data-dependent roundings.  A
+// constant rounded once to float would scale every step's result by the
+// same factor, and the norm would drift linearly with the number of steps.
+// Complex constants travel as float4 (hi.re, hi.im, lo.re, lo.im).
+__device__ __forceinline__ float2 cmul_split(float2 a, float4 w)
 {
+    float re = fmaf(-a.y, w.w, a.x * w.z);
+    re = fmaf(-a.y, w.y, re);
+    re = fmaf(a.x, w.x, re);
+    float im = fmaf(a.y, w.z, a.x * w.w);
+    im = fmaf(a.y, w.x, im);
+    im = fmaf(a.x, w.y, im);
+    return {re, im};
+}
+// a * conj(w)
+__device__ __forceinline__ float2 cmulc_split(float2 a, float4 w)
+{
+    float re = fmaf(a.y, w.w, a.x * w.z);
+    re = fmaf(a.y, w.y, re);
+    re = fmaf(a.x, w.x, re);
+    float im = fmaf(-a.x, w.w, a.y * w.z);
+    im = fmaf(-a.x, w.y, im);
+    im = fmaf(a.y, w.x, im);
+    return {re, im};
+}
+// (cos, sin) in double -> split float4.
+__device__ __forceinline__ float4 split4(double c, double s)
+{
+    const float ch = (float)c, sh = (float)s;
+    return {ch, sh, (float)(c - (double)ch), (float)(s - (double)sh)};
+}
+
+// Product with a constant in the arithmetic's constant form: double2
+// constants for double2 data, split float4 constants for float2 data.
+__device__ __forceinline__ double2 cmul_w(double2 a, double2 w) { return cmul(a, w); }
+__device__ __forceinline__ double2 cmulc_w(double2 a, double2 w) { return cmulc(a, w); }
+__device__ __forceinline__ float2 cmul_w(float2 a, float4 w) { return cmul_split(a, w); }
+__device__ __forceinline__ float2 cmulc_w(float2 a, float4 w) { return cmulc_split(a, w); }
+template <int SIGN, typename V, typename W> __device__ __forceinline__ V ctw_w(V a, W w)
+{
+    return SIGN > 0 ? cmul_w(a, w) : cmulc_w(a, w);
+}
+
+// Constant (table) type for complex register type V.
+template <typename V> struct ConstOf;
+template <> struct ConstOf<double2> { using T = double2; };
+template <> struct ConstOf<float2> { using T = float4; };
+
+// a * exp(SIGN i pi e / 8) for compile-time e.
+template <int SIGN, int E, typename V>
+__device__ __forceinline__ V rot16(V a)
+{
+    using T = typename ScalarOf<V>::T;
     constexpr int e = ((SIGN > 0 ? E : -E) % 16 + 16) % 16;
     constexpr double c8 = 0.92387953251128675613, s8 = 0.38268343236508977173,
                      r2 = 0.70710678118654752440;
@@ -39,19 +95,26 @@ __device__ __forceinline__ double2 rot16(double2 a)
     else if constexpr (e == 4) return {-a.y, a.x};
     else if constexpr (e == 8) return {-a.x, -a.y};
     else if constexpr (e == 12) return {a.y, -a.x};
-    else return {a.x * cs[e] - a.y * sn[e], a.x * sn[e] + a.y * cs[e]};
+    else if constexpr (sizeof(T) == 4) {
+        constexpr float ch = (float)cs[e], sh = (float)sn[e];
+        constexpr float cl = (float)(cs[e] - (double)ch), sl = (float)(sn[e] - (double)sh);
+        return cmul_split(a, float4{ch, sh, cl, sl});
+    } else {
+        constexpr T c = (T)cs[e], t = (T)sn[e];
+        return {a.x * c - a.y * t, a.x * t + a.y * c};
+    }
 }
 
 // In-register DFT of length L in {4, 8, 16} on v[O], v[O+1], ..., natural
 // order in and out, sign SIGN, unnormalised.
-template <int SIGN, int O>
-__device__ __forceinline__ void dft4(double2 *v)
+template <int SIGN, int O, typename V>
+__device__ __forceinline__ void dft4(V *v)
 {
     bfly4<SIGN>(v[O], v[O + 1], v[O + 2], v[O + 3]);
 }
 
-template <int SIGN, int O>
-__device__ __forceinline__ void dft8(double2 *v)
+template <int SIGN, int O, typename V>
+__device__ __forceinline__ void dft8(V *v)
 {
     // n = 2 n1 + n2: DFT4 over n1, twiddle e^{s 2pi i n2 k1 / 8}, DFT2 over n2.
     bfly4<SIGN>(v[O + 0], v[O + 2], v[O + 4], v[O + 6]);
@@ -64,7 +127,7 @@ __device__ __forceinline__ void dft8(double2 *v)
     bfly2(v[O + 4], v[O + 5]);
     bfly2(v[O + 6], v[O + 7]);
     // position 2 k1 + k2 holds X[k1 + 4 k2]
-    double2 t[8];
+    V t[8];
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1)
 #pragma unroll
@@ -73,8 +136,8 @@ __device__ __forceinline__ void dft8(double2 *v)
     for (int i = 0; i < 8; ++i) v[O + i] = t[i];
 }
 
-template <int SIGN, int O>
-__device__ __forceinline__ void dft16(double2 *v)
+template <int SIGN, int O, typename V>
+__device__ __forceinline__ void dft16(V *v)
 {
     // n = 4 n1 + n2: DFT4 over n1, twiddle e^{s 2pi i n2 k1 / 16}, DFT4 over n2.
 #pragma unroll
@@ -91,7 +154,7 @@ __device__ __forceinline__ void dft16(double2 *v)
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) bfly4<SIGN>(v[O + 4 * k1], v[O + 4 * k1 + 1], v[O + 4 * k1 + 2], v[O + 4 * k1 + 3]);
     // position 4 k1 + k2 holds X[k1 + 4 k2]
-    double2 t[16];
+    V t[16];
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1)
 #pragma unroll
@@ -100,8 +163,8 @@ __device__ __forceinline__ void dft16(double2 *v)
     for (int i = 0; i < 16; ++i) v[O + i] = t[i];
 }
 
-template <int L, int SIGN, int O>
-__device__ __forceinline__ void dft(double2 *v)
+template <int L, int SIGN, int O, typename V>
+__device__ __forceinline__ void dft(V *v)
 {
     static_assert(L == 4 || L == 8 || L == 16, "pencil DFT length");
     if constexpr (L == 4) dft4<SIGN, O>(v);
@@ -111,12 +174,12 @@ __device__ __forceinline__ void dft(double2 *v)
 
 // v[O + e] *= W^{SIGN k e} for e in [0, CNT), W = exp(2 pi i / N), from two
 // table entries (W^k, W^{4k}; 4k < N) and at most three products each.
-template <int SIGN, int CNT, int O>
-__device__ __forceinline__ void twiddle_powers(double2 *v, const double2 *__restrict__ tw, int k)
+template <int SIGN, int CNT, int O, typename V>
+__device__ __forceinline__ void twiddle_powers(V *v, const V *__restrict__ tw, int k)
 {
-    const double2 w1 = tw[k];
-    const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
-    double2 w4 = {1, 0}, w8 = {1, 0}, w12 = {1, 0};
+    const V w1 = tw[k];
+    const V w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+    V w4 = {1, 0}, w8 = {1, 0}, w12 = {1, 0};
     if constexpr (CNT > 4) {
         w4 = tw[4 * k];
         w8 = cmul(w4, w4);
@@ -124,11 +187,20 @@ __device__ __forceinline__ void twiddle_powers(double2 *v, const double2 *__rest
     }
 #pragma unroll
     for (int e = 1; e < CNT; ++e) {
-        const double2 lo = (e & 3) == 1 ? w1 : (e & 3) == 2 ? w2 : w3;
-        const double2 hi = (e >> 2) == 1 ? w4 : (e >> 2) == 2 ? w8 : w12;
-        const double2 w = (e & 3) == 0 ? hi : (e >> 2) == 0 ? lo : cmul(hi, lo);
+        const V lo = (e & 3) == 1 ? w1 : (e & 3) == 2 ? w2 : w3;
+        const V hi = (e >> 2) == 1 ? w4 : (e >> 2) == 2 ? w8 : w12;
+        const V w = (e & 3) == 0 ? hi : (e >> 2) == 0 ? lo : cmul(hi, lo);
         v[O + e] = ctw<SIGN>(v[O + e], w);
     }
+}
+
+// v[O + e] *= W^{SIGN k e} for e in [1, CNT) from a table of split
+// constants: tt[e * M] = W^{k e} (the caller offsets tt by its k's row).
+template <int SIGN, int CNT, int O, int M>
+__device__ __forceinline__ void twiddle_table(float2 *v, const float4 *__restrict__ tt)
+{
+#pragma unroll
+    for (int e = 1; e < CNT; ++e) v[O + e] = ctw_w<SIGN>(v[O + e], tt[e * M]);
 }
 
 // sin and cos of th for |th| < kSincosMax: Cody-Waite reduction by pi/2
@@ -181,12 +253,20 @@ template <int M> constexpr int xslots() { return 17 * M; }
 
 // Natural -> spectral, sign SIGN.  xl: this line's exchange slots.  The
 // caller orders LDS between consecutive exchanges of the same slots.
-template <int M, int SIGN>
-__device__ __forceinline__ void forward(double2 (&v)[16], int j, double2 *xl, const double2 *__restrict__ tw)
+// tw: double2 data take exp(2 pi i m / N), m in [0, N), and form the
+// powers they need; float2 data take the split table of every twiddle,
+// entry k1 M + j = W^{j k1} for the forward, then at N + (q M + n2) M + j
+// W^{(j Q + q) n2} for the inverse.
+template <int M, int SIGN, typename V>
+__device__ __forceinline__ void forward(V (&v)[16], int j, V *xl,
+                                        const typename ConstOf<V>::T *__restrict__ tw)
 {
     constexpr int Q = 16 / M;
     dft16<SIGN, 0>(v);                         // v[k1] = sum_n1 x[M n1 + j] ...
-    twiddle_powers<SIGN, 16, 0>(v, tw, j);     // * W^{s j k1}
+    if constexpr (sizeof(v[0].x) == 4)
+        twiddle_table<SIGN, 16, 0, M>(v, tw + j);  // * W^{s j k1}
+    else
+        twiddle_powers<SIGN, 16, 0>(v, tw, j);
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) xl[xslot<M>(k1, j)] = v[k1];
     __syncthreads();
@@ -200,14 +280,18 @@ __device__ __forceinline__ void forward(double2 (&v)[16], int j, double2 *xl, co
 
 // Spectral -> natural, sign SIGN (exact inverse of forward<M, -SIGN> up to
 // the factor N).
-template <int M, int SIGN>
-__device__ __forceinline__ void inverse(double2 (&v)[16], int j, double2 *xl, const double2 *__restrict__ tw)
+template <int M, int SIGN, typename V>
+__device__ __forceinline__ void inverse(V (&v)[16], int j, V *xl,
+                                        const typename ConstOf<V>::T *__restrict__ tw)
 {
     constexpr int Q = 16 / M;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         dft<M, SIGN, 0>(v + q * M);
-        twiddle_powers<SIGN, M, 0>(v + q * M, tw, j * Q + q);
+        if constexpr (sizeof(v[0].x) == 4)
+            twiddle_table<SIGN, M, 0, M>(v + q * M, tw + 16 * M + q * M * M + j);
+        else
+            twiddle_powers<SIGN, M, 0>(v + q * M, tw, j * Q + q);
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q)

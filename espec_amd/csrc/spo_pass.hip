@@ -50,7 +50,19 @@ struct PassArgs {
     int xmap;              // strided pass: give each XCD a contiguous range of tiles
     const double2 *twr;    // pencil kernels: exp(2 pi i m / L), m in [0, L)
     const double2 *kreg;   // pencil kernels: kinetic phases in spectral-layout order
+    const float4 *twr_s;   // single-precision pencils: split twiddle table (2 L per axis)
+    const float4 *kreg_s;  // and split kinetic phases (kreg's layout)
 };
+
+// Constants of the pencil kernels in the form their arithmetic takes:
+// double2 for FP64 storage, split float4 (hi, lo) for FP32 storage.
+template <typename S> using PenW = typename pencil::ConstOf<V2<S>>::T;
+template <typename S> __device__ __forceinline__ const PenW<S> *pen_twr(const PassArgs &a);
+template <> __device__ __forceinline__ const double2 *pen_twr<double>(const PassArgs &a) { return a.twr; }
+template <> __device__ __forceinline__ const float4 *pen_twr<float>(const PassArgs &a) { return a.twr_s; }
+template <typename S> __device__ __forceinline__ const PenW<S> *pen_kreg(const PassArgs &a);
+template <> __device__ __forceinline__ const double2 *pen_kreg<double>(const PassArgs &a) { return a.kreg; }
+template <> __device__ __forceinline__ const float4 *pen_kreg<float>(const PassArgs &a) { return a.kreg_s; }
 
 template <typename S>
 __device__ __forceinline__ double2 widen(V2<S> v)
@@ -330,7 +342,11 @@ __device__ __forceinline__ void pen_fetch(const PassArgs &a, long long po, long 
 
 // Two waves per SIMD are requested: the exchange slots (17 KiB per wave)
 // cap residency near that anyway.  (Prefetching the next item here costs
-// spills: the row pass holds psi0 and V beside the state.)
+// spills: the row pass holds psi0 and V beside the state.)  The row pass
+// computes in FP64 for either storage: its FP64 sincos of the potential
+// phase dominates the arithmetic, and in single precision the phase would
+// have to stay split to FP64 accuracy anyway, since a fixed per-point
+// rounding of it accumulates coherently over the steps.
 template <typename S, int M, bool INV, bool FWD, bool PHT>
 __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
 {
@@ -441,15 +457,17 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
 }
 
 // Strided pass on pencils: lane = c + C j, line c of the item's C
-// consecutive columns, lane j within the line.
+// consecutive columns, lane j within the line; arithmetic in precision S.
 // PF: load the next item's state while transforming the current one.
 template <typename S, int M, bool PF>
 __global__ __launch_bounds__(64, 2) void pass_strided_pen(PassArgs a)
 {
+    using VS = V2<S>;
     constexpr int C = 64 / M;
-    __shared__ double2 xs[C * pencil::xslots<M>()];
+    __shared__ VS xs[C * pencil::xslots<M>()];
     const int c = threadIdx.x % C, j = threadIdx.x / C;
-    double2 *xl = xs + c * pencil::xslots<M>();
+    VS *xl = xs + c * pencil::xslots<M>();
+    const PenW<S> *twr = pen_twr<S>(a);
     const int groups = a.n / (a.L * C);  // column groups per packet
     const int total = groups * a.npk;
     int w = blockIdx.x;
@@ -470,21 +488,21 @@ __global__ __launch_bounds__(64, 2) void pass_strided_pen(PassArgs a)
     if (PF) pen_fetch<S, false>(a, po, stride, sr, si, sv);
     for (;;) {
         if (!PF) pen_fetch<S, false>(a, po, stride, sr, si, sv);
-        double2 v[16];
+        VS v[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = {(double)sr[r], (double)si[r]};
+        for (int r = 0; r < 16; ++r) v[r] = {sr[r], si[r]};
         const int wn = w + gridDim.x;
         int pkn = pk;
         const long long pon = wn < total ? origin(wn, pkn) : po;
         if (PF && wn < total) pen_fetch<S, false>(a, pon, stride, sr, si, sv);
-        pencil::forward<M, 1>(v, j, xl, a.twr);
-        const double2 *kt = a.kreg + (long long)pk * a.ktps + j;
+        pencil::forward<M, 1>(v, j, xl, twr);
+        const PenW<S> *kt = pen_kreg<S>(a) + (long long)pk * a.ktps + j;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = cmulc(v[r], kt[r * M]);
-        pencil::inverse<M, -1>(v, j, xl, a.twr);
-        V2<S> *gx = reinterpret_cast<V2<S> *>(a.x) + po;
+        for (int r = 0; r < 16; ++r) v[r] = pencil::cmulc_w(v[r], kt[r * M]);
+        pencil::inverse<M, -1>(v, j, xl, twr);
+        VS *gx = reinterpret_cast<VS *>(a.x) + po;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) gx[r * stride] = {(S)v[r].x, (S)v[r].y};
+        for (int r = 0; r < 16; ++r) gx[r * stride] = v[r];
         if (a.last && w == 0 && threadIdx.x == 0) *a.dstep += 1;
         if (wn >= total) break;
         w = wn;
@@ -494,7 +512,65 @@ __global__ __launch_bounds__(64, 2) void pass_strided_pen(PassArgs a)
     }
 }
 
+// Strided pass on pencils through an LDS tile: a workgroup of four
+// wavefronts takes W = 4 C = 256 / M consecutive columns (one 128-byte row
+// segment in single precision), so every global load and store covers
+// whole cache lines; each wavefront transforms C of the columns as
+// register pencils (lane c + C j, register r = row M r + j) in precision
+// S.  The staging tile overlays the exchange slots, which are idle while
+// it is in use.  One item per workgroup; requires S % W == 0.
+template <typename S, int M>
+__global__ __launch_bounds__(256, 2) void pass_strided_tile(PassArgs a)
+{
+    using VS = V2<S>;
+    constexpr int C = 64 / M, L = 16 * M, W = 4 * C, P = W + 1;
+    constexpr int XS = C * pencil::xslots<M>();  // exchange slots of one wavefront
+    static_assert(L * P <= 4 * XS, "tile fits the exchange slots");
+    __shared__ VS xs[4 * XS];
+    VS *tile = xs;
+    const int wv = threadIdx.x / 64, c = threadIdx.x % C, j = (threadIdx.x % 64) / C;
+    VS *xl = xs + wv * XS + c * pencil::xslots<M>();
+    const int tiles = a.S / W;                     // column tiles per slab
+    const int per_pk = a.n / (L * a.S) * tiles;    // items per packet
+    const int total = per_pk * a.npk;
+    int w = blockIdx.x;
+    if (w >= total) return;
+    if (a.xmap) w = (w & 7) * (total >> 3) + (w >> 3);
+    const int pk = w / per_pk, rem = w - pk * per_pk;
+    const int slab = rem / tiles, ct = rem - slab * tiles;
+    VS *gx = reinterpret_cast<VS *>(a.x) + (long long)pk * a.n + (long long)slab * a.S * L +
+             (long long)ct * W;
+    const int tc = threadIdx.x % W, tr = threadIdx.x / W;  // M rows per sweep of 256 threads
+    {
+        VS t[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t[i] = gx[(long long)(tr + M * i) * a.S + tc];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tile[(tr + M * i) * P + tc] = t[i];
+    }
+    __syncthreads();
+    VS v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = tile[(M * r + j) * P + C * wv + c];
+    __syncthreads();  // the exchange slots overlay the tile
+    const PenW<S> *twr = pen_twr<S>(a);
+    pencil::forward<M, 1>(v, j, xl, twr);
+    const PenW<S> *kt = pen_kreg<S>(a) + (long long)pk * a.ktps + j;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = pencil::cmulc_w(v[r], kt[r * M]);
+    pencil::inverse<M, -1>(v, j, xl, twr);
+    __syncthreads();  // every wavefront is done with its exchange slots
+#pragma unroll
+    for (int r = 0; r < 16; ++r) tile[(M * r + j) * P + C * wv + c] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) gx[(long long)(tr + M * i) * a.S + tc] = tile[(tr + M * i) * P + tc];
+    if (a.last && blockIdx.x == 0 && threadIdx.x == 0) *a.dstep += 1;
+}
+
 __global__ void set_step(long long *d, long long v) { *d = v; }
+// One-dimensional grids have no strided pass to advance the step counter.
+__global__ void add_step(long long *d) { *d += 1; }
 
 // cbuf[pk][nc] = sum over row blocks of part[nc][pk][blk], nc in [lo, hi).
 __global__ void reduce_parts(const double *part, double *cbuf, int npk, int xblocks, int maxsamp,
@@ -618,6 +694,8 @@ hipError_t rows(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, double 
     if (p.shape.rows_pen) {
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[0];
         a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[0];
+        a.twr_s = static_cast<const float4 *>(p.twr_s) + 2 * p.twr_off[0];
+        a.kreg_s = static_cast<const float4 *>(p.kreg_s) + p.kax_off[0];
         const dim3 pg(act);
 #define ESPEC_PEN_ROWS2(M_, P_)                                                               \
     if (p.np[0] == 16 * M_ && pht == P_) {                                                    \
@@ -673,9 +751,24 @@ hipError_t strided(const SpoPassPlan &p, int axis, bool last, hipStream_t s)
     const int nt = p.shape.tile_nt;
     a.ept = a.cnt / nt;
     a.last = last ? 1 : 0;
+    if (p.shape.tile_pen == 2) {
+        a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[axis];
+        a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[axis];
+        a.twr_s = static_cast<const float4 *>(p.twr_s) + 2 * p.twr_off[axis];
+        a.kreg_s = static_cast<const float4 *>(p.kreg_s) + p.kax_off[axis];
+        const int total = p.n / (a.L * (4096 / a.L)) * p.npk;
+        a.xmap = (p.shape.xmap && total % 8 == 0) ? 1 : 0;
+        const dim3 tg(total);
+        if (a.L == 64) return launch(pass_strided_tile<R, 4>, tg, 256, 0, s, a);
+        if (a.L == 128) return launch(pass_strided_tile<R, 8>, tg, 256, 0, s, a);
+        if (a.L == 256) return launch(pass_strided_tile<R, 16>, tg, 256, 0, s, a);
+        return hipErrorInvalidValue;
+    }
     if (p.shape.tile_pen) {
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[axis];
         a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[axis];
+        a.twr_s = static_cast<const float4 *>(p.twr_s) + 2 * p.twr_off[axis];
+        a.kreg_s = static_cast<const float4 *>(p.kreg_s) + p.kax_off[axis];
         const int total = p.n / (a.L * (64 / (a.L / 16))) * p.npk;
         const int act = p.shape.grid > 0 ? std::min(total, p.shape.grid) : total;
         a.xmap = (p.shape.xmap && total % 8 == 0) ? 1 : 0;
@@ -714,6 +807,11 @@ hipError_t one_step(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, int
     hipError_t e = rows<R>(p, inv, fwd, s);
     ++*launches;
     if (e != hipSuccess || !fwd) return e;
+    if (p.ndim == 1) {
+        hipLaunchKernelGGL(add_step, dim3(1), dim3(1), 0, s, p.dstep);
+        ++*launches;
+        return hipGetLastError();
+    }
     for (int ax = 1; ax < p.ndim; ++ax) {
         e = strided<R>(p, ax, ax == p.ndim - 1, s);
         ++*launches;
@@ -772,7 +870,7 @@ int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *l
         }
         while (left >= p.graph_steps) {
             ESPEC_HIP_TRY(hipGraphLaunch(p.gexec, s));
-            *launches += p.graph_steps * p.ndim;
+            *launches += p.graph_steps * (p.ndim == 1 ? 2 : p.ndim);
             left -= p.graph_steps;
         }
     }
@@ -820,10 +918,13 @@ bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 bool tile_ok(const SpoPassPlan &p, const PassShape &s)
 {
     if (s.tile_pen) {
+        if (s.tile_pen > 2) return false;
+        // columns per item: 64 / M (one wavefront) or 256 / M (four, tiled)
+        const int cols = s.tile_pen == 2 ? 4096 : 1024;
         for (int a = 1; a < p.ndim; ++a) {
             int stride = 1;
             for (int k = 0; k < a; ++k) stride *= p.np[k];
-            if (!pen_len(p.np[a]) || stride % (1024 / p.np[a])) return false;
+            if (!pen_len(p.np[a]) || stride % (cols / p.np[a])) return false;
         }
         return true;
     }
@@ -924,7 +1025,8 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
         best_ms = 1e30f;
         // {tile, threads, pencil}
         const int tc_[][3] = {{1, 64, 0},  {2, 64, 0},  {4, 64, 0},  {1, 256, 0},  {2, 256, 0},
-                              {4, 256, 0}, {8, 256, 0}, {16, 256, 0}, {p.shape.tile, p.shape.tile_nt, 1}};
+                              {4, 256, 0}, {8, 256, 0}, {16, 256, 0}, {p.shape.tile, p.shape.tile_nt, 1},
+                              {p.shape.tile, p.shape.tile_nt, 2}};
         for (auto &c : tc_) {
             PassShape sh = p.shape;
             sh.tile = c[0];
@@ -935,7 +1037,11 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
             apply_shape(p);
             ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
             float ms = 0;
-            int rc = time_launches(s, reps, [&]() { return strided<R>(p, 1, false, s); }, &ms);
+            // Whole steps: the strided kernels differ in how they leave the
+            // state in the caches for the row pass that follows.
+            int dummy = 0;
+            int rc = time_launches(s, reps, [&]() { return one_step<R>(p, true, true, s, &dummy); },
+                                   &ms);
             if (rc) return rc;
             if (clearly_faster(ms, best_ms)) {
                 best_ms = ms;
@@ -987,11 +1093,21 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
         best_ms = 1e30f;
         int best_grid = 0;
         int best_pf = 0;
-        const bool pen = p.shape.tile_pen;
+        int best_tp = p.shape.tile_pen;
+        // The two pencil forms of the strided passes are compared again
+        // here, with everything else at its final setting.
+        std::vector<int> tps{p.shape.tile_pen};
+        if (p.shape.tile_pen) {
+            PassShape alt = p.shape;
+            alt.tile_pen = 3 - p.shape.tile_pen;
+            if (p.ndim > 1 && tile_ok(p, alt)) tps.push_back(alt.tile_pen);
+        }
+        for (int tp : tps)
         for (int pf : {0, 1})
         for (int gs : {0, 4096, 2048, 1024, 512}) {
             if (gs && (items <= gs || p.shape.spread > 1)) continue;
-            if (pf && !pen) continue;
+            if (pf && tp != 1) continue;  // prefetch: one-wavefront pencils only
+            p.shape.tile_pen = tp;
             p.shape.grid = gs;
             p.shape.pen_pf = pf;
             apply_shape(p);
@@ -1004,8 +1120,10 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
                 best_ms = ms;
                 best_grid = gs;
                 best_pf = pf;
+                best_tp = tp;
             }
         }
+        p.shape.tile_pen = best_tp;
         p.shape.grid = best_grid;
         p.shape.pen_pf = best_pf;
     }
@@ -1049,6 +1167,10 @@ int profile_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, do
         if (err == hipSuccess) err = rows<R>(p, true, true, s);
         disarm();
         kind.push_back(0);
+        if (p.ndim == 1 && err == hipSuccess) {
+            hipLaunchKernelGGL(add_step, dim3(1), dim3(1), 0, s, p.dstep);
+            err = hipGetLastError();
+        }
         for (int ax = 1; ax < p.ndim && err == hipSuccess; ++ax) {
             err = arm();
             if (err == hipSuccess) err = strided<R>(p, ax, ax == p.ndim - 1, s);
@@ -1185,11 +1307,41 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
 shaped:
     apply_shape(p);
     const int max_xblocks = p.n / p.np[0];  // rows = 1
-    DevBuf btw, bk, btwr, bkr, bpart, bstep;
+    // Split single-precision copies: per axis of length L = 16 M, the
+    // forward twiddles W^{j k1} at k1 M + j, then the inverse twiddles
+    // W^{(j Q + q) n2} at L + (q M + n2) M + j (Q = 16 / M); the kinetic
+    // phases entry for entry.
+    std::vector<float> twr_s, kreg_s;
+    auto push_split = [](std::vector<float> &v, double c, double sn) {
+        const float ch = (float)c, sh = (float)sn;
+        v.push_back(ch);
+        v.push_back(sh);
+        v.push_back((float)(c - (double)ch));
+        v.push_back((float)(sn - (double)sh));
+    };
+    for (int a = 0; a < ndim; ++a) {
+        const int L = np[a], M = L / 16, Q = pen_len(L) ? 16 / M : 0;
+        auto wpow = [&](long long m) {
+            push_split(twr_s, std::cos(kTwoPi * (double)(m % L) / L), std::sin(kTwoPi * (double)(m % L) / L));
+        };
+        if (!pen_len(L)) {
+            for (int i = 0; i < 2 * L; ++i) push_split(twr_s, 1.0, 0.0);
+            continue;
+        }
+        for (int k1 = 0; k1 < 16; ++k1)
+            for (int j = 0; j < M; ++j) wpow((long long)j * k1);
+        for (int q = 0; q < Q; ++q)
+            for (int n2 = 0; n2 < M; ++n2)
+                for (int j = 0; j < M; ++j) wpow((long long)(j * Q + q) * n2);
+    }
+    for (size_t i = 0; i < kreg.size(); i += 2) push_split(kreg_s, kreg[i], kreg[i + 1]);
+    DevBuf btw, bk, btwr, bkr, btwf, bkrf, bpart, bstep;
     ESPEC_HIP_TRY(btw.upload(tw.data(), sizeof(double) * tw.size()));
     ESPEC_HIP_TRY(bk.upload(kax.data(), sizeof(double) * kax.size()));
     ESPEC_HIP_TRY(btwr.upload(twr.data(), sizeof(double) * twr.size()));
     ESPEC_HIP_TRY(bkr.upload(kreg.data(), sizeof(double) * kreg.size()));
+    ESPEC_HIP_TRY(btwf.upload(twr_s.data(), sizeof(float) * twr_s.size()));
+    ESPEC_HIP_TRY(bkrf.upload(kreg_s.data(), sizeof(float) * kreg_s.size()));
     ESPEC_HIP_TRY(bpart.alloc(sizeof(double) * 3 * (size_t)maxsamp * npk * max_xblocks));
     ESPEC_HIP_TRY(hipMemset(bpart.p, 0, bpart.bytes));
     ESPEC_HIP_TRY(bstep.alloc(sizeof(long long)));
@@ -1201,6 +1353,10 @@ shaped:
     btwr.p = nullptr;
     p.kreg = bkr.p;
     bkr.p = nullptr;
+    p.twr_s = btwf.p;
+    btwf.p = nullptr;
+    p.kreg_s = bkrf.p;
+    bkrf.p = nullptr;
     p.part = static_cast<double *>(bpart.p);
     bpart.p = nullptr;
     p.dstep = static_cast<long long *>(bstep.p);
@@ -1259,6 +1415,8 @@ void spo_pass_release(SpoPassPlan &p)
     if (p.kax) (void)hipFree(p.kax);
     if (p.twr) (void)hipFree(p.twr);
     if (p.kreg) (void)hipFree(p.kreg);
+    if (p.twr_s) (void)hipFree(p.twr_s);
+    if (p.kreg_s) (void)hipFree(p.kreg_s);
     if (p.part) (void)hipFree(p.part);
     if (p.dstep) (void)hipFree(p.dstep);
     p = SpoPassPlan{};

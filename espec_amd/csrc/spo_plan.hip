@@ -179,7 +179,7 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
     // LDS-resident when the FP64 grid fits in one workgroup's LDS; the V
     // phase table is staged next to it when that fits too.
     const size_t lds_cap = 160 * 1024 - 1024;
-    P->lds = spo_lds_bytes(P->n, P->ref, false, P->nt) <= lds_cap;
+    P->lds = !(flags & ESPEC_MULTIPASS) && spo_lds_bytes(P->n, P->ref, false, P->nt) <= lds_cap;
     P->la.vph_lds = (!P->ref && spo_lds_bytes(P->n, false, true, P->nt) <= lds_cap) ? 1 : 0;
     {
         int ntw = 0;

@@ -41,6 +41,10 @@ extern "C" {
 /* Plans only: sm is [npk][ndim], one set of masses per packet (the
  * kinetic phases become per-packet tables). */
 #define ESPEC_PER_PACKET_MASS 0x4
+/* Split-operator plans only: one pass per axis per step through HBM even
+ * when a packet fits one workgroup's LDS (batches of many small packets,
+ * where the LDS-resident kernel has one workgroup per packet). */
+#define ESPEC_MULTIPASS 0x8
 
 enum {
     ESPEC_OK = 0,

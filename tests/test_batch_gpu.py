@@ -48,6 +48,32 @@ def test_per_packet_masses(gpu, orc, n2d, flags, tol):
     b.close()
 
 
+@pytest.mark.parametrize("kind", ["lds", "multipass", "auto"])
+@pytest.mark.parametrize("n2d", [0, 64])
+@pytest.mark.parametrize("flags,tol", [(E.FP64 | E.EXACT, 1e-10), (E.MIXED | E.EXACT, 1e-6)])
+def test_plan_kinds(gpu, orc, kind, n2d, flags, tol):
+    """Packets that fit one workgroup's LDS on the LDS-resident kernel, on
+    the multi-pass kernels (one-dimensional grids: the step counter is
+    advanced without a strided pass) and on the timed choice of the two;
+    C(t) every second step."""
+    from oracle import EXACT
+    pk = family(n2d, 5, 31 + n2d)
+    nst, lc = 40, 2
+    print(f"case: plan kinds kind={kind} n2d={n2d} flags={flags}", flush=True)
+    b = SpoBatch(pk, flags=flags, lc=lc, kind=kind)
+    if kind != "auto":
+        assert b.groups[0]["plan"].shape()["lds"] == (kind == "lds")
+    b.run(nst)
+    b.sync()
+    for i, p in enumerate(pk):
+        o = orc.pspofft(p.np, p.dt_fs, 0.0, (nst - 1) * p.dt_fs, lc, p.sh, p.sm, p.u, p.v,
+                        p.vpot, EXACT)
+        g = b.get(i)
+        assert len(g["cr"]) == nst // lc + 1
+        assert dC(g, o) <= tol, i
+    b.close()
+
+
 def test_config4_subset(gpu, orc):
     """The uneven mix of the batched configuration (1-D N in 128..1024, 2-D
     64^2..256^2), a 48-packet sample, mixed path."""

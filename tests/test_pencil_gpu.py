@@ -18,6 +18,8 @@ SHAPES = {
     "pen_cols": dict(rows_pen=0, tile_pen=1, pen_pf=0, grid=0, xmap=1),
     "pen_loop_pf": dict(rows_pen=1, tile_pen=1, pen_pf=1, grid=24, xmap=1),
     "pen_loop": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=40, xmap=0),
+    "pen_tile": dict(rows_pen=1, tile_pen=2, pen_pf=0, grid=0, xmap=1),
+    "pen_tile_nox": dict(rows_pen=0, tile_pen=2, pen_pf=0, grid=0, xmap=0),
 }
 
 
@@ -54,7 +56,7 @@ def test_config2_shapes(gpu, orc, shape, flags, tol):
     plan.close()
 
 
-@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "stage"])
+@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "stage", "pen_tile"])
 def test_3d_shapes(gpu, orc, shape):
     """64^3 (every axis a pencil length), two runs continuing one propagation."""
     from oracle import EXACT
@@ -72,7 +74,7 @@ def test_3d_shapes(gpu, orc, shape):
     assert dC(g, o) <= 1e-10
 
 
-@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "pen_loop"])
+@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "pen_loop", "pen_tile"])
 @pytest.mark.parametrize("flags,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-6)])
 def test_batch_per_packet_masses(gpu, orc, shape, flags, tol):
     """Five 128^2 packets with their own masses, potentials and states;
