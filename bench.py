@@ -44,12 +44,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Launch shape of the 256x256 plans: register-pencil row and column passes,
-# prefetching column pass with XCD-contiguous column groups, one workgroup
-# per work item.  Pinned so that the kernels profiled by rocprofv3 are the
-# ones timed here: the autotuner's pick between near-equal shapes can change
-# with the load on the device (a profiler is load).
-C2_SHAPE = dict(rows_pen=1, rows=4, rows_nt=64, tile_pen=1, pen_pf=1, xmap=1, grid=0, spread=1)
+# Launch shape of the 256x256 plans: one 256-point line per wavefront in
+# both passes (radix-4 Stockham in registers), XCD-contiguous columns, one
+# workgroup per line.  Pinned so that the kernels profiled by rocprofv3 are
+# the ones timed here: the autotuner's pick between near-equal shapes can
+# change with the load on the device (a profiler is load).
+C2_SHAPE = dict(rows_pen=2, rows=1, rows_nt=64, tile_pen=3, pen_pf=0, xmap=1, grid=0, spread=1)
 
 
 def timed_run(E, c, flags, steps, warmup):
@@ -91,12 +91,16 @@ def kernel_names(shape, np0, np1, storage="float"):
     arguments stripped) of the row-pass and strided-pass kernels that a
     launch shape selects for steps 2..K."""
     pht = "true" if shape.get("vph") else "false"
-    if shape["rows_pen"]:
+    if shape["rows_pen"] == 2:
+        rows = f"pass_rows_wide<{storage}, true, true, {pht}>"
+    elif shape["rows_pen"]:
         rows = f"pass_rows_pen<{storage}, {np0 // 16}, true, true, {pht}>"
     else:
         ept_r = shape["rows"] * np0 // shape["rows_nt"]
         rows = f"pass_rows<{storage}, {shape['rows_nt']}, true, true, {ept_r}, {pht}>"
-    if shape["tile_pen"] == 2:
+    if shape["tile_pen"] == 3:
+        cols = f"pass_strided_wide<{storage}>"
+    elif shape["tile_pen"] == 2:
         cols = f"pass_strided_tile<{storage}, {np1 // 16}>"
     elif shape["tile_pen"]:
         pf = "true" if shape["pen_pf"] else "false"

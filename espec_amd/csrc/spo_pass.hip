@@ -52,6 +52,7 @@ struct PassArgs {
     const double2 *kreg;   // pencil kernels: kinetic phases in spectral-layout order
     const float4 *twr_s;   // single-precision pencils: split twiddle table (2 L per axis)
     const float4 *kreg_s;  // and split kinetic phases (kreg's layout)
+    const double2 *knat;   // wide lines: kinetic phases in natural frequency order
 };
 
 // Constants of the pencil kernels in the form their arithmetic takes:
@@ -568,6 +569,104 @@ __global__ __launch_bounds__(256, 2) void pass_strided_tile(PassArgs a)
     if (a.last && blockIdx.x == 0 && threadIdx.x == 0) *a.dstep += 1;
 }
 
+// Row pass on wide lines (x length 256): one line per 64-thread workgroup,
+// FP64 arithmetic for either storage; the state between passes keeps x in
+// natural frequency order.  Same sequence as pass_rows_pen.
+template <typename S, bool INV, bool FWD, bool PHT>
+__global__ __launch_bounds__(64) void pass_rows_wide(PassArgs a)
+{
+    __shared__ double2 xs[wide::kSlots];
+    const int j = threadIdx.x;
+    const int lines = a.n / wide::kLine;
+    const int w = blockIdx.x;
+    if (w >= lines * a.npk) return;
+    const int pk = w / lines, line = w - pk * lines;
+    const long long po = (long long)pk * a.n + (long long)line * wide::kLine + j;
+    const long long g = INV ? *a.dstep : 0;
+    const bool samp = INV && (g % a.lc) == 0;
+    const V2<S> *gx = reinterpret_cast<const V2<S> *>(a.x) + po;
+    double2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = widen<S>(gx[64 * r]);
+    if (INV) wide::stockham<-1>(v, j, xs, a.twr);
+    double cr = 0, ci = 0, nr = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double2 ph;
+        if (PHT) {
+            ph = a.vph[po + 64 * r];
+        } else {
+            double sn, cs;
+            pencil::sincos_reduced(a.cst2 * (double)reinterpret_cast<const S *>(a.vpot)[po + 64 * r],
+                                   &sn, &cs);
+            ph = {cs, sn};
+        }
+        if (INV) {
+            v[r] = cmulc(v[r], ph);
+            v[r].x *= a.b;
+            v[r].y *= a.b;
+            if (samp) {
+                const double2 q = widen<S>(reinterpret_cast<const V2<S> *>(a.psi0)[po + 64 * r]);
+                cr += q.x * v[r].x + q.y * v[r].y;
+                ci += q.x * v[r].y - q.y * v[r].x;
+                nr += v[r].x * v[r].x + v[r].y * v[r].y;
+            }
+        }
+        if (FWD) v[r] = cmulc(v[r], ph);
+    }
+    if (FWD) {
+        wide::stockham<1>(v, j, xs, a.twr);
+        const double2 *kt = a.knat + (long long)pk * a.ktps + j;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = cmulc(v[r], kt[64 * r]);
+    }
+    V2<S> *gy = reinterpret_cast<V2<S> *>(a.x) + po;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) gy[64 * r] = {(S)v[r].x, (S)v[r].y};
+    if (samp) {
+        cr = wave_sum(cr);
+        ci = wave_sum(ci);
+        nr = wave_sum(nr);
+        const long long nc = g / a.lc;
+        if (j == 0 && nc < a.maxsamp) {
+            double *c = a.part + (((long long)nc * a.npk + pk) * a.xblocks + line) * 3;
+            c[0] = cr;
+            c[1] = ci;
+            c[2] = nr;
+        }
+    }
+}
+
+// Strided pass on wide lines (axis length 256): one column per 64-thread
+// workgroup, lane j holding rows j + 64 r; FP64 arithmetic.
+template <typename S>
+__global__ __launch_bounds__(64) void pass_strided_wide(PassArgs a)
+{
+    __shared__ double2 xs[wide::kSlots];
+    const int j = threadIdx.x;
+    const int cols = a.n / wide::kLine;  // lines of this axis per packet
+    const int total = cols * a.npk;
+    int w = blockIdx.x;
+    if (w >= total) return;
+    if (a.xmap) w = (w & 7) * (total >> 3) + (w >> 3);
+    const int pk = w / cols, col = w - pk * cols;
+    const long long po = (long long)pk * a.n + (col % a.S) +
+                         (long long)(col / a.S) * a.S * wide::kLine + (long long)j * a.S;
+    const long long st = 64LL * a.S;
+    V2<S> *gx = reinterpret_cast<V2<S> *>(a.x) + po;
+    double2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = widen<S>(gx[r * st]);
+    wide::stockham<1>(v, j, xs, a.twr);
+    const double2 *kt = a.knat + (long long)pk * a.ktps + j;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = cmulc(v[r], kt[64 * r]);
+    wide::stockham<-1>(v, j, xs, a.twr);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) gx[r * st] = {(S)v[r].x, (S)v[r].y};
+    if (a.last && blockIdx.x == 0 && j == 0) *a.dstep += 1;
+}
+
 __global__ void set_step(long long *d, long long v) { *d = v; }
 // One-dimensional grids have no strided pass to advance the step counter.
 __global__ void add_step(long long *d) { *d += 1; }
@@ -691,6 +790,21 @@ hipError_t rows(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, double 
     const bool pht = p.vph != nullptr;
     const int total = p.xblocks * p.npk;
     const int act = p.shape.grid > 0 ? std::min(total, p.shape.grid) : total;
+    if (p.shape.rows_pen == 2) {
+        a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[0];
+        a.knat = static_cast<const double2 *>(p.knat) + p.kax_off[0];
+        const dim3 wg(total);
+#define ESPEC_WIDE_ROWS(P_)                                                                  \
+    if (pht == P_) {                                                                         \
+        if (inv && fwd) return launch(pass_rows_wide<R, true, true, P_>, wg, 64, 0, s, a);   \
+        if (inv) return launch(pass_rows_wide<R, true, false, P_>, wg, 64, 0, s, a);         \
+        return launch(pass_rows_wide<R, false, true, P_>, wg, 64, 0, s, a);                  \
+    }
+        ESPEC_WIDE_ROWS(true)
+        ESPEC_WIDE_ROWS(false)
+#undef ESPEC_WIDE_ROWS
+        return hipErrorInvalidValue;
+    }
     if (p.shape.rows_pen) {
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[0];
         a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[0];
@@ -751,6 +865,14 @@ hipError_t strided(const SpoPassPlan &p, int axis, bool last, hipStream_t s)
     const int nt = p.shape.tile_nt;
     a.ept = a.cnt / nt;
     a.last = last ? 1 : 0;
+    if (p.shape.tile_pen == 3) {
+        a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[axis];
+        a.knat = static_cast<const double2 *>(p.knat) + p.kax_off[axis];
+        const int total = p.n / wide::kLine * p.npk;
+        a.xmap = (p.shape.xmap && total % 8 == 0) ? 1 : 0;
+        if (a.L != wide::kLine) return hipErrorInvalidValue;
+        return launch(pass_strided_wide<R>, dim3(total), 64, 0, s, a);
+    }
     if (p.shape.tile_pen == 2) {
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[axis];
         a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[axis];
@@ -904,6 +1026,10 @@ bool pen_len(int l) { return l == 64 || l == 128 || l == 256; }
 
 bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 {
+    if (s.rows_pen == 2)
+        return p.np[0] == wide::kLine && s.rows == 1 && s.rows_nt == 64 &&
+               (p.vph != nullptr || p.phase_max < pencil::kSincosMax);
+    if (s.rows_pen > 2) return false;
     if (s.rows_pen)
         return pen_len(p.np[0]) && s.rows == 1024 / p.np[0] && s.rows_nt == 64 &&
                (p.n / p.np[0]) % s.rows == 0 &&
@@ -917,8 +1043,13 @@ bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 
 bool tile_ok(const SpoPassPlan &p, const PassShape &s)
 {
+    if (s.tile_pen == 3) {
+        for (int a = 1; a < p.ndim; ++a)
+            if (p.np[a] != wide::kLine) return false;
+        return true;
+    }
     if (s.tile_pen) {
-        if (s.tile_pen > 2) return false;
+        if (s.tile_pen > 3) return false;
         // columns per item: 64 / M (one wavefront) or 256 / M (four, tiled)
         const int cols = s.tile_pen == 2 ? 4096 : 1024;
         for (int a = 1; a < p.ndim; ++a) {
@@ -997,7 +1128,8 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
     float best_ms = 1e30f;
     // {rows, threads, pencil}
     const int rc_[][3] = {{1, 64, 0},  {2, 64, 0},  {4, 64, 0},  {1, 256, 0},
-                          {2, 256, 0}, {4, 256, 0}, {8, 256, 0}, {1024 / p.np[0], 64, 1}};
+                          {2, 256, 0}, {4, 256, 0}, {8, 256, 0}, {1024 / p.np[0], 64, 1},
+                          {1, 64, 2}};
     for (auto &c : rc_) {
         PassShape sh = p.shape;
         sh.rows = c[0];
@@ -1026,7 +1158,7 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
         // {tile, threads, pencil}
         const int tc_[][3] = {{1, 64, 0},  {2, 64, 0},  {4, 64, 0},  {1, 256, 0},  {2, 256, 0},
                               {4, 256, 0}, {8, 256, 0}, {16, 256, 0}, {p.shape.tile, p.shape.tile_nt, 1},
-                              {p.shape.tile, p.shape.tile_nt, 2}};
+                              {p.shape.tile, p.shape.tile_nt, 2}, {p.shape.tile, p.shape.tile_nt, 3}};
         for (auto &c : tc_) {
             PassShape sh = p.shape;
             sh.tile = c[0];
@@ -1097,11 +1229,12 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
         // The two pencil forms of the strided passes are compared again
         // here, with everything else at its final setting.
         std::vector<int> tps{p.shape.tile_pen};
-        if (p.shape.tile_pen) {
-            PassShape alt = p.shape;
-            alt.tile_pen = 3 - p.shape.tile_pen;
-            if (p.ndim > 1 && tile_ok(p, alt)) tps.push_back(alt.tile_pen);
-        }
+        if (p.shape.tile_pen && p.ndim > 1)
+            for (int tp = 1; tp <= 3; ++tp) {
+                PassShape alt = p.shape;
+                alt.tile_pen = tp;
+                if (tp != p.shape.tile_pen && tile_ok(p, alt)) tps.push_back(tp);
+            }
         for (int tp : tps)
         for (int pf : {0, 1})
         for (int gs : {0, 4096, 2048, 1024, 512}) {
@@ -1252,7 +1385,7 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
     kax.resize(2 * (size_t)pstride * nset);
     // Pencil tables: exact twiddles exp(2 pi i m / L), and the kinetic
     // phases at the spectral-layout position of each frequency.
-    std::vector<double> twr, kreg(kax.size());
+    std::vector<double> twr, kreg(kax.size()), knat(kax.size());
     for (int a = 0; a < ndim; ++a) {
         p.twr_off[a] = (int)(twr.size() / 2);
         for (int m = 0; m < np[a]; ++m) {
@@ -1268,6 +1401,8 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                 const int pos = fp[a].perm[k];
                 kax[k0 + 2 * pos] = std::cos(th);
                 kax[k0 + 2 * pos + 1] = std::sin(th);
+                knat[k0 + 2 * k] = std::cos(th);
+                knat[k0 + 2 * k + 1] = std::sin(th);
             }
             if (pen_len(np[a]))
                 for (int pos = 0; pos < np[a]; ++pos) {
@@ -1335,7 +1470,8 @@ shaped:
                 for (int j = 0; j < M; ++j) wpow((long long)(j * Q + q) * n2);
     }
     for (size_t i = 0; i < kreg.size(); i += 2) push_split(kreg_s, kreg[i], kreg[i + 1]);
-    DevBuf btw, bk, btwr, bkr, btwf, bkrf, bpart, bstep;
+    DevBuf btw, bk, btwr, bkr, btwf, bkrf, bkn, bpart, bstep;
+    ESPEC_HIP_TRY(bkn.upload(knat.data(), sizeof(double) * knat.size()));
     ESPEC_HIP_TRY(btw.upload(tw.data(), sizeof(double) * tw.size()));
     ESPEC_HIP_TRY(bk.upload(kax.data(), sizeof(double) * kax.size()));
     ESPEC_HIP_TRY(btwr.upload(twr.data(), sizeof(double) * twr.size()));
@@ -1355,6 +1491,8 @@ shaped:
     bkr.p = nullptr;
     p.twr_s = btwf.p;
     btwf.p = nullptr;
+    p.knat = bkn.p;
+    bkn.p = nullptr;
     p.kreg_s = bkrf.p;
     bkrf.p = nullptr;
     p.part = static_cast<double *>(bpart.p);
@@ -1416,6 +1554,7 @@ void spo_pass_release(SpoPassPlan &p)
     if (p.twr) (void)hipFree(p.twr);
     if (p.kreg) (void)hipFree(p.kreg);
     if (p.twr_s) (void)hipFree(p.twr_s);
+    if (p.knat) (void)hipFree(p.knat);
     if (p.kreg_s) (void)hipFree(p.kreg_s);
     if (p.part) (void)hipFree(p.part);
     if (p.dstep) (void)hipFree(p.dstep);

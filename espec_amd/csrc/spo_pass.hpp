@@ -14,9 +14,11 @@ struct PassShape {
     int spread = 1;                // active workgroups at linear ids 0, spread, 2 spread, ...
     int xmap = 1;                  // contiguous strided tiles per XCD
     int grid = 0;                  // workgroups per pass (0: one per work item)
-    int rows_pen = 0;              // row pass on register pencils (rows = 64 / (np0 / 16))
+    int rows_pen = 0;              // row pass on register pencils (rows = 64 / (np0 / 16)),
+                                   // 2: one 256-point line per wavefront (rows = 1)
     int tile_pen = 0;              // strided passes on register pencils: 1 one wavefront per
-                                   // item, 2 four wavefronts sharing an LDS-staged tile
+                                   // item, 2 four wavefronts sharing an LDS-staged tile,
+                                   // 3 one 256-point column per wavefront
     int pen_pf = 0;                // pencil strided passes prefetch their next work item
 };
 
@@ -39,6 +41,7 @@ struct SpoPassPlan {
     void *kreg = nullptr;  // as kax, in the pencils' spectral-layout order
     void *twr_s = nullptr;   // split float4 twiddles, 2 np[a] per axis at 2 twr_off[a]
     void *kreg_s = nullptr;  // split float4 copy of kreg
+    void *knat = nullptr;    // as kax, in natural frequency order (wide lines)
     int kax_off[3] = {};   // offset of axis a within one packet's set
     int kax_pstride = 0;   // entries between packets' sets (0: one shared set)
     double *part = nullptr;      // [maxsamp][npk][max xblocks][3] row-pass partial sums

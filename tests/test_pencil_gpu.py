@@ -20,13 +20,18 @@ SHAPES = {
     "pen_loop": dict(rows_pen=1, tile_pen=1, pen_pf=0, grid=40, xmap=0),
     "pen_tile": dict(rows_pen=1, tile_pen=2, pen_pf=0, grid=0, xmap=1),
     "pen_tile_nox": dict(rows_pen=0, tile_pen=2, pen_pf=0, grid=0, xmap=0),
+    "wide": dict(rows_pen=2, tile_pen=3, pen_pf=0, grid=0, xmap=1),
+    "wide_rows": dict(rows_pen=2, tile_pen=1, pen_pf=1, grid=0, xmap=1),
+    "wide_cols": dict(rows_pen=1, tile_pen=3, pen_pf=0, grid=0, xmap=0),
 }
 
 
 def pinned(plan, name, np0):
     kw = dict(SHAPES[name])
-    if kw.get("rows_pen"):
+    if kw.get("rows_pen") == 1:
         kw.update(rows=1024 // np0, rows_nt=64)
+    elif kw.get("rows_pen") == 2:
+        kw.update(rows=1, rows_nt=64)
     plan.set_shape(**kw)
     got = plan.shape()
     for k, v in kw.items():
