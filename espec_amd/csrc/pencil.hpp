@@ -305,47 +305,58 @@ __device__ __forceinline__ void inverse(V (&v)[16], int j, V *xl,
 
 }  // namespace pencil
 
-// One 256-point line per wavefront ("wide" lines): lane j holds x[j + 64 r]
-// in register r, four points per lane, and a transform is four radix-4
-// Stockham stages (Govindaraju et al., SC 2008, "High performance discrete
-// Fourier transforms on graphics processors", radix-R Stockham
-// iteration), with an LDS exchange between consecutive stages.  Natural
-// order in and out, so no permutation is needed either way.  A quarter of
-// the pencil's per-lane work on four times the lanes: for single packets,
-// whose passes are the latency of one wavefront's dependent chain.
+// Wide lines: a line of L = 64, 128 or 256 points on L / 4 lanes, four
+// points per lane (lane j holds x[j + (L/4) r] in register r), so that a
+// wavefront holds 256 / L lines.  A transform is radix-4 Stockham stages
+// (Govindaraju et al., SC 2008, "High performance discrete Fourier
+// transforms on graphics processors", radix-R Stockham iteration), with
+// a final radix-2 stage for L = 128 and an LDS exchange between stages.
+// Natural order in and out, so no permutation is needed either way.  A
+// quarter of the pencil's per-lane work on four times the lanes: for
+// single packets, whose passes are the latency of one wavefront's chain.
 namespace wide {
 
-constexpr int kLine = 256;
-constexpr int kSlots = kLine + kLine / 16;
+template <int L> constexpr int slots() { return L + L / 16; }
 __device__ __forceinline__ int slot(int i) { return i + (i >> 4); }
 
-// X[k] = sum_n x[n] exp(SIGN 2 pi i n k / 256), unnormalised; tw[m] =
-// exp(2 pi i m / 256).  Stage s (ns = 4^s): lane j twiddles its inputs by
-// exp(SIGN 2 pi i r (j mod ns) / (4 ns)), takes a 4-point DFT and writes
-// output r to position (j / ns) 4 ns + (j mod ns) + r ns.
-template <int SIGN>
+// X[k] = sum_n x[n] exp(SIGN 2 pi i n k / L), unnormalised; tw[m] =
+// exp(2 pi i m / L).  Radix-4 stage s (ns = 4^s): lane j twiddles its
+// inputs by exp(SIGN 2 pi i r (j mod ns) / (4 ns)), takes a 4-point DFT and
+// writes output r to position (j / ns) 4 ns + (j mod ns) + r ns.  L = 128
+// ends with a radix-2 stage (ns = 64) on the register pairs (0, 2) and
+// (1, 3), butterflies j and j + 32.
+template <int L, int SIGN>
 __device__ __forceinline__ void stockham(double2 (&v)[4], int j, double2 *xl,
                                          const double2 *__restrict__ tw)
 {
+    static_assert(L == 64 || L == 128 || L == 256, "wide line length");
+    constexpr int LW = L / 4, S4 = L == 256 ? 4 : 3;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < S4; ++s) {
         const int ns = 1 << (2 * s);
         const int k = j & (ns - 1);
         if (s > 0) {
-            const int base = k * (64 >> (2 * s));
+            const int base = k * (LW >> (2 * s));
             v[1] = ctw<SIGN>(v[1], tw[base]);
             v[2] = ctw<SIGN>(v[2], tw[2 * base]);
             v[3] = ctw<SIGN>(v[3], tw[3 * base]);
         }
         bfly4<SIGN>(v[0], v[1], v[2], v[3]);
-        if (s == 3) break;  // the last stage writes position j + 64 r
+        if (s == S4 - 1 && L != 128) break;  // the last stage writes position j + LW r
         const int d = ((j >> (2 * s)) << (2 * s + 2)) + k;
         __syncthreads();  // the previous stage's reads of xl are done
 #pragma unroll
         for (int r = 0; r < 4; ++r) xl[slot(d + r * ns)] = v[r];
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = xl[slot(j + 64 * r)];
+        for (int r = 0; r < 4; ++r) v[r] = xl[slot(j + LW * r)];
+    }
+    if constexpr (L == 128) {
+        const double2 a = ctw<SIGN>(v[2], tw[j]), b = ctw<SIGN>(v[3], tw[j + 32]);
+        v[2] = csub(v[0], a);
+        v[0] = cadd(v[0], a);
+        v[3] = csub(v[1], b);
+        v[1] = cadd(v[1], b);
     }
 }
 

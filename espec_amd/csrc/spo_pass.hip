@@ -569,35 +569,38 @@ __global__ __launch_bounds__(256, 2) void pass_strided_tile(PassArgs a)
     if (a.last && blockIdx.x == 0 && threadIdx.x == 0) *a.dstep += 1;
 }
 
-// Row pass on wide lines (x length 256): one line per 64-thread workgroup,
+// Row pass on wide lines (x length L in {64, 128, 256}): 256 / L lines per
+// 64-thread workgroup, lane j + (L/4) c holding x[j + (L/4) r] of line c;
 // FP64 arithmetic for either storage; the state between passes keeps x in
 // natural frequency order.  Same sequence as pass_rows_pen.
-template <typename S, bool INV, bool FWD, bool PHT>
+template <typename S, int L, bool INV, bool FWD, bool PHT>
 __global__ __launch_bounds__(64) void pass_rows_wide(PassArgs a)
 {
-    __shared__ double2 xs[wide::kSlots];
-    const int j = threadIdx.x;
-    const int lines = a.n / wide::kLine;
+    constexpr int LW = L / 4, C = 64 / LW;
+    __shared__ double2 xs[C * wide::slots<L>()];
+    const int j = threadIdx.x % LW, c = threadIdx.x / LW;
+    double2 *xl = xs + c * wide::slots<L>();
+    const int blocks = a.n / (L * C);  // workgroups per packet
     const int w = blockIdx.x;
-    if (w >= lines * a.npk) return;
-    const int pk = w / lines, line = w - pk * lines;
-    const long long po = (long long)pk * a.n + (long long)line * wide::kLine + j;
+    if (w >= blocks * a.npk) return;
+    const int pk = w / blocks, blk = w - pk * blocks;
+    const long long po = (long long)pk * a.n + (long long)(blk * C + c) * L + j;
     const long long g = INV ? *a.dstep : 0;
     const bool samp = INV && (g % a.lc) == 0;
     const V2<S> *gx = reinterpret_cast<const V2<S> *>(a.x) + po;
     double2 v[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = widen<S>(gx[64 * r]);
-    if (INV) wide::stockham<-1>(v, j, xs, a.twr);
+    for (int r = 0; r < 4; ++r) v[r] = widen<S>(gx[LW * r]);
+    if (INV) wide::stockham<L, -1>(v, j, xl, a.twr);
     double cr = 0, ci = 0, nr = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         double2 ph;
         if (PHT) {
-            ph = a.vph[po + 64 * r];
+            ph = a.vph[po + LW * r];
         } else {
             double sn, cs;
-            pencil::sincos_reduced(a.cst2 * (double)reinterpret_cast<const S *>(a.vpot)[po + 64 * r],
+            pencil::sincos_reduced(a.cst2 * (double)reinterpret_cast<const S *>(a.vpot)[po + LW * r],
                                    &sn, &cs);
             ph = {cs, sn};
         }
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(64) void pass_rows_wide(PassArgs a)
             v[r].x *= a.b;
             v[r].y *= a.b;
             if (samp) {
-                const double2 q = widen<S>(reinterpret_cast<const V2<S> *>(a.psi0)[po + 64 * r]);
+                const double2 q = widen<S>(reinterpret_cast<const V2<S> *>(a.psi0)[po + LW * r]);
                 cr += q.x * v[r].x + q.y * v[r].y;
                 ci += q.x * v[r].y - q.y * v[r].x;
                 nr += v[r].x * v[r].x + v[r].y * v[r].y;
@@ -615,56 +618,59 @@ __global__ __launch_bounds__(64) void pass_rows_wide(PassArgs a)
         if (FWD) v[r] = cmulc(v[r], ph);
     }
     if (FWD) {
-        wide::stockham<1>(v, j, xs, a.twr);
+        wide::stockham<L, 1>(v, j, xl, a.twr);
         const double2 *kt = a.knat + (long long)pk * a.ktps + j;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = cmulc(v[r], kt[64 * r]);
+        for (int r = 0; r < 4; ++r) v[r] = cmulc(v[r], kt[LW * r]);
     }
     V2<S> *gy = reinterpret_cast<V2<S> *>(a.x) + po;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) gy[64 * r] = {(S)v[r].x, (S)v[r].y};
+    for (int r = 0; r < 4; ++r) gy[LW * r] = {(S)v[r].x, (S)v[r].y};
     if (samp) {
         cr = wave_sum(cr);
         ci = wave_sum(ci);
         nr = wave_sum(nr);
         const long long nc = g / a.lc;
-        if (j == 0 && nc < a.maxsamp) {
-            double *c = a.part + (((long long)nc * a.npk + pk) * a.xblocks + line) * 3;
-            c[0] = cr;
-            c[1] = ci;
-            c[2] = nr;
+        if (threadIdx.x == 0 && nc < a.maxsamp) {
+            double *cp = a.part + (((long long)nc * a.npk + pk) * a.xblocks + blk) * 3;
+            cp[0] = cr;
+            cp[1] = ci;
+            cp[2] = nr;
         }
     }
 }
 
-// Strided pass on wide lines (axis length 256): one column per 64-thread
-// workgroup, lane j holding rows j + 64 r; FP64 arithmetic.
-template <typename S>
+// Strided pass on wide lines (axis length L): 256 / L consecutive columns
+// per 64-thread workgroup, lane c + C j holding rows j + (L/4) r of column
+// c; FP64 arithmetic.
+template <typename S, int L>
 __global__ __launch_bounds__(64) void pass_strided_wide(PassArgs a)
 {
-    __shared__ double2 xs[wide::kSlots];
-    const int j = threadIdx.x;
-    const int cols = a.n / wide::kLine;  // lines of this axis per packet
-    const int total = cols * a.npk;
+    constexpr int LW = L / 4, C = 64 / LW;
+    __shared__ double2 xs[C * wide::slots<L>()];
+    const int c = threadIdx.x % C, j = threadIdx.x / C;
+    double2 *xl = xs + c * wide::slots<L>();
+    const int groups = a.n / (L * C);  // column groups per packet
+    const int total = groups * a.npk;
     int w = blockIdx.x;
     if (w >= total) return;
     if (a.xmap) w = (w & 7) * (total >> 3) + (w >> 3);
-    const int pk = w / cols, col = w - pk * cols;
-    const long long po = (long long)pk * a.n + (col % a.S) +
-                         (long long)(col / a.S) * a.S * wide::kLine + (long long)j * a.S;
-    const long long st = 64LL * a.S;
+    const int pk = w / groups, col = (w - pk * groups) * C + c;
+    const long long po = (long long)pk * a.n + (col % a.S) + (long long)(col / a.S) * a.S * L +
+                         (long long)j * a.S;
+    const long long st = (long long)LW * a.S;
     V2<S> *gx = reinterpret_cast<V2<S> *>(a.x) + po;
     double2 v[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) v[r] = widen<S>(gx[r * st]);
-    wide::stockham<1>(v, j, xs, a.twr);
+    wide::stockham<L, 1>(v, j, xl, a.twr);
     const double2 *kt = a.knat + (long long)pk * a.ktps + j;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = cmulc(v[r], kt[64 * r]);
-    wide::stockham<-1>(v, j, xs, a.twr);
+    for (int r = 0; r < 4; ++r) v[r] = cmulc(v[r], kt[LW * r]);
+    wide::stockham<L, -1>(v, j, xl, a.twr);
 #pragma unroll
     for (int r = 0; r < 4; ++r) gx[r * st] = {(S)v[r].x, (S)v[r].y};
-    if (a.last && blockIdx.x == 0 && j == 0) *a.dstep += 1;
+    if (a.last && blockIdx.x == 0 && threadIdx.x == 0) *a.dstep += 1;
 }
 
 __global__ void set_step(long long *d, long long v) { *d = v; }
@@ -794,15 +800,18 @@ hipError_t rows(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, double 
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[0];
         a.knat = static_cast<const double2 *>(p.knat) + p.kax_off[0];
         const dim3 wg(total);
-#define ESPEC_WIDE_ROWS(P_)                                                                  \
-    if (pht == P_) {                                                                         \
-        if (inv && fwd) return launch(pass_rows_wide<R, true, true, P_>, wg, 64, 0, s, a);   \
-        if (inv) return launch(pass_rows_wide<R, true, false, P_>, wg, 64, 0, s, a);         \
-        return launch(pass_rows_wide<R, false, true, P_>, wg, 64, 0, s, a);                  \
+#define ESPEC_WIDE_ROWS2(L_, P_)                                                                \
+    if (p.np[0] == L_ && pht == P_) {                                                           \
+        if (inv && fwd) return launch(pass_rows_wide<R, L_, true, true, P_>, wg, 64, 0, s, a);  \
+        if (inv) return launch(pass_rows_wide<R, L_, true, false, P_>, wg, 64, 0, s, a);        \
+        return launch(pass_rows_wide<R, L_, false, true, P_>, wg, 64, 0, s, a);                 \
     }
-        ESPEC_WIDE_ROWS(true)
-        ESPEC_WIDE_ROWS(false)
+#define ESPEC_WIDE_ROWS(L_) ESPEC_WIDE_ROWS2(L_, true) ESPEC_WIDE_ROWS2(L_, false)
+        ESPEC_WIDE_ROWS(64)
+        ESPEC_WIDE_ROWS(128)
+        ESPEC_WIDE_ROWS(256)
 #undef ESPEC_WIDE_ROWS
+#undef ESPEC_WIDE_ROWS2
         return hipErrorInvalidValue;
     }
     if (p.shape.rows_pen) {
@@ -868,10 +877,13 @@ hipError_t strided(const SpoPassPlan &p, int axis, bool last, hipStream_t s)
     if (p.shape.tile_pen == 3) {
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[axis];
         a.knat = static_cast<const double2 *>(p.knat) + p.kax_off[axis];
-        const int total = p.n / wide::kLine * p.npk;
+        const int total = p.n / 256 * p.npk;  // groups of 256 / L columns
         a.xmap = (p.shape.xmap && total % 8 == 0) ? 1 : 0;
-        if (a.L != wide::kLine) return hipErrorInvalidValue;
-        return launch(pass_strided_wide<R>, dim3(total), 64, 0, s, a);
+        const dim3 wg(total);
+        if (a.L == 64) return launch(pass_strided_wide<R, 64>, wg, 64, 0, s, a);
+        if (a.L == 128) return launch(pass_strided_wide<R, 128>, wg, 64, 0, s, a);
+        if (a.L == 256) return launch(pass_strided_wide<R, 256>, wg, 64, 0, s, a);
+        return hipErrorInvalidValue;
     }
     if (p.shape.tile_pen == 2) {
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[axis];
@@ -1027,7 +1039,8 @@ bool pen_len(int l) { return l == 64 || l == 128 || l == 256; }
 bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 {
     if (s.rows_pen == 2)
-        return p.np[0] == wide::kLine && s.rows == 1 && s.rows_nt == 64 &&
+        return pen_len(p.np[0]) && s.rows == 256 / p.np[0] && s.rows_nt == 64 &&
+               (p.n / p.np[0]) % s.rows == 0 &&
                (p.vph != nullptr || p.phase_max < pencil::kSincosMax);
     if (s.rows_pen > 2) return false;
     if (s.rows_pen)
@@ -1044,8 +1057,11 @@ bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 bool tile_ok(const SpoPassPlan &p, const PassShape &s)
 {
     if (s.tile_pen == 3) {
-        for (int a = 1; a < p.ndim; ++a)
-            if (p.np[a] != wide::kLine) return false;
+        for (int a = 1; a < p.ndim; ++a) {
+            int stride = 1;
+            for (int k = 0; k < a; ++k) stride *= p.np[k];
+            if (!pen_len(p.np[a]) || stride % (256 / p.np[a])) return false;
+        }
         return true;
     }
     if (s.tile_pen) {
@@ -1129,7 +1145,7 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
     // {rows, threads, pencil}
     const int rc_[][3] = {{1, 64, 0},  {2, 64, 0},  {4, 64, 0},  {1, 256, 0},
                           {2, 256, 0}, {4, 256, 0}, {8, 256, 0}, {1024 / p.np[0], 64, 1},
-                          {1, 64, 2}};
+                          {256 / p.np[0], 64, 2}};
     for (auto &c : rc_) {
         PassShape sh = p.shape;
         sh.rows = c[0];

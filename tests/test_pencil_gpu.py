@@ -31,7 +31,7 @@ def pinned(plan, name, np0):
     if kw.get("rows_pen") == 1:
         kw.update(rows=1024 // np0, rows_nt=64)
     elif kw.get("rows_pen") == 2:
-        kw.update(rows=1, rows_nt=64)
+        kw.update(rows=256 // np0, rows_nt=64)
     plan.set_shape(**kw)
     got = plan.shape()
     for k, v in kw.items():
@@ -61,7 +61,7 @@ def test_config2_shapes(gpu, orc, shape, flags, tol):
     plan.close()
 
 
-@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "stage", "pen_tile"])
+@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "stage", "pen_tile", "wide", "wide_cols"])
 def test_3d_shapes(gpu, orc, shape):
     """64^3 (every axis a pencil length), two runs continuing one propagation."""
     from oracle import EXACT
@@ -79,7 +79,7 @@ def test_3d_shapes(gpu, orc, shape):
     assert dC(g, o) <= 1e-10
 
 
-@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "pen_loop", "pen_tile"])
+@pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "pen_loop", "pen_tile", "wide"])
 @pytest.mark.parametrize("flags,tol", [(E.FP64, 1e-10), (E.MIXED, 1e-6)])
 def test_batch_per_packet_masses(gpu, orc, shape, flags, tol):
     """Five 128^2 packets with their own masses, potentials and states;
