@@ -92,14 +92,14 @@ def kernel_names(shape, np0, np1, storage="float"):
     launch shape selects for steps 2..K."""
     pht = "true" if shape.get("vph") else "false"
     if shape["rows_pen"] == 2:
-        rows = f"pass_rows_wide<{storage}, true, true, {pht}>"
+        rows = f"pass_rows_wide<{storage}, {np0}, true, true, {pht}>"
     elif shape["rows_pen"]:
         rows = f"pass_rows_pen<{storage}, {np0 // 16}, true, true, {pht}>"
     else:
         ept_r = shape["rows"] * np0 // shape["rows_nt"]
         rows = f"pass_rows<{storage}, {shape['rows_nt']}, true, true, {ept_r}, {pht}>"
     if shape["tile_pen"] == 3:
-        cols = f"pass_strided_wide<{storage}>"
+        cols = f"pass_strided_wide<{storage}, {np1}>"
     elif shape["tile_pen"] == 2:
         cols = f"pass_strided_tile<{storage}, {np1 // 16}>"
     elif shape["tile_pen"]:
