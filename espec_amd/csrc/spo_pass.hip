@@ -518,16 +518,20 @@ __global__ __launch_bounds__(64, 2) void pass_strided_pen(PassArgs a)
 // segment in single precision), so every global load and store covers
 // whole cache lines; each wavefront transforms C of the columns as
 // register pencils (lane c + C j, register r = row M r + j) in precision
-// S.  The staging tile overlays the exchange slots, which are idle while
-// it is in use.  One item per workgroup; requires S % W == 0.
+// S.  The staging tile and the exchange slots share one LDS array: each is
+// idle while the other is in use.  One item per workgroup; requires
+// S % W == 0.
 template <typename S, int M>
 __global__ __launch_bounds__(256, 2) void pass_strided_tile(PassArgs a)
 {
     using VS = V2<S>;
-    constexpr int C = 64 / M, L = 16 * M, W = 4 * C, P = W + 1;
+    // Row pitch of the tile: rows shifted by C elements, so that the 16
+    // lanes of a quarter-wavefront (C columns by 16 / C rows) read distinct
+    // banks when they take their pencils.
+    constexpr int C = 64 / M, L = 16 * M, W = 4 * C, P = C < 16 ? W + C : W;
     constexpr int XS = C * pencil::xslots<M>();  // exchange slots of one wavefront
-    static_assert(L * P <= 4 * XS, "tile fits the exchange slots");
-    __shared__ VS xs[4 * XS];
+    constexpr int NS = L * P > 4 * XS ? L * P : 4 * XS;
+    __shared__ VS xs[NS];
     VS *tile = xs;
     const int wv = threadIdx.x / 64, c = threadIdx.x % C, j = (threadIdx.x % 64) / C;
     VS *xl = xs + wv * XS + c * pencil::xslots<M>();
