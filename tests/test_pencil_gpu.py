@@ -61,6 +61,28 @@ def test_config2_shapes(gpu, orc, shape, flags, tol):
     plan.close()
 
 
+@pytest.mark.parametrize("shape", ["pen", "pen_tile", "wide"])
+def test_config2_phase_from_v(gpu, orc, shape):
+    """Mixed plans above 64 MB form exp(-i cst2 V) in the row pass from V
+    (table-based FP64 sincos) instead of reading a phase table: 80 copies
+    of the 256x256 packet, packets 0 and 79 against the oracle."""
+    from oracle import EXACT
+    c = configs.config2()
+    nst, npk = 40, 80
+    print(f"case: config2 x{npk} phase from V shape={shape}", flush=True)
+    plan = E.SpoPlan(c.np, c.dt_fs, 1, c.sh, c.sm, np.repeat(c.vpot[None], npk, axis=0),
+                     np.repeat(c.u[None], npk, axis=0), None, maxsamp=nst + 1,
+                     flags=E.MIXED | E.EXACT)
+    pinned(plan, shape, c.np[0])
+    assert plan.shape()["vph"] == 0
+    plan.run(nst)
+    o = orc.pspofft(c.np, c.dt_fs, 0.0, (nst - 1) * c.dt_fs, 1, c.sh, c.sm, c.u, c.v, c.vpot,
+                    EXACT)
+    for k in (0, npk - 1):
+        assert dC(plan.get(k, psi=False), o) <= 1e-6, k
+    plan.close()
+
+
 @pytest.mark.parametrize("shape", ["pen", "pen_loop_pf", "stage", "pen_tile", "wide", "wide_cols"])
 def test_3d_shapes(gpu, orc, shape):
     """64^3 (every axis a pencil length), two runs continuing one propagation."""
