@@ -26,6 +26,17 @@ inline int spectral_index(int p, int m)
     return j * (16 / m) + q + 16 * k2;
 }
 
+// Orders this wavefront's LDS accesses around an exchange inside its own
+// slots: a wavefront's LDS instructions execute in issue order, so the
+// compiler only has to keep the writes before the reads (no workgroup
+// barrier, which would also stall the other wavefronts of a workgroup).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Scalar type of a complex register type.
 template <typename V> struct ScalarOf;
 template <> struct ScalarOf<double2> { using T = double; };
@@ -251,8 +262,8 @@ template <int M> __device__ __forceinline__ int xslot(int k1, int n2)
 // Exchange slots per line.
 template <int M> constexpr int xslots() { return 17 * M; }
 
-// Natural -> spectral, sign SIGN.  xl: this line's exchange slots.  The
-// caller orders LDS between consecutive exchanges of the same slots.
+// Natural -> spectral, sign SIGN.  xl: this line's exchange slots, which
+// only this wavefront touches during the transform.
 // tw: double2 data take exp(2 pi i m / N), m in [0, N), and form the
 // powers they need; float2 data take the split table of every twiddle,
 // entry k1 M + j = W^{j k1} for the forward, then at N + (q M + n2) M + j
@@ -269,7 +280,7 @@ __device__ __forceinline__ void forward(V (&v)[16], int j, V *xl,
         twiddle_powers<SIGN, 16, 0>(v, tw, j);
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) xl[xslot<M>(k1, j)] = v[k1];
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
@@ -297,7 +308,7 @@ __device__ __forceinline__ void inverse(V (&v)[16], int j, V *xl,
     for (int q = 0; q < Q; ++q)
 #pragma unroll
         for (int n2 = 0; n2 < M; ++n2) xl[xslot<M>(j * Q + q, n2)] = v[q * M + n2];
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) v[k1] = xl[xslot<M>(k1, j)];
     dft16<SIGN, 0>(v);
@@ -344,10 +355,10 @@ __device__ __forceinline__ void stockham(double2 (&v)[4], int j, double2 *xl,
         bfly4<SIGN>(v[0], v[1], v[2], v[3]);
         if (s == S4 - 1 && L != 128) break;  // the last stage writes position j + LW r
         const int d = ((j >> (2 * s)) << (2 * s + 2)) + k;
-        __syncthreads();  // the previous stage's reads of xl are done
+        pencil::wave_sync();  // the previous stage's reads of xl are done
 #pragma unroll
         for (int r = 0; r < 4; ++r) xl[slot(d + r * ns)] = v[r];
-        __syncthreads();
+        pencil::wave_sync();
 #pragma unroll
         for (int r = 0; r < 4; ++r) v[r] = xl[slot(j + LW * r)];
     }
