@@ -239,7 +239,10 @@ struct LnzArgs {
     double *al, *bt;      // [nvec][m]
     double *beta0;        // [nvec] norm of the start vector
     double *rc;           // [nvec][ncols] Gram coefficients of the current iteration
-    double *part;         // [nvec][nchunk][nb][kPart] partial sums
+    double *part;         // [nvec][nchunk][nb][kPart] partial sums (last-block reductions)
+    double *pa, *pb, *pr; // iteration partials: alpha [nvec][nb][kPart], |w|^2 [nvec][nb][kPart],
+                          // Gram coefficients [nvec][nchunk][nb][kPart]
+    int bfrom;            // lnz_apply takes beta_{i-1} from bt (1) or from pb (0)
     unsigned *ticket;     // [nvec][nchunk]
     int *dead;            // [nvec] start vector was zero (lnz.f:85-86)
     int it;               // current iteration i (0-based)
@@ -322,8 +325,35 @@ __device__ bool last_block(const LnzArgs &a, int vec, int c, const double *v, in
     return true;
 }
 
+// Total over the nb block partials p[b kPart] that the previous kernel
+// wrote, in the fixed order of last_block (block b's to thread b, then
+// block_sum_k), so every block of the consuming kernel gets the same bits.
+// The Lanczos iteration passes its reductions this way: a kernel boundary
+// orders the partials, where a last-block reduction needs a device-scope
+// fence in every block (an L2 write-back on each XCD).
+template <int NT>
+__device__ double sum_parts1(const LnzArgs &a, const double *p, double *red)
+{
+    __shared__ double sh;
+    double v[1] = {(int)threadIdx.x < a.nb ? p[threadIdx.x * kPart] : 0.0};
+    block_sum_k<NT>(v, 1, red);
+    if (threadIdx.x == 0) sh = v[0];
+    __syncthreads();
+    const double t = sh;
+    __syncthreads();  // red and sh are reused by the caller
+    return t;
+}
+
+// Publish this block's k partials (thread 0 holds them after block_sum_k).
+__device__ __forceinline__ void put_parts(double *p, const double *v, int k)
+{
+    if (threadIdx.x == 0)
+        for (int j = 0; j < k; ++j) p[(long long)blockIdx.x * kPart + j] = v[j];
+}
+
 // Start: w = start vector of each recurrence (Re psi for even vec, Im psi
-// for odd vec, or an explicit start), beta0 = |w| (lnz.f:75-86).
+// for odd vec, or an explicit start); |w|^2 partials for lnz_apply, which
+// forms beta0 = |w| (lnz.f:75-86).
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_init(LnzArgs a, const V2<S> *x, const S *start)
 {
@@ -344,11 +374,7 @@ __global__ __launch_bounds__(NT) void lnz_init(LnzArgs a, const V2<S> *x, const 
     }
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
-    double tot[1];
-    if (last_block<NT>(a, vec, 0, v, 1, tot, red) && threadIdx.x == 0) {
-        a.beta0[vec] = sqrt(tot[0]);
-        a.dead[vec] = tot[0] == 0.0;
-    }
+    put_parts(a.pb + (long long)vec * a.nb * kPart, v, 1);
 }
 
 // lnz.f:89-108: v_i = w / beta, w = H v_i - beta v_{i-1}, alpha_i = <v_i, w>.
@@ -364,10 +390,30 @@ __global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
     const S *vprev = reinterpret_cast<const S *>(a.vring) + ((long long)((i + 1) & 1) * a.nvec + vec) * a.n;
     S *B = reinterpret_cast<S *>(a.basis) + (long long)vec * a.ncols * a.n;
     const double *vp = a.vpot + ((long long)(vec >> a.vshift)) * a.n;
-    // A zero start vector leaves the basis at zero (lnz.f:85-86); the
-    // recurrence is then carried as zeros so nothing reads a stale beta.
-    const bool dead = a.dead[vec];
-    const double b0 = dead ? 1.0 : (i == 0 ? a.beta0[vec] : a.bt[vec * a.m + i - 1]);
+    // beta_{i-1} (beta0 at i = 0) from the partials of the kernel before,
+    // or from bt when this launch extends an earlier run.  A zero start
+    // vector leaves the basis at zero (lnz.f:85-86); the recurrence is then
+    // carried as zeros so nothing reads a stale beta.
+    bool dead;
+    double bprev;
+    if (i > 0 && a.bfrom) {
+        dead = a.dead[vec];
+        bprev = a.bt[vec * a.m + i - 1];
+    } else {
+        const double t = sum_parts1<NT>(a, a.pb + (long long)vec * a.nb * kPart, red);
+        bprev = sqrt(t);
+        if (i == 0) {
+            dead = t == 0.0;
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.beta0[vec] = bprev;
+                a.dead[vec] = dead;
+            }
+        } else {
+            dead = a.dead[vec];
+            if (blockIdx.x == 0 && threadIdx.x == 0) a.bt[vec * a.m + i - 1] = bprev;
+        }
+    }
+    const double b0 = dead ? 1.0 : bprev;
     const double rb = 1.0 / b0;
     const bool store = i < a.ncols;
     double acc = 0;
@@ -386,8 +432,7 @@ __global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
     }
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
-    double tot[1];
-    if (last_block<NT>(a, vec, 0, v, 1, tot, red) && threadIdx.x == 0) a.al[vec * a.m + i] = tot[0];
+    put_parts(a.pa + (long long)vec * a.nb * kPart, v, 1);
 }
 
 // lnz.f:110-112 and reort.f:54-67: r_j = <v_j, w - alpha v_i> for the
@@ -406,7 +451,8 @@ __global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
     const S *w = reinterpret_cast<const S *>((i & 1) ? a.w0 : a.w1) + vo;
     const S *vcur = reinterpret_cast<const S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
     const S *B = reinterpret_cast<const S *>(a.basis) + ((long long)vec * a.ncols + j0) * a.n;
-    const double al = a.al[vec * a.m + i];
+    const double al = sum_parts1<NT>(a, a.pa + (long long)vec * a.nb * kPart, red);
+    if (c == 0 && blockIdx.x == 0 && threadIdx.x == 0) a.al[vec * a.m + i] = al;
     double r[kChunk];
     for (int j = 0; j < kChunk; ++j) r[j] = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
@@ -414,9 +460,7 @@ __global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
         for (int j = 0; j < jn; ++j) r[j] += ws * (double)B[(long long)j * a.n + p];
     }
     block_sum_k<NT>(r, jn, red);
-    double tot[kChunk];
-    if (last_block<NT>(a, vec, c, r, jn, tot, red) && threadIdx.x == 0)
-        for (int j = 0; j < jn; ++j) a.rc[(long long)vec * a.ncols + j0 + j] = tot[j];
+    put_parts(a.pr + ((long long)vec * a.nchunk + c) * a.nb * kPart, r, jn);
 }
 
 // w <- (w - alpha v_i) - sum_j r_j v_j (reort.f:69-78), beta_i = |w|.
@@ -432,7 +476,19 @@ __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
     const S *vcur = reinterpret_cast<const S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
     const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.ncols * a.n;
     const double al = a.al[vec * a.m + i];
-    for (int j = threadIdx.x; j < nj; j += NT) rs[j] = a.rc[(long long)vec * a.ncols + j];
+    // Gram coefficient j: wavefront j mod (NT/64) sums the nb partials of
+    // lnz_project, lanes in a fixed order, then a wavefront shuffle tree.
+    {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int j = wv; j < nj; j += NT / 64) {
+            const double *pc = a.pr + ((long long)vec * a.nchunk + j / kChunk) * a.nb * kPart +
+                               j % kChunk;
+            double x = 0;
+            for (int b = lane; b < a.nb; b += 64) x += pc[(long long)b * kPart];
+            x = wave_sum(x);
+            if (lane == 0) rs[j] = x;
+        }
+    }
     __syncthreads();
     double acc = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
@@ -444,8 +500,17 @@ __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
     }
     double v[1] = {acc};
     block_sum_k<NT>(v, 1, red);
-    double tot[1];
-    if (last_block<NT>(a, vec, 0, v, 1, tot, red) && threadIdx.x == 0) a.bt[vec * a.m + i] = sqrt(tot[0]);
+    put_parts(a.pb + (long long)vec * a.nb * kPart, v, 1);
+}
+
+// beta of the last iteration of a run (no lnz_apply follows to form it).
+template <int NT>
+__global__ __launch_bounds__(NT) void lnz_finish(LnzArgs a)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    const int vec = blockIdx.y;
+    const double t = sum_parts1<NT>(a, a.pb + (long long)vec * a.nb * kPart, red);
+    if (threadIdx.x == 0) a.bt[vec * a.m + a.it] = sqrt(t);
 }
 
 struct SilArgs {
@@ -1051,7 +1116,8 @@ struct LnzBufs {
         if ((e = bt.alloc(sizeof(double) * nv * l.m))) return e;
         if ((e = beta0.alloc(sizeof(double) * nv))) return e;
         if ((e = rc.alloc(sizeof(double) * nv * l.ncols))) return e;
-        if ((e = part.alloc(sizeof(double) * nv * l.nchunk * l.nb * kPart))) return e;
+        const size_t np1 = nv * l.nchunk * l.nb * kPart;  // one region of partials
+        if ((e = part.alloc(sizeof(double) * 4 * np1))) return e;
         if ((e = ticket.alloc(sizeof(unsigned) * nv * l.nchunk))) return e;
         if ((e = hipMemset(ticket.p, 0, ticket.bytes))) return e;
         if ((e = dead.alloc(sizeof(int) * nv))) return e;
@@ -1064,6 +1130,10 @@ struct LnzBufs {
         l.beta0 = beta0.as<double>();
         l.rc = rc.as<double>();
         l.part = part.as<double>();
+        l.pa = l.part + np1;
+        l.pb = l.part + 2 * np1;
+        l.pr = l.part + 3 * np1;
+        l.bfrom = 0;
         l.ticket = ticket.as<unsigned>();
         l.dead = dead.as<int>();
         return hipSuccess;
@@ -1141,10 +1211,14 @@ hipError_t lnz_iterations(const LnzArgs &l, hipStream_t s)
     for (int i = 0; i < l.m; ++i) {
         LnzArgs li = l;
         li.it = i;
+        li.bfrom = 0;
         hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
         hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
         hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
     }
+    LnzArgs lf = l;
+    lf.it = l.m - 1;
+    hipLaunchKernelGGL(lnz_finish<256>, dim3(1, l.nvec), dim3(256), 0, s, lf);
     return hipGetLastError();
 }
 
@@ -1246,10 +1320,14 @@ hipError_t ctl_launches(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb,
         for (int i = have; i < iters; ++i) {
             LnzArgs li = l;
             li.it = i;
+            li.bfrom = i == have && have > 0;  // extending: beta_{have-1} is in bt
             hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
             hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
             hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
         }
+        LnzArgs lf = l;
+        lf.it = iters - 1;
+        hipLaunchKernelGGL(lnz_finish<256>, dim3(1, l.nvec), dim3(256), 0, s, lf);
     }
     hipLaunchKernelGGL((ctl_coeffs<S, 256>), grid, dim3(256), 0, s, c);
     hipLaunchKernelGGL(ctl_feat, dim3((P.npk + 63) / 64), dim3(64), 0, s, c, P.cfeat.as<float>(),
@@ -1737,10 +1815,12 @@ int espec_lnz(int ndim, const int *np, int m, int nreort, const double *shm, con
     hipLaunchKernelGGL((lnz_init<double, 256>), dim3(l.nb, 1), dim3(256), 0, nullptr, l,
                        (const double2 *)nullptr, ds.as<double>());
     ESPEC_HIP_TRY(hipGetLastError());
+    // A zero start runs as zeros (the first lnz_apply sets the flag) and
+    // leaves AL, BT untouched (lnz.f:86).
+    ESPEC_HIP_TRY(lnz_iterations<double>(l, nullptr));
     int dead = 0;
     ESPEC_HIP_TRY(hipMemcpy(&dead, bufs.dead.p, sizeof(int), hipMemcpyDeviceToHost));
-    if (!dead) ESPEC_HIP_TRY(lnz_iterations<double>(l, nullptr));
-    if (!dead) {  // a zero start leaves AL, BT untouched (lnz.f:86)
+    if (!dead) {
         ESPEC_HIP_TRY(hipMemcpy(al, bufs.al.p, sizeof(double) * m, hipMemcpyDeviceToHost));
         ESPEC_HIP_TRY(hipMemcpy(bt, bufs.bt.p, sizeof(double) * m, hipMemcpyDeviceToHost));
     }
