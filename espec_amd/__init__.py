@@ -22,6 +22,9 @@ FP64, MIXED = 0x0, 0x1
 EXACT, REF = 0x0, 0x2
 PER_PACKET_MASS = 0x4
 MULTIPASS = 0x8
+# Absorbing boundaries (TPABSOR with ABSORB=.TRUE.; spofft.f:65-118, 162-176,
+# sodfft.f:66-72, 108-116): VABC is an imaginary potential or a mask.
+ABSORB_NONE, ABSORB_VOPTIC, ABSORB_SMOOTHW = 0, 1, 2
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 AMU = 1836.152667539    # electron masses per amu (espec.f:614)
 A0A = 0.5291772083      # angstrom per bohr (espec.f:67)
@@ -55,6 +58,8 @@ def _load():
     L.espec_pspofft.argtypes = prop
     L.espec_digt.argtypes = [ip, i, d, d, d, P(dp), ip]
     L.espec_spo_plan_create.argtypes = [P(vp), ip, i, ip, dp, ip, d, d, d, d, vp, ip, ip]
+    L.espec_spo_plan_create_absorb.argtypes = L.espec_spo_plan_create.argtypes + [ip, vp]
+    L.espec_pspofft_absorb.argtypes = prop + [ip, vp]
     L.espec_spo_plan_run.argtypes = [vp, ip]
     L.espec_spo_plan_sync.argtypes = [vp]
     L.espec_spo_plan_get.argtypes = [vp, ip, vp, vp, vp, ip, P(ip), vp, vp]
@@ -81,6 +86,8 @@ def _load():
     L.espec_sil_plan_get.argtypes = [vp, ip, vp, vp, vp, ip, P(ip), vp, vp]
     L.espec_sil_plan_destroy.argtypes = [vp]
     L.espec_psodfft.argtypes = prop
+    L.espec_psodfft_absorb.argtypes = prop + [ip, vp]
+    L.espec_sod_plan_create_absorb.argtypes = L.espec_spo_plan_create.argtypes + [ip, vp]
     L.espec_sod_plan_create.argtypes = L.espec_spo_plan_create.argtypes
     L.espec_sod_plan_run.argtypes = [vp, ip]
     L.espec_sod_plan_sync.argtypes = [vp]
@@ -145,18 +152,22 @@ def device_count():
     return _load().espec_device_count()
 
 
-def pspofft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT):
+def pspofft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT, absorb=ABSORB_NONE,
+            vabc=None):
     """PSPOFFT (pspofft.f:1-150): returns dict(cr, ci, norm, u, v).
 
     cr/ci are the LNZVC(NC,MPR)/LNZVC(NC,MPI) columns: sample 0 is C(TI),
-    then one sample every lc steps."""
+    then one sample every lc steps.  absorb/vabc: ABSORB=.TRUE. with
+    TPABSOR .VOPTIC or .SMOOTHW and its VABC array (spofft.f:65-118, 162-176)."""
     L = _load()
     maxs = nsteps(ti, tf, dt) // lc + 1
     cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
     u, v = _f64(u1).copy(), _f64(v1).copy()
     ns = C.c_int(0)
-    rc = L.espec_pspofft(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v,
-                         _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags)
+    va = _f64(vabc) if vabc is not None else None
+    rc = L.espec_pspofft_absorb(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v,
+                                _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags,
+                                absorb, va.ctypes.data if va is not None else None)
     _check(rc, "PSPOFFT")
     k = ns.value
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
@@ -179,19 +190,23 @@ def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT)
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
 
 
-def psodfft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT):
+def psodfft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT, absorb=ABSORB_NONE,
+            vabc=None):
     """PSODFFT (psodfft.f:1-200, step sodfft.f): returns dict(cr, ci, norm, u, v).
 
     Sample 0 is C(TI); the two half-strength start-up steps count as the
-    first step, then one sample every lc leapfrog steps."""
+    first step, then one sample every lc leapfrog steps.  absorb/vabc as
+    in pspofft (sodfft.f:66-72, 108-116)."""
     L = _load()
     nst = 1 + nsteps(ti + dt, tf, dt)
     maxs = nst // lc + 2
     cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
     u, v = _f64(u1).copy(), _f64(v1).copy()
     ns = C.c_int(0)
-    rc = L.espec_psodfft(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v,
-                         _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags)
+    va = _f64(vabc) if vabc is not None else None
+    rc = L.espec_psodfft_absorb(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v,
+                                _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags,
+                                absorb, va.ctypes.data if va is not None else None)
     _check(rc, "PSODFFT")
     k = ns.value
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
@@ -432,11 +447,14 @@ class SodPlan(_Plan):
 
     _prefix = "sod"
 
-    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=FP64 | EXACT):
+    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=FP64 | EXACT,
+                 absorb=ABSORB_NONE, vabc=None):
         vp = self._setup(np_, vpot, u, v, maxsamp)
-        rc = self._L.espec_sod_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, dt,
-                                           lc, _v3(sh), _v3(sm), self._vpot, self._u, vp,
-                                           maxsamp, flags)
+        self._vabc = _f64(vabc).ravel() if vabc is not None else None
+        rc = self._L.espec_sod_plan_create_absorb(
+            C.byref(self._h), len(np_), _np3(np_), self.npk, dt, lc, _v3(sh), _v3(sm),
+            self._vpot, self._u, vp, maxsamp, flags, absorb,
+            self._vabc.ctypes.data if self._vabc is not None else None)
         _check(rc, "sod_plan_create")
 
 
@@ -486,8 +504,10 @@ class SpoPlan(_Plan):
 
     _prefix = "spo"
 
-    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=MIXED):
-        """sm: masses per axis, or an [npk][ndim] array of per-packet masses."""
+    def __init__(self, np_, dt, lc, sh, sm, vpot, u, v=None, maxsamp=1, flags=MIXED,
+                 absorb=ABSORB_NONE, vabc=None):
+        """sm: masses per axis, or an [npk][ndim] array of per-packet masses;
+        absorb/vabc: an absorber per packet, vabc shaped like vpot."""
         vp = self._setup(np_, vpot, u, v, maxsamp)
         sm = np.asarray(sm, dtype=np.float64)
         if sm.ndim == 2:
@@ -495,9 +515,11 @@ class SpoPlan(_Plan):
             self._sm = np.ascontiguousarray(sm).ravel()
         else:
             self._sm = _v3(sm)
-        rc = self._L.espec_spo_plan_create(C.byref(self._h), len(np_), _np3(np_), self.npk, dt,
-                                           lc, _v3(sh), self._sm, self._vpot, self._u, vp,
-                                           maxsamp, flags)
+        self._vabc = _f64(vabc).ravel() if vabc is not None else None
+        rc = self._L.espec_spo_plan_create_absorb(
+            C.byref(self._h), len(np_), _np3(np_), self.npk, dt, lc, _v3(sh), self._sm,
+            self._vpot, self._u, vp, maxsamp, flags, absorb,
+            self._vabc.ctypes.data if self._vabc is not None else None)
         _check(rc, "spo_plan_create")
 
     def timing(self):

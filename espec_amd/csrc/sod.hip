@@ -18,6 +18,12 @@
 // so each pass transforms the lines of one axis of psi(t), multiplies by
 // T_a / n_a and accumulates into an FP64 buffer; the last axis' pass also
 // forms psi(t+dt) and the C(t) partial sums (ESPEC_EXACT only).
+//
+// Absorbers (ABSORB=.TRUE.): .VOPTIC adds dt-scaled W psi(t) to the
+// leapfrog sum (sodfft.f:66-72); .SMOOTHW multiplies psi(t+dt) by each
+// point's mask.  The reference never reaches its mask loop (sodfft.f:108
+// compares the 7 characters TPABSOR(1:7) with the 8-character '.SMOOTHW'),
+// so ESPEC_REF propagates .SMOOTHW without absorber.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -47,6 +53,8 @@ struct SodArgs {
     int nsteps, lc, maxsamp;
     long long step0;      // steps done before this launch (the start-up counts as one)
     double *cbuf;         // [npk][maxsamp][3]
+    int absorb;           // ESPEC_ABSORB_*
+    const double *vabc;   // [npk][n] imaginary potential or mask, or null
 };
 
 // new = prev - i h (V c + b x) in the reference's operation order
@@ -56,6 +64,21 @@ __device__ __forceinline__ double2 sod_update(double2 prev, double2 c, double2 x
 {
     const double wr = prev.x + hv * c.y, wi = prev.y - hv * c.x;
     return {wr + hb * x.y, wi - hb * x.x};
+}
+
+// sod_update with an absorber: hw = h W adds hw c to the leapfrog sum
+// (sodfft.f:68-69); mk (1 without a mask) scales the result (sodfft.f:112-115).
+__device__ __forceinline__ double2 sod_update_abs(double2 prev, double2 c, double2 x, double hv,
+                                                  double hb, int absorb, double hw, double mk)
+{
+    double wr = prev.x + hv * c.y, wi = prev.y - hv * c.x;
+    if (absorb == ESPEC_ABSORB_VOPTIC) {
+        wr = wr + hw * c.x;
+        wi = wi + hw * c.y;
+    }
+    double2 r = {wr + hb * x.y, wi - hb * x.x};
+    if (absorb == ESPEC_ABSORB_SMOOTHW) r = {mk * r.x, mk * r.y};
+    return r;
 }
 
 // H-application and update, fast path: x <- c, transform, T, inverse.
@@ -137,7 +160,15 @@ __global__ __launch_bounds__(NT) void sod_lds(SodArgs a)
             }
             for (int i = threadIdx.x; i < n; i += NT) {
                 const double2 xv = REF ? r[i < nh ? i + nh : i - nh] : r[i];
-                const double2 nw = sod_update(p[i], c[i], xv, h * vp[i], h * a.b);
+                double2 nw;
+                if (a.absorb == ESPEC_ABSORB_NONE) {
+                    nw = sod_update(p[i], c[i], xv, h * vp[i], h * a.b);
+                } else {
+                    const double *va = a.vabc + po;
+                    const double w = a.absorb == ESPEC_ABSORB_VOPTIC ? h * va[i] : 0.0;
+                    const double mk = a.absorb != ESPEC_ABSORB_SMOOTHW ? 1.0 : va[i];
+                    nw = sod_update_abs(p[i], c[i], xv, h * vp[i], h * a.b, a.absorb, w, mk);
+                }
                 p[i] = c[i];
                 c[i] = nw;
             }
@@ -199,6 +230,8 @@ struct SodPassArgs {
     const long long *dstep;
     double *cbuf, *part;
     unsigned *ticket;
+    int absorb;           // ESPEC_ABSORB_*
+    const double *vabc;   // [npk][n] or null
 };
 
 // One axis of one step for a tile of lines: forward transform of psi(t)
@@ -271,7 +304,14 @@ __global__ __launch_bounds__(NT) void sod_pass(SodPassArgs a)
         }
         const VS cv = gc[gi], pv = gp[gi];
         const double2 c = {(double)cv.x, (double)cv.y};
-        const double2 nw = sod_update({(double)pv.x, (double)pv.y}, c, r, a.h * vp[gi], a.h);
+        double2 nw;
+        if (a.absorb == ESPEC_ABSORB_NONE) {
+            nw = sod_update({(double)pv.x, (double)pv.y}, c, r, a.h * vp[gi], a.h);
+        } else {
+            const double va = a.vabc[po + gi];
+            nw = sod_update_abs({(double)pv.x, (double)pv.y}, c, r, a.h * vp[gi], a.h, a.absorb,
+                                a.h * va, va);
+        }
         const VS ns = {(S)nw.x, (S)nw.y};
         gp[gi] = cv;
         gc[gi] = ns;
@@ -337,6 +377,8 @@ struct espec_sod_plan {
     long long steps_done = 0;
     hipStream_t stream = nullptr;
     DevBuf cur, prev, psi0, vpot, tc, tw, perm, cbuf, acc, part, ticket, dstep, tax;
+    int absorb = ESPEC_ABSORB_NONE;
+    DevBuf vabc;  // [npk][n] absorber data (VABC)
     SodArgs la{};
     std::vector<SodPassArgs> passes;  // one per axis
     int pass_grid[3] = {0, 0, 0};
@@ -421,9 +463,11 @@ int sod_run_pass(espec_sod_plan &P, int nsteps)
 
 template <typename S>
 int sod_build(espec_sod_plan &P, const double *sh, const double *sm, const double *vpot,
-              const double *u, const double *v)
+              const double *u, const double *v, const double *vabc)
 {
     const int n = P.n, npk = P.npk;
+    if (P.absorb != ESPEC_ABSORB_NONE)
+        ESPEC_HIP_TRY(P.vabc.upload(vabc, sizeof(double) * (size_t)n * npk));
     std::vector<S> xs = interleave<S>(u, v, (size_t)n * npk);
     ESPEC_HIP_TRY(P.cur.upload(xs.data(), sizeof(S) * xs.size()));
     ESPEC_HIP_TRY(P.prev.upload(xs.data(), sizeof(S) * xs.size()));
@@ -456,6 +500,8 @@ int sod_build(espec_sod_plan &P, const double *sh, const double *sm, const doubl
         a.lc = P.lc;
         a.maxsamp = P.maxsamp;
         a.cbuf = P.cbuf.as<double>();
+        a.absorb = P.absorb;
+        a.vabc = P.vabc.as<double>();
         return ESPEC_OK;
     }
     // Multi-pass tables: per axis T_a / n_a in digit-reversed order.
@@ -508,6 +554,8 @@ int sod_build(espec_sod_plan &P, const double *sh, const double *sm, const doubl
         pa.nblk = P.pass_grid[a];
         pa.dstep = P.dstep.as<long long>();
         pa.cbuf = P.cbuf.as<double>();
+        pa.absorb = P.absorb;
+        pa.vabc = P.vabc.as<double>();
         maxblk = std::max(maxblk, P.pass_grid[a]);
         st *= na;
     }
@@ -531,8 +579,18 @@ int espec_sod_plan_create(espec_sod_plan **out, int ndim, const int *np, int npk
                           int lc, const double *sh, const double *sm, const double *vpot,
                           const double *u, const double *v, int maxsamp, int flags)
 {
+    return espec_sod_plan_create_absorb(out, ndim, np, npk, dt, lc, sh, sm, vpot, u, v, maxsamp,
+                                        flags, ESPEC_ABSORB_NONE, nullptr);
+}
+
+int espec_sod_plan_create_absorb(espec_sod_plan **out, int ndim, const int *np, int npk,
+                                 double dt, int lc, const double *sh, const double *sm,
+                                 const double *vpot, const double *u, const double *v,
+                                 int maxsamp, int flags, int absorb, const double *vabc)
+{
     if (!out || ndim < 1 || ndim > 3 || !np || npk < 1 || lc < 1 || !sh || !sm || !vpot || !u ||
-        maxsamp < 1)
+        maxsamp < 1 || absorb < ESPEC_ABSORB_NONE || absorb > ESPEC_ABSORB_SMOOTHW ||
+        (absorb != ESPEC_ABSORB_NONE && !vabc))
         return ESPEC_ERR_ARG;
     *out = nullptr;
     for (int a = 0; a < ndim; ++a)
@@ -546,14 +604,16 @@ int espec_sod_plan_create(espec_sod_plan **out, int ndim, const int *np, int npk
     P->maxsamp = maxsamp;
     P->mixed = (flags & ESPEC_MIXED) != 0;
     P->ref = (flags & ESPEC_REF) != 0;
+    // ESPEC_REF: the reference's .SMOOTHW branch is unreachable (sodfft.f:108).
+    P->absorb = (P->ref && absorb == ESPEC_ABSORB_SMOOTHW) ? ESPEC_ABSORB_NONE : absorb;
     if (P->ref && P->mixed) return ESPEC_ERR_UNSUPPORTED;
     P->nt = P->n <= 1024 ? 64 : 256;
     P->lds = P->n <= kSodLdsMax;
     if (!P->lds && P->ref) return ESPEC_ERR_UNSUPPORTED;
     P->la.cst = 2.0 * dt / kShfs;  // psodfft.f:80
     ESPEC_HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
-    int rc = P->mixed ? sod_build<float>(*P, sh, sm, vpot, u, v)
-                      : sod_build<double>(*P, sh, sm, vpot, u, v);
+    int rc = P->mixed ? sod_build<float>(*P, sh, sm, vpot, u, v, vabc)
+                      : sod_build<double>(*P, sh, sm, vpot, u, v, vabc);
     if (rc) return rc;
     ESPEC_HIP_TRY(launch_spo_sample(P->cur.p, P->psi0.p, P->n, npk, P->mixed,
                                     P->cbuf.as<double>(), maxsamp, 0, P->stream));
@@ -670,12 +730,22 @@ int espec_psodfft(int ndim, const int *np, double dt, double ti, double tf, int 
                   const double *sh, const double *sm, double *u1, double *v1, const double *vpot,
                   double *cr, double *ci, double *nrm, int maxsamp, int *nsamp, int flags)
 {
+    return espec_psodfft_absorb(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, cr, ci, nrm,
+                                maxsamp, nsamp, flags, ESPEC_ABSORB_NONE, nullptr);
+}
+
+int espec_psodfft_absorb(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                         const double *sh, const double *sm, double *u1, double *v1,
+                         const double *vpot, double *cr, double *ci, double *nrm, int maxsamp,
+                         int *nsamp, int flags, int absorb, const double *vabc)
+{
     if (!u1 || !v1 || !cr || !ci || maxsamp < 1) return ESPEC_ERR_ARG;
     // The start-up covers TI+DT; the leapfrog loop is DO T=TI+2DT,TF+DT,DT
     // (psodfft.f:131).
     const int nsteps = 1 + fortran_trip(ti + 2 * dt, tf + dt, dt);
     espec_sod_plan *P = nullptr;
-    int rc = espec_sod_plan_create(&P, ndim, np, 1, dt, lc, sh, sm, vpot, u1, v1, maxsamp, flags);
+    int rc = espec_sod_plan_create_absorb(&P, ndim, np, 1, dt, lc, sh, sm, vpot, u1, v1, maxsamp,
+                                          flags, absorb, vabc);
     if (rc) return rc;
     rc = espec_sod_plan_run(P, nsteps);
     if (!rc) rc = espec_sod_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u1, v1);

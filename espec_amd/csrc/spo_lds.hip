@@ -11,6 +11,10 @@
 // inverse stage that ends step k and the forward stage that starts step
 // k+1 touch the same elements), the T phase into the innermost one.
 //
+// Absorbers: .VOPTIC's damping exp(cst2 W) is folded into the V phase
+// table (both half phases); .SMOOTHW's mask multiplies psi after the
+// closing half phase (spofft.f:162-176), before the C(t) sample.
+//
 // spo_lds_ref (ESPEC_REF, FP64): the reference's order of operations,
 // including the half-swap of the flattened array around the V phases
 // (spofft.f:120-133,178-191), natural-order FFTs per axis with the f32
@@ -44,6 +48,7 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
     const V *__restrict__ gvg = reinterpret_cast<const V *>(a.vph) + po;
     const V *__restrict__ gkg = reinterpret_cast<const V *>(a.kin) + (a.kin_pp ? po : 0);
     const V *__restrict__ twg = reinterpret_cast<const V *>(a.tw);
+    const double *__restrict__ mk = a.mask ? a.mask + po : nullptr;
     const int n = a.n;
     const double b = a.b;
     const DevAxis &ax0 = a.ax[0];
@@ -85,6 +90,10 @@ __global__ __launch_bounds__(NT) void spo_lds_fast(SpoLdsArgs a)
             v = cmulc(v, w);
             v.x *= b;
             v.y *= b;
+            if (mk) {  // .SMOOTHW (spofft.f:171-174)
+                v.x *= mk[i];
+                v.y *= mk[i];
+            }
             if (samp) {
                 VS p = g0[i];
                 cr += (double)p.x * v.x + (double)p.y * v.y;
@@ -132,6 +141,7 @@ __global__ __launch_bounds__(NT) void spo_lds_ref(SpoLdsArgs a)
     const V *__restrict__ gv = reinterpret_cast<const V *>(a.vph) + po;
     const V *__restrict__ gk = reinterpret_cast<const V *>(a.kin) + (a.kin_pp ? po : 0);
     const V *__restrict__ tw = reinterpret_cast<const V *>(a.tw);
+    const double *__restrict__ mk = a.mask ? a.mask + po : nullptr;
     const int n = a.n, nh = n / 2;
     const double b = a.b;
 
@@ -170,8 +180,14 @@ __global__ __launch_bounds__(NT) void spo_lds_ref(SpoLdsArgs a)
             V p = x[i], q = x[i + nh];
             V w1 = gv[i + nh], w2 = gv[i];
             double c1 = w1.x * b, s1 = w1.y * b, c2 = w2.x * b, s2 = w2.y * b;
-            x[i + nh] = {c1 * p.x + s1 * p.y, c1 * p.y - s1 * p.x};
-            x[i] = {c2 * q.x + s2 * q.y, c2 * q.y - s2 * q.x};
+            V r1 = {c1 * p.x + s1 * p.y, c1 * p.y - s1 * p.x};
+            V r2 = {c2 * q.x + s2 * q.y, c2 * q.y - s2 * q.x};
+            if (mk) {  // .SMOOTHW: the mask of the stored point (spofft.f:171-174)
+                r1 = {mk[i + nh] * r1.x, mk[i + nh] * r1.y};
+                r2 = {mk[i] * r2.x, mk[i] * r2.y};
+            }
+            x[i + nh] = r1;
+            x[i] = r2;
         }
         __syncthreads();
         const long long g = a.step0 + st + 1;

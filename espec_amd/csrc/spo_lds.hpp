@@ -23,6 +23,7 @@ struct SpoLdsArgs {
     int vph_lds;           // stage the V phase table in LDS (fast kernel)
     int tab_lds;           // stage kinetic phases and twiddles in LDS (fast kernel)
     int ntw;               // twiddle entries, all axes
+    const double *mask;    // [npk][n] .SMOOTHW mask after the closing V half phase, or null
 };
 
 size_t spo_lds_bytes(int n, bool ref, bool vph_lds, int nt, bool tab_lds = false, int ntw = 0);

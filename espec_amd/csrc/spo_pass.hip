@@ -53,6 +53,8 @@ struct PassArgs {
     const float4 *twr_s;   // single-precision pencils: split twiddle table (2 L per axis)
     const float4 *kreg_s;  // and split kinetic phases (kreg's layout)
     const double2 *knat;   // wide lines: kinetic phases in natural frequency order
+    const double *mask;    // [npk][n] .SMOOTHW mask after the closing V half phase, or null
+                           // (row pass on LDS stages only)
 };
 
 // Constants of the pencil kernels in the form their arithmetic takes:
@@ -195,6 +197,11 @@ __global__ __launch_bounds__(NT) void pass_rows(PassArgs a)
                 v = cmulc(v, wph);
                 v.x *= b;
                 v.y *= b;
+                if (a.mask) {  // .SMOOTHW (spofft.f:171-174)
+                    const double mk = a.mask[po + i];
+                    v.x *= mk;
+                    v.y *= mk;
+                }
                 if (samp) {
                     const V p = p0[i];
                     cr += p.x * v.x + p.y * v.y;
@@ -760,6 +767,7 @@ PassArgs base_args(const SpoPassPlan &p)
     a.psi0 = p.psi0;
     a.vpot = p.vpot;
     a.vph = static_cast<const double2 *>(p.vph);
+    a.mask = p.mask;
     a.cst2 = p.cst2;
     a.part = p.part;
     a.dstep = p.dstep;
@@ -1042,6 +1050,7 @@ bool pen_len(int l) { return l == 64 || l == 128 || l == 256; }
 
 bool rows_ok(const SpoPassPlan &p, const PassShape &s)
 {
+    if (p.mask && s.rows_pen) return false;  // the mask is applied by pass_rows only
     if (s.rows_pen == 2)
         return pen_len(p.np[0]) && s.rows == 256 / p.np[0] && s.rows_nt == 64 &&
                (p.n / p.np[0]) % s.rows == 0 &&
@@ -1132,7 +1141,9 @@ inline bool clearly_faster(float ms, float best) { return ms < 0.97f * best; }
 template <typename R>
 int tune_typed(SpoPassPlan &p, hipStream_t s)
 {
-    const TuneKey key{p.ndim, p.np[0], p.np[1], p.np[2], p.npk, p.mixed};
+    // The candidates depend on the phase table and the mask as well.
+    const TuneKey key{p.ndim, p.np[0], p.np[1], p.np[2], p.npk,
+                      p.mixed | (p.vph ? 2 : 0) | (p.mask ? 4 : 0)};
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tuned.find(key);
@@ -1357,10 +1368,11 @@ int profile_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, do
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   const void *vph, double *cbuf, int maxsamp, double phase_max, int tune,
-                   hipStream_t s)
+                   const void *vph, const double *mask, double *cbuf, int maxsamp,
+                   double phase_max, int tune, hipStream_t s)
 {
     p.phase_max = phase_max;
+    p.mask = mask;
     // taxes holds ndim kinetic axes for one shared set of masses, or
     // ndim per packet (packet-major) when the masses differ per packet.
     const int nset = (int)taxes.size() / ndim;

@@ -32,6 +32,7 @@ struct SpoPassPlan {
     void *x = nullptr;
     const void *psi0 = nullptr, *vpot = nullptr;  // storage precision
     const void *vph = nullptr;  // FP64 V phase table, or null for V + sincos
+    const double *mask = nullptr;  // .SMOOTHW mask [npk][n] (forces the LDS-stage row pass)
     double *cbuf = nullptr;
     void *tw = nullptr;    // double2 twiddles, axis a at tw_off[a], ntw[a] entries
     int tw_off[3] = {}, ntw[3] = {};
@@ -57,8 +58,8 @@ struct SpoPassPlan {
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   const void *vph, double *cbuf, int maxsamp, double phase_max, int tune,
-                   hipStream_t s);
+                   const void *vph, const double *mask, double *cbuf, int maxsamp,
+                   double phase_max, int tune, hipStream_t s);
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s,
                  int *launches);
 int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, double *ms,

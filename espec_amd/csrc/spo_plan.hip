@@ -21,6 +21,8 @@ struct espec_spo_plan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf x, psi0, vph, kin, tw, perm, cbuf;
     DevBuf vst;  // multi-pass: the potential in storage precision
+    int absorb = ESPEC_ABSORB_NONE;
+    DevBuf mask;  // .SMOOTHW: FP64 [npk][n] mask applied after the closing V half phase
     SpoLdsArgs la{};
     SpoPassPlan pp{};  // multi-pass plan for grids that do not fit in LDS
     float last_ms = 0;
@@ -41,10 +43,31 @@ namespace {
 // kernels multiply by (twiddles, V and T phases) is FP64.
 template <typename S>
 int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const double *vpot,
-                 const double *u, const double *v)
+                 const double *u, const double *v, const double *vabc)
 {
     const int n = P.n, npk = P.npk;
     const double cst1 = P.dt / kShfs, cst2 = P.dt / (2.0 * kShfs);
+    // V phase tables of all packets; .VOPTIC scales each entry by the
+    // damping exp(cst2 VABC) of its point, as the reference scales DCOS
+    // and DSIN (spofft.f:68-71, 100-103), so both half phases absorb.
+    auto vphase = [&]() {
+        std::vector<double> vph((size_t)2 * n * npk);
+        for (int p = 0; p < npk; ++p) {
+            std::vector<double> t = phase_table<double>(vpot + (size_t)p * n, n, cst2);
+            if (P.absorb == ESPEC_ABSORB_VOPTIC)
+                for (int i = 0; i < n; ++i) {
+                    const double e = std::exp(cst2 * vabc[(size_t)p * n + i]);
+                    t[2 * i] *= e;
+                    t[2 * i + 1] *= e;
+                }
+            std::memcpy(vph.data() + (size_t)2 * n * p, t.data(), sizeof(double) * 2 * n);
+        }
+        return vph;
+    };
+    // .SMOOTHW: the mask multiplies psi after the closing half phase
+    // (spofft.f:162-176).
+    if (P.absorb == ESPEC_ABSORB_SMOOTHW)
+        ESPEC_HIP_TRY(P.mask.upload(vabc, sizeof(double) * (size_t)n * npk));
     std::vector<FftPlan> fp;
     std::vector<double> tw;
     std::vector<int> perm;
@@ -69,14 +92,11 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         // and always with double-precision storage: there the row pass
         // cannot hold V, psi0 and the state across the transform without
         // spilling, and the table removes V and the sincos from it.
-        const bool table =
-            !P.mixed || (size_t)n * npk * sizeof(double2) <= (size_t)64 << 20;
+        // .VOPTIC always uses the table: its damping rides on it.
+        const bool table = !P.mixed || P.absorb == ESPEC_ABSORB_VOPTIC ||
+                           (size_t)n * npk * sizeof(double2) <= (size_t)64 << 20;
         if (table) {
-            std::vector<double> vph((size_t)2 * n * npk);
-            for (int p = 0; p < npk; ++p) {
-                std::vector<double> t = phase_table<double>(vpot + (size_t)p * n, n, cst2);
-                std::memcpy(vph.data() + (size_t)2 * n * p, t.data(), sizeof(double) * 2 * n);
-            }
+            std::vector<double> vph = vphase();
             ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(double) * vph.size()));
         }
         std::vector<std::vector<double>> taxes;
@@ -87,8 +107,9 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         double vmax = 0;
         for (size_t i = 0; i < (size_t)n * npk; ++i) vmax = std::max(vmax, std::fabs(vpot[i]));
         return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, cst2, P.x.p,
-                              P.psi0.p, P.vst.p, table ? P.vph.p : nullptr, P.cbuf.as<double>(),
-                              P.maxsamp, std::fabs(cst2) * vmax, 1, P.stream);
+                              P.psi0.p, P.vst.p, table ? P.vph.p : nullptr, P.mask.as<double>(),
+                              P.cbuf.as<double>(), P.maxsamp, std::fabs(cst2) * vmax, 1,
+                              P.stream);
     }
     // Kinetic phase: natural order for the reference kernel; for the fused
     // kernel, position of frequency k is sum_a perm_a[k_a] * stride_a.
@@ -102,11 +123,8 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         kin.insert(kin.end(), k.begin(), k.end());
     }
     P.la.kin_pp = nset > 1 ? 1 : 0;
-    std::vector<double> vph((size_t)2 * n * npk);
-    for (int p = 0; p < npk; ++p) {
-        std::vector<double> t = phase_table<double>(vpot + (size_t)p * n, n, cst2);
-        std::memcpy(vph.data() + (size_t)2 * n * p, t.data(), sizeof(double) * 2 * n);
-    }
+    P.la.mask = P.mask.as<double>();
+    std::vector<double> vph = vphase();
     ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(double) * vph.size()));
     ESPEC_HIP_TRY(P.kin.upload(kin.data(), sizeof(double) * kin.size()));
     ESPEC_HIP_TRY(P.tw.upload(tw.data(), sizeof(double) * tw.size()));
@@ -155,8 +173,18 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
                           int lc, const double *sh, const double *sm, const double *vpot,
                           const double *u, const double *v, int maxsamp, int flags)
 {
+    return espec_spo_plan_create_absorb(out, ndim, np, npk, dt, lc, sh, sm, vpot, u, v, maxsamp,
+                                        flags, ESPEC_ABSORB_NONE, nullptr);
+}
+
+int espec_spo_plan_create_absorb(espec_spo_plan **out, int ndim, const int *np, int npk,
+                                 double dt, int lc, const double *sh, const double *sm,
+                                 const double *vpot, const double *u, const double *v,
+                                 int maxsamp, int flags, int absorb, const double *vabc)
+{
     if (!out || ndim < 1 || ndim > 3 || !np || npk < 1 || lc < 1 || !sh || !sm || !vpot || !u ||
-        maxsamp < 1)
+        maxsamp < 1 || absorb < ESPEC_ABSORB_NONE || absorb > ESPEC_ABSORB_SMOOTHW ||
+        (absorb != ESPEC_ABSORB_NONE && !vabc))
         return ESPEC_ERR_ARG;
     *out = nullptr;
     for (int a = 0; a < ndim; ++a)
@@ -173,6 +201,7 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
     P->mixed = (flags & ESPEC_MIXED) != 0;
     P->ref = (flags & ESPEC_REF) != 0;
     P->per_packet_mass = (flags & ESPEC_PER_PACKET_MASS) != 0;
+    P->absorb = absorb;
     if (P->ref && P->mixed) return ESPEC_ERR_UNSUPPORTED;
     P->b = 1.0 / P->n;
     P->nt = P->n <= 1024 ? 64 : 256;
@@ -192,8 +221,8 @@ int espec_spo_plan_create(espec_spo_plan **out, int ndim, const int *np, int npk
     ESPEC_HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     ESPEC_HIP_TRY(hipEventCreate(&P->ev0));
     ESPEC_HIP_TRY(hipEventCreate(&P->ev1));
-    int rc = P->mixed ? build_tables<float>(*P, sh, sm, vpot, u, v)
-                      : build_tables<double>(*P, sh, sm, vpot, u, v);
+    int rc = P->mixed ? build_tables<float>(*P, sh, sm, vpot, u, v, vabc)
+                      : build_tables<double>(*P, sh, sm, vpot, u, v, vabc);
     if (rc) return rc;
     if (!P->lds) {
         rc = spo_pass_prepare(P->pp, lc);
@@ -407,11 +436,21 @@ int espec_pspofft(int ndim, const int *np, double dt, double ti, double tf, int 
                   const double *sh, const double *sm, double *u1, double *v1, const double *vpot,
                   double *cr, double *ci, double *nrm, int maxsamp, int *nsamp, int flags)
 {
+    return espec_pspofft_absorb(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, cr, ci, nrm,
+                                maxsamp, nsamp, flags, ESPEC_ABSORB_NONE, nullptr);
+}
+
+int espec_pspofft_absorb(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                         const double *sh, const double *sm, double *u1, double *v1,
+                         const double *vpot, double *cr, double *ci, double *nrm, int maxsamp,
+                         int *nsamp, int flags, int absorb, const double *vabc)
+{
     if (!u1 || !v1 || !cr || !ci || maxsamp < 1) return ESPEC_ERR_ARG;
     // DO T=TI+DT,TF+DT,DT (pspofft.f:110).
     const int nsteps = fortran_trip(ti + dt, tf + dt, dt);
     espec_spo_plan *P = nullptr;
-    int rc = espec_spo_plan_create(&P, ndim, np, 1, dt, lc, sh, sm, vpot, u1, v1, maxsamp, flags);
+    int rc = espec_spo_plan_create_absorb(&P, ndim, np, 1, dt, lc, sh, sm, vpot, u1, v1, maxsamp,
+                                          flags, absorb, vabc);
     if (rc) return rc;
     rc = espec_spo_plan_run(P, nsteps);
     if (!rc) rc = espec_spo_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u1, v1);

@@ -35,6 +35,18 @@ module espec_capi
       integer(c_int) :: rc
     end function
 
+    function espec_spo_plan_create_absorb(plan, ndim, np, npk, dt, lc, sh, sm, vpot, u, v, &
+                                          maxsamp, flags, absorb, vabc) &
+        bind(C, name='espec_spo_plan_create_absorb') result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr) :: plan
+      integer(c_int), value :: ndim, npk, lc, maxsamp, flags, absorb
+      integer(c_int) :: np(*)
+      real(c_double), value :: dt
+      real(c_double) :: sh(*), sm(*), vpot(*), u(*), v(*), vabc(*)
+      integer(c_int) :: rc
+    end function
+
     function espec_sod_plan_create(plan, ndim, np, npk, dt, lc, sh, sm, vpot, u, v, maxsamp, &
                                    flags) bind(C, name='espec_sod_plan_create') result(rc)
       import :: c_int, c_double, c_ptr
@@ -43,6 +55,18 @@ module espec_capi
       integer(c_int) :: np(*)
       real(c_double), value :: dt
       real(c_double) :: sh(*), sm(*), vpot(*), u(*), v(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_sod_plan_create_absorb(plan, ndim, np, npk, dt, lc, sh, sm, vpot, u, v, &
+                                          maxsamp, flags, absorb, vabc) &
+        bind(C, name='espec_sod_plan_create_absorb') result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr) :: plan
+      integer(c_int), value :: ndim, npk, lc, maxsamp, flags, absorb
+      integer(c_int) :: np(*)
+      real(c_double), value :: dt
+      real(c_double) :: sh(*), sm(*), vpot(*), u(*), v(*), vabc(*)
       integer(c_int) :: rc
     end function
 
@@ -237,6 +261,18 @@ contains
     case ('MIXED', 'mixed')
       mode_flags = ESPEC_MIXED + ESPEC_EXACT
     end select
+  end function
+
+  ! Absorber kind of ABSORB/TPABSOR as PSPOFFTAUX and PSODFFTAUX test it
+  ! (spofft.f:65,162; sodfft.f:66,108): ESPEC_ABSORB_NONE when ABSORB is
+  ! false or TPABSOR names neither .VOPTIC nor .SMOOTHW.
+  integer function absorb_of(absorb, tpabsor)
+    logical, intent(in) :: absorb
+    character(len=*), intent(in) :: tpabsor
+    absorb_of = 0
+    if (.not. absorb) return
+    if (starts(tpabsor, '.VOPTIC')) absorb_of = 1
+    if (starts(tpabsor, '.SMOOTHW')) absorb_of = 2
   end function
 
   logical function starts(s, p)

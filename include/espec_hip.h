@@ -84,6 +84,30 @@ int espec_pspofft(int ndim, const int *np, double dt, double ti, double tf,
                   double *v1, const double *vpot, double *cr, double *ci,
                   double *nrm, int maxsamp, int *nsamp, int flags);
 
+/* Absorbing boundaries: PSPOFFT/PSODFFT with ABSORB=.TRUE.; TPABSOR picks
+ * the kind and VABC carries its per-point data (absorb.f:1-100):
+ *   ESPEC_ABSORB_VOPTIC   vabc = imaginary potential W (hartree, <= 0 to
+ *                         absorb): both V half phases are scaled by
+ *                         exp(cst2 W) (spofft.f:65-118); the leapfrog adds
+ *                         cst W psi(t) (sodfft.f:66-72);
+ *   ESPEC_ABSORB_SMOOTHW  vabc = mask in [0, 1]: psi is multiplied by it
+ *                         after the closing V half phase (spofft.f:162-176)
+ *                         or after the leapfrog update (sodfft.f:108-116).
+ *                         The reference's SOD never takes that branch (its
+ *                         test compares TPABSOR(1:7) with '.SMOOTHW'), so
+ *                         with ESPEC_REF PSODFFT runs without absorber. */
+enum {
+    ESPEC_ABSORB_NONE = 0,
+    ESPEC_ABSORB_VOPTIC = 1,
+    ESPEC_ABSORB_SMOOTHW = 2
+};
+/* espec_pspofft with an absorber; vabc [N] (may be NULL for NONE). */
+int espec_pspofft_absorb(int ndim, const int *np, double dt, double ti,
+                         double tf, int lc, const double *sh, const double *sm,
+                         double *u1, double *v1, const double *vpot,
+                         double *cr, double *ci, double *nrm, int maxsamp,
+                         int *nsamp, int flags, int absorb, const double *vabc);
+
 /* Replaces PSODFFT (psodfft.f:1-3) / PSODFFTAUX (sodfft.f:3-4): second-
  * order-differencing leapfrog with the FFT kinetic operator, including
  * the two half-strength start-up steps (psodfft.f:120-123).  Arguments as
@@ -92,6 +116,12 @@ int espec_psodfft(int ndim, const int *np, double dt, double ti, double tf,
                   int lc, const double *sh, const double *sm, double *u1,
                   double *v1, const double *vpot, double *cr, double *ci,
                   double *nrm, int maxsamp, int *nsamp, int flags);
+/* espec_psodfft with an absorber (sodfft.f:66-72, 108-116); vabc [N]. */
+int espec_psodfft_absorb(int ndim, const int *np, double dt, double ti,
+                         double tf, int lc, const double *sh, const double *sm,
+                         double *u1, double *v1, const double *vpot,
+                         double *cr, double *ci, double *nrm, int maxsamp,
+                         int *nsamp, int flags, int absorb, const double *vabc);
 
 /* Replaces FFT(A,B,NTOT,N,NSPAN,ISN) (fft3d.f:1): one variate of an in-place
  * multivariate transform, isn > 0 -> exp(+i ...), unnormalised; n a power
@@ -198,6 +228,14 @@ int espec_spo_plan_create(espec_spo_plan **plan, int ndim, const int *np,
                           const double *sm, const double *vpot,
                           const double *u, const double *v, int maxsamp,
                           int flags);
+/* As espec_spo_plan_create with an absorber per packet (ABSORB=.TRUE.,
+ * spofft.f:65-118, 162-176): vabc [npk][N], see ESPEC_ABSORB_*. */
+int espec_spo_plan_create_absorb(espec_spo_plan **plan, int ndim,
+                                 const int *np, int npk, double dt, int lc,
+                                 const double *sh, const double *sm,
+                                 const double *vpot, const double *u,
+                                 const double *v, int maxsamp, int flags,
+                                 int absorb, const double *vabc);
 /* Enqueue nsteps steps on the plan's stream (asynchronous). */
 int espec_spo_plan_run(espec_spo_plan *plan, int nsteps);
 int espec_spo_plan_sync(espec_spo_plan *plan);
@@ -326,6 +364,13 @@ int espec_sod_plan_create(espec_sod_plan **plan, int ndim, const int *np,
                           const double *sm, const double *vpot,
                           const double *u, const double *v, int maxsamp,
                           int flags);
+/* As espec_sod_plan_create with an absorber per packet, vabc [npk][N]. */
+int espec_sod_plan_create_absorb(espec_sod_plan **plan, int ndim,
+                                 const int *np, int npk, double dt, int lc,
+                                 const double *sh, const double *sm,
+                                 const double *vpot, const double *u,
+                                 const double *v, int maxsamp, int flags,
+                                 int absorb, const double *vabc);
 int espec_sod_plan_run(espec_sod_plan *plan, int nsteps);
 int espec_sod_plan_sync(espec_sod_plan *plan);
 int espec_sod_plan_get(espec_sod_plan *plan, int pk, double *cr, double *ci,

@@ -289,24 +289,47 @@ void orc_digt(int ndim, const int *np, const double *sh, const double *sm,
 /* -------------------------------------------------- propagator steps */
 
 /* spofft.f:120-133 and 178-191: V half-phase fused with the swap of the
- * two halves of the flattened array; post != 0 also scales by b. */
+ * two halves of the flattened array; post != 0 also scales by b.
+ * Absorbers (spofft.f:65-118, 162-176): ORC_ABS_VOPTIC multiplies both
+ * half-phases' cosine and sine by exp(cst2 vabc) of the point they act
+ * on; ORC_ABS_SMOOTHW multiplies the result of the post half-phase by the
+ * mask vabc of the point it is stored at. */
 static void vphase_swap(int n, double cst2, double b, int post, double *u,
-                        double *v, const double *vpot)
+                        double *v, const double *vpot, int absorb,
+                        const double *vabc)
 {
     int nh = n / 2;
     for (int i = 0; i < nh; ++i) {
         /* spofft.f:120-133 (pre) / 178-191 (post, reading the swapped
          * positions and scaling by b). */
-        double th1 = cst2 * vpot[post ? i + nh : i];
+        int p1 = post ? i + nh : i, p2 = post ? i : i + nh;
+        double th1 = cst2 * vpot[p1];
         double c1 = cos(th1), s1 = sin(th1);
-        double th2 = cst2 * vpot[post ? i : i + nh];
+        double th2 = cst2 * vpot[p2];
         double c2 = cos(th2), s2 = sin(th2);
+        if (absorb == ORC_ABS_VOPTIC) {
+            /* spofft.f:68-71, 76-79, 100-103, 108-111 */
+            double e1 = exp(cst2 * vabc[p1]), e2 = exp(cst2 * vabc[p2]);
+            c1 = c1 * e1;
+            s1 = s1 * e1;
+            c2 = c2 * e2;
+            s2 = s2 * e2;
+        }
         double uaux, vaux;
         if (post) {
             uaux = c1 * b * u[i] + s1 * b * v[i];
             vaux = c1 * b * v[i] - s1 * b * u[i];
-            u[i] = c2 * b * u[i + nh] + s2 * b * v[i + nh];
-            v[i] = c2 * b * v[i + nh] - s2 * b * u[i + nh];
+            double un = c2 * b * u[i + nh] + s2 * b * v[i + nh];
+            double vn = c2 * b * v[i + nh] - s2 * b * u[i + nh];
+            if (absorb == ORC_ABS_SMOOTHW) {
+                /* spofft.f:171-174: the mask of the stored point */
+                un = vabc[i] * un;
+                vn = vabc[i] * vn;
+                uaux = vabc[i + nh] * uaux;
+                vaux = vabc[i + nh] * vaux;
+            }
+            u[i] = un;
+            v[i] = vn;
         } else {
             uaux = c1 * u[i] + s1 * v[i];
             vaux = c1 * v[i] - s1 * u[i];
@@ -318,12 +341,13 @@ static void vphase_swap(int n, double cst2, double b, int post, double *u,
     }
 }
 
-void orc_spo_step(int ndim, const int *np, double b, double cst1, double cst2,
-                  double *u, double *v, const double *tc, const double *vpot,
-                  int mode)
+void orc_spo_step_abs(int ndim, const int *np, double b, double cst1,
+                      double cst2, double *u, double *v, const double *tc,
+                      const double *vpot, int absorb, const double *vabc,
+                      int mode)
 {
     int n = ntot_of(ndim, np);
-    vphase_swap(n, cst2, b, 0, u, v, vpot);
+    vphase_swap(n, cst2, b, 0, u, v, vpot, absorb, vabc);
     fft_all(ndim, np, u, v, +1, mode);
     for (int i = 0; i < n; ++i) {
         double th = cst1 * tc[i];
@@ -339,18 +363,37 @@ void orc_spo_step(int ndim, const int *np, double b, double cst1, double cst2,
         }
     }
     fft_all(ndim, np, u, v, -1, mode);
-    vphase_swap(n, cst2, b, 1, u, v, vpot);
+    vphase_swap(n, cst2, b, 1, u, v, vpot, absorb, vabc);
 }
 
-/* sodfft.f:3-129 without absorber. */
-void orc_sod_step(int ndim, const int *np, double b, double cst, double *u1,
-                  double *u2, double *v1, double *v2, const double *tc,
-                  const double *vpot, double *wr, double *wi, int mode)
+void orc_spo_step(int ndim, const int *np, double b, double cst1, double cst2,
+                  double *u, double *v, const double *tc, const double *vpot,
+                  int mode)
+{
+    orc_spo_step_abs(ndim, np, b, cst1, cst2, u, v, tc, vpot, ORC_ABS_NONE, NULL, mode);
+}
+
+/* sodfft.f:3-129.  ORC_ABS_VOPTIC adds cst vabc psi(t) to the leapfrog
+ * sum (sodfft.f:66-72).  The mask loop of ORC_ABS_SMOOTHW
+ * (sodfft.f:108-116) is never reached in the reference: it compares the
+ * 7 characters TPABSOR(1:7) with the 8-character '.SMOOTHW', which cannot
+ * match, so ORC_REF propagates without absorber; ORC_EXACT multiplies
+ * psi(t+dt) by the mask of each point (the loop itself would use the mask
+ * of point i for point i + N/2 as well). */
+void orc_sod_step_abs(int ndim, const int *np, double b, double cst, double *u1,
+                      double *u2, double *v1, double *v2, const double *tc,
+                      const double *vpot, int absorb, const double *vabc,
+                      double *wr, double *wi, int mode)
 {
     int n = ntot_of(ndim, np), nh = n / 2;
     for (int i = 0; i < n; ++i) {
-        wr[i] = u1[i] + cst * vpot[i] * v2[i];
-        wi[i] = v1[i] - cst * vpot[i] * u2[i];
+        if (absorb == ORC_ABS_VOPTIC) {
+            wr[i] = u1[i] + cst * vpot[i] * v2[i] + cst * vabc[i] * u2[i];
+            wi[i] = v1[i] - cst * vpot[i] * u2[i] + cst * vabc[i] * v2[i];
+        } else {
+            wr[i] = u1[i] + cst * vpot[i] * v2[i];
+            wi[i] = v1[i] - cst * vpot[i] * u2[i];
+        }
         u1[i] = u2[i];
         v1[i] = v2[i];
     }
@@ -367,11 +410,27 @@ void orc_sod_step(int ndim, const int *np, double b, double cst, double *u1,
     for (int i = 0; i < nh; ++i) {
         double uaux = wr[i + nh] + cst * b * v2[i];
         double vaux = wi[i + nh] - cst * b * u2[i];
-        u2[i] = wr[i] + cst * b * v2[i + nh];
-        v2[i] = wi[i] - cst * b * u2[i + nh];
+        double un = wr[i] + cst * b * v2[i + nh];
+        double vn = wi[i] - cst * b * u2[i + nh];
+        if (absorb == ORC_ABS_SMOOTHW && mode == ORC_EXACT) {
+            un = vabc[i] * un;
+            vn = vabc[i] * vn;
+            uaux = vabc[i + nh] * uaux;
+            vaux = vabc[i + nh] * vaux;
+        }
+        u2[i] = un;
+        v2[i] = vn;
         u2[i + nh] = uaux;
         v2[i + nh] = vaux;
     }
+}
+
+void orc_sod_step(int ndim, const int *np, double b, double cst, double *u1,
+                  double *u2, double *v1, double *v2, const double *tc,
+                  const double *vpot, double *wr, double *wi, int mode)
+{
+    orc_sod_step_abs(ndim, np, b, cst, u1, u2, v1, v2, tc, vpot, ORC_ABS_NONE, NULL, wr, wi,
+                     mode);
 }
 
 /* Fortran 77 trip count of DO T=M1,M2,M3: MAX(INT((M2-M1+M3)/M3),0). */
@@ -431,10 +490,11 @@ static void corr(int n, const double *u0, const double *v0, const double *u,
 }
 
 /* pspofft.f:72-145 with PRTCRL='.NO' (no print-time energy checks). */
-int orc_pspofft(int ndim, const int *np, double dt, double ti, double tf,
-                int lc, const double *sh, const double *sm, double *u1,
-                double *v1, const double *vpot, double *cr, double *ci,
-                int maxsamp, double *e, double *nrm, int mode)
+int orc_pspofft_abs(int ndim, const int *np, double dt, double ti, double tf,
+                    int lc, const double *sh, const double *sm, double *u1,
+                    double *v1, const double *vpot, int absorb,
+                    const double *vabc, double *cr, double *ci, int maxsamp,
+                    double *e, double *nrm, int mode)
 {
     int n = ntot_of(ndim, np);
     double cst1 = dt / ORC_SHFS, cst2 = dt / (2.0 * ORC_SHFS), b;
@@ -450,7 +510,7 @@ int orc_pspofft(int ndim, const int *np, double dt, double ti, double tf,
     }
     int ns = orc_nsteps(ti, tf, dt);
     for (int s = 0; s < ns; ++s) {
-        orc_spo_step(ndim, np, b, cst1, cst2, u1, v1, tc, vpot, mode);
+        orc_spo_step_abs(ndim, np, b, cst1, cst2, u1, v1, tc, vpot, absorb, vabc, mode);
         if (++mc == lc) {
             ++nc;
             mc = 0;
@@ -465,12 +525,22 @@ int orc_pspofft(int ndim, const int *np, double dt, double ti, double tf,
     return nc + 1;
 }
 
-/* psodfft.f:75-200 with PRTCRL='.NO': two half-strength start-up steps
- * from psi0, then leapfrog from TI+2DT. */
-int orc_psodfft(int ndim, const int *np, double dt, double ti, double tf,
+int orc_pspofft(int ndim, const int *np, double dt, double ti, double tf,
                 int lc, const double *sh, const double *sm, double *u1,
                 double *v1, const double *vpot, double *cr, double *ci,
-                int maxsamp, int mode)
+                int maxsamp, double *e, double *nrm, int mode)
+{
+    return orc_pspofft_abs(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, ORC_ABS_NONE,
+                           NULL, cr, ci, maxsamp, e, nrm, mode);
+}
+
+/* psodfft.f:75-200 with PRTCRL='.NO': two half-strength start-up steps
+ * from psi0, then leapfrog from TI+2DT. */
+int orc_psodfft_abs(int ndim, const int *np, double dt, double ti, double tf,
+                    int lc, const double *sh, const double *sm, double *u1,
+                    double *v1, const double *vpot, int absorb,
+                    const double *vabc, double *cr, double *ci, int maxsamp,
+                    int mode)
 {
     int n = ntot_of(ndim, np);
     double cst = 2.0 * dt / ORC_SHFS, b;
@@ -484,8 +554,10 @@ int orc_psodfft(int ndim, const int *np, double dt, double ti, double tf,
     orc_digt(ndim, np, sh, sm, tc, &b, mode);
     int nc = 0, mc;
     if (maxsamp > 0) corr(n, u0, v0, u2, v2, &cr[0], &ci[0]);
-    orc_sod_step(ndim, np, b, cst / 2.0, u1, u2, v1, v2, tc, vpot, wr, wi, mode);
-    orc_sod_step(ndim, np, b, cst / 2.0, u1, u2, v1, v2, tc, vpot, wr, wi, mode);
+    orc_sod_step_abs(ndim, np, b, cst / 2.0, u1, u2, v1, v2, tc, vpot, absorb, vabc, wr, wi,
+                     mode);
+    orc_sod_step_abs(ndim, np, b, cst / 2.0, u1, u2, v1, v2, tc, vpot, absorb, vabc, wr, wi,
+                     mode);
     mc = 1;
     if (mc == lc) {
         ++nc;
@@ -496,7 +568,7 @@ int orc_psodfft(int ndim, const int *np, double dt, double ti, double tf,
     memcpy(v1, v0, sizeof(double) * n);
     int ns = trip(ti + 2 * dt, tf + dt, dt);
     for (int s = 0; s < ns; ++s) {
-        orc_sod_step(ndim, np, b, cst, u1, u2, v1, v2, tc, vpot, wr, wi, mode);
+        orc_sod_step_abs(ndim, np, b, cst, u1, u2, v1, v2, tc, vpot, absorb, vabc, wr, wi, mode);
         if (++mc == lc) {
             ++nc;
             mc = 0;
@@ -509,6 +581,15 @@ int orc_psodfft(int ndim, const int *np, double dt, double ti, double tf,
     memcpy(v1, v2, sizeof(double) * n);
     free(w);
     return nc + 1;
+}
+
+int orc_psodfft(int ndim, const int *np, double dt, double ti, double tf,
+                int lc, const double *sh, const double *sm, double *u1,
+                double *v1, const double *vpot, double *cr, double *ci,
+                int maxsamp, int mode)
+{
+    return orc_psodfft_abs(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, ORC_ABS_NONE,
+                           NULL, cr, ci, maxsamp, mode);
 }
 
 /* --------------------------------------------------- FD Hamiltonian */

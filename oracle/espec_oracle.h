@@ -25,6 +25,10 @@ extern "C" {
 #endif
 
 enum { ORC_EXACT = 0, ORC_REF = 1 };
+/* Absorbing boundaries (TPABSOR): none, .VOPTIC (vabc = imaginary
+ * potential, hartree; spofft.f:65-118, sodfft.f:66-72) and .SMOOTHW
+ * (vabc = mask; spofft.f:162-176, sodfft.f:108-116). */
+enum { ORC_ABS_NONE = 0, ORC_ABS_VOPTIC = 1, ORC_ABS_SMOOTHW = 2 };
 
 /* Atomic unit of time in fs used by every propagator (pspofft.f:76). */
 #define ORC_SHFS 2.4188843243e-2
@@ -50,6 +54,16 @@ void orc_sod_step(int ndim, const int *np, double b, double cst, double *u1,
                   double *u2, double *v1, double *v2, const double *tc,
                   const double *vpot, double *wr, double *wi, int mode);
 
+/* PSPOFFTAUX / PSODFFTAUX with an absorber (see ORC_ABS_*). */
+void orc_spo_step_abs(int ndim, const int *np, double b, double cst1,
+                      double cst2, double *u, double *v, const double *tc,
+                      const double *vpot, int absorb, const double *vabc,
+                      int mode);
+void orc_sod_step_abs(int ndim, const int *np, double b, double cst, double *u1,
+                      double *u2, double *v1, double *v2, const double *tc,
+                      const double *vpot, int absorb, const double *vabc,
+                      double *wr, double *wi, int mode);
+
 /* Trip count of the reference's REAL DO loop DO T=TI+DT,TF+DT,DT. */
 int orc_nsteps(double ti, double tf, double dt);
 
@@ -60,11 +74,23 @@ int orc_pspofft(int ndim, const int *np, double dt, double ti, double tf,
                 int lc, const double *sh, const double *sm, double *u1,
                 double *v1, const double *vpot, double *cr, double *ci,
                 int maxsamp, double *e, double *nrm, int mode);
+/* As orc_pspofft with ABSORB=.TRUE. and TPABSOR selected by absorb; vabc
+ * [N] is the VABC argument (absorb.f). */
+int orc_pspofft_abs(int ndim, const int *np, double dt, double ti, double tf,
+                    int lc, const double *sh, const double *sm, double *u1,
+                    double *v1, const double *vpot, int absorb,
+                    const double *vabc, double *cr, double *ci, int maxsamp,
+                    double *e, double *nrm, int mode);
 /* psodfft.f:1-211 PSODFFT with PRTCRL='.NO'. */
 int orc_psodfft(int ndim, const int *np, double dt, double ti, double tf,
                 int lc, const double *sh, const double *sm, double *u1,
                 double *v1, const double *vpot, double *cr, double *ci,
                 int maxsamp, int mode);
+int orc_psodfft_abs(int ndim, const int *np, double dt, double ti, double tf,
+                    int lc, const double *sh, const double *sm, double *u1,
+                    double *v1, const double *vpot, int absorb,
+                    const double *vabc, double *cr, double *ci, int maxsamp,
+                    int mode);
 
 /* fcorrel.f:1 / ecnorm.f:1 / eigenergfft.f:1 / eigenerg.f:1. */
 double orc_fcorrel(int n, const double *a, const double *b);

@@ -16,6 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 EXACT, REF = 0, 1
+ABS_NONE, ABS_VOPTIC, ABS_SMOOTHW = 0, 1, 2  # TPABSOR .VOPTIC / .SMOOTHW
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -62,6 +63,15 @@ class Oracle:
         L.orc_psodfft.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
                                   _d, _d, _ip, _ip]
         L.orc_psodfft.restype = _ip
+        L.orc_spo_step_abs.argtypes = [_ip, _i, _dp, _dp, _dp, _d, _d, _d, _d, _ip, _d, _ip]
+        L.orc_sod_step_abs.argtypes = [_ip, _i, _dp, _dp, _d, _d, _d, _d, _d, _d, _ip, _d,
+                                       _d, _d, _ip]
+        L.orc_pspofft_abs.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
+                                      _ip, _d, _d, _d, _ip, C.c_void_p, C.c_void_p, _ip]
+        L.orc_pspofft_abs.restype = _ip
+        L.orc_psodfft_abs.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
+                                      _ip, _d, _d, _d, _ip, _ip]
+        L.orc_psodfft_abs.restype = _ip
         L.orc_fcorrel.argtypes = [_ip, _d, _d]
         L.orc_fcorrel.restype = _dp
         L.orc_efft4.argtypes = [_ip, _i, _d, _d, _d, _d, _ip]
@@ -121,6 +131,23 @@ class Oracle:
                               _f64(vpot), mode)
         return u, v
 
+    def spo_step_abs(self, np_, b, cst1, cst2, u, v, tc, vpot, absorb, vabc, mode=REF):
+        """PSPOFFTAUX with ABSORB=.TRUE. (spofft.f:65-118, 162-176)."""
+        u, v = _f64(u).copy(), _f64(v).copy()
+        self.lib.orc_spo_step_abs(len(np_), _np3(np_), b, cst1, cst2, u, v, _f64(tc),
+                                  _f64(vpot), absorb, _f64(vabc), mode)
+        return u, v
+
+    def sod_step_abs(self, np_, b, cst, u1, u2, v1, v2, tc, vpot, absorb, vabc, mode=REF):
+        """PSODFFTAUX with ABSORB=.TRUE. (sodfft.f:66-72, 108-116); returns
+        (u1, u2, v1, v2) after the step."""
+        u1, u2, v1, v2 = (_f64(a).copy() for a in (u1, u2, v1, v2))
+        n = u1.size
+        wr, wi = np.zeros(n), np.zeros(n)
+        self.lib.orc_sod_step_abs(len(np_), _np3(np_), b, cst, u1, u2, v1, v2, _f64(tc),
+                                  _f64(vpot), absorb, _f64(vabc), wr, wi, mode)
+        return u1, u2, v1, v2
+
     def nsteps(self, ti, tf, dt):
         return self.lib.orc_nsteps(ti, tf, dt)
 
@@ -145,6 +172,32 @@ class Oracle:
         u, v = _f64(u).copy(), _f64(v).copy()
         k = self.lib.orc_psodfft(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm),
                                  u, v, _f64(vpot), cr, ci, ns, mode)
+        return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
+
+    def pspofft_abs(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, absorb, vabc, mode=REF,
+                    energies=False):
+        """PSPOFFT with ABSORB=.TRUE.: absorb ABS_VOPTIC (vabc = imaginary
+        potential) or ABS_SMOOTHW (vabc = mask); spofft.f:65-118, 162-176."""
+        ns = self.nsteps(ti, tf, dt) // max(lc, 1) + 2
+        cr, ci = np.zeros(ns), np.zeros(ns)
+        e, nr = np.zeros(ns), np.zeros(ns)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        k = self.lib.orc_pspofft_abs(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm),
+                                     u, v, _f64(vpot), absorb, _f64(vabc), cr, ci, ns,
+                                     e.ctypes.data if energies else None,
+                                     nr.ctypes.data if energies else None, mode)
+        out = dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
+        if energies:
+            out.update(e=e[:k], norm=nr[:k])
+        return out
+
+    def psodfft_abs(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, absorb, vabc, mode=REF):
+        """PSODFFT with ABSORB=.TRUE. (sodfft.f:66-72, 108-116)."""
+        ns = self.nsteps(ti, tf, dt) // max(lc, 1) + 3
+        cr, ci = np.zeros(ns), np.zeros(ns)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        k = self.lib.orc_psodfft_abs(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm),
+                                     u, v, _f64(vpot), absorb, _f64(vabc), cr, ci, ns, mode)
         return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
 
     def plnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, mode=REF):
@@ -257,6 +310,9 @@ class Reference:
         L.ref_pspofft.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _d,
                                   _d, _ip, P(_ip)]
         L.ref_psodfft.argtypes = L.ref_pspofft.argtypes
+        L.ref_pspofft_abs.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
+                                      _ip, _d, _d, _d, _ip, P(_ip)]
+        L.ref_psodfft_abs.argtypes = L.ref_pspofft_abs.argtypes
         L.ref_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d,
                                _d, _d, _ip, P(_ip)]
         L.ref_fft.argtypes = [_d, _d, _ip, _ip, _ip, _ip]
@@ -287,6 +343,24 @@ class Reference:
 
     def pspofft(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot):
         return self._prop(self.lib.ref_pspofft, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, 0)
+
+    def _prop_abs(self, fn, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, absorb, vabc, extra):
+        nst = int(((tf + dt) - (ti + dt) + dt) / dt)
+        nsamp = nst // max(lc, 1) + 1 + extra
+        cr, ci = np.zeros(nsamp), np.zeros(nsamp)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        info = _ip(0)
+        fn(len(np_), _np3(np_), dt, ti, tf, lc, _v3(sh), _v3(sm), u, v, _f64(vpot), absorb,
+           _f64(vabc), cr, ci, nsamp, C.byref(info))
+        return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
+
+    def pspofft_abs(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, absorb, vabc):
+        return self._prop_abs(self.lib.ref_pspofft_abs, np_, dt, ti, tf, lc, sh, sm, u, v,
+                              vpot, absorb, vabc, 0)
+
+    def psodfft_abs(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, absorb, vabc):
+        return self._prop_abs(self.lib.ref_psodfft_abs, np_, dt, ti, tf, lc, sh, sm, u, v,
+                              vpot, absorb, vabc, 1)
 
     def psodfft(self, np_, dt, ti, tf, lc, sh, sm, u, v, vpot):
         return self._prop(self.lib.ref_psodfft, np_, dt, ti, tf, lc, sh, sm, u, v, vpot, 1)

@@ -89,6 +89,79 @@ contains
     deallocate(tc, vabc, lnzvc, u2, v2, wr, wi)
   end subroutine ref_psodfft
 
+  ! TPABSOR of absorber kind 1 (.VOPTIC) or 2 (.SMOOTHW).
+  subroutine absstr(kind, s)
+    integer(c_int), value :: kind
+    character(len=8) :: s
+    s = '.NONE   '
+    if (kind == 1) s = '.VOPTIC '
+    if (kind == 2) s = '.SMOOTHW'
+  end subroutine absstr
+
+  ! PSPOFFT with ABSORB=.TRUE., TPABSOR from kind and the caller's VABC.
+  subroutine ref_pspofft_abs(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, &
+                             kind, vabc, cr, ci, nsamp, info) bind(C, name='ref_pspofft_abs')
+    integer(c_int), value :: ndim, lc, nsamp, kind
+    integer(c_int) :: np(3)
+    real(c_double), value :: dt, ti, tf
+    real(c_double) :: sh(3), sm(3), u1(*), v1(*), vpot(*), vabc(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims, tpab
+    integer :: n, mxdct, i
+    real(8), allocatable :: tc(:), lnzvc(:,:), w1(:), w2(:)
+    real(8) :: shm(3), xi(3), xf(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(tc(n), lnzvc(mxdct, 2), w1(n), w2(n))
+    lnzvc = 0d0
+    shm = 0d0; xi = 0d0; xf = 0d0; var = 0d0
+    call dimstr(ndim, dims)
+    call absstr(kind, tpab)
+    call PSPOFFT(.true., dims, '.NO     ', tpab, info, lc, 0, 2, 1, &
+                 mxdct, n, dt, ti, tf, 1d30, np, shm, sh, sm, u1, v1, xi, xf, &
+                 tc, vpot, vabc, lnzvc, w1, w2, var)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, 1)
+      ci(i) = lnzvc(i, 2)
+    end do
+    deallocate(tc, lnzvc, w1, w2)
+  end subroutine ref_pspofft_abs
+
+  ! PSODFFT with ABSORB=.TRUE., TPABSOR from kind and the caller's VABC.
+  subroutine ref_psodfft_abs(ndim, np, dt, ti, tf, lc, sh, sm, u1, v1, vpot, &
+                             kind, vabc, cr, ci, nsamp, info) bind(C, name='ref_psodfft_abs')
+    integer(c_int), value :: ndim, lc, nsamp, kind
+    integer(c_int) :: np(3)
+    real(c_double), value :: dt, ti, tf
+    real(c_double) :: sh(3), sm(3), u1(*), v1(*), vpot(*), vabc(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims, tpab
+    integer :: n, mxdct, i
+    real(8), allocatable :: tc(:), lnzvc(:,:), u2(:), v2(:), wr(:), wi(:)
+    real(8) :: shm(3), xi(3), xf(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(tc(n), lnzvc(mxdct, 2), u2(n), v2(n), wr(n), wi(n))
+    lnzvc = 0d0
+    shm = 0d0; xi = 0d0; xf = 0d0; var = 0d0
+    call dimstr(ndim, dims)
+    call absstr(kind, tpab)
+    call PSODFFT(.true., dims, '.NO     ', tpab, info, lc, 0, 2, 1, &
+                 mxdct, n, dt, ti, tf, 1d30, np, shm, sh, sm, u1, u2, v1, v2, &
+                 xi, xf, tc, vpot, vabc, wr, wi, var, lnzvc)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, 1)
+      ci(i) = lnzvc(i, 2)
+    end do
+    deallocate(tc, lnzvc, u2, v2, wr, wi)
+  end subroutine ref_psodfft_abs
+
   ! PLNZ (plnz.f:1-156); Krylov order mp, DSTEVX tolerance abstol.
   subroutine ref_plnz(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, &
                       cr, ci, nsamp, info) bind(C, name='ref_plnz')
