@@ -941,6 +941,12 @@ __global__ __launch_bounds__(NT) void ctl_coeffs(CtlArgs c)
         return;
     }
     const double b0 = a.beta0[vec];
+    __shared__ double tq_[kMaxKrylov], u2[kMaxKrylov];
+    __shared__ double pc[2][kMaxKrylov], ps[2][kMaxKrylov];
+    __shared__ double ce[2][kMaxKrylov], se[2][kMaxKrylov];  // cos, sin(lam_j tend): T_m, T_{m-1}
+    __shared__ double yr[2][kMaxKrylov], yi[2][kMaxKrylov];  // exp(-i tend T) e1: T_m, T_{m-1}
+    if (threadIdx.x == 0)
+        for (int j = 0; j < m; ++j) tq_[j] = tot[j];
     if (threadIdx.x < 64) {  // wavefront 0: T_m; wavefront 1: T_{m-1}
         const int r = leading_ql_wave(a, vec, m, d[0], ee[0], z[0]);
         if (threadIdx.x == 0) bad = r;
@@ -952,37 +958,78 @@ __global__ __launch_bounds__(NT) void ctl_coeffs(CtlArgs c)
         c.est4[vec] = krylov_est(4, d4, z4, a.bt[vec * ms + 3], b0, c.tfeat);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (bad) *c.info = bad;
-        const double *dd = d[0], *zz = z[0];
-        double u2[kMaxKrylov], pc[kMaxKrylov], ps[kMaxKrylov];
-        for (int j = 0; j < m; ++j) {  // Q^T c
-            double t = 0;
-            for (int i = 0; i < m; ++i) t += zz[j * m + i] * tot[i];
-            u2[j] = t;
-        }
+    // Wavefront 0 forms the coefficients and the estimate, lane j taking
+    // eigenvalue j (lanes 32 + j those of T_{m-1}) and then row i of the
+    // products; every sum keeps the ascending order of the serial form.
+    const double *dd = d[0], *zz = z[0];
+    const double tend = c.nsub * c.tau;
+    const int k = threadIdx.x;
+    if (k < m) {
+        double t = 0;  // (Q^T c)_k
+        for (int i = 0; i < m; ++i) t += zz[k * m + i] * tq_[i];
+        u2[k] = t;
         for (int q = 0; q < c.nsub; ++q) {
             const double tq = (q + 1) * c.tau;
+            const double cs = cos(dd[k] * tq), sn = sin(dd[k] * tq);
+            pc[q][k] = cs * t;
+            ps[q][k] = ((vec & 1) ? sn : -sn) * t;
+        }
+        ce[0][k] = cos(dd[k] * tend);
+        se[0][k] = sin(dd[k] * tend);
+    } else if (k >= 32 && k < 32 + m - 1) {
+        ce[1][k - 32] = cos(d[1][k - 32] * tend);
+        se[1][k - 32] = sin(d[1][k - 32] * tend);
+    }
+    __syncthreads();
+    if (k < m) {
+        for (int q = 0; q < c.nsub; ++q) {  // Q (...)
+            double tc = 0, ts = 0;
             for (int j = 0; j < m; ++j) {
-                const double cs = cos(dd[j] * tq), sn = sin(dd[j] * tq);
-                pc[j] = cs * u2[j];
-                ps[j] = ((vec & 1) ? sn : -sn) * u2[j];
+                tc += pc[q][j] * zz[j * m + k];
+                ts += ps[q][j] * zz[j * m + k];
             }
             double *cq = coef + q * 2 * ms;
-            for (int i = 0; i < m; ++i) {  // Q (...)
-                double tc = 0, ts = 0;
-                for (int j = 0; j < m; ++j) {
-                    tc += pc[j] * zz[j * m + i];
-                    ts += ps[j] * zz[j * m + i];
-                }
-                cq[i] = tc;
-                cq[ms + i] = ts;
-            }
+            cq[k] = tc;
+            cq[ms + k] = ts;
         }
-        const double tend = c.nsub * c.tau;
-        const double res = krylov_est(m, dd, zz, a.bt[vec * ms + m - 1], b0, tend);
-        const double dif = krylov_diff(m - 1, d[1], z[1], m, dd, zz, b0, tend);
-        c.est[vec] = fmax(res, dif);
+        double sr = 0, si = 0;  // y_m[k]
+        for (int j = 0; j < m; ++j) {
+            const double w = zz[j * m + k] * zz[j * m];
+            sr += w * ce[0][j];
+            si -= w * se[0][j];
+        }
+        yr[0][k] = sr;
+        yi[0][k] = si;
+    } else if (k >= 32 && k < 32 + m - 1) {
+        const int i = k - 32, m1 = m - 1;
+        const double *z1 = z[1];
+        double sr = 0, si = 0;  // y_{m-1}[i]
+        for (int j = 0; j < m1; ++j) {
+            const double w = z1[j * m1 + i] * z1[j * m1];
+            sr += w * ce[1][j];
+            si -= w * se[1][j];
+        }
+        yr[1][i] = sr;
+        yi[1][i] = si;
+    }
+    __syncthreads();
+    if (k == 0) {
+        if (bad) *c.info = bad;
+        // residual estimate of order m (krylov_est) and beta0 |y_{m-1} - y_m|
+        double sr = 0, si = 0;
+        for (int j = 0; j < m; ++j) {
+            const double w = zz[j * m + m - 1] * zz[j * m];
+            sr += w * ce[0][j];
+            si -= w * se[0][j];
+        }
+        const double res = b0 * a.bt[vec * ms + m - 1] * sqrt(sr * sr + si * si);
+        double sq = 0;
+        for (int i = 0; i < m; ++i) {
+            const double dr = yr[0][i] - (i < m - 1 ? yr[1][i] : 0.0);
+            const double di = yi[0][i] - (i < m - 1 ? yi[1][i] : 0.0);
+            sq += dr * dr + di * di;
+        }
+        c.est[vec] = fmax(res, b0 * sqrt(sq));
     }
 }
 
