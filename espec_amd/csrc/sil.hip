@@ -478,15 +478,32 @@ __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
     const double al = a.al[vec * a.m + i];
     // Gram coefficient j: wavefront j mod (NT/64) sums the nb partials of
     // lnz_project, lanes in a fixed order, then a wavefront shuffle tree.
+    // A wavefront takes eight of its columns at a time, so that their
+    // loads and shuffle trees overlap.
     {
+        constexpr int NW = NT / 64, G = 8;
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int j = wv; j < nj; j += NT / 64) {
-            const double *pc = a.pr + ((long long)vec * a.nchunk + j / kChunk) * a.nb * kPart +
-                               j % kChunk;
-            double x = 0;
-            for (int b = lane; b < a.nb; b += 64) x += pc[(long long)b * kPart];
-            x = wave_sum(x);
-            if (lane == 0) rs[j] = x;
+        for (int j0 = wv; j0 < nj; j0 += G * NW) {
+            double x[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) x[q] = 0;
+            for (int b = lane; b < a.nb; b += 64) {
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    const int j = j0 + q * NW;
+                    if (j < nj)
+                        x[q] += a.pr[((long long)vec * a.nchunk + j / kChunk) * a.nb * kPart +
+                                     (long long)b * kPart + j % kChunk];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                const int j = j0 + q * NW;
+                if (j < nj) {
+                    const double t = wave_sum(x[q]);
+                    if (lane == 0) rs[j] = t;
+                }
+            }
         }
     }
     __syncthreads();
