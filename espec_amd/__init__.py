@@ -25,6 +25,8 @@ MULTIPASS = 0x8
 # Absorbing boundaries (TPABSOR with ABSORB=.TRUE.; spofft.f:65-118, 162-176,
 # sodfft.f:66-72, 108-116): VABC is an imaginary potential or a mask.
 ABSORB_NONE, ABSORB_VOPTIC, ABSORB_SMOOTHW = 0, 1, 2
+# Laser pulse shapes of EF (ef.f): EFC .NONE/.NULL, .GAUS, .GGAUS, .SIN2.
+PULSE_NONE, PULSE_GAUS, PULSE_GGAUS, PULSE_SIN2 = 0, 1, 2, 3
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 AMU = 1836.152667539    # electron masses per amu (espec.f:614)
 A0A = 0.5291772083      # angstrom per bohr (espec.f:67)
@@ -81,6 +83,11 @@ def _load():
     L.espec_bw_probe.argtypes = [C.c_size_t, ip, P(dp), P(dp)]
     L.espec_plnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, d, vp, ip, P(ip), ip]
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
+    L.espec_sil_plan_set_pulse.argtypes = [vp, d, d, ip]
+    L.espec_ef.argtypes = [ip, dp, dp, dp, dp, dp, dp, ip, dp]
+    L.espec_ef.restype = dp
+    L.espec_pplnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, ip, dp, dp, dp, dp,
+                              dp, dp, ip, vp, d, d, vp, ip, P(ip), ip]
     L.espec_sil_plan_run.argtypes = [vp, ip]
     L.espec_sil_plan_sync.argtypes = [vp]
     L.espec_sil_plan_get.argtypes = [vp, ip, vp, vp, vp, ip, P(ip), vp, vp]
@@ -188,6 +195,31 @@ def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT)
     _check(rc, "PLNZ")
     k = ns.value
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
+
+
+def ef(efc, e0, t0, td, tp, omg, sni, kl, t):
+    """EF (ef.f): the laser field at time t (fs) for pulse shape efc."""
+    return _load().espec_ef(efc, e0, t0, td, tp, omg, sni, kl, t)
+
+
+def pplnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vp, dm, pulse, flags=FP64 | EXACT):
+    """PPLNZ (pplnz.f:1-340) time loop: PLNZ steps with VEFF = VP - DM EF(T)
+    (chpot.f:58-62) at each step's end time.  pulse = (efc, e0, t0, td, tp,
+    omg, sni, kl).  Returns dict(cr, ci, norm, u, v, veff); veff is the
+    last VEFF, which the reference leaves in VPOT."""
+    L = _load()
+    maxs = nsteps(ti, tf - dt / 2, dt) // lc + 1
+    cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
+    u, v = _f64(u1).copy(), _f64(v1).copy()
+    ve = np.zeros(u.size)
+    ns = C.c_int(0)
+    efc, e0, t0, td, tp, omg, sni, kl = pulse
+    rc = L.espec_pplnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0), u, v,
+                       _f64(vp), _f64(dm), efc, e0, t0, td, tp, omg, sni, kl, ve.ctypes.data,
+                       cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags)
+    _check(rc, "PPLNZ")
+    k = ns.value
+    return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v, veff=ve)
 
 
 def psodfft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT, absorb=ABSORB_NONE,
@@ -395,6 +427,14 @@ class SilPlan(_Plan):
                                            vp, maxsamp, flags)
         _check(rc, "sil_plan_create")
         self.mp = mp
+
+    def set_pulse(self, dm, efield):
+        """Pulse-driven steps from now on (PPLNZ): step g of the plan uses
+        VPOT - dm * efield[g-1]; dm shaped like vpot."""
+        self._dm = _f64(dm).ravel()
+        self._ef = _f64(efield).ravel()
+        _check(self._L.espec_sil_plan_set_pulse(self._h, self._dm, self._ef, self._ef.size),
+               "sil_plan_set_pulse")
 
     def adaptive(self, ndt, params=None, tol=1e-10, m_fix=10):
         """Advance ndt base steps with the learned controller choosing the

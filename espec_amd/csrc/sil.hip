@@ -1216,6 +1216,10 @@ struct espec_sil_plan {
     int nb = 0;
     hipStream_t stream = nullptr;
     DevBuf x, psi0, vpot, coef, info, cbuf, dstep;
+    // pulse-driven steps (PPLNZ): dipole [npk][n], field per step, V - DM E
+    DevBuf dm, efield, veff;
+    bool pulse = false;
+    long long nfield = 0;
     LnzBufs lb;
     SilArgs sa{};
     hipGraphExec_t gexec = nullptr;
@@ -1286,11 +1290,30 @@ hipError_t lnz_iterations(const LnzArgs &l, hipStream_t s)
     return hipGetLastError();
 }
 
+// Effective potential of a pulse-driven step (chpot.f:58-62):
+// VEFF = VP - DM EF(T) with the field at the step's end time; the step
+// that completes step count g + 1 reads efield[g] (g = *dstep).
+__global__ __launch_bounds__(256) void sil_field(const double *vp, const double *dm,
+                                                 const double *efield, const long long *dstep,
+                                                 double *veff, long long n)
+{
+    const double e = efield[*dstep];
+    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < n; p += gridDim.x * 256LL)
+        veff[p] = vp[p] - __dmul_rn(dm[p], e);
+}
+
 template <typename S>
 hipError_t sil_step(espec_sil_plan &P, hipStream_t s)
 {
     const LnzArgs &l = P.sa.l;
     dim3 grid(P.nb, l.nvec);
+    if (P.pulse) {
+        const long long tot = (long long)P.n * P.npk;
+        hipLaunchKernelGGL(sil_field, dim3((unsigned)std::min<long long>(1024, (tot + 255) / 256)),
+                           dim3(256), 0, s, P.vpot.as<double>(), P.dm.as<double>(),
+                           P.efield.as<double>(), P.dstep.as<long long>(), P.veff.as<double>(),
+                           tot);
+    }
     hipLaunchKernelGGL((lnz_init<S, 256>), grid, dim3(256), 0, s, l, (const V2<S> *)P.x.p,
                        (const S *)nullptr);
     hipError_t e = lnz_iterations<S>(l, s);
@@ -1680,6 +1703,7 @@ int espec_sil_plan_run(espec_sil_plan *P, int nsteps)
 {
     if (!P || nsteps < 0) return ESPEC_ERR_ARG;
     if (nsteps == 0) return ESPEC_OK;
+    if (P->pulse && P->steps_done + nsteps > P->nfield) return ESPEC_ERR_ARG;
     return P->mixed ? sil_run_typed<float>(*P, nsteps) : sil_run_typed<double>(*P, nsteps);
 }
 
@@ -1738,6 +1762,7 @@ int espec_sil_plan_adaptive(espec_sil_plan *P, const float *params, int ndt, dou
 {
     if (!P || ndt < 0 || !(tol > 0) || m_fix < kLabM0 || m_fix > P->m || P->lc != 1 || P->ref)
         return ESPEC_ERR_ARG;
+    if (P->pulse) return ESPEC_ERR_UNSUPPORTED;
     if (ndt == 0) return ESPEC_OK;
     int rc = ctl_setup(*P);
     if (rc) return rc;
@@ -1779,6 +1804,7 @@ int espec_sil_plan_trace(espec_sil_plan *P, int nsteps, int mprop, float *feat, 
     if (!P || nsteps < 1 || !feat || !lab || P->m < kLabM0 + kLabM || P->lc != 1 || P->ref ||
         mprop < kLabM0 + 1 || mprop > P->m)
         return ESPEC_ERR_ARG;
+    if (P->pulse) return ESPEC_ERR_UNSUPPORTED;
     int rc = ctl_setup(*P);
     if (rc) return rc;
     P->drop_graph();
@@ -1808,6 +1834,78 @@ int espec_plnz(int ndim, const int *np, int mp, double abstol, double dt, double
     rc = espec_sil_plan_run(P, nsteps);
     if (!rc) rc = espec_sil_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u, v);
     espec_sil_plan_destroy(P);
+    return rc;
+}
+
+int espec_sil_plan_set_pulse(espec_sil_plan *P, const double *dm, const double *efield,
+                             int nfield)
+{
+    if (!P || !dm || !efield || nfield < 0) return ESPEC_ERR_ARG;
+    const size_t tot = (size_t)P->n * P->npk;
+    ESPEC_HIP_TRY(P->dm.upload(dm, sizeof(double) * tot));
+    ESPEC_HIP_TRY(P->efield.upload(efield, sizeof(double) * (size_t)(nfield > 0 ? nfield : 1)));
+    if (!P->veff.p) ESPEC_HIP_TRY(P->veff.alloc(sizeof(double) * tot));
+    P->nfield = nfield;
+    P->pulse = true;
+    P->sa.l.vpot = P->veff.as<double>();
+    P->drop_graph();  // the step's launches change
+    return ESPEC_OK;
+}
+
+double espec_ef(int efc, double e0, double t0, double td, double tp, double omg, double sni,
+                int kl, double t)
+{
+    // ef.f: FAT, FATE (V/cm -> a.u./D) and FATT (eV -> rad/fs) as the
+    // reference writes them; .GGAUS takes the default-REAL (f32) 0.693147181.
+    const double pi = 3.14159265358979323846;
+    const double fat = 2.1132e-9, fate = 2.29371276e+17 * 3.335664e-28, fatt = 1.5192676;
+    switch (efc) {
+    case ESPEC_PULSE_NONE:
+        return 0.0;
+    case ESPEC_PULSE_GAUS:
+        return e0 * std::exp(-(4.0 * 0.693147181 * (t - t0) * (t - t0) / (tp * tp))) *
+               std::cos(omg * t * fatt + sni) * fate;
+    case ESPEC_PULSE_GGAUS: {
+        const double a1 = 1.0 / (2.0 * kl);
+        const double tpa = tp / std::pow((double)0.693147181f, a1);
+        return fat * std::sqrt(e0 * std::exp(-std::pow((t - t0) / tpa, 2.0 * kl))) *
+               std::cos(omg * t * fatt + sni);
+    }
+    case ESPEC_PULSE_SIN2:
+        return e0 * std::cos(pi * (t - td) / tp) * std::cos(pi * (t - td) / tp) *
+               std::cos(omg * (t - td) * fatt + sni) * fate;
+    default:
+        return std::nan("");
+    }
+}
+
+int espec_pplnz(int ndim, const int *np, int mp, double abstol, double dt, double ti, double tf,
+                int lc, const double *shm, double *u, double *v, const double *vp,
+                const double *dm, int efc, double e0, double t0, double td, double tp,
+                double omg, double sni, int kl, double *veff, double *cr, double *ci,
+                double *nrm, int maxsamp, int *nsamp, int flags)
+{
+    if (!u || !v || !vp || !dm || !cr || !ci || maxsamp < 1 || efc < ESPEC_PULSE_NONE ||
+        efc > ESPEC_PULSE_SIN2)
+        return ESPEC_ERR_ARG;
+    // DO T=TI+DT,TF+DT/2,DT (pplnz.f:190), T advanced by DT each pass.
+    const int nsteps = fortran_trip(ti + dt, tf + dt / 2, dt);
+    std::vector<double> ef((size_t)std::max(nsteps, 1));
+    double t = ti + dt;
+    for (int k = 0; k < nsteps; ++k, t += dt) ef[k] = espec_ef(efc, e0, t0, td, tp, omg, sni, kl, t);
+    espec_sil_plan *P = nullptr;
+    int rc = espec_sil_plan_create(&P, ndim, np, 1, mp, abstol, dt, lc, shm, vp, u, v, maxsamp,
+                                   flags);
+    if (rc) return rc;
+    rc = espec_sil_plan_set_pulse(P, dm, ef.data(), nsteps);
+    if (!rc) rc = espec_sil_plan_run(P, nsteps);
+    if (!rc) rc = espec_sil_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u, v);
+    espec_sil_plan_destroy(P);
+    if (!rc && veff) {  // VPOT as the reference leaves it: the last VEFF
+        const double e = nsteps > 0 ? ef[nsteps - 1] : 0.0;
+        const int n = ntot(ndim, np);
+        for (int i = 0; i < n; ++i) veff[i] = nsteps > 0 ? vp[i] - dm[i] * e : vp[i];
+    }
     return rc;
 }
 

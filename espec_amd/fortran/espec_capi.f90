@@ -70,6 +70,22 @@ module espec_capi
       integer(c_int) :: rc
     end function
 
+    function espec_sil_plan_set_pulse(plan, dm, efield, nfield) &
+        bind(C, name='espec_sil_plan_set_pulse') result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: plan
+      real(c_double) :: dm(*), efield(*)
+      integer(c_int), value :: nfield
+      integer(c_int) :: rc
+    end function
+
+    function espec_ef(efc, e0, t0, td, tp, omg, sni, kl, t) bind(C, name='espec_ef') result(r)
+      import :: c_int, c_double
+      integer(c_int), value :: efc, kl
+      real(c_double), value :: e0, t0, td, tp, omg, sni, t
+      real(c_double) :: r
+    end function
+
     function espec_sil_plan_create(plan, ndim, np, npk, mp, abstol, dt, lc, shm, vpot, u, v, &
                                    maxsamp, flags) bind(C, name='espec_sil_plan_create') result(rc)
       import :: c_int, c_double, c_ptr
@@ -273,6 +289,22 @@ contains
     if (.not. absorb) return
     if (starts(tpabsor, '.VOPTIC')) absorb_of = 1
     if (starts(tpabsor, '.SMOOTHW')) absorb_of = 2
+  end function
+
+  ! Pulse shape code of EFC as EF tests it (ef.f:40-58); -1 where the
+  ! reference stops ("Desired laser pulse wasn't found").
+  integer function pulse_of(efc)
+    character(len=*), intent(in) :: efc
+    pulse_of = -1
+    if (starts(efc, '.GAUS')) then
+      pulse_of = 1
+    else if (starts(efc, '.GGAUS')) then
+      pulse_of = 2
+    else if (starts(efc, '.SIN2')) then
+      pulse_of = 3
+    else if (starts(efc, '.NONE') .or. starts(efc, '.NULL')) then
+      pulse_of = 0
+    end if
   end function
 
   logical function starts(s, p)

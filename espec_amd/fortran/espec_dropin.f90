@@ -330,6 +330,164 @@ subroutine PLNZ(ABSORB, CHANGE, DIM, PRTCRL, TPABSOR, INFO, LC, LMTREORT, MP, MP
 1021 format(1X,F14.8,3X,E12.6,2(4X,F10.7),4X,F10.8,4X,F8.6)
 end subroutine PLNZ
 
+! pplnz.f:1-340: PLNZ steps with VEFF = VP - DM EF(T) (chpot.f:58-62) at
+! each step's end time.  The movie, potential and pulse files (PRTVEFF,
+! PRTEIGVC2, PRTPULS, PRTCRL '.LASTWP') are not written: INFO = 6.
+subroutine PPLNZ(ABSORB, CHANGE, PRTVEFF, PRTEIGVC2, PRTPULS, DIM, EFC, PRTCRL, TPABSOR, INFO, &
+                 LC, LMTREORT, MP, MPI, MPR, MXDCT, N, NSHOT, KL, NPR, KP, ABSTOL, DT, TI, TF, &
+                 TOL, E0, T0, TD, TP, OMG, SNI, NP, ND, IWORK, SH, SHM, VPOT, VABC, U, V, XI, &
+                 XF, XP, DM, WU, WV, WP, WORK, VAR, LNZVC)
+  use iso_c_binding
+  use espec_capi
+  implicit none
+  logical :: ABSORB, CHANGE, PRTVEFF, PRTEIGVC2, PRTPULS
+  character(len=*) :: DIM, EFC, PRTCRL, TPABSOR
+  integer :: INFO, LC, LMTREORT, MP, MPI, MPR, MXDCT, N, NSHOT, KL, NPR, KP
+  real(8) :: ABSTOL, DT, TI, TF, TOL, E0, T0, TD, TP, OMG, SNI
+  integer :: NP(*), ND(*), IWORK(*)
+  real(8) :: SH(*), SHM(*), VPOT(*), VABC(*), U(*), V(*), XI(*), XF(*), XP(*), DM(*)
+  real(8) :: WU(*), WV(*), WP(*), WORK(*), VAR(*), LNZVC(MXDCT, *)
+  type(c_ptr) :: plan
+  integer :: ndim, flags, nst, maxs, ns, rc, k, i, j, step, chunk, efk, nc
+  real(8) :: t, cr, ci, er, ei, et, et0, fnr, fni, fnt, f0, ef
+  real(8) :: ct(7)
+  real(8), allocatable :: u0(:), v0(:), vp(:), crs(:), cis(:), nrs(:), efield(:)
+  logical :: every, prt
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  efk = pulse_of(EFC)
+  if (ndim == 0 .or. PRTVEFF .or. PRTEIGVC2 .or. PRTPULS .or. starts(PRTCRL, '.LASTWP')) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  if (efk < 0) then
+    write(*,'(A)') '<<<>>> Desired laser pulse wasn''t found. <<<>>>'
+    INFO = ESPEC_ERR_ARG
+    return
+  end if
+  flags = mode_flags()
+  nst = trip(TI + DT, TF + DT / 2, DT)  ! pplnz.f:190
+  maxs = nst / LC + 1
+  allocate(u0(N), v0(N), vp(N), crs(maxs), cis(maxs), nrs(maxs), efield(max(nst, 1)))
+  u0 = U(1:N)
+  v0 = V(1:N)
+  vp = VPOT(1:N)
+  t = TI
+  do k = 1, nst
+    t = t + DT
+    efield(k) = espec_ef(efk, E0, T0, TD, TP, OMG, SNI, KL, t)
+  end do
+  rc = espec_sil_plan_create(plan, ndim, NP, 1, MP, ABSTOL, DT, LC, SHM, vp, U, V, maxs, flags)
+  if (rc == 0) rc = espec_sil_plan_set_pulse(plan, DM, efield, nst)
+  if (rc /= 0) then
+    INFO = rc
+    return
+  end if
+  every = starts(PRTCRL, '.YES')
+  prt = every .or. starts(PRTCRL, '.PARTIAL')
+  t = TI
+  step = 0
+  if (prt) then  ! pplnz.f:100-160
+    call project(0)
+    rc = espec_eigenerg(ndim, NP, SHM, U, VPOT, er)
+    rc = espec_eigenerg(ndim, NP, SHM, V, VPOT, ei)
+    et = er + ei
+    et0 = et
+    f0 = fnr**2
+    write(*,*)
+    if (ndim == 1) then
+      write(*,*) 'Auto-correlation function:'
+    else
+      write(*,*) 'Partial auto-correlation function:'
+    end if
+    write(*,*) '=================================='
+    write(*,1011) ('*|P', j - 1, '|^2*', j = 1, NPR), '*|c', 0, '|^2*'
+    write(*,1021) t, et, et0, cr, ci, (ct(j), j = 1, NPR), f0
+    chunk = LC
+    if (every) chunk = 1
+    do while (step < nst)
+      k = min(chunk, nst - step)
+      rc = espec_sil_plan_run(plan, k)
+      if (rc /= 0) exit
+      do i = 1, k
+        t = t + DT
+      end do
+      step = step + k
+      if (every .or. mod(step, LC) == 0) then
+        rc = espec_sil_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U, V)
+        if (rc /= 0) exit
+        ef = efield(step)
+        do i = 1, N
+          VPOT(i) = vp(i) - DM(i) * ef
+        end do
+        call project(1)
+        rc = espec_eigenerg(ndim, NP, SHM, U, VPOT, er)
+        rc = espec_eigenerg(ndim, NP, SHM, V, VPOT, ei)
+        et = er + ei
+        rc = espec_eigenerg(ndim, NP, SHM, U, vp, er)
+        rc = espec_eigenerg(ndim, NP, SHM, V, vp, ei)
+        et0 = er + ei
+        fnt = fnr
+        if (.not. ABSORB .and. abs((fnt - f0) / f0) * 1.0d2 > TOL) INFO = 1
+        write(*,1021) t, et, et0, cr, ci, (ct(j), j = 1, NPR), fnt
+      end if
+    end do
+  else
+    rc = espec_sil_plan_run(plan, nst)
+  end if
+  if (rc == 0) rc = espec_sil_plan_get(plan, 0, crs, cis, nrs, maxs, ns, U, V)
+  if (rc /= 0) then
+    if (rc == 5) write(*,'(A)') '<<<>>> Diagonalization error <<<>>>'
+    INFO = rc
+  else
+    ! C(TI) is stored only with PRTCRL .YES/.PARTIAL (pplnz.f:130-134)
+    nc = 1
+    if (.not. prt) nc = 2
+    do k = nc, ns
+      LNZVC(k, MPR) = crs(k)
+      LNZVC(k, MPI) = cis(k)
+    end do
+    ! VPOT is left holding the last step's VEFF (chpot.f:58-62)
+    if (nst > 0) then
+      ef = efield(nst)
+      do i = 1, N
+        VPOT(i) = vp(i) - DM(i) * ef
+      end do
+    end if
+  end if
+  rc = espec_sil_plan_destroy(plan)
+  deallocate(u0, v0, vp, crs, cis, nrs, efield)
+1011 format(/,8X,'*t/fs*',5X,'*ET/a.u.*',6X,'*E0/a.u.*',6X,'*Pr(t)*',4X,'*Pi(t)*',3X, &
+            7(A3,I1,A4,3X))
+1021 format(1X,F14.6,3X,2(E12.6,3X),2(F8.5,3X),10(F8.6,3X))
+contains
+  ! C(t), the populations |<W_k|psi>|^2 of LNZVC(:,KP+k) and the norm
+  ! (pplnz.f:101-140); pass 0 is the initial state.
+  subroutine project(pass)
+    integer, intent(in) :: pass
+    integer :: q, ii
+    real(8) :: pr, pi
+    do q = 2, min(NPR, 7)
+      pr = 0d0
+      pi = 0d0
+      do ii = 1, N
+        pr = pr + LNZVC(ii, KP + q - 1) * U(ii)
+      end do
+      do ii = 1, N
+        pi = pi + LNZVC(ii, KP + q - 1) * V(ii)
+      end do
+      ct(q) = pr**2 + pi**2
+    end do
+    call corr(N, u0, v0, U, V, cr, ci, fnr, fni)
+    ct(1) = cr**2 + ci**2
+    if (pass == 0) then
+      LNZVC(1, MPR) = cr
+      LNZVC(1, MPI) = ci
+    end if
+  end subroutine
+end subroutine PPLNZ
+
 ! lanczs.f:1-190: eigenvectors into EIGVC(:,1..IU-IL+1) and LNZVC(:,1..IU-IL+1).
 subroutine LANCZS(CHANGE, DIM, IL, IU, INFO, NREORT, LMTREORT, N, NSEED, MXDCT, ABSTOL, IWK, &
                   NP, AL, BT, EIGVL, SHM, V0, VI, VPOT, WK1, WK2, VAR, LNZVC, EIGVC)

@@ -150,6 +150,38 @@ int espec_plnz(int ndim, const int *np, int mp, double abstol, double dt,
                double *v, const double *vpot, double *cr, double *ci,
                double *nrm, int maxsamp, int *nsamp, int flags);
 
+/* ---------------------------------------------------------------------
+ * Pulse-driven short-iterative Lanczos.  Replaces PPLNZ (pplnz.f:1-5):
+ * every step of DO T=TI+DT,TF+DT/2,DT (pplnz.f:190) first forms
+ * VEFF = VP - DM EF(T) (CHPOT, chpot.f:58-62) with the field at the step's
+ * end time T, then takes one PLNZAUX step (sil.f:3-5) with VEFF.
+ *   vp, dm [N]           static potential and dipole function
+ *   efc .. kl            EF's pulse (ef.f): ESPEC_PULSE_*, E0, T0, TD, TP,
+ *                        OMG, SNI, KL
+ *   veff [N]             out (may be NULL): the last VEFF, which PPLNZ
+ *                        leaves in VPOT
+ *   cr, ci, nrm          C(t) as espec_plnz; sample 0 is C(TI) here, where
+ *                        the reference stores it only when PRTCRL is .YES
+ *                        or .PARTIAL (pplnz.f:130-134)
+ * Other arguments as espec_plnz.
+ * ------------------------------------------------------------------- */
+enum {
+    ESPEC_PULSE_NONE = 0,  /* EFC .NONE / .NULL */
+    ESPEC_PULSE_GAUS = 1,  /* .GAUS  */
+    ESPEC_PULSE_GGAUS = 2, /* .GGAUS */
+    ESPEC_PULSE_SIN2 = 3   /* .SIN2  */
+};
+int espec_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
+                double ti, double tf, int lc, const double *shm, double *u,
+                double *v, const double *vp, const double *dm, int efc,
+                double e0, double t0, double td, double tp, double omg,
+                double sni, int kl, double *veff, double *cr, double *ci,
+                double *nrm, int maxsamp, int *nsamp, int flags);
+/* Replaces EF (ef.f:1-60): the field at time t (fs); NaN for an unknown
+ * efc, where the reference stops. */
+double espec_ef(int efc, double e0, double t0, double td, double tp,
+                double omg, double sni, int kl, double t);
+
 /* Replaces AU(DIM,N,NP,SHM,VPOT,U,WORK,VAR) (au.f:3) -> AU_1D (au1d.f:1),
  * AU_2D (au2di.f:1), AU_3D (au3d.f:1): hu = H u, 4th-order FD kinetic with
  * Dirichlet truncation plus V.  Runs on the device in FP64. */
@@ -273,6 +305,12 @@ int espec_sil_plan_create(espec_sil_plan **plan, int ndim, const int *np,
                           const double *shm, const double *vpot,
                           const double *u, const double *v, int maxsamp,
                           int flags);
+/* Make every later step of the plan pulse-driven (PPLNZ): step g (1-based
+ * over the plan's life) uses VEFF = VPOT - dm E_g with E_g = efield[g-1];
+ * dm [npk][N].  Runs past nfield steps return ESPEC_ERR_ARG; adaptive and
+ * trace runs of a pulse-driven plan return ESPEC_ERR_UNSUPPORTED. */
+int espec_sil_plan_set_pulse(espec_sil_plan *plan, const double *dm,
+                             const double *efield, int nfield);
 int espec_sil_plan_run(espec_sil_plan *plan, int nsteps);
 int espec_sil_plan_sync(espec_sil_plan *plan);
 /* Returns ESPEC_ERR_DIAG if a tridiagonal eigensolve failed to converge

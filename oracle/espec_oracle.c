@@ -925,6 +925,73 @@ int orc_plnz(int ndim, const int *np, int mp, double abstol, double dt,
     return nc + 1;
 }
 
+/* ef.f:1-60 EF: the laser field at time t (fs) in the reference's units.
+ * efc: 0 .NONE/.NULL, 1 .GAUS, 2 .GGAUS, 3 .SIN2.  .GGAUS raises the
+ * default-REAL (f32) literal 0.693147181 to 1/(2 KL), as the reference
+ * does. */
+double orc_ef(int efc, double e0, double t0, double td, double tp, double omg,
+              double sni, int kl, double t)
+{
+    const double pi = 3.14159265358979323846;
+    const double fat = 2.1132e-9, fate = 2.29371276e+17 * 3.335664e-28, fatt = 1.5192676;
+    double ef;
+    if (efc == 1) {
+        ef = e0 * exp(-(4.0 * 0.693147181 * (t - t0) * (t - t0) / (tp * tp))) *
+             cos(omg * t * fatt + sni);
+        ef = ef * fate;
+    } else if (efc == 2) {
+        double a1 = 1.0 / (2.0 * kl);
+        double tpa = tp / pow((double)0.693147181f, a1);
+        ef = fat * sqrt(e0 * exp(-pow((t - t0) / tpa, 2.0 * kl))) * cos(omg * t * fatt + sni);
+    } else if (efc == 3) {
+        ef = e0 * cos(pi * (t - td) / tp) * cos(pi * (t - td) / tp) *
+             cos(omg * (t - td) * fatt + sni);
+        ef = ef * fate;
+    } else {
+        ef = 0.0;
+    }
+    return ef;
+}
+
+/* pplnz.f:72-300 with PRTCRL='.NO': DO T=TI+DT,TF+DT/2,DT, each step
+ * VEFF = VP - DM EF(T) at the step's end time T (chpot.f:58-62) and one
+ * PLNZAUX step with VEFF.  veff (may be NULL) receives the last VEFF,
+ * which the reference leaves in VPOT. */
+int orc_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
+              double ti, double tf, int lc, const double *shm, double *u,
+              double *v, const double *vp, const double *dm, int efc,
+              double e0, double t0, double td, double tp, double omg,
+              double sni, int kl, double *veff, double *cr, double *ci,
+              int maxsamp, int mode)
+{
+    int n = ntot_of(ndim, np);
+    double cst = dt / ORC_SHFS;
+    double *u0 = malloc(sizeof(double) * 3 * n), *v0 = u0 + n, *ve = u0 + 2 * n;
+    memcpy(u0, u, sizeof(double) * n);
+    memcpy(v0, v, sizeof(double) * n);
+    memcpy(ve, vp, sizeof(double) * n);
+    int nc = 0, mc = 0;
+    if (maxsamp > 0) corr(n, u0, v0, u, v, &cr[0], &ci[0]);
+    int ns = trip(ti + dt, tf + dt / 2, dt);
+    double t = ti + dt;
+    for (int s = 0; s < ns; ++s, t += dt) {
+        double e = orc_ef(efc, e0, t0, td, tp, omg, sni, kl, t);
+        for (int i = 0; i < n; ++i) ve[i] = vp[i] - dm[i] * e;
+        if (orc_plnz_step(ndim, np, mp, abstol, cst, shm, ve, u, v, mode)) {
+            free(u0);
+            return -1;
+        }
+        if (++mc == lc) {
+            ++nc;
+            mc = 0;
+            if (nc < maxsamp) corr(n, u0, v0, u, v, &cr[nc], &ci[nc]);
+        }
+    }
+    if (veff) memcpy(veff, ve, sizeof(double) * n);
+    free(u0);
+    return nc + 1;
+}
+
 /* rans.f:32-39. */
 double orc_ran(int *nseed)
 {

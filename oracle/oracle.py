@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 EXACT, REF = 0, 1
 ABS_NONE, ABS_VOPTIC, ABS_SMOOTHW = 0, 1, 2  # TPABSOR .VOPTIC / .SMOOTHW
+PULSE_NONE, PULSE_GAUS, PULSE_GGAUS, PULSE_SIN2 = 0, 1, 2, 3  # EFC (ef.f)
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -91,6 +92,11 @@ class Oracle:
         L.orc_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d,
                                _d, _d, _d, _ip, _ip]
         L.orc_plnz.restype = _ip
+        L.orc_ef.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
+        L.orc_ef.restype = _dp
+        L.orc_pplnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _ip,
+                                _dp, _dp, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _ip, _ip]
+        L.orc_pplnz.restype = _ip
         L.orc_ran.argtypes = [C.POINTER(_ip)]
         L.orc_ran.restype = _dp
         L.orc_lanczs.argtypes = [_ip, _i, _ip, _ip, _ip, _dp, _d, _d, _d, _d]
@@ -210,6 +216,25 @@ class Oracle:
             raise RuntimeError("diagonalization error")
         return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
 
+    def ef(self, efc, e0, t0, td, tp, omg, sni, kl, t):
+        """EF (ef.f): efc PULSE_NONE/GAUS/GGAUS/SIN2."""
+        return self.lib.orc_ef(efc, e0, t0, td, tp, omg, sni, kl, t)
+
+    def pplnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vp, dm, pulse, mode=REF):
+        """PPLNZ (pplnz.f) with PRTCRL='.NO'; pulse = (efc, e0, t0, td, tp,
+        omg, sni, kl).  Returns dict(cr, ci, u, v, veff)."""
+        n = int(np.prod(np_))
+        ns = self.nsteps(ti, tf - dt / 2, dt) // max(lc, 1) + 2
+        cr, ci, ve = np.zeros(ns), np.zeros(ns), np.zeros(n)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        efc, e0, t0, td, tp, omg, sni, kl = pulse
+        k = self.lib.orc_pplnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc,
+                               _v3(shm, 0.0), u, v, _f64(vp), _f64(dm), efc, e0, t0, td, tp,
+                               omg, sni, kl, ve, cr, ci, ns, mode)
+        if k < 0:
+            raise RuntimeError("diagonalization error")
+        return dict(cr=cr[:k], ci=ci[:k], u=u, v=v, veff=ve)
+
     def plnz_step(self, np_, mp, abstol, cst, shm, u, v, vpot, mode=REF):
         u, v = _f64(u).copy(), _f64(v).copy()
         info = self.lib.orc_plnz_step(len(np_), _np3(np_), mp, abstol, cst,
@@ -327,6 +352,10 @@ class Reference:
         L.ref_hwhm.argtypes = [_ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_spectrumtd.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_ran.argtypes = [P(_ip)]
+        L.ref_ef.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
+        L.ref_ef.restype = _dp
+        L.ref_pplnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
+                                _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _ip, P(_ip)]
         L.ref_ran.restype = _dp
         L.ref_lanczs.argtypes = [_ip, _i, _ip, _ip, _ip, _dp, _d, _d, _d, _d, _ip, P(_ip)]
 
@@ -374,6 +403,22 @@ class Reference:
         self.lib.ref_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0),
                           u, v, _f64(vpot), cr, ci, nsamp, C.byref(info))
         return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
+
+    def ef(self, efc, e0, t0, td, tp, omg, sni, kl, t):
+        return self.lib.ref_ef(efc, e0, t0, td, tp, omg, sni, kl, t)
+
+    def pplnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vp, dm, pulse):
+        n = int(np.prod(np_))
+        nst = int(((tf + dt / 2) - (ti + dt) + dt) / dt)
+        nsamp = nst // max(lc, 1) + 1
+        cr, ci, ve = np.zeros(nsamp), np.zeros(nsamp), np.zeros(n)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        info = _ip(0)
+        efc, e0, t0, td, tp, omg, sni, kl = pulse
+        self.lib.ref_pplnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0), u, v,
+                           _f64(vp), _f64(dm), efc, e0, t0, td, tp, omg, sni, kl, ve, cr, ci,
+                           nsamp, C.byref(info))
+        return dict(cr=cr, ci=ci, u=u, v=v, veff=ve, info=info.value)
 
     def fft(self, a, b, ntot, n, nspan, isn):
         a, b = _f64(a).copy(), _f64(b).copy()

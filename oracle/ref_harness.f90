@@ -196,6 +196,71 @@ contains
     deallocate(iwork, vabc, lnzvc, wu, wv, wp, work)
   end subroutine ref_plnz
 
+  ! EFC string of pulse kind 0..3 (ef.f).
+  subroutine efcstr(efc, s)
+    integer(c_int), value :: efc
+    character(len=8) :: s
+    s = '.NONE   '
+    if (efc == 1) s = '.GAUS   '
+    if (efc == 2) s = '.GGAUS  '
+    if (efc == 3) s = '.SIN2   '
+  end subroutine efcstr
+
+  ! EF (ef.f): the field at time t.
+  function ref_ef(efc, e0, t0, td, tp, omg, sni, kl, t) bind(C, name='ref_ef') result(r)
+    integer(c_int), value :: efc, kl
+    real(c_double), value :: e0, t0, td, tp, omg, sni, t
+    real(c_double) :: r
+    character(len=8) :: s
+    integer :: k
+    real(8) :: a0, a1, a2, a3, a4, a5, a6, EF
+    external EF
+    call efcstr(efc, s)
+    k = kl
+    a0 = e0; a1 = t0; a2 = td; a3 = tp; a4 = omg; a5 = sni; a6 = t
+    r = EF(s, a0, a1, a2, a3, a4, a5, k, a6)
+  end function ref_ef
+
+  ! PPLNZ (pplnz.f:1-340) with PRTCRL='.NO', no projections (NPR = 0);
+  ! veff receives VPOT as the reference leaves it (the last VEFF).
+  subroutine ref_pplnz(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vp, dm, &
+                       efc, e0, t0, td, tp, omg, sni, kl, veff, cr, ci, nsamp, info) &
+      bind(C, name='ref_pplnz')
+    integer(c_int), value :: ndim, mp, lc, nsamp, efc, kl
+    integer(c_int) :: np(3)
+    real(c_double), value :: abstol, dt, ti, tf, e0, t0, td, tp, omg, sni
+    real(c_double) :: shm(3), u(*), v(*), vp(*), dm(*), veff(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims, efcs
+    integer :: n, mxdct, i, nd(3)
+    integer, allocatable :: iwork(:)
+    real(8), allocatable :: vabc(:), lnzvc(:,:), wu(:), wv(:), wp(:), work(:), vpot(:)
+    real(8) :: xi(3), xf(3), xp(3), sh(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(iwork(5*mp + 16), vabc(n), lnzvc(mxdct, mp + 2), wu(n), wv(n), &
+             wp(n), work(max(n, 5*mp + 16)), vpot(n))
+    lnzvc = 0d0
+    vabc = 1d0
+    vpot = vp(1:n)
+    xi = 0d0; xf = 0d0; xp = 0d0; sh = 0d0; var = 0d0; nd = 0
+    call dimstr(ndim, dims)
+    call efcstr(efc, efcs)
+    call PPLNZ(.false., .false., .false., .false., .false., dims, efcs, '.NO     ', &
+               '.NONE   ', info, lc, mp, mp, mp + 2, mp + 1, mxdct, n, 1, kl, 0, 0, &
+               abstol, dt, ti, tf, 1d30, e0, t0, td, tp, omg, sni, np, nd, iwork, sh, &
+               shm, vpot, vabc, u, v, xi, xf, xp, dm, wu, wv, wp, work, var, lnzvc)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, mp + 1)
+      ci(i) = lnzvc(i, mp + 2)
+    end do
+    veff(1:n) = vpot(1:n)
+    deallocate(iwork, vabc, lnzvc, wu, wv, wp, work, vpot)
+  end subroutine ref_pplnz
+
   ! FFT (fft3d.f:1): Singleton multivariate in-place transform.
   subroutine ref_fft(a, b, ntot, n, nspan, isn) bind(C, name='ref_fft')
     real(c_double) :: a(*), b(*)

@@ -17,14 +17,15 @@ program dropin_driver
   real(8) :: ti, tf, dt, abstol, tol, sc
   real(8) :: sh(3), sm(3), shm(3)
   real(8), allocatable :: vpot(:), u(:), v(:), u2(:), v2(:), tc(:), w1(:), w2(:), w3(:), w4(:)
-  real(8), allocatable :: lnzvc(:, :), var(:), xi(:), xf(:), vabc(:)
+  real(8), allocatable :: lnzvc(:, :), var(:), xi(:), xf(:), vabc(:), xp(:), dm(:)
   integer, allocatable :: iwork(:)
+  integer :: nd(3)
 
   read(*,*) method, dim, n
   read(*,*) ti, tf, dt, mf, mp, abstol, tol
   read(*,*) np, sh, sm, shm
   allocate(vpot(n), u(n), v(n), u2(n), v2(n), tc(n), w1(n), w2(n), w3(n), w4(n), var(8), &
-           xi(3), xf(3), vabc(n), iwork(5 * n + 16))
+           xi(3), xf(3), vabc(n), iwork(5 * n + 16), xp(3), dm(n))
   do i = 1, n
     read(*,*) vpot(i), u(i), v(i)
   end do
@@ -47,6 +48,9 @@ program dropin_driver
   xf = 0
   vabc = 1
   var = 0
+  xp = 0
+  dm = 0
+  nd = 0
   info = 0
   select case (trim(method))
   case ('PFSPO')
@@ -60,6 +64,11 @@ program dropin_driver
     call PLNZ(.false., .true., dim, '.PARTIAL', '.NONE', info, lc, 10, mp, mpi, mpr, mxdct, n, &
               abstol, dt, ti, tf, tol, np, iwork, shm, vpot, vabc, u, v, xi, xf, w1, w2, w3, w4, &
               var, lnzvc)
+  case ('PPSIL')  ! espec.f:1509-1518 with no laser (EFC .NONE), one population
+    call PPLNZ(.false., .true., .false., .false., .false., dim, '.NONE', '.PARTIAL', '.NONE', &
+               info, lc, 10, mp, mpi, mpr, mxdct, n, 1, 1, 1, 0, abstol, dt, ti, tf, tol, &
+               0d0, 0d0, 0d0, 1d0, 0d0, 0d0, np, nd, iwork, sh, shm, vpot, vabc, u, v, xi, xf, &
+               xp, dm, w1, w2, w3, w4, var, lnzvc)
   end select
   nsamp = 2**(mf - 1) + 1
   do i = 1, nsamp

@@ -61,6 +61,12 @@ def write(name, rows):
     print(f"{path}: {len(rows)} rows")
 
 
+def pplnz_table(tt):
+    """Rows t/fs, ET, E0, Pr, Pi, |P0|^2, norm of results.test14."""
+    lines = open(os.path.join(tt, "results.test14")).read().splitlines()
+    return block(lines, "Auto-correlation function", 7, ["Eigenvector propagated"])
+
+
 def main(ref):
     tt = os.path.join(ref, "ttests")
     stops = ["Correlation Function", "Spectrum:", "Partial auto", "Total correlation",
@@ -77,6 +83,8 @@ def main(ref):
             rows = block(lines, title, nc, [s for s in stops if not title.startswith(s)], lim)
             if rows:
                 write(f"test{t}_{name}", rows)
+    # test14 (.PPSIL): PPLNZ's table, 7 columns (pplnz.f:1011/1021)
+    write("test14_table", pplnz_table(tt))
     ep = open(os.path.join(tt, "eigpot_test3.inp")).read().splitlines()
     sec = [i for i, ln in enumerate(ep) if ln.startswith("#")]
     write("eigpot_test3_psi", numeric_rows(ep[sec[0] + 1:sec[1]], 2))

@@ -67,6 +67,17 @@ def ground_state(v, shm):
 def test_case(t):
     """Return dict(np_, sh, sm, shm, vpot, u, v, dt, ti, tf, mf, method, mp)."""
     m = 7.49 * AMU
+    if t == 14:
+        # .PPSIL (PPLNZ) without a laser field: test12's surfaces on 201
+        # points, LANCZS ground state of the initial surface, ABSTOL 1e-6.
+        n = 201
+        x, sh = grid(n, 0.5, 2.8)
+        shm = 1.0 / (24.0 * m * sh * sh)
+        v0 = ohs(x, 1.146, 1943.79, m)
+        v1 = ohs(x, 1.211, 1629.23, m)
+        psi = ground_state(v0, shm)
+        return dict(np_=[n], sh=[sh], sm=[m], shm=[shm], vpot=v1, u=psi, v=np.zeros(n),
+                    dt=2e-3, ti=0.0, tf=250.0, mf=12, method="PPSIL", mp=10, abstol=1e-6)
     if t in (12, 10):
         n = 256 if t == 12 else 250
         x, sh = grid(n, 0.5, 2.8)

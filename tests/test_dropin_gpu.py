@@ -52,7 +52,7 @@ def run_driver(c, mode="REF", timeout=240):
             samples.append((float(p[2]), float(p[3])))
         elif ln.startswith("INFO"):
             info = int(p[1])
-        elif len(p) == 6:
+        elif len(p) == (7 if c["method"] == "PPSIL" else 6):
             try:
                 table.append([float(x.replace("D", "E")) for x in p])
             except ValueError:
@@ -82,3 +82,21 @@ def test_dropin_reproduces_published_table(gpu, t, bar, ebar):
     assert np.abs(table[:, 1] / gold[:, 1] - 1).max() <= ebar   # energy, 6 digits
     # the LNZVC columns hold the same samples as the printed rows
     assert np.abs(samples[:, 0] - table[:, 2]).max() <= 5e-8
+
+
+def test_dropin_pplnz_reproduces_published_table(gpu):
+    """test14 (.PPSIL, no laser): PPLNZ's own table (t, ET, E0, Pr, Pi,
+    |P0|^2, norm; pplnz.f:1021) over its 2049 rows.  ABSTOL = 1e-6
+    bisection, as in test4, so C(t) agrees to a few 1e-6."""
+    c = S.test_case(14)
+    table, samples, info = run_driver(c)
+    gold = S.load("test14_table")
+    assert table.shape == gold.shape == (2049, 7), (table.shape, gold.shape)
+    assert info == 0
+    assert np.abs(table[:, 0] - gold[:, 0]).max() <= 2e-6   # F14.6, t accumulated
+    d = np.abs(table[:, 3:5] - gold[:, 3:5]).max(axis=0)
+    print(f"max |dPr| {d[0]:.2e} |dPi| {d[1]:.2e}", flush=True)
+    assert d.max() <= 1.01e-5          # one unit of F8.5's last digit (bisection drift)
+    assert np.abs(table[:, 5:7] - gold[:, 5:7]).max() <= 1e-5
+    assert np.abs(table[:, 1:3] / gold[:, 1:3] - 1).max() <= 1e-5  # energies, 6 digits
+    assert np.abs(samples[:, 0] - table[:, 3]).max() <= 5e-6
