@@ -27,6 +27,9 @@ MULTIPASS = 0x8
 ABSORB_NONE, ABSORB_VOPTIC, ABSORB_SMOOTHW = 0, 1, 2
 # Laser pulse shapes of EF (ef.f): EFC .NONE/.NULL, .GAUS, .GGAUS, .SIN2.
 PULSE_NONE, PULSE_GAUS, PULSE_GGAUS, PULSE_SIN2 = 0, 1, 2, 3
+# INIEIGVC forms of INIT_COND (init_cond.f): .CALC, .GETR/.GETR1, .GETR2,
+# .GETC/.GETC1, .GETC2.
+INIT_CALC, INIT_GETR, INIT_GETR2, INIT_GETC, INIT_GETC2 = 0, 1, 2, 3, 4
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 AMU = 1836.152667539    # electron masses per amu (espec.f:614)
 A0A = 0.5291772083      # angstrom per bohr (espec.f:67)
@@ -85,6 +88,7 @@ def _load():
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
     L.espec_sil_plan_set_pulse.argtypes = [vp, d, d, ip]
     L.espec_ef.argtypes = [ip, dp, dp, dp, dp, dp, dp, ip, dp]
+    L.espec_init_cond.argtypes = [ip, i, ip, d, d, d, d, d, d, d]
     L.espec_ef.restype = dp
     L.espec_pplnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, ip, dp, dp, dp, dp,
                               dp, dp, ip, vp, d, d, vp, ip, P(ip), ip]
@@ -195,6 +199,20 @@ def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT)
     _check(rc, "PLNZ")
     k = ns.value
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
+
+
+def init_cond(np_, mode, xi, sh, x0, ra, rk, u=None, v=None):
+    """INIT_COND (init_cond.f): Gaussian initial packet (u, v) on the
+    device; u/v carry the 1-D eigenvector for the .GET* modes."""
+    n = int(np.prod(np_))
+    uu, vv = np.zeros(n), np.zeros(n)
+    if u is not None:
+        uu[:len(u)] = u
+    if v is not None:
+        vv[:len(v)] = v
+    _check(_load().espec_init_cond(len(np_), _np3(np_), mode, _v3(xi, 0.0), _v3(sh),
+                                   _v3(x0, 0.0), _v3(ra), _v3(rk, 0.0), uu, vv), "INIT_COND")
+    return uu, vv
 
 
 def ef(efc, e0, t0, td, tp, omg, sni, kl, t):

@@ -177,6 +177,34 @@ int espec_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
                 double e0, double t0, double td, double tp, double omg,
                 double sni, int kl, double *veff, double *cr, double *ci,
                 double *nrm, int maxsamp, int *nsamp, int flags);
+/* ---------------------------------------------------------------------
+ * Gaussian initial packets.  Replaces INIT_COND (init_cond.f:1-200) with
+ * GET_GAUSS (gauss.f:1-20): rnorm exp(-ln2 (c/rA)^2) (cos, sin)(k c) per
+ * axis, c = XI + (i-1) SH - X0, normalised over the axis.
+ *   mode ESPEC_INIT_CALC   1-D packet (U = cos g, V = sin g), or the 2-D
+ *                          product the reference forms (U + iV = the
+ *                          complex conjugate of the product of the 1-D
+ *                          packets)
+ *        ESPEC_INIT_GETR   2-D: the real eigenvector u[0..np0) along x
+ *                          times a Gaussian along y (.GETR, .GETR1)
+ *        ESPEC_INIT_GETR2  eigenvector u[0..np1) along y, Gaussian along x
+ *        ESPEC_INIT_GETC   complex eigenvector u + iv along x (.GETC, .GETC1)
+ *        ESPEC_INIT_GETC2  complex eigenvector along y
+ *   xi, sh, x0, ra, rk [ndim]  grid start and step, centre, HWHM and wave
+ *                              number per axis (a.u.)
+ *   u, v [N]               out (in: the eigenvector for the .GET* modes)
+ * ------------------------------------------------------------------- */
+enum {
+    ESPEC_INIT_CALC = 0,
+    ESPEC_INIT_GETR = 1,
+    ESPEC_INIT_GETR2 = 2,
+    ESPEC_INIT_GETC = 3,
+    ESPEC_INIT_GETC2 = 4
+};
+int espec_init_cond(int ndim, const int *np, int mode, const double *xi,
+                    const double *sh, const double *x0, const double *ra,
+                    const double *rk, double *u, double *v);
+
 /* Replaces EF (ef.f:1-60): the field at time t (fs); NaN for an unknown
  * efc, where the reference stops. */
 double espec_ef(int efc, double e0, double t0, double td, double tp,

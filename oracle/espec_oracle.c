@@ -992,6 +992,97 @@ int orc_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
     return nc + 1;
 }
 
+/* gauss.f GET_GAUSS: rnorm exp(-ln2 (coord/ra)^2), ln2 to 8 digits. */
+static double get_gauss(double ra, double coord, double rnorm)
+{
+    double q = coord / ra;
+    return rnorm * exp(-(0.69314718 * (q * q)));
+}
+
+/* 1/sqrt(sum of GET_GAUSS^2) over an axis (init_cond.f:60-66). */
+static double gauss_norm(int np, double xi, double sh, double x0, double ra)
+{
+    double r = 0;
+    for (int i = 0; i < np; ++i) {
+        double g = get_gauss(ra, xi + i * sh - x0, 1.0);
+        r = r + g * g;
+    }
+    return 1.0 / sqrt(r);
+}
+
+/* init_cond.f:1-200 INIT_COND.  mode: ORC_INIT_CALC (1-D or 2-D Gaussian
+ * packets), ORC_INIT_GETR/GETR1 (2-D: the real 1-D eigenvector u[0..np1)
+ * along axis 1 times a Gaussian along axis 2), ORC_INIT_GETR2 (eigenvector
+ * along axis 2, Gaussian along axis 1), ORC_INIT_GETC/GETC1/GETC2 (the same
+ * with the complex eigenvector u + i v).  Returns -1 for a combination the
+ * reference rejects. */
+int orc_init_cond(int ndim, const int *np, int mode, const double *xi,
+                  const double *sh, const double *x0, const double *ra,
+                  const double *rk, double *u, double *v)
+{
+    if (ndim == 1 && mode == ORC_INIT_CALC) {
+        double rn = gauss_norm(np[0], xi[0], sh[0], x0[0], ra[0]);
+        for (int i = 0; i < np[0]; ++i) {
+            double c = xi[0] + i * sh[0] - x0[0];
+            double g = get_gauss(ra[0], c, rn);
+            double a = rk[0] * c;
+            u[i] = cos(a) * g;
+            v[i] = sin(a) * g;
+        }
+        return 0;
+    }
+    if (ndim != 2) return -1;
+    /* eig: axis of the given factor (w1, w2); gx: the Gaussian's axis */
+    int eig, gx;
+    double *w1 = malloc(sizeof(double) * 2 * (np[0] > np[1] ? np[0] : np[1]));
+    double *w2 = w1 + (np[0] > np[1] ? np[0] : np[1]);
+    if (mode == ORC_INIT_CALC) {
+        eig = 0;
+        gx = 1;
+        double rn1 = gauss_norm(np[0], xi[0], sh[0], x0[0], ra[0]);
+        for (int i = 0; i < np[0]; ++i) {
+            double c = xi[0] + i * sh[0] - x0[0];
+            double g = get_gauss(ra[0], c, rn1);
+            double a = rk[0] * c;
+            w1[i] = cos(a) * g;
+            w2[i] = -sin(a) * g;
+        }
+    } else if (mode == ORC_INIT_GETR || mode == ORC_INIT_GETC) {
+        eig = 0;
+        gx = 1;
+    } else if (mode == ORC_INIT_GETR2 || mode == ORC_INIT_GETC2) {
+        eig = 1;
+        gx = 0;
+    } else {
+        free(w1);
+        return -1;
+    }
+    if (mode != ORC_INIT_CALC)
+        for (int i = 0; i < np[eig]; ++i) {
+            w1[i] = u[i];
+            w2[i] = (mode == ORC_INIT_GETC || mode == ORC_INIT_GETC2) ? v[i] : 0.0;
+        }
+    double rn = gauss_norm(np[gx], xi[gx], sh[gx], x0[gx], ra[gx]);
+    for (int j = 0; j < np[gx]; ++j) {
+        double c = xi[gx] + j * sh[gx] - x0[gx];
+        double g = get_gauss(ra[gx], c, rn);
+        double a = rk[gx] * c;
+        double ct = cos(a) * g, st = sin(a) * g;
+        for (int i = 0; i < np[eig]; ++i) {
+            int k = eig == 0 ? j * np[0] + i : i * np[0] + j;
+            if (mode == ORC_INIT_GETR || mode == ORC_INIT_GETR2) {
+                u[k] = w1[i] * ct;
+                v[k] = -w1[i] * st;
+            } else {
+                u[k] = w1[i] * ct + w2[i] * st;
+                v[k] = -w1[i] * st + w2[i] * ct;
+            }
+        }
+    }
+    free(w1);
+    return 0;
+}
+
 /* rans.f:32-39. */
 double orc_ran(int *nseed)
 {

@@ -29,6 +29,10 @@ enum { ORC_EXACT = 0, ORC_REF = 1 };
  * potential, hartree; spofft.f:65-118, sodfft.f:66-72) and .SMOOTHW
  * (vabc = mask; spofft.f:162-176, sodfft.f:108-116). */
 enum { ORC_ABS_NONE = 0, ORC_ABS_VOPTIC = 1, ORC_ABS_SMOOTHW = 2 };
+/* INIEIGVC choices of INIT_COND (init_cond.f): .CALC, .GETR/.GETR1,
+ * .GETR2, .GETC/.GETC1, .GETC2. */
+enum { ORC_INIT_CALC = 0, ORC_INIT_GETR = 1, ORC_INIT_GETR2 = 2, ORC_INIT_GETC = 3,
+       ORC_INIT_GETC2 = 4 };
 
 /* Atomic unit of time in fs used by every propagator (pspofft.f:76). */
 #define ORC_SHFS 2.4188843243e-2
@@ -139,6 +143,11 @@ int orc_plnz(int ndim, const int *np, int mp, double abstol, double dt,
              int maxsamp, int mode);
 /* ef.f EF (efc 0 .NONE/.NULL, 1 .GAUS, 2 .GGAUS, 3 .SIN2) and pplnz.f
  * PPLNZ with PRTCRL='.NO' (veff may be NULL). */
+/* init_cond.f INIT_COND (see ORC_INIT_*); u, v [N] out (in: the 1-D
+ * eigenvector for the .GET* modes). */
+int orc_init_cond(int ndim, const int *np, int mode, const double *xi,
+                  const double *sh, const double *x0, const double *ra,
+                  const double *rk, double *u, double *v);
 double orc_ef(int efc, double e0, double t0, double td, double tp, double omg,
               double sni, int kl, double t);
 int orc_pplnz(int ndim, const int *np, int mp, double abstol, double dt,

@@ -18,6 +18,7 @@ REF_DIR = os.path.join(HERE, "_ref")
 EXACT, REF = 0, 1
 ABS_NONE, ABS_VOPTIC, ABS_SMOOTHW = 0, 1, 2  # TPABSOR .VOPTIC / .SMOOTHW
 PULSE_NONE, PULSE_GAUS, PULSE_GGAUS, PULSE_SIN2 = 0, 1, 2, 3  # EFC (ef.f)
+INIT_CALC, INIT_GETR, INIT_GETR2, INIT_GETC, INIT_GETC2 = 0, 1, 2, 3, 4  # INIEIGVC
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -92,6 +93,8 @@ class Oracle:
         L.orc_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d,
                                _d, _d, _d, _ip, _ip]
         L.orc_plnz.restype = _ip
+        L.orc_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
+        L.orc_init_cond.restype = _ip
         L.orc_ef.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
         L.orc_ef.restype = _dp
         L.orc_pplnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _ip,
@@ -215,6 +218,21 @@ class Oracle:
         if k < 0:
             raise RuntimeError("diagonalization error")
         return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
+
+    def init_cond(self, np_, mode, xi, sh, x0, ra, rk, u=None, v=None):
+        """INIT_COND (init_cond.f): (u, v) of the initial packet; u, v carry
+        the 1-D eigenvector for the .GET* modes."""
+        n = int(np.prod(np_))
+        uu, vv = np.zeros(n), np.zeros(n)
+        if u is not None:
+            uu[:len(u)] = u
+        if v is not None:
+            vv[:len(v)] = v
+        k = self.lib.orc_init_cond(len(np_), _np3(np_), mode, _v3(xi, 0.0), _v3(sh), _v3(x0, 0.0),
+                                   _v3(ra), _v3(rk, 0.0), uu, vv)
+        if k < 0:
+            raise ValueError("INIT_COND: unsupported combination")
+        return uu, vv
 
     def ef(self, efc, e0, t0, td, tp, omg, sni, kl, t):
         """EF (ef.f): efc PULSE_NONE/GAUS/GGAUS/SIN2."""
@@ -352,6 +370,7 @@ class Reference:
         L.ref_hwhm.argtypes = [_ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_spectrumtd.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_ran.argtypes = [P(_ip)]
+        L.ref_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
         L.ref_ef.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
         L.ref_ef.restype = _dp
         L.ref_pplnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d,
@@ -406,6 +425,17 @@ class Reference:
 
     def ef(self, efc, e0, t0, td, tp, omg, sni, kl, t):
         return self.lib.ref_ef(efc, e0, t0, td, tp, omg, sni, kl, t)
+
+    def init_cond(self, np_, mode, xi, sh, x0, ra, rk, u=None, v=None):
+        n = int(np.prod(np_))
+        uu, vv = np.zeros(n + 1), np.zeros(n + 1)
+        if u is not None:
+            uu[:len(u)] = u
+        if v is not None:
+            vv[:len(v)] = v
+        self.lib.ref_init_cond(len(np_), _np3(np_), mode, _v3(xi, 0.0), _v3(sh), _v3(x0, 0.0),
+                               _v3(ra), _v3(rk, 0.0), uu, vv)
+        return uu[:n], vv[:n]
 
     def pplnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vp, dm, pulse):
         n = int(np.prod(np_))

@@ -261,6 +261,36 @@ contains
     deallocate(iwork, vabc, lnzvc, wu, wv, wp, work, vpot)
   end subroutine ref_pplnz
 
+  ! INIT_COND (init_cond.f): mode 0 .CALC, 1 .GETR, 2 .GETR2, 3 .GETC, 4 .GETC2.
+  subroutine ref_init_cond(ndim, np, mode, xi, sh, x0, ra, rk, u, v) &
+      bind(C, name='ref_init_cond')
+    integer(c_int), value :: ndim, mode
+    integer(c_int) :: np(3)
+    real(c_double) :: xi(3), sh(3), x0(3), ra(3), rk(3), u(*), v(*)
+    character(len=8) :: dims, ini
+    integer :: n
+    real(8), allocatable :: w1(:), w2(:)
+    real(8) :: var(4)
+    n = max(np(1), np(2), np(3)) + 1
+    allocate(w1(n), w2(n))
+    var = 0d0
+    call dimstr(ndim, dims)
+    select case (mode)
+    case (1)
+      ini = '.GETR   '
+    case (2)
+      ini = '.GETR2  '
+    case (3)
+      ini = '.GETC   '
+    case (4)
+      ini = '.GETC2  '
+    case default
+      ini = '.CALC   '
+    end select
+    call INIT_COND(dims, ini, np, rk, ra, sh, xi, x0, u, v, w1, w2, var)
+    deallocate(w1, w2)
+  end subroutine ref_init_cond
+
   ! FFT (fft3d.f:1): Singleton multivariate in-place transform.
   subroutine ref_fft(a, b, ntot, n, nspan, isn) bind(C, name='ref_fft')
     real(c_double) :: a(*), b(*)
