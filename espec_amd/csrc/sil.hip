@@ -274,22 +274,30 @@ __device__ __forceinline__ double apply_h(const LnzArgs &a, const S *u, const do
     return acc;
 }
 
-// Sum k values per thread over the block (thread 0 receives the sums).
-template <int NT>
-__device__ void block_sum_k(double *v, int k, double *red)
+// Sum k <= K values per thread over the block (thread 0 receives the
+// sums).  The loops run to the array's compile-time length with the count
+// as a predicate, so the array stays in registers (a runtime-indexed array
+// would live in scratch memory).
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_k(double (&v)[K], int k, double *red)
 {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int j = 0; j < k; ++j) {
-        double x = wave_sum(v[j]);
-        if (lane == 0) red[w * kPart + j] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int j = 0; j < k; ++j) {
-            double s = 0;
-            for (int q = 0; q < NT / 64; ++q) s += red[q * kPart + j];
-            v[j] = s;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (j < k) {
+            const double x = wave_sum(v[j]);
+            if (lane == 0) red[w * kPart + j] = x;
         }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < k) {
+                double s = 0;
+                for (int q = 0; q < NT / 64; ++q) s += red[q * kPart + j];
+                v[j] = s;
+            }
+    }
 }
 
 // Publish k block partials of (recurrence vec, chunk c); returns true in
@@ -297,15 +305,17 @@ __device__ void block_sum_k(double *v, int k, double *red)
 // totals.  No block waits.  The last block sums the partials in parallel
 // (block b's go to thread b, nb <= NT) and then in the fixed order of
 // block_sum_k, so results are deterministic.
-template <int NT>
-__device__ bool last_block(const LnzArgs &a, int vec, int c, const double *v, int k, double *tot,
-                           double *red)
+template <int NT, int K>
+__device__ bool last_block(const LnzArgs &a, int vec, int c, const double (&v)[K], int k,
+                           double *tot, double *red)
 {
     __shared__ bool last;
     const long long slot = (long long)vec * a.nchunk + c;
     double *p = a.part + (slot * a.nb + blockIdx.x) * kPart;
     if (threadIdx.x == 0) {
-        for (int j = 0; j < k; ++j) p[j] = v[j];
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < k) p[j] = v[j];
         __threadfence();
         unsigned t = atomicAdd(&a.ticket[slot], 1u);
         last = (t == (unsigned)a.nb - 1);
@@ -313,13 +323,16 @@ __device__ bool last_block(const LnzArgs &a, int vec, int c, const double *v, in
     __syncthreads();
     if (!last) return false;
     __threadfence();
-    double s[kPart];
+    double s[K];
     const double *q = a.part + (slot * a.nb + threadIdx.x) * kPart;
     const bool mine = (int)threadIdx.x < a.nb;
-    for (int j = 0; j < k; ++j) s[j] = mine ? q[j] : 0.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) s[j] = (mine && j < k) ? q[j] : 0.0;
     block_sum_k<NT>(s, k, red);
     if (threadIdx.x == 0) {
-        for (int j = 0; j < k; ++j) tot[j] = s[j];
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < k) tot[j] = s[j];
         a.ticket[slot] = 0;
     }
     return true;
@@ -345,10 +358,14 @@ __device__ double sum_parts1(const LnzArgs &a, const double *p, double *red)
 }
 
 // Publish this block's k partials (thread 0 holds them after block_sum_k).
-__device__ __forceinline__ void put_parts(double *p, const double *v, int k)
+template <int K>
+__device__ __forceinline__ void put_parts(double *p, const double (&v)[K], int k)
 {
-    if (threadIdx.x == 0)
-        for (int j = 0; j < k; ++j) p[(long long)blockIdx.x * kPart + j] = v[j];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < k) p[(long long)blockIdx.x * kPart + j] = v[j];
+    }
 }
 
 // Start: w = start vector of each recurrence (Re psi for even vec, Im psi
@@ -457,7 +474,9 @@ __global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
     for (int j = 0; j < kChunk; ++j) r[j] = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
         const double ws = (double)(S)((double)w[p] - al * (double)vcur[p]);
-        for (int j = 0; j < jn; ++j) r[j] += ws * (double)B[(long long)j * a.n + p];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j)
+            if (j < jn) r[j] += ws * (double)B[(long long)j * a.n + p];
     }
     block_sum_k<NT>(r, jn, red);
     put_parts(a.pr + ((long long)vec * a.nchunk + c) * a.nb * kPart, r, jn);
@@ -558,7 +577,9 @@ __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
         V2<S> v = x0[p];
         const double u = (vec & 1) ? (double)v.y : (double)v.x;
-        for (int j = 0; j < m; ++j) c[j] += (double)B[(long long)j * a.n + p] * u;
+#pragma unroll
+        for (int j = 0; j < kMaxKrylov; ++j)
+            if (j < m) c[j] += (double)B[(long long)j * a.n + p] * u;
     }
     block_sum_k<NT>(c, m, red);
     double tot[kMaxKrylov];
@@ -943,7 +964,9 @@ __global__ __launch_bounds__(NT) void ctl_coeffs(CtlArgs c)
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
         const V2<S> v = x0[p];
         const double u = (vec & 1) ? (double)v.y : (double)v.x;
-        for (int j = 0; j < m; ++j) cp[j] += (double)B[(long long)j * a.n + p] * u;
+#pragma unroll
+        for (int j = 0; j < kMaxKrylov; ++j)
+            if (j < m) cp[j] += (double)B[(long long)j * a.n + p] * u;
     }
     block_sum_k<NT>(cp, m, red);
     double tot[kMaxKrylov];
