@@ -72,7 +72,7 @@ def main(ref):
     stops = ["Correlation Function", "Spectrum:", "Partial auto", "Total correlation",
              "Calculating", "Propagating", "Eigenvector propagated"]
     limits = {"table": {6: 1200}, "corr": {6: 0}, "spec": {6: 0}}
-    for t in (4, 5, 6, 8, 10, 12):
+    for t in (4, 5, 6, 8, 10, 12, 13):
         lines = open(os.path.join(tt, f"results.test{t}")).read().splitlines()
         for name, title, nc in (("table", "Partial auto-correlation function", 6),
                                 ("corr", "Correlation Function", 3),
