@@ -30,6 +30,8 @@ PULSE_NONE, PULSE_GAUS, PULSE_GGAUS, PULSE_SIN2 = 0, 1, 2, 3
 # INIEIGVC forms of INIT_COND (init_cond.f): .CALC, .GETR/.GETR1, .GETR2,
 # .GETC/.GETC1, .GETC2.
 INIT_CALC, INIT_GETR, INIT_GETR2, INIT_GETC, INIT_GETC2 = 0, 1, 2, 3, 4
+# Absorber shapes of ABSORBINGBC (absorb.f).
+ABG_SMOOTHW, ABG_SMRS, ABG_VOPTIC_LINEAR, ABG_VOPTIC_KOSLOFF, ABG_VOPTIC_POLIN = 1, 2, 3, 4, 5
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 AMU = 1836.152667539    # electron masses per amu (espec.f:614)
 A0A = 0.5291772083      # angstrom per bohr (espec.f:67)
@@ -89,6 +91,7 @@ def _load():
     L.espec_sil_plan_set_pulse.argtypes = [vp, d, d, ip]
     L.espec_ef.argtypes = [ip, dp, dp, dp, dp, dp, dp, ip, dp]
     L.espec_init_cond.argtypes = [ip, i, ip, d, d, d, d, d, d, d]
+    L.espec_absorbingbc.argtypes = [ip, i, ip, d, d, d, d, d, d, d, P(dp)]
     L.espec_ef.restype = dp
     L.espec_pplnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, ip, dp, dp, dp, dp,
                               dp, dp, ip, vp, d, d, vp, ip, P(ip), ip]
@@ -199,6 +202,15 @@ def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT)
     _check(rc, "PLNZ")
     k = ns.value
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
+
+
+def absorbingbc(np_, kind, al, ar, sh, xi, xf, voi):
+    """ABSORBINGBC (absorb.f): (vabc, alpha) on the device."""
+    va, alpha = np.zeros(int(np.prod(np_))), C.c_double(0.0)
+    _check(_load().espec_absorbingbc(len(np_), _np3(np_), kind, _v3(al, 0.0), _v3(ar, 0.0),
+                                     _v3(sh), _v3(xi, 0.0), _v3(xf, 0.0), _v3(voi, 0.0), va,
+                                     C.byref(alpha)), "ABSORBINGBC")
+    return va, alpha.value
 
 
 def init_cond(np_, mode, xi, sh, x0, ra, rk, u=None, v=None):

@@ -17,7 +17,9 @@ LANCZS on the device (.CALC; .MTRXDIAG is served by the same full-
 dimension LANCZS) or read from the file (.GETR), the LC/TF rule
 (espec.f:1324-1353), the propagation (.PFSPO, .PFSOD, .PSIL, .PPSIL on the
 device), the C(t) assembly (espec.f:1546-1566), HWHM and SPECTRUMTD.
-Every numerical stage runs through libespec_hip.so.
+The initial state, the propagation, the assembly and the spectrum run
+through libespec_hip.so; the potentials' closed forms are evaluated on
+the host, as the reference's generators evaluate them once per run.
 """
 import os
 from dataclasses import dataclass, field

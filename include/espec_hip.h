@@ -101,6 +101,26 @@ enum {
     ESPEC_ABSORB_VOPTIC = 1,
     ESPEC_ABSORB_SMOOTHW = 2
 };
+/* Replaces ABSORBINGBC (absorb.f:1-130): the VABC array of an absorber
+ * shape on the grid (x fastest), X = XI + (I-1) SH per axis.
+ *   kind  ESPEC_ABG_SMOOTHW (.SMOOTHW mask, 1-D and 2-D), ESPEC_ABG_SMRS
+ *         (.SMRS mask), ESPEC_ABG_VOPTIC_LINEAR, _KOSLOFF, _POLIN
+ *         (imaginary potentials), the last four 1-D only
+ *   al, ar, sh, xi, xf, voi [ndim]  the reference's arguments
+ *   alpha out (may be NULL): 1 - 0.1 (XF - AR) for the 1-D .SMOOTHW
+ * ESPEC_ERR_UNSUPPORTED where the reference stops (3-D, 2-D non-SMOOTHW). */
+enum {
+    ESPEC_ABG_SMOOTHW = 1,
+    ESPEC_ABG_SMRS = 2,
+    ESPEC_ABG_VOPTIC_LINEAR = 3,
+    ESPEC_ABG_VOPTIC_KOSLOFF = 4,
+    ESPEC_ABG_VOPTIC_POLIN = 5
+};
+int espec_absorbingbc(int ndim, const int *np, int kind, const double *al,
+                      const double *ar, const double *sh, const double *xi,
+                      const double *xf, const double *voi, double *vabc,
+                      double *alpha);
+
 /* espec_pspofft with an absorber; vabc [N] (may be NULL for NONE). */
 int espec_pspofft_absorb(int ndim, const int *np, double dt, double ti,
                          double tf, int lc, const double *sh, const double *sm,

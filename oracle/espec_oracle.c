@@ -1083,6 +1083,92 @@ int orc_init_cond(int ndim, const int *np, int mode, const double *xi,
     return 0;
 }
 
+/* absorb.f:1-130 ABSORBINGBC: VABC on the grid (x fastest) for kind
+ * ORC_ABG_SMOOTHW (.SMOOTHW mask; 1-D and 2-D), ORC_ABG_SMRS (.SMRS
+ * mask), ORC_ABG_VOPTIC_LINEAR, ORC_ABG_VOPTIC_KOSLOFF, ORC_ABG_VOPTIC_POLIN
+ * (imaginary potentials; 1-D).  alpha receives 1 - 0.1 (XF - AR) (the
+ * default-REAL 0.1) for the 1-D .SMOOTHW.  Returns -1 where the reference
+ * stops. */
+int orc_absorbingbc(int ndim, const int *np, int kind, const double *al,
+                    const double *ar, const double *sh, const double *xi,
+                    const double *xf, const double *voi, double *vabc,
+                    double *alpha)
+{
+    if (ndim == 1) {
+        for (int i = 0; i < np[0]; ++i) {
+            double x1 = xi[0] + (i + 1 - 1.0) * sh[0], v;
+            int left = x1 < al[0], right = x1 > ar[0];
+            switch (kind) {
+            case ORC_ABG_SMOOTHW:
+                if (left)
+                    v = exp(voi[0] * pow(x1 - al[0], voi[1]) / (x1 + sh[0] - xi[0]));
+                else if (right)
+                    v = exp(-(voi[0] * (x1 - ar[0]) / pow(xf[0] + sh[0] - x1, voi[1])));
+                else
+                    v = 1.0;
+                break;
+            case ORC_ABG_SMRS:
+                if (left)
+                    v = 1.0 - voi[0] * ((x1 - al[0]) * (x1 - al[0]));
+                else if (right)
+                    v = 1.0 - voi[0] * ((x1 - ar[0]) * (x1 - ar[0]));
+                else
+                    v = 1.0;
+                break;
+            case ORC_ABG_VOPTIC_LINEAR:
+                if (left)
+                    v = -voi[0] * (x1 - al[0]) / (x1 + sh[0] - xi[0]);
+                else if (right)
+                    v = voi[0] * (x1 - ar[0]) / (xf[0] + sh[0] - x1);
+                else
+                    v = 0.0;
+                break;
+            case ORC_ABG_VOPTIC_KOSLOFF:
+                /* absorb.f: the argument of COSH does not depend on X1 */
+                if (left)
+                    v = voi[0] / cosh(xi[0] - al[0] / voi[1]);
+                else if (right)
+                    v = voi[0] / cosh(xf[0] - al[0] / voi[1]);
+                else
+                    v = 0.0;
+                break;
+            case ORC_ABG_VOPTIC_POLIN:
+                if (left)
+                    v = voi[0] * pow(x1 - al[0], voi[1]);
+                else if (right)
+                    v = voi[0] * pow(x1 - ar[0], voi[1]);
+                else
+                    v = 0.0;
+                break;
+            default:
+                return -1;
+            }
+            vabc[i] = v;
+        }
+        if (kind == ORC_ABG_SMOOTHW && alpha) *alpha = 1 - (double)0.1f * (xf[0] - ar[0]);
+        return 0;
+    }
+    if (ndim != 2 || kind != ORC_ABG_SMOOTHW) return -1;
+    for (int j = 0; j < np[1]; ++j) {
+        double x2 = xi[1] + (j + 1 - 1.0) * sh[1];
+        for (int i = 0; i < np[0]; ++i) {
+            double x1 = xi[0] + (i + 1 - 1.0) * sh[0], v;
+            if (x1 < al[0])
+                v = exp(voi[0] * (x1 - al[0]) / (x1 + sh[0] - xi[0]));
+            else if (x2 < al[1])
+                v = exp(voi[1] * (x2 - al[1]) / (x2 + sh[1] - xi[1]));
+            else if (x1 > ar[0])
+                v = exp(-voi[0] * (x1 - ar[0]) / (xf[0] + sh[0] - x1));
+            else if (x2 > ar[1])
+                v = exp(-voi[1] * (x2 - ar[1]) / (xf[1] + sh[1] - x2));
+            else
+                v = 1.0;
+            vabc[(size_t)j * np[0] + i] = v;
+        }
+    }
+    return 0;
+}
+
 /* rans.f:32-39. */
 double orc_ran(int *nseed)
 {

@@ -29,6 +29,9 @@ enum { ORC_EXACT = 0, ORC_REF = 1 };
  * potential, hartree; spofft.f:65-118, sodfft.f:66-72) and .SMOOTHW
  * (vabc = mask; spofft.f:162-176, sodfft.f:108-116). */
 enum { ORC_ABS_NONE = 0, ORC_ABS_VOPTIC = 1, ORC_ABS_SMOOTHW = 2 };
+/* Absorber shapes of ABSORBINGBC (absorb.f). */
+enum { ORC_ABG_SMOOTHW = 1, ORC_ABG_SMRS = 2, ORC_ABG_VOPTIC_LINEAR = 3,
+       ORC_ABG_VOPTIC_KOSLOFF = 4, ORC_ABG_VOPTIC_POLIN = 5 };
 /* INIEIGVC choices of INIT_COND (init_cond.f): .CALC, .GETR/.GETR1,
  * .GETR2, .GETC/.GETC1, .GETC2. */
 enum { ORC_INIT_CALC = 0, ORC_INIT_GETR = 1, ORC_INIT_GETR2 = 2, ORC_INIT_GETC = 3,
@@ -148,6 +151,11 @@ int orc_plnz(int ndim, const int *np, int mp, double abstol, double dt,
 int orc_init_cond(int ndim, const int *np, int mode, const double *xi,
                   const double *sh, const double *x0, const double *ra,
                   const double *rk, double *u, double *v);
+/* absorb.f ABSORBINGBC (see ORC_ABG_*); alpha may be NULL. */
+int orc_absorbingbc(int ndim, const int *np, int kind, const double *al,
+                    const double *ar, const double *sh, const double *xi,
+                    const double *xf, const double *voi, double *vabc,
+                    double *alpha);
 double orc_ef(int efc, double e0, double t0, double td, double tp, double omg,
               double sni, int kl, double t);
 int orc_pplnz(int ndim, const int *np, int mp, double abstol, double dt,

@@ -19,6 +19,8 @@ EXACT, REF = 0, 1
 ABS_NONE, ABS_VOPTIC, ABS_SMOOTHW = 0, 1, 2  # TPABSOR .VOPTIC / .SMOOTHW
 PULSE_NONE, PULSE_GAUS, PULSE_GGAUS, PULSE_SIN2 = 0, 1, 2, 3  # EFC (ef.f)
 INIT_CALC, INIT_GETR, INIT_GETR2, INIT_GETC, INIT_GETC2 = 0, 1, 2, 3, 4  # INIEIGVC
+# ABSORBINGBC shapes (absorb.f)
+ABG_SMOOTHW, ABG_SMRS, ABG_VOPTIC_LINEAR, ABG_VOPTIC_KOSLOFF, ABG_VOPTIC_POLIN = 1, 2, 3, 4, 5
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -93,6 +95,8 @@ class Oracle:
         L.orc_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d,
                                _d, _d, _d, _ip, _ip]
         L.orc_plnz.restype = _ip
+        L.orc_absorbingbc.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d, C.POINTER(_dp)]
+        L.orc_absorbingbc.restype = _ip
         L.orc_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
         L.orc_init_cond.restype = _ip
         L.orc_ef.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
@@ -218,6 +222,17 @@ class Oracle:
         if k < 0:
             raise RuntimeError("diagonalization error")
         return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
+
+    def absorbingbc(self, np_, kind, al, ar, sh, xi, xf, voi):
+        """ABSORBINGBC (absorb.f): (vabc, alpha)."""
+        n = int(np.prod(np_))
+        va, alpha = np.zeros(n), _dp(0.0)
+        k = self.lib.orc_absorbingbc(len(np_), _np3(np_), kind, _v3(al, 0.0), _v3(ar, 0.0), _v3(sh),
+                                     _v3(xi, 0.0), _v3(xf, 0.0), _v3(voi, 0.0), va,
+                                     C.byref(alpha))
+        if k < 0:
+            raise ValueError("ABSORBINGBC: unsupported combination")
+        return va, alpha.value
 
     def init_cond(self, np_, mode, xi, sh, x0, ra, rk, u=None, v=None):
         """INIT_COND (init_cond.f): (u, v) of the initial packet; u, v carry
@@ -370,6 +385,7 @@ class Reference:
         L.ref_hwhm.argtypes = [_ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_spectrumtd.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_ran.argtypes = [P(_ip)]
+        L.ref_absorbingbc.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d, P(_dp)]
         L.ref_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
         L.ref_ef.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
         L.ref_ef.restype = _dp
@@ -425,6 +441,13 @@ class Reference:
 
     def ef(self, efc, e0, t0, td, tp, omg, sni, kl, t):
         return self.lib.ref_ef(efc, e0, t0, td, tp, omg, sni, kl, t)
+
+    def absorbingbc(self, np_, kind, al, ar, sh, xi, xf, voi):
+        n = int(np.prod(np_))
+        va, alpha = np.zeros(n), _dp(0.0)
+        self.lib.ref_absorbingbc(len(np_), _np3(np_), kind, _v3(al, 0.0), _v3(ar, 0.0), _v3(sh),
+                                 _v3(xi, 0.0), _v3(xf, 0.0), _v3(voi, 0.0), va, C.byref(alpha))
+        return va, alpha.value
 
     def init_cond(self, np_, mode, xi, sh, x0, ra, rk, u=None, v=None):
         n = int(np.prod(np_))

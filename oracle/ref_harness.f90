@@ -291,6 +291,36 @@ contains
     deallocate(w1, w2)
   end subroutine ref_init_cond
 
+  ! ABSORBINGBC (absorb.f): kind 1 .SMOOTHW, 2 .SMRS, 3 .VOPTIC_LINEAR,
+  ! 4 .VOPTIC_KOSLOFF, 5 .VOPTIC_POLIN.
+  subroutine ref_absorbingbc(ndim, np, kind, al, ar, sh, xi, xf, voi, vabc, alpha) &
+      bind(C, name='ref_absorbingbc')
+    integer(c_int), value :: ndim, kind
+    integer(c_int) :: np(3)
+    real(c_double) :: al(3), ar(3), sh(3), xi(3), xf(3), voi(3), vabc(*), alpha
+    character(len=8) :: dims
+    character(len=16) :: tp
+    integer :: n, i
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    call dimstr(ndim, dims)
+    select case (kind)
+    case (1)
+      tp = '.SMOOTHW        '
+    case (2)
+      tp = '.SMRS           '
+    case (3)
+      tp = '.VOPTIC_LINEAR  '
+    case (4)
+      tp = '.VOPTIC_KOSLOFF '
+    case default
+      tp = '.VOPTIC_POLIN   '
+    end select
+    call ABSORBINGBC(dims, tp, n, np, al, ar, sh, xi, xf, voi, vabc, alpha)
+  end subroutine ref_absorbingbc
+
   ! FFT (fft3d.f:1): Singleton multivariate in-place transform.
   subroutine ref_fft(a, b, ntot, n, nspan, isn) bind(C, name='ref_fft')
     real(c_double) :: a(*), b(*)
