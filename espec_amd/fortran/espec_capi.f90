@@ -8,6 +8,8 @@ module espec_capi
 
   integer, parameter :: ESPEC_FP64 = 0, ESPEC_MIXED = 1, ESPEC_EXACT = 0, ESPEC_REF = 2
   integer, parameter :: ESPEC_OK = 0, ESPEC_ERR_ARG = 2, ESPEC_ERR_UNSUPPORTED = 6
+  integer, parameter :: ESPEC_ABG_SMOOTHW = 1, ESPEC_ABG_SMRS = 2, ESPEC_ABG_VOPTIC_LINEAR = 3, &
+                        ESPEC_ABG_VOPTIC_KOSLOFF = 4, ESPEC_ABG_VOPTIC_POLIN = 5
 
   interface
     function espec_strerror(code) bind(C, name='espec_strerror') result(p)
@@ -76,6 +78,15 @@ module espec_capi
       type(c_ptr), value :: plan
       real(c_double) :: dm(*), efield(*)
       integer(c_int), value :: nfield
+      integer(c_int) :: rc
+    end function
+
+    function espec_absorbingbc(ndim, np, kind, al, ar, sh, xi, xf, voi, vabc, alpha) &
+        bind(C, name='espec_absorbingbc') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, kind
+      integer(c_int) :: np(*)
+      real(c_double) :: al(*), ar(*), sh(*), xi(*), xf(*), voi(*), vabc(*), alpha
       integer(c_int) :: rc
     end function
 

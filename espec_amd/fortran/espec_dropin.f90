@@ -552,6 +552,39 @@ subroutine LANCZS(CHANGE, DIM, IL, IU, INFO, NREORT, LMTREORT, N, NSEED, MXDCT, 
 end subroutine LANCZS
 
 ! fft3d.f:1
+! absorb.f:1-130: VABC of TPABSOR on the grid, on the device (N is not
+! read; the grid is NP); VABC is left as it was where the reference has
+! no shape (3-D, 2-D other than .SMOOTHW, unknown TPABSOR).
+subroutine ABSORBINGBC(DIM, TPABSOR, N, NP, AL, AR, SH, XI, XF, VOI, VABC, ALPHA)
+  use espec_capi
+  implicit none
+  character(len=*) :: DIM, TPABSOR
+  integer :: N
+  integer :: NP(*)
+  real(8) :: AL(*), AR(*), SH(*), XI(*), XF(*), VOI(*), VABC(*), ALPHA
+  integer :: rc, kind, nd
+  nd = dim_of(DIM)
+  kind = 0
+  select case (trim(TPABSOR))
+  case ('.SMOOTHW')
+    kind = ESPEC_ABG_SMOOTHW
+  case ('.SMRS')
+    kind = ESPEC_ABG_SMRS
+  case ('.VOPTIC_LINEAR')
+    kind = ESPEC_ABG_VOPTIC_LINEAR
+  case ('.VOPTIC_KOSLOFF')
+    kind = ESPEC_ABG_VOPTIC_KOSLOFF
+  case ('.VOPTIC_POLIN')
+    kind = ESPEC_ABG_VOPTIC_POLIN
+  end select
+  if (nd == 0 .or. kind == 0) then
+    write(*,'(A)') '<<<>>> ABSORBINGBC: no absorber of this DIM/TPABSOR'
+    return
+  end if
+  rc = espec_absorbingbc(nd, NP, kind, AL, AR, SH, XI, XF, VOI, VABC, ALPHA)
+  if (rc /= 0) write(*,'(A,I3)') '<<<>>> ABSORBINGBC on the device failed, code ', rc
+end subroutine ABSORBINGBC
+
 subroutine FFT(A, B, NTOT, N, NSPAN, ISN)
   use espec_capi
   implicit none

@@ -47,3 +47,29 @@ def test_absorbingbc_rejects_unsupported(gpu):
                       [0.7, 0.7], [1, 1])
     with pytest.raises(E.EspecError):
         E.absorbingbc([8], 9, [1], [2], [0.1], [0], [0.7], [1, 1])
+
+
+def test_absorbingbc_fortran_dropin(gpu, orc):
+    """ABSORBINGBC(DIM, TPABSOR, N, NP, AL, AR, SH, XI, XF, VOI, VABC, ALPHA)
+    of libespec_f.so with the reference's argument list: by reference, the
+    two CHARACTER lengths appended by value (gfortran's convention)."""
+    import ctypes as C
+    from espec_amd import build as B
+    E._load()
+    lib = C.CDLL(B.FLIB)
+    n = 300
+    sh, xi = 0.02, 0.0
+    xf = xi + (n - 1) * sh
+    d = lambda *v: np.array(v, dtype=np.float64)
+    va, alpha = np.zeros(n), C.c_double(0.0)
+    dim, tp = b".1D", b".VOPTIC_LINEAR"
+    f = lib.absorbingbc_
+    f.restype = None
+    f(C.c_char_p(dim), C.c_char_p(tp), C.byref(C.c_int(n)),
+      np.array([n, 1, 1], dtype=np.int32).ctypes.data_as(C.c_void_p),
+      *[a.ctypes.data_as(C.c_void_p) for a in (d(1.0), d(xf - 1.0), d(sh), d(xi), d(xf),
+                                              d(0.4, 1.0), va)],
+      C.byref(alpha), C.c_size_t(len(dim)), C.c_size_t(len(tp)))
+    o, _ = orc.absorbingbc([n], E.ABG_VOPTIC_LINEAR, [1.0], [xf - 1.0], [sh], [xi], [xf],
+                           [0.4, 1.0])
+    assert close(va, o) and np.any(o != 0)
