@@ -99,7 +99,11 @@ extern "C" int espec_absorbingbc(int ndim, const int *np, int kind, const double
         a.xf[d] = xf[d];
         a.voi[d] = voi[d];
     }
-    if (ndim == 1) a.voi[1] = voi[1];
+    // The 1-D shapes with an exponent or width in VOI(2) read it
+    // (absorb.f:66,72,123,133); the others take one VOI.
+    if (ndim == 1 && (kind == ESPEC_ABG_SMOOTHW || kind == ESPEC_ABG_VOPTIC_KOSLOFF ||
+                      kind == ESPEC_ABG_VOPTIC_POLIN))
+        a.voi[1] = voi[1];
     const size_t n = (size_t)a.np0 * a.np1;
     DevBuf d;
     ESPEC_HIP_TRY(d.alloc(sizeof(double) * n));

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "api_common.hpp"
@@ -728,6 +729,20 @@ __global__ __launch_bounds__(256) void dot_kernel(LnzArgs a, const double *u1, c
     if (last_block<256>(a, 0, 0, v, 1, tot, red) && threadIdx.x == 0) *out = tot[0];
 }
 
+// Scratch of espec_fcorrel_noalloc, which allocates nothing: the block
+// partials of dot_kernel (at most kScrBlocks blocks of kPart slots) and
+// its ticket counter, in one array per device for the life of the process.
+// Host code reaches it only through the address hipGetSymbolAddress gives
+// for the current device (scratch_address); the array's own name is a host
+// stand-in, not a device address.
+constexpr int kScrBlocks = 256;  // blocks_for() never exceeds this
+constexpr size_t kScrTicket = sizeof(double) * kScrBlocks * kPart;
+constexpr size_t kScrBytes = kScrTicket + 256;
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+__global__ void scratch_clear_ticket(unsigned *ticket) { *ticket = 0u; }
+__global__ void scratch_zero_out(double *out) { *out = 0.0; }
+
 // DSTEVX('V','A') for one tridiagonal: QL vectors, QL or bisection values.
 __global__ __launch_bounds__(64) void dstevx_kernel(int m, double *d, double *e, double *e2,
                                                     double abstol, double *w, double *z,
@@ -1245,8 +1260,9 @@ struct espec_sil_plan {
     long long nfield = 0;
     LnzBufs lb;
     SilArgs sa{};
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    // step graphs of 8 steps and of one step (runs replay graphs only)
+    hipGraphExec_t gexec = nullptr, gexec1 = nullptr;
+    hipGraph_t graph = nullptr, graph1 = nullptr;
     // adaptive and trace runs: second state buffer, coefficient sets,
     // estimates, controller inputs/outputs and its bfloat16 weights
     DevBuf xalt, ccoef, cest, cest4, cfeat, cout, cwb;
@@ -1258,8 +1274,10 @@ struct espec_sil_plan {
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
+        if (gexec1) (void)hipGraphExecDestroy(gexec1);
+        if (graph1) (void)hipGraphDestroy(graph1);
+        gexec = gexec1 = nullptr;
+        graph = graph1 = nullptr;
     }
     ~espec_sil_plan()
     {
@@ -1352,6 +1370,22 @@ hipError_t sil_step(espec_sil_plan &P, hipStream_t s)
 }
 
 template <typename S>
+int sil_capture(espec_sil_plan &P, int steps, hipGraph_t *g, hipGraphExec_t *x)
+{
+    if (*x) return ESPEC_OK;
+    hipStream_t cs;
+    ESPEC_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    for (int k = 0; e == hipSuccess && k < steps; ++k) e = sil_step<S>(P, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, g);
+    (void)hipStreamDestroy(cs);
+    ESPEC_HIP_TRY(e);
+    ESPEC_HIP_TRY(e2);
+    ESPEC_HIP_TRY(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
+    return ESPEC_OK;
+}
+
+template <typename S>
 int sil_run_typed(espec_sil_plan &P, int nsteps)
 {
     hipLaunchKernelGGL(sil_set_step, dim3(1), dim3(1), 0, P.stream, P.dstep.as<long long>(),
@@ -1360,23 +1394,18 @@ int sil_run_typed(espec_sil_plan &P, int nsteps)
     const int gsteps = 8;
     int left = nsteps;
     if (left >= gsteps) {
-        if (!P.gexec) {
-            hipStream_t cs;
-            ESPEC_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-            for (int k = 0; e == hipSuccess && k < gsteps; ++k) e = sil_step<S>(P, cs);
-            hipError_t e2 = hipStreamEndCapture(cs, &P.graph);
-            (void)hipStreamDestroy(cs);
-            ESPEC_HIP_TRY(e);
-            ESPEC_HIP_TRY(e2);
-            ESPEC_HIP_TRY(hipGraphInstantiate(&P.gexec, P.graph, nullptr, nullptr, 0));
-        }
+        const int rc = sil_capture<S>(P, gsteps, &P.graph, &P.gexec);
+        if (rc) return rc;
         while (left >= gsteps) {
             ESPEC_HIP_TRY(hipGraphLaunch(P.gexec, P.stream));
             left -= gsteps;
         }
     }
-    for (; left > 0; --left) ESPEC_HIP_TRY(sil_step<S>(P, P.stream));
+    if (left > 0) {
+        const int rc = sil_capture<S>(P, 1, &P.graph1, &P.gexec1);
+        if (rc) return rc;
+        for (; left > 0; --left) ESPEC_HIP_TRY(hipGraphLaunch(P.gexec1, P.stream));
+    }
     P.steps_done += nsteps;
     return ESPEC_OK;
 }
@@ -1974,6 +2003,58 @@ int espec_fcorrel(int n, const double *u1, const double *u2, double *out)
                        b.as<double>(), res.as<double>());
     ESPEC_HIP_TRY(hipGetLastError());
     ESPEC_HIP_TRY(hipMemcpy(out, res.p, sizeof(double), hipMemcpyDeviceToHost));
+    return ESPEC_OK;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+static int scratch_address(void **out)
+{
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    int dev = 0;
+    ESPEC_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = addr.find(dev);
+    if (it == addr.end()) {
+        void *p = nullptr;
+        ESPEC_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return ESPEC_ERR_HIP;
+        it = addr.emplace(dev, p).first;
+    }
+    *out = it->second;
+    return ESPEC_OK;
+}
+
+int espec_fcorrel_noalloc(int n, const double *u1, const double *u2, double *out, void *stream)
+{
+    if (n < 0 || !u1 || !u2 || !out) return ESPEC_ERR_ARG;
+    if (n > ESPEC_NOALLOC_MAX_N) {
+        std::fprintf(stderr, "espec_fcorrel_noalloc: n = %d exceeds the maximum %d\n", n,
+                     ESPEC_NOALLOC_MAX_N);
+        return ESPEC_ERR_ARG;
+    }
+    void *scr = nullptr;
+    const int rc = scratch_address(&scr);
+    if (rc) return rc;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (n == 0) {
+        hipLaunchKernelGGL(scratch_zero_out, dim3(1), dim3(1), 0, s, out);
+        ESPEC_HIP_TRY(hipGetLastError());
+        return ESPEC_OK;
+    }
+    LnzArgs l{};
+    l.n = n;
+    l.nb = blocks_for(n);
+    l.nchunk = 1;
+    if (l.nb > kScrBlocks) return ESPEC_ERR_ARG;
+    l.part = static_cast<double *>(scr);
+    l.ticket = reinterpret_cast<unsigned *>(static_cast<char *>(scr) + kScrTicket);
+    // The ticket starts at zero (a kernel, not a memset, so that the call
+    // can be captured into a graph); the last block leaves it at zero too.
+    hipLaunchKernelGGL(scratch_clear_ticket, dim3(1), dim3(1), 0, s, l.ticket);
+    hipLaunchKernelGGL(dot_kernel, dim3(l.nb), dim3(256), 0, s, l, u1, u2, out);
+    ESPEC_HIP_TRY(hipGetLastError());
     return ESPEC_OK;
 }
 

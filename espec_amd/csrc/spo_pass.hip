@@ -968,32 +968,33 @@ hipError_t one_step(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, int
 
 void drop_graph(SpoPassPlan &p)
 {
-    if (p.gexec) (void)hipGraphExecDestroy(p.gexec);
-    if (p.graph) (void)hipGraphDestroy(p.graph);
-    p.gexec = nullptr;
-    p.graph = nullptr;
-    p.graph_steps = 0;
+    for (int j = 0; j < SpoPassPlan::kGraphLevels; ++j) {
+        if (p.gexec[j]) (void)hipGraphExecDestroy(p.gexec[j]);
+        if (p.graph[j]) (void)hipGraphDestroy(p.graph[j]);
+        p.gexec[j] = nullptr;
+        p.graph[j] = nullptr;
+    }
 }
 
-// Record kGraphSteps full steps (row pass + strided passes each) into a
-// graph; replays advance the device step counter themselves.  Capture
-// records without executing, so it costs no steps.
-constexpr int kGraphSteps = 32;
-
+// Record 2^j full steps (row pass + strided passes each) into a graph;
+// replays advance the device step counter themselves.  Capture records
+// without executing, so it costs no steps.  Graphs of 1, 2, 4, ..., 32
+// steps let a run of any length replay graphs only (a 20-step run is
+// 16 + 2 + 1), so short runs do not fall back to single launches.
 template <typename R>
-int capture_typed(SpoPassPlan &p)
+int capture_typed(SpoPassPlan &p, int j)
 {
+    if (p.gexec[j]) return ESPEC_OK;
     hipStream_t cs;
     ESPEC_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     int dummy = 0;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    for (int k = 0; e == hipSuccess && k < kGraphSteps; ++k) e = one_step<R>(p, true, true, cs, &dummy);
-    hipError_t e2 = hipStreamEndCapture(cs, &p.graph);
+    for (int k = 0; e == hipSuccess && k < (1 << j); ++k) e = one_step<R>(p, true, true, cs, &dummy);
+    hipError_t e2 = hipStreamEndCapture(cs, &p.graph[j]);
     (void)hipStreamDestroy(cs);
     ESPEC_HIP_TRY(e);
     ESPEC_HIP_TRY(e2);
-    ESPEC_HIP_TRY(hipGraphInstantiate(&p.gexec, p.graph, nullptr, nullptr, 0));
-    p.graph_steps = kGraphSteps;
+    ESPEC_HIP_TRY(hipGraphInstantiate(&p.gexec[j], p.graph[j], nullptr, nullptr, 0));
     return ESPEC_OK;
 }
 
@@ -1006,21 +1007,18 @@ int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *l
     // Step 1 of the run: V half phase + forward x, then the other axes.
     ESPEC_HIP_TRY(one_step<R>(p, false, true, s, launches));
     // Steps 2..nsteps: finish the previous step and start the next.  The
-    // same launch sequence repeats, so it is captured once into a graph of
-    // graph_steps steps and replayed.
+    // same launch sequence repeats, so it replays from the step graphs of
+    // 32, 16, ..., 1 steps, largest first.
     int left = nsteps - 1;
-    if (left >= kGraphSteps) {
-        if (!p.gexec) {
-            const int rc = capture_typed<R>(p);
-            if (rc) return rc;
-        }
-        while (left >= p.graph_steps) {
-            ESPEC_HIP_TRY(hipGraphLaunch(p.gexec, s));
-            *launches += p.graph_steps * (p.ndim == 1 ? 2 : p.ndim);
-            left -= p.graph_steps;
-        }
+    while (left > 0) {
+        int j = SpoPassPlan::kGraphLevels - 1;
+        while ((1 << j) > left) --j;
+        const int rc = capture_typed<R>(p, j);
+        if (rc) return rc;
+        ESPEC_HIP_TRY(hipGraphLaunch(p.gexec[j], s));
+        *launches += (1 << j) * (p.ndim == 1 ? 2 : p.ndim);
+        left -= 1 << j;
     }
-    for (; left > 0; --left) ESPEC_HIP_TRY(one_step<R>(p, true, true, s, launches));
     // Finish the last step: inverse x, V half phase, 1/N, last sample.
     ESPEC_HIP_TRY(one_step<R>(p, true, false, s, launches));
     // Fold the row-block partial sums of this run's samples into cbuf.
@@ -1552,8 +1550,11 @@ int spo_pass_prepare(SpoPassPlan &p, int lc)
 {
     if (lc != p.lc) drop_graph(p);
     p.lc = lc;
-    if (p.gexec) return ESPEC_OK;
-    return p.mixed ? capture_typed<float>(p) : capture_typed<double>(p);
+    for (int j = 0; j < SpoPassPlan::kGraphLevels; ++j) {
+        const int rc = p.mixed ? capture_typed<float>(p, j) : capture_typed<double>(p, j);
+        if (rc) return rc;
+    }
+    return ESPEC_OK;
 }
 
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, int *launches)

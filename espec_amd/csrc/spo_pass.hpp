@@ -50,9 +50,11 @@ struct SpoPassPlan {
     double b = 1, cst2 = 0;
     double phase_max = 0;  // max |cst2 V| over the batch (range of the pencils' sincos)
     int lc = 1;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    int graph_steps = 0;
+    // Step graphs of 2^j full steps, j = 0..kGraphLevels-1 (captured by
+    // spo_pass_prepare); a run replays them largest first.
+    static constexpr int kGraphLevels = 6;
+    hipGraphExec_t gexec[kGraphLevels] = {};
+    hipGraph_t graph[kGraphLevels] = {};
 };
 
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,

@@ -553,8 +553,13 @@ end subroutine LANCZS
 
 ! fft3d.f:1
 ! absorb.f:1-130: VABC of TPABSOR on the grid, on the device (N is not
-! read; the grid is NP); VABC is left as it was where the reference has
-! no shape (3-D, 2-D other than .SMOOTHW, unknown TPABSOR).
+! read; the grid is NP).  DIM and TPABSOR are matched on their leading
+! characters in the reference's order (absorb.f:62-156), so '.SMOOTHW_au'
+! or a TPABSOR that carries a trailing comment select the same shape.
+! Where the reference STOPs (a 1-D shape it does not know, the 2-D
+! .VOPTIC, 3-D, an unknown DIM) the drop-in stops too, as it does when
+! the device fails; a 2-D TPABSOR that is neither .SMOOTHW nor .VOPTIC
+! leaves VABC as it was, as the reference does.
 subroutine ABSORBINGBC(DIM, TPABSOR, N, NP, AL, AR, SH, XI, XF, VOI, VABC, ALPHA)
   use espec_capi
   implicit none
@@ -563,26 +568,47 @@ subroutine ABSORBINGBC(DIM, TPABSOR, N, NP, AL, AR, SH, XI, XF, VOI, VABC, ALPHA
   integer :: NP(*)
   real(8) :: AL(*), AR(*), SH(*), XI(*), XF(*), VOI(*), VABC(*), ALPHA
   integer :: rc, kind, nd
-  nd = dim_of(DIM)
   kind = 0
-  select case (trim(TPABSOR))
-  case ('.SMOOTHW')
-    kind = ESPEC_ABG_SMOOTHW
-  case ('.SMRS')
-    kind = ESPEC_ABG_SMRS
-  case ('.VOPTIC_LINEAR')
-    kind = ESPEC_ABG_VOPTIC_LINEAR
-  case ('.VOPTIC_KOSLOFF')
-    kind = ESPEC_ABG_VOPTIC_KOSLOFF
-  case ('.VOPTIC_POLIN')
-    kind = ESPEC_ABG_VOPTIC_POLIN
-  end select
-  if (nd == 0 .or. kind == 0) then
-    write(*,'(A)') '<<<>>> ABSORBINGBC: no absorber of this DIM/TPABSOR'
-    return
+  if (starts(DIM, '.1D')) then
+    nd = 1
+    if (starts(TPABSOR, '.SMOOTHW')) then
+      kind = ESPEC_ABG_SMOOTHW
+    else if (starts(TPABSOR, '.SMRS')) then
+      kind = ESPEC_ABG_SMRS
+    else if (starts(TPABSOR, '.VOPTIC_LINEAR')) then
+      kind = ESPEC_ABG_VOPTIC_LINEAR
+    else if (starts(TPABSOR, '.VOPTIC_KOSLOFF')) then
+      kind = ESPEC_ABG_VOPTIC_KOSLOFF
+    else if (starts(TPABSOR, '.VOPTIC_POLIN')) then
+      kind = ESPEC_ABG_VOPTIC_POLIN
+    else
+      write(*,*)
+      error stop 'ABSORBINGBC: no 1-D absorber of this TPABSOR'
+    end if
+  else if (starts(DIM, '.2D')) then
+    nd = 2
+    if (starts(TPABSOR, '.SMOOTHW')) then
+      kind = ESPEC_ABG_SMOOTHW
+    else if (starts(TPABSOR, '.VOPTIC')) then
+      write(*,*) '<<<>>> Absorbing bondary conditions .VOPTIC '
+      write(*,*) '          is not implemente in the code <<<>>>'
+      error stop
+    else
+      return
+    end if
+  else if (starts(DIM, '.3D')) then
+    write(*,*) '<<<>>> Absorbing bondary conditions is not '
+    write(*,*) '      implemente for the 3-dimensional code <<<>>>'
+    error stop
+  else
+    write(*,*) '<<<>>> Dimension error <<<>>>'
+    error stop
   end if
   rc = espec_absorbingbc(nd, NP, kind, AL, AR, SH, XI, XF, VOI, VABC, ALPHA)
-  if (rc /= 0) write(*,'(A,I3)') '<<<>>> ABSORBINGBC on the device failed, code ', rc
+  if (rc /= 0) then
+    write(*,'(A,I3)') '<<<>>> ABSORBINGBC on the device failed, code ', rc
+    error stop
+  end if
 end subroutine ABSORBINGBC
 
 subroutine FFT(A, B, NTOT, N, NSPAN, ISN)

@@ -87,6 +87,8 @@ class Spc:
     window: str = ".NONE"
     wp: float = 0.0
     hwhm: tuple = None  # (CHWHM, DE, E1I, ADE) in eV
+    tpabsor: str = ".NULL"  # *ABSORBING shape (ABSORB false by default, espec.f)
+    efc: str = ".NONE"  # *PULSE_AND_DIPOLE pulse shape
     other: dict = field(default_factory=dict)
 
 
@@ -179,6 +181,10 @@ def parse(text, base=None):
                 s.wp = _num(_values(nxt())[0])
             elif key == "*HWHM":
                 s.hwhm = tuple(_num(t) for t in _values(nxt())[:4])
+            elif key.startswith("*ABSORBING"):  # rdinput.f:526-548
+                s.tpabsor = _word(nxt())
+            elif key.startswith("*PULSE_AND_DIPOLE"):  # rdinput.f:549-575
+                s.efc = _word(nxt())
             else:
                 s.other[key] = _word(nxt()) if i + 1 < len(lines) else ""
         i += 1
@@ -269,6 +275,13 @@ def run(s, flags=FP64 | REF):
             raise ValueError("run() covers .SPECTRUM .TD and .ONLYSPEC on 1-D grids")
         if not s.propag.startswith(PROPAGATORS):
             raise ValueError(f"propagator {s.propag} is not supported here")
+        # An absorber or a laser changes the propagation (absorb.f, ef.f);
+        # run() propagates without either, so it refuses inputs that ask
+        # for them rather than return a different C(t).
+        if not s.tpabsor.startswith(".NULL"):
+            raise ValueError(f"*ABSORBING {s.tpabsor}: absorbers are not run by run()")
+        if not s.efc.startswith((".NONE", ".NULL")):
+            raise ValueError(f"*PULSE_AND_DIPOLE {s.efc}: laser pulses are not run by run()")
         m = s.mass[0] * AMU
         n = s.np[0]
         if s.potential.startswith(".FILE"):
@@ -306,7 +319,7 @@ def run(s, flags=FP64 | REF):
                       np.zeros(n), (PULSE_NONE, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0), flags)
         mc = 1 << (s.mf - 1)
         u, v = assemble_corr(s.mf, r["cr"][:mc], r["ci"][:mc])
-        out = dict(lc=lc, tf=tf, cr=r["cr"], ci=r["ci"])
+        out = dict(lc=lc, tf=tf, cr=r["cr"], ci=r["ci"], psi0=(u0, v00))
     mc = 1 << (s.mf - 1)
     dtf = tf / mc  # espec.f:1600-1601
     out["t"] = -tf + np.arange(2 * mc) * dtf

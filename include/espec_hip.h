@@ -106,7 +106,9 @@ enum {
  *   kind  ESPEC_ABG_SMOOTHW (.SMOOTHW mask, 1-D and 2-D), ESPEC_ABG_SMRS
  *         (.SMRS mask), ESPEC_ABG_VOPTIC_LINEAR, _KOSLOFF, _POLIN
  *         (imaginary potentials), the last four 1-D only
- *   al, ar, sh, xi, xf, voi [ndim]  the reference's arguments
+ *   al, ar, sh, xi, xf [ndim]  the reference's arguments
+ *   voi   [ndim], except [2] for the 1-D .SMOOTHW, _KOSLOFF and _POLIN,
+ *         which read VOI(2) as an exponent or width
  *   alpha out (may be NULL): 1 - 0.1 (XF - AR) for the 1-D .SMOOTHW
  * ESPEC_ERR_UNSUPPORTED where the reference stops (3-D, 2-D non-SMOOTHW). */
 enum {
@@ -261,6 +263,19 @@ int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort,
 
 /* Replaces FCORREL(N,U1,U2) (fcorrel.f:1): sum u1*u2 in FP64 on device. */
 int espec_fcorrel(int n, const double *u1, const double *u2, double *out);
+/* No counterpart in the reference: espec_fcorrel on device buffers that
+ * allocates nothing, for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * u1, u2 [n] and out [1] are device pointers; the call is asynchronous on
+ * `stream` (a hipStream_t, NULL for the default stream).  Its block
+ * partials and ticket live in a static device array of the library (about
+ * 70 KiB per device, held until the process ends), so calls that overlap
+ * on one device race; calls in one stream are ordered.  n above
+ * ESPEC_NOALLOC_MAX_N returns ESPEC_ERR_ARG.  The sum is bit-identical to
+ * espec_fcorrel's (same kernel, same block count). */
+#define ESPEC_NOALLOC_MAX_N (1 << 30)
+int espec_fcorrel_noalloc(int n, const double *u1, const double *u2, double *out,
+                          void *stream);
 /* Replaces ECNORM(N,US) (ecnorm.f:1): sqrt(sum u^2). */
 int espec_ecnorm(int n, const double *u, double *out);
 /* Replaces EFFT4(NDIM,N,NP,B,U,V,WU,WV,TC,VPOT) (eigenergfft.f:1): <psi|V|psi>

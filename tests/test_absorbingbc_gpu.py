@@ -73,3 +73,35 @@ def test_absorbingbc_fortran_dropin(gpu, orc):
     o, _ = orc.absorbingbc([n], E.ABG_VOPTIC_LINEAR, [1.0], [xf - 1.0], [sh], [xi], [xf],
                            [0.4, 1.0])
     assert close(va, o) and np.any(o != 0)
+
+
+@pytest.mark.parametrize("tp,kind", [(b".SMOOTHW_au     ", E.ABG_SMOOTHW),
+                                     (b".SMRS_au", E.ABG_SMRS),
+                                     (b".VOPTIC_POLIN ! comment", E.ABG_VOPTIC_POLIN)])
+def test_absorbingbc_fortran_dropin_prefix(gpu, orc, tp, kind):
+    """TPABSOR is matched on its leading characters as absorb.f:63-131 does:
+    espec.f:652's '.SMOOTHW_au' and '.SMRS_au', and a CHARACTER*16 value
+    read with '(A)' that still holds a trailing comment, select the shape
+    of their prefix."""
+    import ctypes as C
+    from espec_amd import build as B
+    E._load()
+    lib = C.CDLL(B.FLIB)
+    n = 257
+    sh, xi = 0.02, 0.0
+    xf = xi + (n - 1) * sh
+    d = lambda *v: np.array(v, dtype=np.float64)
+    va, alpha = np.full(n, -7.0), C.c_double(0.0)
+    dim = b".1D"
+    print(f"case: ABSORBINGBC drop-in TPABSOR={tp!r}", flush=True)
+    f = lib.absorbingbc_
+    f.restype = None
+    f(C.c_char_p(dim), C.c_char_p(tp), C.byref(C.c_int(n)),
+      np.array([n, 1, 1], dtype=np.int32).ctypes.data_as(C.c_void_p),
+      *[a.ctypes.data_as(C.c_void_p) for a in (d(1.0), d(xf - 1.0), d(sh), d(xi), d(xf),
+                                              d(0.4, 3.0), va)],
+      C.byref(alpha), C.c_size_t(len(dim)), C.c_size_t(len(tp)))
+    o, al = orc.absorbingbc([n], kind, [1.0], [xf - 1.0], [sh], [xi], [xf], [0.4, 3.0])
+    assert close(va, o)
+    if kind == E.ABG_SMOOTHW:
+        assert alpha.value == al

@@ -71,3 +71,27 @@ def test_file_sections_test4():
     psi, pot = I._sections(os.path.join(INPUTS, s.pot_file))[:2]
     c = S.test_case(4)
     assert np.array_equal(psi[:, 1], c["u"]) and np.array_equal(pot[:, 1], c["vpot"])
+
+
+@pytest.mark.parametrize("block,attr,word", [
+    ("*ABSORBING\n.SMOOTHW\n0.5\n1.0 2.0\n", "tpabsor", ".SMOOTHW"),
+    ("*ABSORBING\n.VOPTIC_LINEAR\n0.5\n1.0 2.0\n", "tpabsor", ".VOPTIC_LINEAR"),
+    ("*PULSE_AND_DIPOLE\n.GAUS\n1e12 10 3.0 20 0\n.NULL\n", "efc", ".GAUS")])
+def test_absorber_and_pulse_are_refused_by_run(block, attr, word):
+    """*ABSORBING (rdinput.f:526) other than .NULL and *PULSE_AND_DIPOLE
+    (rdinput.f:549) other than .NONE/.NULL change the propagation; run()
+    refuses them instead of propagating without them."""
+    text = open(os.path.join(INPUTS, "input.test12")).read()
+    k = text.index("*PROPAG")
+    s = I.parse(text[:k] + block + text[k:], INPUTS)
+    assert getattr(s, attr) == word
+    with pytest.raises(ValueError, match=attr == "tpabsor" and "ABSORBING" or "PULSE"):
+        I.run(s)
+
+
+def test_null_absorber_and_no_pulse_parse():
+    text = open(os.path.join(INPUTS, "input.test12")).read()
+    k = text.index("*PROPAG")
+    s = I.parse(text[:k] + "*ABSORBING\n.NULL\n0\n0 0\n*PULSE_AND_DIPOLE\n.NONE\n0\n.NULL\n"
+                + text[k:], INPUTS)
+    assert (s.tpabsor, s.efc) == (".NULL", ".NONE")

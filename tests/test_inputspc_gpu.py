@@ -67,17 +67,30 @@ def test_input_test4_psil_file_hwhm(gpu):
 
 
 def test_input_test14_ppsil_lanczs(gpu):
-    """Initial state from LANCZS on the device, PPLNZ without a laser.  The
-    table prints C(t) with 5 decimals; both the initial LANCZS state and
-    the 124928 propagation steps use ABSTOL = 1e-6 bisection, whose
-    interval drift reaches about 1e-5 here, so the bar is two units of
-    the last printed digit."""
+    """Initial state from LANCZS on the device, PPLNZ without a laser.
+
+    The table prints C(t) with 5 decimals (F8.5), so the unrounded C(t)
+    here differs from a printed value by up to half a unit, 5e-6, on top
+    of any real difference.  The real difference is the drift of the
+    ABSTOL = 1e-6 bisection eigenvalues (dstebz.f): each step's Krylov
+    eigenvalues are interval midpoints, which rounding-level differences
+    between builds move by up to ABSTOL.  Against the 7-decimal tables of
+    the same kind of run (test4 at order 10, test13 at order 15, 124928
+    steps) that drift measures 3.9e-6 and 5.4e-6; the allowance here is
+    about three times the larger, 1.5e-5.  Bar: 5e-6 + 1.5e-5 = 2.0e-5.
+    The LANCZS initial state itself is checked against the dense-eigh
+    ground state, so it contributes nothing: for unit states a, b and a
+    unitary step U, |<a|Ua> - <b|Ub>| <= 2 |a - b|."""
     s, r = run(14)
     tab = S.load("test14_table")
     k = len(tab)
     d = max(np.abs(r["cr"][:k] - tab[:, 3]).max(), np.abs(r["ci"][:k] - tab[:, 4]).max())
-    print(f"|dC| {d:.1e}", flush=True)
-    assert d <= 2.01e-5
+    exact = S.test_case(14)["u"]
+    u0 = r["psi0"][0]
+    dist = min(np.linalg.norm(u0 - exact), np.linalg.norm(u0 + exact))
+    print(f"|dC| {d:.2e}  |psi0 - psi_exact| {dist:.2e}", flush=True)
+    assert dist <= 1e-10
+    assert d <= 5e-6 + 1.5e-5
 
 
 @pytest.mark.parametrize("t,method,bar", [(5, ".PFSOD", 2.01e-7), (6, ".PFSPO", 2.01e-7)])
