@@ -87,6 +87,12 @@ def _load():
     L.espec_mlp_train.argtypes = [f, f, f, ip, ip, C.c_float, C.c_uint, f]
     L.espec_bw_probe.argtypes = [C.c_size_t, ip, P(dp), P(dp)]
     L.espec_plnz.argtypes = [ip, i, ip, dp, dp, dp, dp, ip, d, d, d, d, d, d, vp, ip, P(ip), ip]
+    L.espec_plnz_absorb.argtypes = L.espec_plnz.argtypes + [ip, vp]
+    L.espec_pspofftaux.argtypes = [ip, i, dp, dp, dp, d, d, d, d, ip, vp, ip]
+    L.espec_psodfftaux.argtypes = [ip, i, dp, dp, d, d, d, d, d, d, ip, vp, ip]
+    L.espec_reort.argtypes = [ip, ip, ip, d, d]
+    L.espec_fftnt.argtypes = [d, d, ip, ip, ip, ip]
+    L.espec_sil_plan_set_absorb.argtypes = [vp, ip, vp]
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
     L.espec_sil_plan_set_pulse.argtypes = [vp, d, d, ip]
     L.espec_ef.argtypes = [ip, dp, dp, dp, dp, dp, dp, ip, dp]
@@ -187,18 +193,23 @@ def pspofft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT, absor
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
 
 
-def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT):
+def plnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vpot, flags=FP64 | EXACT,
+         absorb=ABSORB_NONE, vabc=None):
     """PLNZ (plnz.f:1-140, step sil.f:70-181): returns dict(cr, ci, norm, u, v).
 
     shm is the FD kinetic prefactor per axis (espec.f:630); mp the Krylov
-    order; abstol the DSTEVX tolerance (<= 0: QL, > 0: bisection)."""
+    order; abstol the DSTEVX tolerance (<= 0: QL, > 0: bisection).
+    absorb=ABSORB_SMOOTHW with the mask vabc: PLNZAUX's .SMOOTHW absorber
+    (sil.f:171-180)."""
     L = _load()
     maxs = nsteps(ti, tf, dt) // lc + 1
     cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
     u, v = _f64(u1).copy(), _f64(v1).copy()
     ns = C.c_int(0)
-    rc = L.espec_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0), u, v,
-                      _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags)
+    va = None if vabc is None else _f64(vabc)
+    rc = L.espec_plnz_absorb(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0), u,
+                             v, _f64(vpot), cr, ci, nr.ctypes.data, maxs, C.byref(ns), flags,
+                             absorb, None if va is None else va.ctypes.data)
     _check(rc, "PLNZ")
     k = ns.value
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v)
@@ -279,6 +290,45 @@ def fft(a, b, ntot, n, nspan, isn, flags=REF):
     a, b = _f64(a).copy(), _f64(b).copy()
     _check(_load().espec_fft(a, b, ntot, n, nspan, isn, flags), "FFT")
     return a, b
+
+
+def pspofftaux(np_, b, cst1, cst2, tc, u, v, vpot, flags=REF, absorb=ABSORB_NONE, vabc=None):
+    """PSPOFFTAUX (spofft.f:3-203): one split-operator step on the device
+    with the caller's kinetic grid tc; returns (u, v)."""
+    u, v = _f64(u).copy(), _f64(v).copy()
+    va = None if vabc is None else _f64(vabc)
+    _check(_load().espec_pspofftaux(len(np_), _np3(np_), b, cst1, cst2, _f64(tc), u, v,
+                                    _f64(vpot), absorb, None if va is None else va.ctypes.data,
+                                    flags), "PSPOFFTAUX")
+    return u, v
+
+
+def psodfftaux(np_, b, cst, tc, u1, u2, v1, v2, vpot, flags=REF, absorb=ABSORB_NONE,
+               vabc=None):
+    """PSODFFTAUX (sodfft.f:3-129): one leapfrog step; returns (u1, u2, v1, v2)."""
+    u1, u2, v1, v2 = (_f64(a).copy() for a in (u1, u2, v1, v2))
+    va = None if vabc is None else _f64(vabc)
+    _check(_load().espec_psodfftaux(len(np_), _np3(np_), b, cst, _f64(tc), u1, u2, v1, v2,
+                                    _f64(vpot), absorb, None if va is None else va.ctypes.data,
+                                    flags), "PSODFFTAUX")
+    return u1, u2, v1, v2
+
+
+def reort(basis, w):
+    """REORT (reort.f:1-83): w - V (V^T w) for the columns basis[j] (rows
+    of the [m][n] array are the vectors); returns the new w."""
+    basis = np.ascontiguousarray(basis, dtype=np.float64)
+    w = _f64(w).copy()
+    m, n = basis.shape
+    _check(_load().espec_reort(n, m, n, w, basis), "REORT")
+    return w
+
+
+def fftnt(x, y, m, itype, flags=REF):
+    """FFTNT(X,Y,N,M,ITYPE) (cfft.f:6-106) on the device: returns (x, y)."""
+    x, y = _f64(x).copy(), _f64(y).copy()
+    _check(_load().espec_fftnt(x, y, len(x), m, itype, flags), "FFTNT")
+    return x, y
 
 
 def au(np_, shm, vpot, u):
@@ -465,6 +515,14 @@ class SilPlan(_Plan):
         self._ef = _f64(efield).ravel()
         _check(self._L.espec_sil_plan_set_pulse(self._h, self._dm, self._ef, self._ef.size),
                "sil_plan_set_pulse")
+
+    def set_absorb(self, absorb, vabc=None):
+        """PLNZAUX's .SMOOTHW mask (sil.f:171-180) for every packet, vabc
+        [npk][n]; ABSORB_NONE removes it."""
+        self._va = None if vabc is None else _f64(vabc)
+        _check(self._L.espec_sil_plan_set_absorb(
+            self._h, absorb, None if self._va is None else self._va.ctypes.data),
+            "sil_plan_set_absorb")
 
     def adaptive(self, ndt, params=None, tol=1e-10, m_fix=10):
         """Advance ndt base steps with the learned controller choosing the

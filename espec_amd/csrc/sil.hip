@@ -558,6 +558,7 @@ struct SilArgs {
     int ref;              // reproduce sil.f:98-99
     double *coef;         // [nvec][2][m]: cos part, sin part in the Krylov basis
     int *info;            // [1] eigensolve failure flag
+    const double *mask;   // [npk][n] .SMOOTHW mask applied to the new state, or null
 };
 
 // Projection c_j = <v_j, start> (DGEMV 'T', sil.f:86-87), then in the last
@@ -680,7 +681,12 @@ __global__ __launch_bounds__(NT) void sil_combine(SilArgs s, V2<S> *x, const V2<
             v += cf[2 * m + j] * bi;
             wu += cf[3 * m + j] * bi;
         }
-        const double nu = u + wu, nv = wv + v;
+        double nu = u + wu, nv = wv + v;
+        if (s.mask) {  // sil.f:171-175: U = VABC (U + WU), V = VABC (WV + V)
+            const double mk = s.mask[(long long)pk * a.n + p];
+            nu = mk * nu;
+            nv = mk * nv;
+        }
         V2<S> o;
         o.x = (S)nu;
         o.y = (S)nv;
@@ -1256,6 +1262,7 @@ struct espec_sil_plan {
     DevBuf x, psi0, vpot, coef, info, cbuf, dstep;
     // pulse-driven steps (PPLNZ): dipole [npk][n], field per step, V - DM E
     DevBuf dm, efield, veff;
+    DevBuf mask;  // .SMOOTHW VABC [npk][n] (PLNZAUX with ABSORB), when set
     bool pulse = false;
     long long nfield = 0;
     LnzBufs lb;
@@ -1814,7 +1821,7 @@ int espec_sil_plan_adaptive(espec_sil_plan *P, const float *params, int ndt, dou
 {
     if (!P || ndt < 0 || !(tol > 0) || m_fix < kLabM0 || m_fix > P->m || P->lc != 1 || P->ref)
         return ESPEC_ERR_ARG;
-    if (P->pulse) return ESPEC_ERR_UNSUPPORTED;
+    if (P->pulse || P->sa.mask) return ESPEC_ERR_UNSUPPORTED;
     if (ndt == 0) return ESPEC_OK;
     int rc = ctl_setup(*P);
     if (rc) return rc;
@@ -1856,7 +1863,7 @@ int espec_sil_plan_trace(espec_sil_plan *P, int nsteps, int mprop, float *feat, 
     if (!P || nsteps < 1 || !feat || !lab || P->m < kLabM0 + kLabM || P->lc != 1 || P->ref ||
         mprop < kLabM0 + 1 || mprop > P->m)
         return ESPEC_ERR_ARG;
-    if (P->pulse) return ESPEC_ERR_UNSUPPORTED;
+    if (P->pulse || P->sa.mask) return ESPEC_ERR_UNSUPPORTED;
     int rc = ctl_setup(*P);
     if (rc) return rc;
     P->drop_graph();
@@ -1904,6 +1911,41 @@ int espec_sil_plan_set_pulse(espec_sil_plan *P, const double *dm, const double *
     return ESPEC_OK;
 }
 
+int espec_sil_plan_set_absorb(espec_sil_plan *P, int absorb, const double *vabc)
+{
+    if (!P || absorb < ESPEC_ABSORB_NONE || absorb > ESPEC_ABSORB_SMOOTHW ||
+        (absorb == ESPEC_ABSORB_SMOOTHW && !vabc))
+        return ESPEC_ERR_ARG;
+    // PLNZAUX tests TPABSOR(1:8) = '.SMOOTHW' only (sil.f:171): .VOPTIC has
+    // no effect on the Lanczos step.
+    if (absorb == ESPEC_ABSORB_SMOOTHW) {
+        ESPEC_HIP_TRY(P->mask.upload(vabc, sizeof(double) * (size_t)P->n * P->npk));
+        P->sa.mask = P->mask.as<double>();
+    } else {
+        P->sa.mask = nullptr;
+    }
+    P->drop_graph();  // the captured combine carries the mask pointer
+    return ESPEC_OK;
+}
+
+int espec_plnz_absorb(int ndim, const int *np, int mp, double abstol, double dt, double ti,
+                      double tf, int lc, const double *shm, double *u, double *v,
+                      const double *vpot, double *cr, double *ci, double *nrm, int maxsamp,
+                      int *nsamp, int flags, int absorb, const double *vabc)
+{
+    if (!u || !v || !cr || !ci || maxsamp < 1) return ESPEC_ERR_ARG;
+    const int nsteps = fortran_trip(ti + dt, tf + dt, dt);  // plnz.f:109
+    espec_sil_plan *P = nullptr;
+    int rc = espec_sil_plan_create(&P, ndim, np, 1, mp, abstol, dt, lc, shm, vpot, u, v, maxsamp,
+                                   flags);
+    if (rc) return rc;
+    rc = espec_sil_plan_set_absorb(P, absorb, vabc);
+    if (!rc) rc = espec_sil_plan_run(P, nsteps);
+    if (!rc) rc = espec_sil_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u, v);
+    espec_sil_plan_destroy(P);
+    return rc;
+}
+
 double espec_ef(int efc, double e0, double t0, double td, double tp, double omg, double sni,
                 int kl, double t)
 {
@@ -1931,11 +1973,12 @@ double espec_ef(int efc, double e0, double t0, double td, double tp, double omg,
     }
 }
 
-int espec_pplnz(int ndim, const int *np, int mp, double abstol, double dt, double ti, double tf,
-                int lc, const double *shm, double *u, double *v, const double *vp,
-                const double *dm, int efc, double e0, double t0, double td, double tp,
-                double omg, double sni, int kl, double *veff, double *cr, double *ci,
-                double *nrm, int maxsamp, int *nsamp, int flags)
+int espec_pplnz_absorb(int ndim, const int *np, int mp, double abstol, double dt, double ti,
+                       double tf, int lc, const double *shm, double *u, double *v,
+                       const double *vp, const double *dm, int efc, double e0, double t0,
+                       double td, double tp, double omg, double sni, int kl, double *veff,
+                       double *cr, double *ci, double *nrm, int maxsamp, int *nsamp, int flags,
+                       int absorb, const double *vabc)
 {
     if (!u || !v || !vp || !dm || !cr || !ci || maxsamp < 1 || efc < ESPEC_PULSE_NONE ||
         efc > ESPEC_PULSE_SIN2)
@@ -1950,15 +1993,33 @@ int espec_pplnz(int ndim, const int *np, int mp, double abstol, double dt, doubl
                                    flags);
     if (rc) return rc;
     rc = espec_sil_plan_set_pulse(P, dm, ef.data(), nsteps);
+    if (!rc) rc = espec_sil_plan_set_absorb(P, absorb, vabc);
     if (!rc) rc = espec_sil_plan_run(P, nsteps);
     if (!rc) rc = espec_sil_plan_get(P, 0, cr, ci, nrm, maxsamp, nsamp, u, v);
-    espec_sil_plan_destroy(P);
-    if (!rc && veff) {  // VPOT as the reference leaves it: the last VEFF
-        const double e = nsteps > 0 ? ef[nsteps - 1] : 0.0;
-        const int n = ntot(ndim, np);
-        for (int i = 0; i < n; ++i) veff[i] = nsteps > 0 ? vp[i] - dm[i] * e : vp[i];
+    // VPOT as the reference leaves it: the VEFF of the last step, which the
+    // device formed (sil_field), or VP when no step ran.
+    if (!rc && veff) {
+        const size_t by = sizeof(double) * (size_t)P->n;
+        if (nsteps > 0) {
+            hipError_t e = hipMemcpy(veff, P->veff.p, by, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = ESPEC_ERR_HIP;
+        } else {
+            std::memcpy(veff, vp, by);
+        }
     }
+    espec_sil_plan_destroy(P);
     return rc;
+}
+
+int espec_pplnz(int ndim, const int *np, int mp, double abstol, double dt, double ti, double tf,
+                int lc, const double *shm, double *u, double *v, const double *vp,
+                const double *dm, int efc, double e0, double t0, double td, double tp,
+                double omg, double sni, int kl, double *veff, double *cr, double *ci,
+                double *nrm, int maxsamp, int *nsamp, int flags)
+{
+    return espec_pplnz_absorb(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vp, dm, efc, e0,
+                              t0, td, tp, omg, sni, kl, veff, cr, ci, nrm, maxsamp, nsamp, flags,
+                              ESPEC_ABSORB_NONE, nullptr);
 }
 
 int espec_au(int ndim, const int *np, const double *shm, const double *vpot, const double *u,

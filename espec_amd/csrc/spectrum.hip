@@ -67,6 +67,15 @@ __global__ __launch_bounds__(256) void fftnt_stage_lds(double2 *x, int n, int k0
     for (int i = threadIdx.x; i < nb; i += 256) g[i] = s[i];
 }
 
+// FFTNT's input conjugation for ITYPE = -1 (cfft.f:40-44), into the
+// interleaved work array.
+__global__ __launch_bounds__(256) void fftnt_prep(int n, int itype, const double *x,
+                                                  const double *y, double2 *z)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        z[i] = {x[i], itype == -1 ? -y[i] : y[i]};
+}
+
 __device__ __forceinline__ int bitrev(int i, int m) { return (int)(__brev((unsigned)i) >> (32 - m)); }
 
 // ------------------------------------------------- spectrum pipeline
@@ -358,6 +367,34 @@ int espec_spectrumtd(int iwin, int mf, double dt, double tf, double wp, double d
             if (wi) wi[k] = v[i];
             en = en + enc;
         }
+    return ESPEC_OK;
+}
+
+int espec_fftnt(double *x, double *y, int n, int m, int itype, int flags)
+{
+    if (!x || !y || m < 0 || m > 28 || n != (1 << m)) return ESPEC_ERR_ARG;
+    if (n == 1) return ESPEC_OK;  // cfft.f:38
+    const bool ref = (flags & ESPEC_REF) != 0;
+    SpecArgs a{};
+    a.n = n;
+    a.m = m;
+    a.itype = itype;
+    std::vector<double> tw = fftnt_twiddles(n, ref);
+    DevBuf du, dv, dx, dt_;
+    ESPEC_HIP_TRY(du.upload(x, sizeof(double) * n));
+    ESPEC_HIP_TRY(dv.upload(y, sizeof(double) * n));
+    ESPEC_HIP_TRY(dx.alloc(sizeof(double2) * n));
+    ESPEC_HIP_TRY(dt_.upload(tw.data(), sizeof(double) * tw.size()));
+    const int gb = std::min(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(fftnt_prep, dim3(gb), dim3(256), 0, nullptr, n, itype, du.as<double>(),
+                       dv.as<double>(), dx.as<double2>());
+    ESPEC_HIP_TRY(hipGetLastError());
+    ESPEC_HIP_TRY(run_fftnt(dx.as<double2>(), n, m, dt_.as<double2>(), nullptr));
+    hipLaunchKernelGGL(fftnt_finish, dim3(gb), dim3(256), 0, nullptr, a, dx.as<double2>(),
+                       du.as<double>(), dv.as<double>());
+    ESPEC_HIP_TRY(hipGetLastError());
+    ESPEC_HIP_TRY(hipMemcpy(x, du.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    ESPEC_HIP_TRY(hipMemcpy(y, dv.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return ESPEC_OK;
 }
 

@@ -8,6 +8,7 @@ module espec_capi
 
   integer, parameter :: ESPEC_FP64 = 0, ESPEC_MIXED = 1, ESPEC_EXACT = 0, ESPEC_REF = 2
   integer, parameter :: ESPEC_OK = 0, ESPEC_ERR_ARG = 2, ESPEC_ERR_UNSUPPORTED = 6
+  integer, parameter :: ESPEC_ABSORB_NONE = 0, ESPEC_ABSORB_VOPTIC = 1, ESPEC_ABSORB_SMOOTHW = 2
   integer, parameter :: ESPEC_ABG_SMOOTHW = 1, ESPEC_ABG_SMRS = 2, ESPEC_ABG_VOPTIC_LINEAR = 3, &
                         ESPEC_ABG_VOPTIC_KOSLOFF = 4, ESPEC_ABG_VOPTIC_POLIN = 5
 
@@ -174,6 +175,69 @@ module espec_capi
     function espec_sil_plan_destroy(plan) bind(C, name='espec_sil_plan_destroy') result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: plan
+      integer(c_int) :: rc
+    end function
+
+    function espec_sil_plan_set_absorb(plan, absorb, vabc) &
+        bind(C, name='espec_sil_plan_set_absorb') result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: plan
+      integer(c_int), value :: absorb
+      real(c_double) :: vabc(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_pspofftaux(ndim, np, b, cst1, cst2, tc, u, v, vpot, absorb, vabc, flags) &
+        bind(C, name='espec_pspofftaux') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, absorb, flags
+      integer(c_int) :: np(*)
+      real(c_double), value :: b, cst1, cst2
+      real(c_double) :: tc(*), u(*), v(*), vpot(*), vabc(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_psodfftaux(ndim, np, b, cst, tc, u1, u2, v1, v2, vpot, absorb, vabc, flags) &
+        bind(C, name='espec_psodfftaux') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, absorb, flags
+      integer(c_int) :: np(*)
+      real(c_double), value :: b, cst
+      real(c_double) :: tc(*), u1(*), u2(*), v1(*), v2(*), vpot(*), vabc(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_reort(n, m, ld, w, basis) bind(C, name='espec_reort') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: n, m, ld
+      real(c_double) :: w(*), basis(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_fftnt(x, y, n, m, itype, flags) bind(C, name='espec_fftnt') result(rc)
+      import :: c_int, c_double
+      real(c_double) :: x(*), y(*)
+      integer(c_int), value :: n, m, itype, flags
+      integer(c_int) :: rc
+    end function
+
+    function espec_plnz_absorb(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, cr, ci, &
+                               nrm, maxsamp, nsamp, flags, absorb, vabc) &
+        bind(C, name='espec_plnz_absorb') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, mp, lc, maxsamp, flags, absorb
+      integer(c_int) :: np(*), nsamp
+      real(c_double), value :: abstol, dt, ti, tf
+      real(c_double) :: shm(*), u(*), v(*), vpot(*), cr(*), ci(*), nrm(*), vabc(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_lnz(ndim, np, m, nreort, shm, vpot, start, al, bt, basis) &
+        bind(C, name='espec_lnz') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, m, nreort
+      integer(c_int) :: np(*)
+      real(c_double) :: shm(*), vpot(*), start(*), al(*), bt(*), basis(*)
       integer(c_int) :: rc
     end function
 

@@ -10,7 +10,7 @@
 !   - the coupled-surface DIM variants return INFO = 6 (not on the
 !     device); absorbing boundaries (ABSORB with TPABSOR .VOPTIC or
 !     .SMOOTHW and VABC) run on the device in PSPOFFT and PSODFFT, and
-!     PLNZ ignores ABSORB as the reference does (sil.f never reads it);
+!     PLNZ and PPLNZ apply the .SMOOTHW mask of PLNZAUX (sil.f:171-180);
 !   - ESPEC_MODE in the environment selects the arithmetic (REF: the
 !     reference's own rounding, the default; EXACT; MIXED).
 
@@ -251,7 +251,7 @@ subroutine PLNZ(ABSORB, CHANGE, DIM, PRTCRL, TPABSOR, INFO, LC, LMTREORT, MP, MP
 
   INFO = 0
   ndim = dim_of(DIM)
-  if (ndim == 0) then  ! ABSORB is not used by PLNZ/PLNZAUX (sil.f:3-185)
+  if (ndim == 0) then
     INFO = ESPEC_ERR_UNSUPPORTED
     return
   end if
@@ -265,6 +265,15 @@ subroutine PLNZ(ABSORB, CHANGE, DIM, PRTCRL, TPABSOR, INFO, LC, LMTREORT, MP, MP
   if (rc /= 0) then
     INFO = rc
     return
+  end if
+  ! PLNZAUX masks the state for ABSORB with .SMOOTHW (sil.f:171-180)
+  if (absorb_of(ABSORB, TPABSOR) == ESPEC_ABSORB_SMOOTHW) then
+    rc = espec_sil_plan_set_absorb(plan, ESPEC_ABSORB_SMOOTHW, VABC)
+    if (rc /= 0) then
+      INFO = rc
+      rc = espec_sil_plan_destroy(plan)
+      return
+    end if
   end if
   every = starts(PRTCRL, '.YES')
   prt = every .or. starts(PRTCRL, '.PARTIAL')
@@ -380,6 +389,9 @@ subroutine PPLNZ(ABSORB, CHANGE, PRTVEFF, PRTEIGVC2, PRTPULS, DIM, EFC, PRTCRL, 
   end do
   rc = espec_sil_plan_create(plan, ndim, NP, 1, MP, ABSTOL, DT, LC, SHM, vp, U, V, maxs, flags)
   if (rc == 0) rc = espec_sil_plan_set_pulse(plan, DM, efield, nst)
+  ! PPLNZ hands ABSORB to PLNZAUX (pplnz.f:208), which masks for .SMOOTHW
+  if (rc == 0 .and. absorb_of(ABSORB, TPABSOR) == ESPEC_ABSORB_SMOOTHW) &
+    rc = espec_sil_plan_set_absorb(plan, ESPEC_ABSORB_SMOOTHW, VABC)
   if (rc /= 0) then
     INFO = rc
     return
@@ -773,3 +785,149 @@ subroutine SPECTRUMTD(TPWIND, M, DT, TF, WP, DE, U, V)
 1011 format(5X,'*E (eV)*',11X,'*W*',11X,'*FFTr*',10X,'*FFTi*')
 1021 format(1X,4(F12.7,4X))
 end subroutine SPECTRUMTD
+
+! sil.f:3-185: one short-iterative Lanczos step of U + iV over CST (a.u.),
+! on the device (espec_plnz_absorb with TI = TF = 0, DT = CST SHFS, one
+! step), with PLNZAUX's .SMOOTHW mask for ABSORB.  A failed eigensolve
+! returns INFO = 5 where the reference stops ("Diagonalization error").
+! LNZVC, IWORK and the work arrays are the library's business: the
+! Krylov basis is not left in LNZVC.
+subroutine PLNZAUX(ABSORB, CHANGE, DIM, TPABSOR, INFO, LMTREORT, MP, MXDCT, N, ABSTOL, CST, &
+                   NP, IWORK, SHM, VPOT, VABC, U, V, WU, WV, WP, WORK, VAR, LNZVC)
+  use espec_capi
+  implicit none
+  logical :: ABSORB, CHANGE
+  character(len=*) :: DIM, TPABSOR
+  integer :: INFO, LMTREORT, MP, MXDCT, N
+  real(8) :: ABSTOL, CST
+  integer :: NP(*), IWORK(*)
+  real(8) :: SHM(*), VPOT(*), VABC(*), U(*), V(*), WU(*), WV(*), WP(*), WORK(*), VAR(*)
+  real(8) :: LNZVC(MXDCT, *)
+  real(8), parameter :: SHFS = 2.4188843243d-2
+  real(8) :: cr(2), ci(2), nrm(2)
+  integer :: ndim, ns, rc
+  INFO = 0
+  ndim = dim_of(DIM)
+  if (ndim == 0) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  rc = espec_plnz_absorb(ndim, NP, MP, ABSTOL, CST * SHFS, 0.0d0, 0.0d0, 1, SHM, U, V, VPOT, &
+                         cr, ci, nrm, 2, ns, mode_flags(), absorb_of(ABSORB, TPABSOR), VABC)
+  if (rc /= 0) then
+    if (rc == 5) write(*,'(A)') '<<<>>> Diagonalization error <<<>>>'
+    INFO = rc
+  end if
+end subroutine PLNZAUX
+
+! lnz.f:4-129: M Lanczos steps from VI with one classical Gram-Schmidt pass
+! against the first NREORT vectors, on the device (espec_lnz).  AL, BT and
+! LNZVC(:,1..NREORT) as the reference leaves them (columns past M zero);
+! with M <= NREORT, V0 and VI hold the last two Lanczos vectors as in the
+! reference.  WORK is scratch.
+subroutine LNZ(CHANGE, DIM, N, M, NREORT, LMTREORT, MXDCT, NP, SHM, VPOT, V0, VI, WORK, AL, BT, &
+               VAR, LNZVC)
+  use espec_capi
+  implicit none
+  logical :: CHANGE
+  character(len=*) :: DIM
+  integer :: N, M, NREORT, LMTREORT, MXDCT
+  integer :: NP(*)
+  real(8) :: SHM(*), VPOT(*), V0(*), VI(*), WORK(*), AL(*), BT(*), VAR(*), LNZVC(MXDCT, *)
+  real(8), allocatable :: basis(:)
+  integer :: ndim, rc, j, nre
+  if (NREORT > LMTREORT) then  ! lnz.f:64-73
+    write(*,*) 'Error NREORT > LMTREORT'
+    write(*,*) 'NREORT=', NREORT, '  LMTREORT=', LMTREORT
+    if (CHANGE) then
+      NREORT = LMTREORT
+      write(*,*) 'Modifing NREORT to correct value!'
+      write(*,*) 'NREORT=', NREORT, '  LMTREORT=', LMTREORT
+    end if
+  end if
+  ndim = dim_of(DIM)
+  if (ndim == 0 .or. M < 1 .or. NREORT < 1) then
+    write(*,'(A)') '<<<>>> LNZ: not on the device for this DIM/M/NREORT'
+    return
+  end if
+  nre = NREORT
+  allocate(basis(N * nre))
+  rc = espec_lnz(ndim, NP, M, nre, SHM, VPOT, VI, AL, BT, basis)
+  if (rc /= 0) then
+    write(*,'(A,I3)') '<<<>>> LNZ on the device failed, code ', rc
+    deallocate(basis)
+    return
+  end if
+  do j = 1, nre
+    LNZVC(1:N, j) = basis((j - 1) * N + 1:j * N)
+  end do
+  if (M <= nre) then
+    VI(1:N) = basis((M - 1) * N + 1:M * N)
+    if (M > 1) then
+      V0(1:N) = basis((M - 2) * N + 1:(M - 1) * N)
+    else
+      V0(1:N) = 0.0d0
+    end if
+  end if
+  deallocate(basis)
+end subroutine LNZ
+
+! reort.f:1-83: WORK <- WORK - V (V^T WORK) over the first M columns of
+! EIGVC (leading dimension MXDCT), on the device (espec_reort).
+subroutine REORT(N, M, LMTREORT, MXDCT, WORK, EIGVC)
+  use espec_capi
+  implicit none
+  integer :: N, M, LMTREORT, MXDCT
+  real(8) :: WORK(*), EIGVC(MXDCT, *)
+  integer :: rc
+  rc = espec_reort(N, M, MXDCT, WORK, EIGVC)
+  if (rc /= 0) write(*,'(A,I3)') '<<<>>> REORT on the device failed, code ', rc
+end subroutine REORT
+
+! spofft.f:3-203: one split-operator step with the caller's TC (from DIGT),
+! on the device (espec_pspofftaux).
+subroutine PSPOFFTAUX(ABSORB, TPABSOR, NDIM, N, B, B1, B2, CST1, CST2, NP, U, V, TC, VPOT, VABC)
+  use espec_capi
+  implicit none
+  logical :: ABSORB
+  character(len=*) :: TPABSOR
+  integer :: NDIM, N
+  real(8) :: B, B1, B2, CST1, CST2
+  integer :: NP(*)
+  real(8) :: U(*), V(*), TC(*), VPOT(*), VABC(*)
+  integer :: rc
+  rc = espec_pspofftaux(NDIM, NP, B, CST1, CST2, TC, U, V, VPOT, absorb_of(ABSORB, TPABSOR), &
+                        VABC, mode_flags())
+  if (rc /= 0) write(*,'(A,I3)') '<<<>>> PSPOFFTAUX on the device failed, code ', rc
+end subroutine PSPOFFTAUX
+
+! sodfft.f:3-129: one leapfrog step (U1,V1: psi(t-dt) -> psi(t); U2,V2:
+! psi(t) -> psi(t+dt)), on the device (espec_psodfftaux); WRKR and WRKI are
+! not used.
+subroutine PSODFFTAUX(ABSORB, TPABSOR, NDIM, N, B, CST, NP, U1, U2, V1, V2, TC, VPOT, VABC, &
+                      WRKR, WRKI)
+  use espec_capi
+  implicit none
+  logical :: ABSORB
+  character(len=*) :: TPABSOR
+  integer :: NDIM, N
+  real(8) :: B, CST
+  integer :: NP(*)
+  real(8) :: U1(*), U2(*), V1(*), V2(*), TC(*), VPOT(*), VABC(*), WRKR(*), WRKI(*)
+  integer :: rc
+  rc = espec_psodfftaux(NDIM, NP, B, CST, TC, U1, U2, V1, V2, VPOT, absorb_of(ABSORB, TPABSOR), &
+                        VABC, mode_flags())
+  if (rc /= 0) write(*,'(A,I3)') '<<<>>> PSODFFTAUX on the device failed, code ', rc
+end subroutine PSODFFTAUX
+
+! cfft.f:6-106: radix-2 transform of N = 2**M points on the device
+! (espec_fftnt); ITYPE = -1 backward with 1/N.
+subroutine FFTNT(X, Y, N, M, ITYPE)
+  use espec_capi
+  implicit none
+  integer :: N, M, ITYPE
+  real(8) :: X(*), Y(*)
+  integer :: rc
+  rc = espec_fftnt(X, Y, N, M, ITYPE, mode_flags())
+  if (rc /= 0) write(*,'(A,I3)') '<<<>>> FFTNT on the device failed, code ', rc
+end subroutine FFTNT

@@ -145,6 +145,38 @@ int espec_psodfft_absorb(int ndim, const int *np, double dt, double ti,
                          double *cr, double *ci, double *nrm, int maxsamp,
                          int *nsamp, int flags, int absorb, const double *vabc);
 
+/* Replaces PSPOFFTAUX(ABSORB,TPABSOR,NDIM,N,B,B1,B2,CST1,CST2,NP,U,V,TC,VPOT,
+ * VABC) (spofft.f:3-4): one split-operator step of u + i v with the caller's
+ * kinetic grid tc [N] (as DIGT fills it), kinetic constant cst1 (DT/SHFS)
+ * and half-potential constant cst2, scaled by b; absorb ESPEC_ABSORB_* with
+ * vabc [N] (.VOPTIC potential or .SMOOTHW mask, spofft.f:65-118,162-176).
+ * ESPEC_REF keeps the reference's kinetic update and f32 twiddles.  B1, B2
+ * are not read by the reference.  Runs on the device. */
+int espec_pspofftaux(int ndim, const int *np, double b, double cst1, double cst2,
+                     const double *tc, double *u, double *v, const double *vpot, int absorb,
+                     const double *vabc, int flags);
+
+/* Replaces PSODFFTAUX(ABSORB,TPABSOR,NDIM,N,B,CST,NP,U1,U2,V1,V2,TC,VPOT,VABC,
+ * WRKR,WRKI) (sodfft.f:3-4): one leapfrog step; (u1, v1) = psi(t - dt) on
+ * entry and psi(t) on return, (u2, v2) = psi(t) on entry and psi(t + dt) on
+ * return.  The work arrays are the library's.  .SMOOTHW masks in
+ * ESPEC_EXACT only (the reference's mask branch is unreachable,
+ * sodfft.f:108).  Runs on the device. */
+int espec_psodfftaux(int ndim, const int *np, double b, double cst, const double *tc,
+                     double *u1, double *u2, double *v1, double *v2, const double *vpot,
+                     int absorb, const double *vabc, int flags);
+
+/* Replaces REORT(N,M,LMTREORT,MXDCT,WORK,EIGVC) (reort.f:1): one classical
+ * Gram-Schmidt pass, r = V^T w on the FP64 matrix cores, then w -= V r in
+ * the reference's column order; basis is column-major with leading
+ * dimension ld >= n (MXDCT).  Runs on the device. */
+int espec_reort(int n, int m, int ld, double *w, const double *basis);
+
+/* Replaces FFTNT(X,Y,N,M,ITYPE) (cfft.f:6): radix-2 DIF transform of
+ * n = 2^m points, itype = -1 backward with 1/N; ESPEC_REF keeps the f32
+ * TWOPI (cfft.f:35).  Runs on the device. */
+int espec_fftnt(double *x, double *y, int n, int m, int itype, int flags);
+
 /* Replaces FFT(A,B,NTOT,N,NSPAN,ISN) (fft3d.f:1): one variate of an in-place
  * multivariate transform, isn > 0 -> exp(+i ...), unnormalised; n a power
  * of two.  Runs on the device. */
@@ -171,6 +203,17 @@ int espec_plnz(int ndim, const int *np, int mp, double abstol, double dt,
                double ti, double tf, int lc, const double *shm, double *u,
                double *v, const double *vpot, double *cr, double *ci,
                double *nrm, int maxsamp, int *nsamp, int flags);
+
+/* espec_plnz with PLNZ's ABSORB: absorb ESPEC_ABSORB_SMOOTHW multiplies the
+ * state by vabc [N] after every step, before C(t) is sampled, as PLNZAUX
+ * does for TPABSOR '.SMOOTHW' (sil.f:171-180); ESPEC_ABSORB_VOPTIC and
+ * ESPEC_ABSORB_NONE propagate without absorber (sil.f tests only
+ * .SMOOTHW). */
+int espec_plnz_absorb(int ndim, const int *np, int mp, double abstol, double dt,
+                      double ti, double tf, int lc, const double *shm, double *u,
+                      double *v, const double *vpot, double *cr, double *ci,
+                      double *nrm, int maxsamp, int *nsamp, int flags, int absorb,
+                      const double *vabc);
 
 /* ---------------------------------------------------------------------
  * Pulse-driven short-iterative Lanczos.  Replaces PPLNZ (pplnz.f:1-5):
@@ -199,6 +242,16 @@ int espec_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
                 double e0, double t0, double td, double tp, double omg,
                 double sni, int kl, double *veff, double *cr, double *ci,
                 double *nrm, int maxsamp, int *nsamp, int flags);
+/* espec_pplnz with ABSORB: PPLNZ hands ABSORB/TPABSOR to PLNZAUX
+ * (pplnz.f:208), which masks the state for .SMOOTHW (sil.f:171-180); as
+ * espec_plnz_absorb. */
+int espec_pplnz_absorb(int ndim, const int *np, int mp, double abstol, double dt,
+                       double ti, double tf, int lc, const double *shm, double *u,
+                       double *v, const double *vp, const double *dm, int efc,
+                       double e0, double t0, double td, double tp, double omg,
+                       double sni, int kl, double *veff, double *cr, double *ci,
+                       double *nrm, int maxsamp, int *nsamp, int flags, int absorb,
+                       const double *vabc);
 /* ---------------------------------------------------------------------
  * Gaussian initial packets.  Replaces INIT_COND (init_cond.f:1-200) with
  * GET_GAUSS (gauss.f:1-20): rnorm exp(-ln2 (c/rA)^2) (cos, sin)(k c) per
@@ -374,6 +427,11 @@ int espec_sil_plan_create(espec_sil_plan **plan, int ndim, const int *np,
  * trace runs of a pulse-driven plan return ESPEC_ERR_UNSUPPORTED. */
 int espec_sil_plan_set_pulse(espec_sil_plan *plan, const double *dm,
                              const double *efield, int nfield);
+/* The absorber of PLNZAUX for every packet of the plan: ESPEC_ABSORB_SMOOTHW
+ * with vabc [npk][N] masks the state after each step (sil.f:171-180);
+ * ESPEC_ABSORB_NONE (or _VOPTIC, which PLNZAUX ignores) removes it.
+ * Adaptive and trace runs of a masked plan return ESPEC_ERR_UNSUPPORTED. */
+int espec_sil_plan_set_absorb(espec_sil_plan *plan, int absorb, const double *vabc);
 int espec_sil_plan_run(espec_sil_plan *plan, int nsteps);
 int espec_sil_plan_sync(espec_sil_plan *plan);
 /* Returns ESPEC_ERR_DIAG if a tridiagonal eigensolve failed to converge

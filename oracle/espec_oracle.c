@@ -842,9 +842,22 @@ static int tridiag(int mp, const double *al, const double *bt, double abstol,
  * separately, each projected, rotated by exp(-i lambda cst), and
  * recombined.  ORC_REF keeps sil.f:98-99, where -sin multiplies the
  * already cos-scaled coefficient. */
+/* sil.f:3-185 with the .SMOOTHW mask of sil.f:171-180 when mask is not
+ * NULL: psi = VABC (U + WU) + i VABC (WV + V). */
+static int plnz_step_mask(int ndim, const int *np, int mp, double abstol, double cst,
+                          const double *shm, const double *vpot, double *u, double *v,
+                          const double *mask, int mode);
+
 int orc_plnz_step(int ndim, const int *np, int mp, double abstol, double cst,
                   const double *shm, const double *vpot, double *u, double *v,
                   int mode)
+{
+    return plnz_step_mask(ndim, np, mp, abstol, cst, shm, vpot, u, v, NULL, mode);
+}
+
+static int plnz_step_mask(int ndim, const int *np, int mp, double abstol, double cst,
+                          const double *shm, const double *vpot, double *u, double *v,
+                          const double *mask, int mode)
 {
     int n = ntot_of(ndim, np), info;
     if (mp < 1 || mp > 64) return -2;
@@ -887,8 +900,13 @@ int orc_plnz_step(int ndim, const int *np, int mp, double abstol, double cst,
     gemv_n(mp, mp, q, mp, wvs, work);
     gemv_n(n, mp, basis, n, work, wu);
     for (int i = 0; i < n; ++i) {
-        u[i] = u[i] + wu[i];
-        v[i] = wv[i] + v[i];
+        if (mask) {
+            u[i] = mask[i] * (u[i] + wu[i]);
+            v[i] = mask[i] * (wv[i] + v[i]);
+        } else {
+            u[i] = u[i] + wu[i];
+            v[i] = wv[i] + v[i];
+        }
     }
 done:
     free(basis);
@@ -902,6 +920,15 @@ int orc_plnz(int ndim, const int *np, int mp, double abstol, double dt,
              double *v, const double *vpot, double *cr, double *ci,
              int maxsamp, int mode)
 {
+    return orc_plnz_abs(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, NULL, cr, ci,
+                        maxsamp, mode);
+}
+
+int orc_plnz_abs(int ndim, const int *np, int mp, double abstol, double dt,
+                 double ti, double tf, int lc, const double *shm, double *u,
+                 double *v, const double *vpot, const double *mask, double *cr,
+                 double *ci, int maxsamp, int mode)
+{
     int n = ntot_of(ndim, np);
     double cst = dt / ORC_SHFS;
     double *u0 = malloc(sizeof(double) * 2 * n), *v0 = u0 + n;
@@ -911,7 +938,7 @@ int orc_plnz(int ndim, const int *np, int mp, double abstol, double dt,
     if (maxsamp > 0) corr(n, u0, v0, u, v, &cr[0], &ci[0]);
     int ns = orc_nsteps(ti, tf, dt);
     for (int s = 0; s < ns; ++s) {
-        if (orc_plnz_step(ndim, np, mp, abstol, cst, shm, vpot, u, v, mode)) {
+        if (plnz_step_mask(ndim, np, mp, abstol, cst, shm, vpot, u, v, mask, mode)) {
             free(u0);
             return -1;
         }
@@ -964,6 +991,17 @@ int orc_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
               double sni, int kl, double *veff, double *cr, double *ci,
               int maxsamp, int mode)
 {
+    return orc_pplnz_abs(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vp, dm, efc, e0, t0,
+                         td, tp, omg, sni, kl, NULL, veff, cr, ci, maxsamp, mode);
+}
+
+int orc_pplnz_abs(int ndim, const int *np, int mp, double abstol, double dt,
+                  double ti, double tf, int lc, const double *shm, double *u,
+                  double *v, const double *vp, const double *dm, int efc,
+                  double e0, double t0, double td, double tp, double omg,
+                  double sni, int kl, const double *mask, double *veff, double *cr,
+                  double *ci, int maxsamp, int mode)
+{
     int n = ntot_of(ndim, np);
     double cst = dt / ORC_SHFS;
     double *u0 = malloc(sizeof(double) * 3 * n), *v0 = u0 + n, *ve = u0 + 2 * n;
@@ -977,7 +1015,7 @@ int orc_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
     for (int s = 0; s < ns; ++s, t += dt) {
         double e = orc_ef(efc, e0, t0, td, tp, omg, sni, kl, t);
         for (int i = 0; i < n; ++i) ve[i] = vp[i] - dm[i] * e;
-        if (orc_plnz_step(ndim, np, mp, abstol, cst, shm, ve, u, v, mode)) {
+        if (plnz_step_mask(ndim, np, mp, abstol, cst, shm, ve, u, v, mask, mode)) {
             free(u0);
             return -1;
         }

@@ -144,6 +144,12 @@ int orc_plnz(int ndim, const int *np, int mp, double abstol, double dt,
              double ti, double tf, int lc, const double *shm, double *u,
              double *v, const double *vpot, double *cr, double *ci,
              int maxsamp, int mode);
+/* orc_plnz with the .SMOOTHW mask of PLNZAUX (sil.f:171-180) when mask is
+ * not NULL. */
+int orc_plnz_abs(int ndim, const int *np, int mp, double abstol, double dt,
+                 double ti, double tf, int lc, const double *shm, double *u,
+                 double *v, const double *vpot, const double *mask, double *cr,
+                 double *ci, int maxsamp, int mode);
 /* ef.f EF (efc 0 .NONE/.NULL, 1 .GAUS, 2 .GGAUS, 3 .SIN2) and pplnz.f
  * PPLNZ with PRTCRL='.NO' (veff may be NULL). */
 /* init_cond.f INIT_COND (see ORC_INIT_*); u, v [N] out (in: the 1-D
@@ -164,6 +170,13 @@ int orc_pplnz(int ndim, const int *np, int mp, double abstol, double dt,
               double e0, double t0, double td, double tp, double omg,
               double sni, int kl, double *veff, double *cr, double *ci,
               int maxsamp, int mode);
+/* orc_pplnz with the .SMOOTHW mask of PLNZAUX (pplnz.f:208, sil.f:171-180). */
+int orc_pplnz_abs(int ndim, const int *np, int mp, double abstol, double dt,
+                  double ti, double tf, int lc, const double *shm, double *u,
+                  double *v, const double *vp, const double *dm, int efc,
+                  double e0, double t0, double td, double tp, double omg,
+                  double sni, int kl, const double *mask, double *veff, double *cr,
+                  double *ci, int maxsamp, int mode);
 
 /* rans.f:3 RAN: NSEED=(NSEED*1029+221591) mod 2^20, RAN=NSEED/2^20. */
 double orc_ran(int *nseed);

@@ -95,6 +95,9 @@ class Oracle:
         L.orc_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d,
                                _d, _d, _d, _ip, _ip]
         L.orc_plnz.restype = _ip
+        L.orc_plnz_abs.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d,
+                                   _d, _d, _d, _d, _ip, _ip]
+        L.orc_plnz_abs.restype = _ip
         L.orc_absorbingbc.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d, C.POINTER(_dp)]
         L.orc_absorbingbc.restype = _ip
         L.orc_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
@@ -213,12 +216,19 @@ class Oracle:
                                      u, v, _f64(vpot), absorb, _f64(vabc), cr, ci, ns, mode)
         return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
 
-    def plnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, mode=REF):
+    def plnz(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, mode=REF, mask=None):
+        """PLNZ (plnz.f) with PRTCRL='.NO'; mask: the .SMOOTHW VABC that
+        PLNZAUX applies after every step with ABSORB (sil.f:171-180)."""
         ns = self.nsteps(ti, tf, dt) // max(lc, 1) + 2
         cr, ci = np.zeros(ns), np.zeros(ns)
         u, v = _f64(u).copy(), _f64(v).copy()
-        k = self.lib.orc_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc,
-                              _v3(shm, 0.0), u, v, _f64(vpot), cr, ci, ns, mode)
+        if mask is None:
+            k = self.lib.orc_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc,
+                                  _v3(shm, 0.0), u, v, _f64(vpot), cr, ci, ns, mode)
+        else:
+            k = self.lib.orc_plnz_abs(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc,
+                                      _v3(shm, 0.0), u, v, _f64(vpot), _f64(mask), cr, ci, ns,
+                                      mode)
         if k < 0:
             raise RuntimeError("diagonalization error")
         return dict(cr=cr[:k], ci=ci[:k], u=u, v=v)
@@ -373,6 +383,8 @@ class Reference:
         L.ref_psodfft_abs.argtypes = L.ref_pspofft_abs.argtypes
         L.ref_plnz.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d,
                                _d, _d, _ip, P(_ip)]
+        L.ref_plnz_smoothw.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d,
+                                       _d, _d, _d, _ip, P(_ip)]
         L.ref_fft.argtypes = [_d, _d, _ip, _ip, _ip, _ip]
         L.ref_fftnt.argtypes = [_d, _d, _ip, _ip, _ip]
         L.ref_digt.argtypes = [_ip, _i, _d, _d, _d, P(_dp)]
@@ -437,6 +449,17 @@ class Reference:
         info = _ip(0)
         self.lib.ref_plnz(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc, _v3(shm, 0.0),
                           u, v, _f64(vpot), cr, ci, nsamp, C.byref(info))
+        return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
+
+    def plnz_smoothw(self, np_, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, mask):
+        nst = int(((tf + dt) - (ti + dt) + dt) / dt)
+        nsamp = nst // max(lc, 1) + 1
+        cr, ci = np.zeros(nsamp), np.zeros(nsamp)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        info = _ip(0)
+        self.lib.ref_plnz_smoothw(len(np_), _np3(np_), mp, abstol, dt, ti, tf, lc,
+                                  _v3(shm, 0.0), u, v, _f64(vpot), _f64(mask), cr, ci, nsamp,
+                                  C.byref(info))
         return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
 
     def ef(self, efc, e0, t0, td, tp, omg, sni, kl, t):

@@ -196,6 +196,40 @@ contains
     deallocate(iwork, vabc, lnzvc, wu, wv, wp, work)
   end subroutine ref_plnz
 
+  ! PLNZ with ABSORB and TPABSOR '.SMOOTHW': PLNZAUX multiplies each step's
+  ! state by the mask (sil.f:171-180).
+  subroutine ref_plnz_smoothw(ndim, np, mp, abstol, dt, ti, tf, lc, shm, u, v, vpot, &
+                              mask, cr, ci, nsamp, info) bind(C, name='ref_plnz_smoothw')
+    integer(c_int), value :: ndim, mp, lc, nsamp
+    integer(c_int) :: np(3)
+    real(c_double), value :: abstol, dt, ti, tf
+    real(c_double) :: shm(3), u(*), v(*), vpot(*), mask(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims
+    integer :: n, mxdct, i
+    integer, allocatable :: iwork(:)
+    real(8), allocatable :: lnzvc(:,:), wu(:), wv(:), wp(:), work(:)
+    real(8) :: xi(3), xf(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(iwork(5*mp + 16), lnzvc(mxdct, mp + 2), wu(n), wv(n), &
+             wp(n), work(max(n, 5*mp + 16)))
+    lnzvc = 0d0
+    xi = 0d0; xf = 0d0; var = 0d0
+    call dimstr(ndim, dims)
+    call PLNZ(.true., .false., dims, '.NO     ', '.SMOOTHW', info, lc, mp, &
+              mp, mp + 2, mp + 1, mxdct, n, abstol, dt, ti, tf, 1d30, np, &
+              iwork, shm, vpot, mask, u, v, xi, xf, wu, wv, wp, work, var, lnzvc)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, mp + 1)
+      ci(i) = lnzvc(i, mp + 2)
+    end do
+    deallocate(iwork, lnzvc, wu, wv, wp, work)
+  end subroutine ref_plnz_smoothw
+
   ! EFC string of pulse kind 0..3 (ef.f).
   subroutine efcstr(efc, s)
     integer(c_int), value :: efc

@@ -1,0 +1,90 @@
+"""The reference's own main program (espec.f, compiled from its sources)
+with the hot path linked to the drop-in (oracle/_ref/espec_dropin.x, built
+by `make -C oracle dropin`), run on the GPU on the reference's regression
+inputs: its printed C(t) table, correlation function and spectrum against
+the published results (tests/golden, extracted by make_fixtures.py).
+
+Everything before and after the propagation (input reader, potentials,
+MTRXDIAG, the file handling of espec.f) is the reference's own code; the
+propagator, the initial LANCZS states, C(t) assembly pieces, HWHM and
+SPECTRUMTD are the drop-in's.  Bars as in test_inputspc_gpu.py: the
+printed 7th decimal for the FFT propagators (two units where the initial
+state also comes from ABSTOL = 1e-6 bisection), a few 1e-6 for the
+ABSTOL = 1e-6 Lanczos runs."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import make_fixtures as MF
+import setups as S
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "oracle", "_ref", "espec_dropin.x")
+INPUTS = os.path.join(ROOT, "tests", "golden", "inputs")
+STOPS = ["Correlation Function", "Spectrum:", "Partial auto", "Total correlation",
+         "Calculating", "Propagating", "Eigenvector propagated"]
+
+
+def run_main(t, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.skip("oracle/_ref/espec_dropin.x not built (make -C oracle dropin)")
+    shutil.copy(os.path.join(INPUTS, f"input.test{t}"), tmp_path / "input.spc")
+    for f in os.listdir(INPUTS):
+        if not f.startswith("input."):
+            shutil.copy(os.path.join(INPUTS, f), tmp_path / f)
+    print(f"case: espec_dropin.x input.test{t}", flush=True)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    out = r.stdout.splitlines()
+    assert "Propagation operator error" not in r.stdout, r.stdout[-3000:]
+    return out
+
+
+def arr(rows):
+    return np.array([[float(x.replace("D", "E")) for x in r.split()] for r in rows])
+
+
+def blocks(lines, table_cols=6, table_title="Partial auto-correlation function"):
+    tab = arr(MF.block(lines, table_title, table_cols,
+                       [s for s in STOPS if not table_title.startswith(s)]))
+    cor = arr(MF.block(lines, "Correlation Function", 3, [s for s in STOPS if s != "Correlation Function"]))
+    spec = arr(MF.block(lines, "Spectrum:", 4, [s for s in STOPS if s != "Spectrum:"]))
+    return tab, cor, spec
+
+
+def test_main_test12_pfspo(gpu, tmp_path):
+    """.PFSPO, 2.5e6 steps of a 256-point packet, MTRXDIAG initial state."""
+    tab, cor, spec = blocks(run_main(12, tmp_path))
+    gt, gc, gs = S.load("test12_table"), S.load("test12_corr"), S.load("test12_spec")
+    assert tab.shape == gt.shape and cor.shape == gc.shape and spec.shape == gs.shape
+    d = [np.abs(tab[:, 2:4] - gt[:, 2:4]).max(), np.abs(cor[:, 1:] - gc[:, 1:]).max(),
+         np.abs(spec[:, 1:] - gs[:, 1:]).max()]
+    print(f"test12 |dC| table {d[0]:.1e} corr {d[1]:.1e} spectrum {d[2]:.1e}", flush=True)
+    assert max(d) <= 1.01e-7
+    assert np.abs(spec[:, 0] - gs[:, 0]).max() <= 6e-8
+
+
+def test_main_test4_psil(gpu, tmp_path):
+    """.PSIL order 10 from a .FILE potential and .GETR state, ABSTOL 1e-6."""
+    tab, cor, spec = blocks(run_main(4, tmp_path))
+    gt, gc = S.load("test4_table"), S.load("test4_corr")
+    assert tab.shape == gt.shape and cor.shape == gc.shape
+    d = max(np.abs(tab[:, 2:4] - gt[:, 2:4]).max(), np.abs(cor[:, 1:] - gc[:, 1:]).max())
+    print(f"test4 |dC| {d:.1e}", flush=True)
+    assert d <= 5e-6
+
+
+def test_main_test14_ppsil(gpu, tmp_path):
+    """.PPSIL (PPLNZ) from the LANCZS ground state; PPLNZ prints its own
+    7-column table (t, ET, E0, Pr, Pi, |P0|^2, norm) with F8.5 C(t)."""
+    lines = run_main(14, tmp_path)
+    tab = arr(MF.block(lines, "Auto-correlation function", 7, ["Eigenvector propagated"]))
+    gt = S.load("test14_table")
+    assert tab.shape == gt.shape
+    d = np.abs(tab[:, 3:5] - gt[:, 3:5]).max()
+    print(f"test14 |dP| {d:.1e}", flush=True)
+    assert d <= 1.01e-5  # one unit of F8.5's last digit (bisection drift)

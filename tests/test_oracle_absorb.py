@@ -177,3 +177,46 @@ def test_spo_mask_index(orc, case):
             changed = np.nonzero((gu != bu) | (gv != bv))[0].tolist()
             assert changed == [j]
             assert gu[j] == 0.25 * bu[j] and gv[j] == 0.25 * bv[j]
+
+
+# ------------------------------------------------------------ PLNZ/PLNZAUX
+# PLNZAUX multiplies the new state by VABC when ABSORB and TPABSOR is
+# .SMOOTHW (sil.f:171-180); PLNZ samples C(t) from that masked state.
+
+def _plnz_case(c):
+    shm = [1.0 / (24.0 * c["m"] * c["sh"] ** 2)]
+    return shm, 2e-3, 60
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_plnz_smoothw_matches_reference(orc, ref, case, k):
+    c = case
+    shm, dt, nst = _plnz_case(c)
+    mask = c["mask"][k]
+    tf = (nst - 1) * dt
+    o = orc.plnz([c["n"]], 10, 0.0, dt, 0.0, tf, 1, shm, c["u"], c["v"], c["vpot"], REF,
+                 mask=mask)
+    r = ref.plnz_smoothw([c["n"]], 10, 0.0, dt, 0.0, tf, 1, shm, c["u"], c["v"], c["vpot"],
+                         mask)
+    assert r["info"] == 0
+    assert cplx_err(o["cr"], o["ci"], r["cr"], r["ci"]) <= 1e-12
+    assert np.abs(o["u"] - r["u"]).max() <= 1e-12 and np.abs(o["v"] - r["v"]).max() <= 1e-12
+    # the mask touches the run: it differs from the run without absorber
+    p = orc.plnz([c["n"]], 10, 0.0, dt, 0.0, tf, 1, shm, c["u"], c["v"], c["vpot"], REF)
+    assert cplx_err(o["cr"], o["ci"], p["cr"], p["ci"]) > 1e-9
+
+
+def test_plnz_smoothw_reductions(orc, case):
+    """A unit mask is the plain step bit for bit; a constant mask m scales
+    sample k by m^k (the step is linear in the state)."""
+    c = case
+    shm, dt, nst = _plnz_case(c)
+    tf = (nst - 1) * dt
+    args = ([c["n"]], 10, 0.0, dt, 0.0, tf, 1, shm, c["u"], c["v"], c["vpot"], EXACT)
+    p = orc.plnz(*args)
+    one = orc.plnz(*args, mask=np.ones(c["n"]))
+    assert np.array_equal(p["cr"], one["cr"]) and np.array_equal(p["u"], one["u"])
+    m = 0.999
+    q = orc.plnz(*args, mask=np.full(c["n"], m))
+    k = np.arange(len(p["cr"]))
+    assert np.abs((q["cr"] + 1j * q["ci"]) - m ** k * (p["cr"] + 1j * p["ci"])).max() <= 1e-13
