@@ -1345,9 +1345,12 @@ __global__ __launch_bounds__(256) void sil_field(const double *vp, const double 
                                                  const double *efield, const long long *dstep,
                                                  double *veff, long long n)
 {
+    // The product is rounded before the difference, as CHPOT's host code
+    // does, so that VEFF matches the reference bit for bit.
+#pragma clang fp contract(off)
     const double e = efield[*dstep];
     for (long long p = blockIdx.x * 256LL + threadIdx.x; p < n; p += gridDim.x * 256LL)
-        veff[p] = vp[p] - __dmul_rn(dm[p], e);
+        veff[p] = __dsub_rn(vp[p], __dmul_rn(dm[p], e));
 }
 
 template <typename S>

@@ -178,7 +178,8 @@ constexpr int kReortBlocks = 64;    // workgroups per group of 16 vectors
 // (x, g16) sums points p = 4 q + k over its share of q with MFMA, wave by
 // wave, then writes its 16 partials to part[g16][x][16].  Lane l feeds
 // A[l % 16][l / 16] = V[p0 + l / 16][j0 + l % 16] and B[l / 16][*] =
-// w[p0 + l / 16]; output register r of lane l is row 4 (l / 16) + r.
+// w[p0 + l / 16]; in the FP64 16x16x4 layout, output register r of lane l
+// holds row 4 r + l / 16 (column l % 16).
 __global__ __launch_bounds__(64 * kReortWaves) void reort_project(int n, int m, int ld,
                                                                   const double *w,
                                                                   const double *basis,
@@ -198,10 +199,10 @@ __global__ __launch_bounds__(64 * kReortWaves) void reort_project(int n, int m, 
         const double bw = pok ? w[p] : 0.0;
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bw, acc, 0, 0, 0);
     }
-    // Column 0 of the result (lanes 0, 16, 32, 48) holds rows 4 (l/16) + r.
+    // Column 0 of the result (lanes 0, 16, 32, 48) holds rows 4 r + l/16.
     if ((lane & 15) == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[wv][4 * (lane >> 4) + r] = acc[r];
+        for (int r = 0; r < 4; ++r) red[wv][4 * r + (lane >> 4)] = acc[r];
     }
     __syncthreads();
     if (threadIdx.x < 16) {

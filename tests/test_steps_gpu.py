@@ -152,7 +152,8 @@ def test_sil_plan_smoothw_batched(gpu, orc):
     masks = np.stack([p["mask"], np.roll(p["mask"], 7), np.ones(128)])
     u = np.stack([p["u"], np.roll(p["u"], 3), p["u"]])
     dt, nst = 2e-3, 40
-    P = E.SilPlan([128], 8, 0.0, dt, 1, shm, p["vpot"], u, None, maxsamp=nst + 1,
+    P = E.SilPlan([128], 8, 0.0, dt, 1, shm, np.stack([p["vpot"]] * 3), u, None,
+                  maxsamp=nst + 1,
                   flags=E.FP64 | E.EXACT)
     P.set_absorb(E.ABSORB_SMOOTHW, masks)
     P.run(15)
