@@ -1349,8 +1349,10 @@ __global__ __launch_bounds__(256) void sil_field(const double *vp, const double 
     // does, so that VEFF matches the reference bit for bit.
 #pragma clang fp contract(off)
     const double e = efield[*dstep];
-    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < n; p += gridDim.x * 256LL)
-        veff[p] = __dsub_rn(vp[p], __dmul_rn(dm[p], e));
+    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < n; p += gridDim.x * 256LL) {
+        const double f = dm[p] * e;
+        veff[p] = vp[p] - f;
+    }
 }
 
 template <typename S>
