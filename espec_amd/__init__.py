@@ -91,6 +91,9 @@ def _load():
     L.espec_pspofftaux.argtypes = [ip, i, dp, dp, dp, d, d, d, d, ip, vp, ip]
     L.espec_psodfftaux.argtypes = [ip, i, dp, dp, d, d, d, d, d, d, ip, vp, ip]
     L.espec_reort.argtypes = [ip, ip, ip, d, d]
+    L.espec_ppsod.argtypes = [ip, i, dp, dp, dp, ip, d, d, d, d, d, ip, dp, dp, dp, dp, dp, dp,
+                              ip, ip, vp, ip, vp, vp, vp, vp, d, d, vp, vp, vp, vp, ip, P(ip),
+                              ip]
     L.espec_fftnt.argtypes = [d, d, ip, ip, ip, ip]
     L.espec_sil_plan_set_absorb.argtypes = [vp, ip, vp]
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
@@ -261,6 +264,41 @@ def pplnz(np_, mp, abstol, dt, ti, tf, lc, shm, u1, v1, vp, dm, pulse, flags=FP6
     _check(rc, "PPLNZ")
     k = ns.value
     return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v, veff=ve)
+
+
+def ppsod(np_, dt, ti, tf, lc, shm, u1, v1, vp, dm, pulse, flags=FP64 | EXACT,
+          absorb=ABSORB_NONE, vabc=None, proj=None, energies=False):
+    """PPSOD (ppsod.f:1-548, step PSODAUX sod.f:3-122): the pulse-driven FD
+    leapfrog on the device.  pulse = (efc, e0, t0, td, tp, omg, sni, kl).
+    Returns dict(cr, ci, norm, u, v, uprev, vprev, veff[, et, et0][, proj])."""
+    L = _load()
+    n = int(np.prod(np_))
+    nf = int(np.floor((tf + dt / 2 - (ti + dt)) / dt + 0.5))
+    maxs = (nf + 1) // lc + 1
+    cr, ci, nr = np.zeros(maxs), np.zeros(maxs), np.zeros(maxs)
+    u, v = _f64(u1).copy(), _f64(v1).copy()
+    up, vq, ve = np.zeros(n), np.zeros(n), np.zeros(n)
+    et, et0 = (np.zeros(maxs), np.zeros(maxs)) if energies else (None, None)
+    w = None if proj is None else np.ascontiguousarray(proj, dtype=np.float64).reshape(-1, n)
+    npr = 0 if w is None else w.shape[0]
+    pr = np.zeros((maxs, max(npr, 1), 2))
+    va = None if vabc is None else _f64(vabc)
+    ns = C.c_int(0)
+    efc, e0, t0, td, tp, omg, sni, kl = pulse
+    ptr = lambda a: None if a is None else a.ctypes.data
+    rc = L.espec_ppsod(len(np_), _np3(np_), dt, ti, tf, lc, _v3(shm, 0.0), u, v, _f64(vp),
+                       _f64(dm), efc, e0, t0, td, tp, omg, sni, kl, absorb, ptr(va), npr, ptr(w),
+                       up.ctypes.data, vq.ctypes.data, ve.ctypes.data, cr, ci, nr.ctypes.data,
+                       ptr(et), ptr(et0), pr.ctypes.data if npr else None, maxs, C.byref(ns),
+                       flags)
+    _check(rc, "PPSOD")
+    k = ns.value
+    out = dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v, uprev=up, vprev=vq, veff=ve)
+    if energies:
+        out.update(et=et[:k], et0=et0[:k])
+    if npr:
+        out["proj"] = pr[:k, :npr]
+    return out
 
 
 def psodfft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT, absorb=ABSORB_NONE,

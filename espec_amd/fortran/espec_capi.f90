@@ -241,6 +241,19 @@ module espec_capi
       integer(c_int) :: rc
     end function
 
+    function espec_ppsod(ndim, np, dt, ti, tf, lc, shm, u, v, vp, dm, efc, e0, t0, td, tp, &
+                         omg, sni, kl, absorb, vabc, npr, proj, uprev, vprev, veff, cr, ci, &
+                         nrm, et, et0, pr, maxsamp, nsamp, flags) &
+        bind(C, name='espec_ppsod') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, lc, efc, kl, absorb, npr, maxsamp, flags
+      integer(c_int) :: np(*), nsamp
+      real(c_double), value :: dt, ti, tf, e0, t0, td, tp, omg, sni
+      real(c_double) :: shm(*), u(*), v(*), vp(*), dm(*), vabc(*), proj(*), uprev(*), vprev(*)
+      real(c_double) :: veff(*), cr(*), ci(*), nrm(*), et(*), et0(*), pr(*)
+      integer(c_int) :: rc
+    end function
+
     function espec_fft(a, b, ntot, n, nspan, isn, flags) bind(C, name='espec_fft') result(rc)
       import :: c_int, c_double
       real(c_double) :: a(*), b(*)

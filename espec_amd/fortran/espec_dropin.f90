@@ -931,3 +931,139 @@ subroutine FFTNT(X, Y, N, M, ITYPE)
   rc = espec_fftnt(X, Y, N, M, ITYPE, mode_flags())
   if (rc /= 0) write(*,'(A,I3)') '<<<>>> FFTNT on the device failed, code ', rc
 end subroutine FFTNT
+
+! ppsod.f:1-548: the pulse-driven finite-difference leapfrog (PSODAUX,
+! sod.f:3-122) on the device (espec_ppsod), with the reference's table
+! (FORMAT 1011/1012/1021: t, ET, E0, Pr, Pi, |c|^2, the populations of
+! LNZVC(:,KP+1..KP+NPR) and the norm), every printed quantity formed on the
+! device.  LNZVC(:,MPR/MPI) receives C(t) where the reference stores it:
+! C(TI) and the first step's sample only when PRTCRL prints (so with
+! PRTCRL '.NO' and LC = 1 nothing is stored, ppsod.f:276-296).  U2, V2 end
+! as psi(TF), U1, V1 as psi(TF - DT) and VPOT as the last VEFF.  The movie,
+! potential and pulse files (PRTVEFF, PRTEIGVC2, PRTPULS) return INFO = 6;
+! a TPABSOR that PSODAUX does not know stops, as the reference does.
+subroutine PPSOD(ABSORB, PRTVEFF, PRTEIGVC2, PRTPULS, DIM, EFC, PRTCRL, TPABSOR, INFO, LC, MP, &
+                 MPI, MPR, MXDCT, N, NSHOT, NPR, KP, NISPG, E0, T0, TD, TP, OMG, SNI, KL, DT, &
+                 TI, TF, TOL, NP, ND, SH, SHM, U1, U2, V1, V2, XI, XP, XF, DM, VPOT, VABC, &
+                 WORK, VAR, LNZVC, PRTREGION, NREG, RANGE, PRTONLYREIM)
+  use espec_capi
+  implicit none
+  logical :: ABSORB, PRTVEFF, PRTEIGVC2, PRTPULS, PRTREGION, PRTONLYREIM
+  character(len=*) :: DIM, EFC, PRTCRL, TPABSOR
+  integer :: INFO, LC, MP, MPI, MPR, MXDCT, N, NSHOT, NPR, KP, NISPG, KL, NREG
+  real(8) :: DT, TI, TF, TOL, E0, T0, TD, TP, OMG, SNI
+  integer :: NP(*), ND(*)
+  real(8) :: SH(*), SHM(*), U1(*), U2(*), V1(*), V2(*), XI(*), XP(*), XF(*), DM(*), VPOT(*)
+  real(8) :: VABC(*), WORK(*), VAR(*), LNZVC(MXDCT, *), RANGE(5, 7)
+  integer :: ndim, efk, kind, npj, nf, nstep, lcd, maxs, ns, rc, s, j, step, nc, m
+  real(8) :: t, f0, ct(7)
+  real(8), allocatable :: vp(:), w(:), cr(:), ci(:), nr(:), et(:), et0(:), pr(:)
+  logical :: every, prt
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  efk = pulse_of(EFC)
+  if (ndim == 0 .or. PRTVEFF .or. PRTEIGVC2 .or. PRTPULS) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  if (efk < 0) then
+    write(*,'(A)') '<<<>>> Desired laser pulse wasn''t found. <<<>>>'
+    INFO = ESPEC_ERR_ARG
+    return
+  end if
+  kind = ESPEC_ABSORB_NONE
+  if (ABSORB) then  ! sod.f:55-120
+    if (starts(TPABSOR, '.VOPTIC')) then
+      kind = ESPEC_ABSORB_VOPTIC
+    else if (starts(TPABSOR, '.SMOOTHW') .or. starts(TPABSOR, '.SMRS')) then
+      kind = ESPEC_ABSORB_SMOOTHW
+    else
+      write(*,*) '<<<>>> SOD propagation error <<<>>>'
+      error stop
+    end if
+  end if
+  every = starts(PRTCRL, '.YES')
+  prt = every .or. starts(PRTCRL, '.PARTIAL')
+  npj = 0
+  if (prt) npj = max(0, min(NPR, 6))
+  nf = nint((TF + DT / 2 - (TI + DT)) / DT)  ! ppsod.f:355
+  nstep = max(nf, 0) + 1                     ! the start-up counts as one step
+  lcd = LC
+  if (every) lcd = 1
+  maxs = nstep / lcd + 1
+  allocate(vp(N), w(max(1, N * npj)), cr(maxs), ci(maxs), nr(maxs), et(maxs), et0(maxs), &
+           pr(max(1, 2 * npj * maxs)))
+  vp = VPOT(1:N)
+  do j = 1, npj
+    w((j - 1) * N + 1:j * N) = LNZVC(1:N, KP + j)
+  end do
+  ! psi0 comes in U1, V1 (ppsod.f:85-92); U2, V2 carry it to the device
+  U2(1:N) = U1(1:N)
+  V2(1:N) = V1(1:N)
+  rc = espec_ppsod(ndim, NP, DT, TI, TF, lcd, SHM, U2, V2, vp, DM, efk, E0, T0, TD, TP, OMG, &
+                   SNI, KL, kind, VABC, npj, w, U1, V1, VPOT, cr, ci, nr, et, et0, pr, maxs, &
+                   ns, ESPEC_FP64)
+  if (rc /= 0) then
+    INFO = rc
+    deallocate(vp, w, cr, ci, nr, et, et0, pr)
+    return
+  end if
+  deallocate(vp)
+  ! LNZVC(:,MPR/MPI): sample s sits at step s*lcd; the reference stores
+  ! the steps that are multiples of LC, C(TI) and step 1 only when printing.
+  nc = 1
+  do s = 0, ns - 1
+    step = s * lcd
+    if (mod(step, LC) /= 0) cycle
+    if (.not. prt .and. (s == 0 .or. LC == 1)) cycle
+    nc = step / LC + 1
+    LNZVC(nc, MPR) = cr(s + 1)
+    LNZVC(nc, MPI) = ci(s + 1)
+  end do
+  if (prt) then
+    f0 = nr(1)
+    write(*,*)
+    if (DIM == '.1D') then
+      write(*,*) 'Auto-correlation function:'
+    else
+      write(*,*) 'Partial auto-correlation function:'
+    end if
+    write(*,*) '=================================='
+    if (NPR + KP - 1 > 10) then
+      write(*,1012) '*|P', NISPG - 1, '|^2*', ('*|P', j - 1 + KP, '|^2*', j = 1, NPR), &
+                    '*|c', NISPG - 1, '|^2*'
+    else
+      write(*,1011) '*|P', NISPG - 1, '|^2*', ('*|P', j - 1 + KP, '|^2*', j = 1, NPR), &
+                    ' |c', NISPG - 1, '|^2 '
+    end if
+    t = TI  ! T = TI + DT after the start-up, then T = T + DT (ppsod.f:256, 358)
+    m = 0
+    do s = 0, ns - 1
+      do while (m < s * lcd)
+        if (m == 0) then
+          t = TI + DT
+        else
+          t = t + DT
+        end if
+        m = m + 1
+      end do
+      ct(1) = cr(s + 1)**2 + ci(s + 1)**2
+      do j = 1, npj
+        ct(j + 1) = pr(2 * (s * npj + j) - 1)**2 + pr(2 * (s * npj + j))**2
+      end do
+      ! ppsod.f:456-460: the norm check of the loop's printed rows
+      if (s >= 2 .or. (s == 1 .and. lcd > 1)) then
+        if (.not. ABSORB .and. abs((nr(s + 1) - f0) / f0) * 1.0d2 > TOL) INFO = 1
+      end if
+      write(*,1021) t, et(s + 1), et0(s + 1), cr(s + 1), ci(s + 1), (ct(j), j = 1, NPR + 1), &
+                    nr(s + 1)
+    end do
+  end if
+  deallocate(w, cr, ci, nr, et, et0, pr)
+1011 format(/,8X,'*t/fs*',5X,'*ET/a.u.*',6X,'*E0/a.u.*',6X,'*Pr(t)*',4X,'*Pi(t)*',3X, &
+            8(A3,I1,A4,3X))
+1012 format(/,8X,'*t/fs*',5X,'*ET/a.u.*',6X,'*E0/a.u.*',6X,'*Pr(t)*',4X,'*Pi(t)*',3X, &
+            8(A3,I2,A4,3X))
+1021 format(1X,F14.6,3X,2(E12.6,3X),2(F8.5,3X),10(F8.6,3X))
+end subroutine PPSOD

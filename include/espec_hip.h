@@ -253,6 +253,35 @@ int espec_pplnz_absorb(int ndim, const int *np, int mp, double abstol, double dt
                        double *nrm, int maxsamp, int *nsamp, int flags, int absorb,
                        const double *vabc);
 /* ---------------------------------------------------------------------
+ * Pulse-driven finite-difference leapfrog.  Replaces PPSOD (ppsod.f:1-548)
+ * with its step PSODAUX (sod.f:3-122): psi(t+dt) = psi(t-dt) - i CST H psi(t),
+ * CST = 2 DT/SHFS, H the FD operator of AU with VEFF = VP - DM EF(T) at each
+ * step's end time; start-up by a quarter- and a half-strength step at the
+ * fields of (TI+DT)/2 and TI+DT, then NINT((TF+DT/2-TI-DT)/DT) steps.
+ *   u, v            in: psi0; out: psi(TF) (U2, V2)
+ *   uprev, vprev    out (may be NULL): psi(TF - DT) (U1, V1)
+ *   absorb, vabc    ESPEC_ABSORB_VOPTIC / _SMOOTHW (sod.f:55-120); .SMRS is
+ *                   the .SMOOTHW form
+ *   npr, proj       0..6 real vectors [npr][N] to project on (W1..W6)
+ *   veff            out (may be NULL): the last VEFF (VPOT on return)
+ *   cr, ci, nrm     C(t) = <psi0|psi>, <psi|psi> at sample 0 (TI) and then
+ *                   every lc steps, the start-up counting as one step (the
+ *                   reference stores these when PRTCRL prints)
+ *   et, et0         out (may be NULL): <psi|H|psi> with VEFF and with VP
+ *                   (EIGENERG of U2 plus V2) at each sample
+ *   pr              out (may be NULL): [nsamp][npr][2] (<W,U>, <W,V>)
+ * FP64 only (ESPEC_MIXED returns ESPEC_ERR_UNSUPPORTED).  Runs on the
+ * device. */
+int espec_ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                const double *shm, double *u, double *v, const double *vp,
+                const double *dm, int efc, double e0, double t0, double td,
+                double tp, double omg, double sni, int kl, int absorb,
+                const double *vabc, int npr, const double *proj, double *uprev,
+                double *vprev, double *veff, double *cr, double *ci, double *nrm,
+                double *et, double *et0, double *pr, int maxsamp, int *nsamp,
+                int flags);
+
+/* ---------------------------------------------------------------------
  * Gaussian initial packets.  Replaces INIT_COND (init_cond.f:1-200) with
  * GET_GAUSS (gauss.f:1-20): rnorm exp(-ln2 (c/rA)^2) (cos, sin)(k c) per
  * axis, c = XI + (i-1) SH - X0, normalised over the axis.

@@ -1365,3 +1365,109 @@ void orc_spectrumtd(int iwin, int mf, double dt, double tf, double wp,
         en = en + enc;
     }
 }
+
+/* sod.f:3-122 PSODAUX: one finite-difference leapfrog step,
+ * psi(t+dt) = psi(t-dt) - i cst H psi(t) with H from AU (the 4th-order FD
+ * kinetic operator plus vpot).  (u1, v1) = psi(t-dt) on entry and psi(t)
+ * on return, (u2, v2) = psi(t) on entry and psi(t+dt) on return.  The real
+ * part is advanced first with H applied to v2; the imaginary part then
+ * uses H applied to u1, which by then holds the old u2.  Absorbers
+ * (sod.f:55-120): ORC_ABS_VOPTIC subtracts cst vabc psi(t); ORC_ABS_SMOOTHW
+ * (and .SMRS) multiplies the new value by vabc. */
+void orc_psodaux(int ndim, const int *np, double cst, const double *shm, double *u1,
+                 double *u2, double *v1, double *v2, const double *vpot, int absorb,
+                 const double *vabc)
+{
+    int n = ntot_of(ndim, np);
+    double *w = malloc(sizeof(double) * n);
+    orc_au(ndim, np, shm, vpot, v2, w);
+    for (int i = 0; i < n; ++i) {
+        double ua = u2[i];
+        if (absorb == ORC_ABS_VOPTIC)
+            u2[i] = u1[i] + cst * w[i] - cst * vabc[i] * ua;
+        else if (absorb == ORC_ABS_SMOOTHW)
+            u2[i] = vabc[i] * (u1[i] + cst * w[i]);
+        else
+            u2[i] = u1[i] + cst * w[i];
+        u1[i] = ua;
+    }
+    orc_au(ndim, np, shm, vpot, u1, w);
+    for (int i = 0; i < n; ++i) {
+        double va = v2[i];
+        if (absorb == ORC_ABS_VOPTIC)
+            v2[i] = v1[i] - cst * w[i] - cst * vabc[i] * va;
+        else if (absorb == ORC_ABS_SMOOTHW)
+            v2[i] = vabc[i] * (v1[i] - cst * w[i]);
+        else
+            v2[i] = v1[i] - cst * w[i];
+        v1[i] = va;
+    }
+    free(w);
+}
+
+static void chpot(int n, const double *vp, const double *dm, double e, double *ve)
+{
+    for (int i = 0; i < n; ++i) ve[i] = vp[i] - dm[i] * e; /* chpot.f:58-62 */
+}
+
+/* ppsod.f:1-548: the pulse-driven FD leapfrog.  CST = 2 DT/SHFS; start-up
+ * with a quarter-strength step at the field of (T+DT)/2 and a half-
+ * strength step at T+DT (ppsod.f:246-254), psi(t-dt) reset to psi0
+ * (ppsod.f:348-351), then NINT((TF+DT/2-T)/DT) full steps with T advanced
+ * by DT and the field of each step's end time (ppsod.f:355-362).  C(t)
+ * samples: sample 0 is C(TI), then one every LC steps counting the start-
+ * up as step 1 (the reference's storage when PRTCRL prints).  nrm the
+ * norm <psi|psi> at each sample; veff the last VEFF. */
+int orc_ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
+              const double *shm, double *u, double *v, const double *vp, const double *dm,
+              int efc, double e0, double t0, double td, double tp, double omg, double sni,
+              int kl, int absorb, const double *vabc, double *veff, double *cr, double *ci,
+              double *nrm, int maxsamp)
+{
+    int n = ntot_of(ndim, np);
+    double cst = 2.0 * dt / ORC_SHFS;
+    double *u0 = malloc(sizeof(double) * 5 * (size_t)n), *v0 = u0 + n, *u1 = u0 + 2 * n,
+           *v1 = u0 + 3 * n, *ve = u0 + 4 * n;
+    memcpy(u0, u, sizeof(double) * n);
+    memcpy(v0, v, sizeof(double) * n);
+    memcpy(u1, u, sizeof(double) * n);
+    memcpy(v1, v, sizeof(double) * n);
+    memcpy(ve, vp, sizeof(double) * n);
+    int nc = 0, mc;
+    if (maxsamp > 0) {
+        corr(n, u0, v0, u, v, &cr[0], &ci[0]);
+        nrm[0] = orc_fcorrel(n, u, u) + orc_fcorrel(n, v, v);
+    }
+    double t = ti;
+    chpot(n, vp, dm, orc_ef(efc, e0, t0, td, tp, omg, sni, kl, (t + dt) / 2.0), ve);
+    orc_psodaux(ndim, np, cst / 4.0, shm, u1, u, v1, v, ve, absorb, vabc);
+    chpot(n, vp, dm, orc_ef(efc, e0, t0, td, tp, omg, sni, kl, t + dt), ve);
+    orc_psodaux(ndim, np, cst / 2.0, shm, u1, u, v1, v, ve, absorb, vabc);
+    mc = 1;
+    if (mc == lc) {
+        mc = 0;
+        if (++nc < maxsamp) {
+            corr(n, u0, v0, u, v, &cr[nc], &ci[nc]);
+            nrm[nc] = orc_fcorrel(n, u, u) + orc_fcorrel(n, v, v);
+        }
+    }
+    memcpy(u1, u0, sizeof(double) * n);
+    memcpy(v1, v0, sizeof(double) * n);
+    t = ti + dt;
+    int nf = (int)lround((tf + dt / 2.0 - t) / dt);
+    for (int k = 1; k <= nf; ++k) {
+        t = t + dt;
+        chpot(n, vp, dm, orc_ef(efc, e0, t0, td, tp, omg, sni, kl, t), ve);
+        orc_psodaux(ndim, np, cst, shm, u1, u, v1, v, ve, absorb, vabc);
+        if (++mc == lc) {
+            mc = 0;
+            if (++nc < maxsamp) {
+                corr(n, u0, v0, u, v, &cr[nc], &ci[nc]);
+                nrm[nc] = orc_fcorrel(n, u, u) + orc_fcorrel(n, v, v);
+            }
+        }
+    }
+    if (veff) memcpy(veff, ve, sizeof(double) * n);
+    free(u0);
+    return nc + 1;
+}

@@ -210,4 +210,17 @@ void orc_spectrumtd(int iwin, int mf, double dt, double tf, double wp,
 #ifdef __cplusplus
 }
 #endif
+
+/* PSODAUX (sod.f:3-122): one FD leapfrog step, psi(t+dt) = psi(t-dt)
+ * - i cst H psi(t); absorb ORC_ABS_* with vabc. */
+void orc_psodaux(int ndim, const int *np, double cst, const double *shm, double *u1,
+                 double *u2, double *v1, double *v2, const double *vpot, int absorb,
+                 const double *vabc);
+/* PPSOD (ppsod.f:1-548) time loop: returns the number of C(t) samples
+ * (sample 0 = C(TI), then every LC steps); u, v receive psi(TF). */
+int orc_ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
+              const double *shm, double *u, double *v, const double *vp, const double *dm,
+              int efc, double e0, double t0, double td, double tp, double omg, double sni,
+              int kl, int absorb, const double *vabc, double *veff, double *cr, double *ci,
+              double *nrm, int maxsamp);
 #endif

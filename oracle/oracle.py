@@ -98,6 +98,10 @@ class Oracle:
         L.orc_plnz_abs.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d,
                                    _d, _d, _d, _d, _ip, _ip]
         L.orc_plnz_abs.restype = _ip
+        L.orc_psodaux.argtypes = [_ip, _i, _dp, _d, _d, _d, _d, _d, _d, _ip, _d]
+        L.orc_ppsod.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _ip, _dp, _dp,
+                                _dp, _dp, _dp, _dp, _ip, _ip, _d, _d, _d, _d, _d, _ip]
+        L.orc_ppsod.restype = _ip
         L.orc_absorbingbc.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d, C.POINTER(_dp)]
         L.orc_absorbingbc.restype = _ip
         L.orc_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
@@ -278,6 +282,29 @@ class Oracle:
             raise RuntimeError("diagonalization error")
         return dict(cr=cr[:k], ci=ci[:k], u=u, v=v, veff=ve)
 
+    def psodaux(self, np_, cst, shm, u1, u2, v1, v2, vpot, absorb=ABS_NONE, vabc=None):
+        """PSODAUX (sod.f:3-122): one FD leapfrog step; returns (u1, u2, v1, v2)."""
+        u1, u2, v1, v2 = (_f64(a).copy() for a in (u1, u2, v1, v2))
+        va = np.zeros(u1.size) if vabc is None else _f64(vabc)
+        self.lib.orc_psodaux(len(np_), _np3(np_), cst, _v3(shm, 0.0), u1, u2, v1, v2,
+                             _f64(vpot), absorb, va)
+        return u1, u2, v1, v2
+
+    def ppsod(self, np_, dt, ti, tf, lc, shm, u, v, vp, dm, pulse, absorb=ABS_NONE, vabc=None):
+        """PPSOD (ppsod.f:1-548): dict(cr, ci, norm, u, v, veff); sample 0 is
+        C(TI), then one every LC steps (the start-up counting as one)."""
+        n = int(np.prod(np_))
+        nf = int(np.floor((tf + dt / 2 - (ti + dt)) / dt + 0.5))
+        ns = (nf + 1) // max(lc, 1) + 2
+        cr, ci, nr, ve = np.zeros(ns), np.zeros(ns), np.zeros(ns), np.zeros(n)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        va = np.zeros(n) if vabc is None else _f64(vabc)
+        efc, e0, t0, td, tp, omg, sni, kl = pulse
+        k = self.lib.orc_ppsod(len(np_), _np3(np_), dt, ti, tf, lc, _v3(shm, 0.0), u, v,
+                               _f64(vp), _f64(dm), efc, e0, t0, td, tp, omg, sni, kl, absorb,
+                               va, ve, cr, ci, nr, ns)
+        return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v, veff=ve)
+
     def plnz_step(self, np_, mp, abstol, cst, shm, u, v, vpot, mode=REF):
         u, v = _f64(u).copy(), _f64(v).copy()
         info = self.lib.orc_plnz_step(len(np_), _np3(np_), mp, abstol, cst,
@@ -385,6 +412,8 @@ class Reference:
                                _d, _d, _ip, P(_ip)]
         L.ref_plnz_smoothw.argtypes = [_ip, _i, _ip, _dp, _dp, _dp, _dp, _ip, _d, _d, _d, _d,
                                        _d, _d, _d, _ip, P(_ip)]
+        L.ref_ppsod.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _ip, _dp, _dp,
+                                _dp, _dp, _dp, _dp, _ip, _ip, _d, _d, _d, _d, _ip, P(_ip)]
         L.ref_fft.argtypes = [_d, _d, _ip, _ip, _ip, _ip]
         L.ref_fftnt.argtypes = [_d, _d, _ip, _ip, _ip]
         L.ref_digt.argtypes = [_ip, _i, _d, _d, _d, P(_dp)]
@@ -461,6 +490,22 @@ class Reference:
                                   _v3(shm, 0.0), u, v, _f64(vpot), _f64(mask), cr, ci, nsamp,
                                   C.byref(info))
         return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
+
+    def ppsod(self, np_, dt, ti, tf, lc, shm, u, v, vp, dm, pulse, absorb=0, vabc=None,
+              nsamp=None):
+        n = int(np.prod(np_))
+        if nsamp is None:
+            nf = int(np.floor((tf + dt / 2 - (ti + dt)) / dt + 0.5))
+            nsamp = (nf + 1) // max(lc, 1) + 1
+        cr, ci, ve = np.zeros(nsamp), np.zeros(nsamp), np.zeros(n)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        va = np.ones(n) if vabc is None else _f64(vabc)
+        info = _ip(0)
+        efc, e0, t0, td, tp, omg, sni, kl = pulse
+        self.lib.ref_ppsod(len(np_), _np3(np_), dt, ti, tf, lc, _v3(shm, 0.0), u, v, _f64(vp),
+                           _f64(dm), efc, e0, t0, td, tp, omg, sni, kl, absorb, va, ve, cr, ci,
+                           nsamp, C.byref(info))
+        return dict(cr=cr, ci=ci, u=u, v=v, veff=ve, info=info.value)
 
     def ef(self, efc, e0, t0, td, tp, omg, sni, kl, t):
         return self.lib.ref_ef(efc, e0, t0, td, tp, omg, sni, kl, t)

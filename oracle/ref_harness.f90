@@ -295,6 +295,53 @@ contains
     deallocate(iwork, vabc, lnzvc, wu, wv, wp, work, vpot)
   end subroutine ref_pplnz
 
+  ! PPSOD (ppsod.f:1-548) with PRTCRL='.PARTIAL' (the reference stores its
+  ! C(t) samples only when it prints), no projections (NPR = 0); absorb 0
+  ! none, 1 .VOPTIC, 2 .SMOOTHW with vabc.  u, v receive U2, V2 (psi(TF)),
+  ! veff the last VEFF.
+  subroutine ref_ppsod(ndim, np, dt, ti, tf, lc, shm, u, v, vp, dm, efc, e0, t0, td, tp, &
+                       omg, sni, kl, absorb, vabcin, veff, cr, ci, nsamp, info) &
+      bind(C, name='ref_ppsod')
+    integer(c_int), value :: ndim, lc, nsamp, efc, kl, absorb
+    integer(c_int) :: np(3)
+    real(c_double), value :: dt, ti, tf, e0, t0, td, tp, omg, sni
+    real(c_double) :: shm(3), u(*), v(*), vp(*), dm(*), vabcin(*), veff(*), cr(*), ci(*)
+    integer(c_int) :: info
+    character(len=8) :: dims, efcs, tpab
+    integer :: n, mxdct, i, nd(3)
+    real(8), allocatable :: vabc(:), lnzvc(:,:), u1(:), u2(:), v1(:), v2(:), work(:), vpot(:)
+    real(8) :: xi(3), xf(3), xp(3), sh(3), var(4), range(5, 7)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    mxdct = max(n, nsamp + 2)
+    allocate(vabc(n), lnzvc(mxdct, 4), u1(n), u2(n), v1(n), v2(n), work(n), vpot(n))
+    lnzvc = 0d0
+    vabc = 1d0
+    if (absorb /= 0) vabc = vabcin(1:n)
+    tpab = '.NONE   '
+    if (absorb == 1) tpab = '.VOPTIC '
+    if (absorb == 2) tpab = '.SMOOTHW'
+    vpot = vp(1:n)
+    u1 = u(1:n); v1 = v(1:n); u2 = 0d0; v2 = 0d0
+    xi = 0d0; xf = 0d0; xp = 0d0; sh = 0d0; var = 0d0; nd = 0; range = 0d0
+    call dimstr(ndim, dims)
+    call efcstr(efc, efcs)
+    call PPSOD(absorb /= 0, .false., .false., .false., dims, efcs, '.PARTIAL', tpab, info, &
+               lc, 2, 4, 3, mxdct, n, 1000000, 0, 0, 1, e0, t0, td, tp, omg, sni, kl, dt, &
+               ti, tf, 1d30, np, nd, sh, shm, u1, u2, v1, v2, xi, xp, xf, dm, vpot, vabc, &
+               work, var, lnzvc, .false., 0, range, .false.)
+    do i = 1, nsamp
+      cr(i) = lnzvc(i, 3)
+      ci(i) = lnzvc(i, 4)
+    end do
+    u(1:n) = u2
+    v(1:n) = v2
+    veff(1:n) = vpot(1:n)
+    deallocate(vabc, lnzvc, u1, u2, v1, v2, work, vpot)
+  end subroutine ref_ppsod
+
   ! INIT_COND (init_cond.f): mode 0 .CALC, 1 .GETR, 2 .GETR2, 3 .GETC, 4 .GETC2.
   subroutine ref_init_cond(ndim, np, mode, xi, sh, x0, ra, rk, u, v) &
       bind(C, name='ref_init_cond')
