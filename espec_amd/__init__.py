@@ -32,6 +32,9 @@ PULSE_NONE, PULSE_GAUS, PULSE_GGAUS, PULSE_SIN2 = 0, 1, 2, 3
 INIT_CALC, INIT_GETR, INIT_GETR2, INIT_GETC, INIT_GETC2 = 0, 1, 2, 3, 4
 # Absorber shapes of ABSORBINGBC (absorb.f).
 ABG_SMOOTHW, ABG_SMRS, ABG_VOPTIC_LINEAR, ABG_VOPTIC_KOSLOFF, ABG_VOPTIC_POLIN = 1, 2, 3, 4, 5
+# Return codes of the C ABI (INFO of the drop-ins).
+ESPEC_OK, ESPEC_ERR_ARG, ESPEC_ERR_HIP, ESPEC_ERR_NOMEM = 0, 2, 3, 4
+ESPEC_ERR_DIAG, ESPEC_ERR_UNSUPPORTED = 5, 6
 SHFS = 2.4188843243e-2  # fs per atomic unit of time (pspofft.f:76)
 AMU = 1836.152667539    # electron masses per amu (espec.f:614)
 A0A = 0.5291772083      # angstrom per bohr (espec.f:67)
@@ -91,6 +94,8 @@ def _load():
     L.espec_pspofftaux.argtypes = [ip, i, dp, dp, dp, d, d, d, d, ip, vp, ip]
     L.espec_psodfftaux.argtypes = [ip, i, dp, dp, d, d, d, d, d, d, ip, vp, ip]
     L.espec_reort.argtypes = [ip, ip, ip, d, d]
+    L.espec_lanczsg.argtypes = [ip, i, ip, ip, ip, dp, d, d, d, d, vp, ip, P(ip), ip]
+    L.espec_mtrxdiag.argtypes = [ip, i, ip, ip, dp, d, d, d, d, ip, P(ip)]
     L.espec_ppsod.argtypes = [ip, i, dp, dp, dp, ip, d, d, d, d, d, ip, dp, dp, dp, dp, dp, dp,
                               ip, ip, vp, ip, vp, vp, vp, vp, d, d, vp, vp, vp, vp, ip, P(ip),
                               ip]
@@ -360,6 +365,36 @@ def reort(basis, w):
     m, n = basis.shape
     _check(_load().espec_reort(n, m, n, w, basis), "REORT")
     return w
+
+
+def lanczsg(np_, il, iu, maxint, abstol, shm, vpot, flags=REF, logcap=4096):
+    """LANCZSG (lanczsc.f:1-153) on the device: (eigvl, eigvc[ns][n], log
+    [(state, restart, energy, change)]).  Raises EspecError with INFO=5
+    where the reference stops without convergence."""
+    n = int(np.prod(np_))
+    ns = iu - il + 1
+    ev, vc = np.zeros(ns), np.zeros(ns * n)
+    lg = np.zeros(4 * logcap)
+    nl = C.c_int(0)
+    rc = _load().espec_lanczsg(len(np_), _np3(np_), il, iu, maxint, abstol, _v3(shm, 0.0),
+                               _f64(vpot), ev, vc, lg.ctypes.data, logcap, C.byref(nl), flags)
+    log = lg[:4 * min(nl.value, logcap)].reshape(-1, 4)
+    if rc:
+        err = EspecError(rc, "LANCZSG")
+        err.log = log
+        raise err
+    return ev, vc.reshape(ns, n), log
+
+
+def mtrxdiag(np_, il, iu, abstol, shm, vpot):
+    """MTRXDIAG (mtrxdiag.f:1-89) on the device: (eigvl, eigvc[ns][n])."""
+    n = int(np.prod(np_))
+    ns = iu - il + 1
+    ev, vc = np.zeros(ns), np.zeros(ns * n)
+    m = C.c_int(0)
+    _check(_load().espec_mtrxdiag(len(np_), _np3(np_), il, iu, abstol, _v3(shm, 0.0), _f64(vpot),
+                                  ev, vc, n, C.byref(m)), "MTRXDIAG")
+    return ev[:m.value], vc.reshape(ns, n)[:m.value]
 
 
 def fftnt(x, y, m, itype, flags=REF):

@@ -749,6 +749,72 @@ static __device__ __align__(256) unsigned char scratch[kScrBytes];
 __global__ void scratch_clear_ticket(unsigned *ticket) { *ticket = 0u; }
 __global__ void scratch_zero_out(double *out) { *out = 0.0; }
 
+// LANCZSG's Ritz vector (lanczsc.f:100-106): out_J = sum_I z_I v_I(J) over
+// the m stored Lanczos vectors, in the reference's order.
+__global__ __launch_bounds__(256) void ritz_vector(int n, int m, const double *basis,
+                                                   const double *z, double *out)
+{
+#pragma clang fp contract(off)
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+        double s = 0.0;
+        for (int i = 0; i < m; ++i) s = s + z[i] * basis[(long long)i * n + j];
+        out[j] = s;
+    }
+}
+
+// sum |a - s b| with s = sign <a, b> in one workgroup (fixed order):
+// LANCZSG's convergence measure with the sign of b matched to a.
+__global__ __launch_bounds__(1024) void diff_sum(int n, const double *a, const double *b,
+                                                 double *out)
+{
+    __shared__ double red[3][16];
+    double d0 = 0, d1 = 0, ab = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        d0 += fabs(a[i] - b[i]);
+        d1 += fabs(a[i] + b[i]);
+        ab += a[i] * b[i];
+    }
+    d0 = wave_sum(d0);
+    d1 = wave_sum(d1);
+    ab = wave_sum(ab);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = d0;
+        red[1][w] = d1;
+        red[2][w] = ab;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, t = 0;
+        for (int k = 0; k < 16; ++k) {
+            s0 += red[0][k];
+            s1 += red[1][k];
+            t += red[2][k];
+        }
+        out[0] = t >= 0 ? s0 : s1;
+        out[1] = t;
+    }
+}
+
+// a <- a / |a| (ECNORM, ecnorm.f) in one workgroup.
+__global__ __launch_bounds__(1024) void normalise(int n, double *a)
+{
+    __shared__ double red[16];
+    __shared__ double nrm;
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += a[i] * a[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < 16; ++k) t += red[k];
+        nrm = sqrt(t);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) a[i] = a[i] / nrm;
+}
+
 // DSTEVX('V','A') for one tridiagonal: QL vectors, QL or bisection values.
 __global__ __launch_bounds__(64) void dstevx_kernel(int m, double *d, double *e, double *e2,
                                                     double abstol, double *w, double *z,
@@ -2183,6 +2249,125 @@ int espec_dstevx(int m, const double *d, const double *e, double abstol, double 
     ESPEC_HIP_TRY(hipMemcpy(w, dw.p, sizeof(double) * m, hipMemcpyDeviceToHost));
     if (z) ESPEC_HIP_TRY(hipMemcpy(z, dz.p, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost));
     return info ? ESPEC_ERR_DIAG : ESPEC_OK;
+}
+
+int espec_lanczsg(int ndim, const int *np, int il, int iu, int maxint, double abstol,
+                  const double *shm, const double *vpot, double *eigvl, double *eigvc,
+                  double *log, int logcap, int *nlog, int flags)
+{
+    LnzArgs l{};
+    if (!vpot || !eigvl || !eigvc || maxint < 1 || !set_grid(l, ndim, np, shm))
+        return ESPEC_ERR_ARG;
+    const int m = 40;  // lanczsc.f:79 sets NREORT = 40 whatever the caller asked
+    const int n = l.n, ns = iu - il + 1;
+    if (n <= m || il < 1 || iu > m || ns < 1 || ns > 64) return ESPEC_ERR_ARG;
+    const bool ref = (flags & ESPEC_REF) != 0;
+    DevBuf dv, dvi, daux, dcoef, de2, dw, dwork, dinfo, dsum;
+    ESPEC_HIP_TRY(dv.upload(vpot, sizeof(double) * n));
+    ESPEC_HIP_TRY(dvi.alloc(sizeof(double) * n));
+    ESPEC_HIP_TRY(daux.alloc(sizeof(double) * n));
+    ESPEC_HIP_TRY(dcoef.alloc(sizeof(double) * (size_t)ns * m));
+    ESPEC_HIP_TRY(de2.alloc(sizeof(double) * m));
+    ESPEC_HIP_TRY(dw.alloc(sizeof(double) * ns));
+    ESPEC_HIP_TRY(dwork.alloc(sizeof(double) * (size_t)ns * 6 * m));
+    ESPEC_HIP_TRY(dinfo.alloc(sizeof(int)));
+    ESPEC_HIP_TRY(dsum.alloc(sizeof(double) * 2));
+    l.vpot = dv.as<double>();
+    l.vshift = 0;
+    l.nvec = 1;
+    l.m = m;
+    l.nb = blocks_for(n);
+    l.ncols = m;
+    l.nchunk = (m + kChunk - 1) / kChunk;
+    l.nreort = m;
+    LnzBufs bufs;
+    ESPEC_HIP_TRY(bufs.alloc(l, sizeof(double)));
+    // RAN with NSEED = 1 (lanczsc.f:78-82): the first N values are drawn
+    // and overwritten, each state then starts from the next N.
+    long long seed = 1;
+    auto ran = [&]() {
+        seed = (seed * 1029 + 221591) % 1048576;
+        return (double)seed / 1048576.0;
+    };
+    std::vector<double> h(n), wstate(ns);
+    for (int i = 0; i < n; ++i) (void)ran();
+    int nl = 0;
+    for (int k = 0; k < ns; ++k) {
+        for (int i = 0; i < n; ++i) h[i] = ran();
+        ESPEC_HIP_TRY(hipMemcpy(dvi.p, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        int kk = 0;
+        double sum = 0;
+        do {
+            ++kk;
+            ESPEC_HIP_TRY(hipMemcpy(daux.p, dvi.p, sizeof(double) * n, hipMemcpyDeviceToDevice));
+            // LNZ(M = NREORT = 40) from VI (lanczsc.f:96), DSTEVX('V','I') (:98)
+            hipLaunchKernelGGL((lnz_init<double, 256>), dim3(l.nb, 1), dim3(256), 0, nullptr, l,
+                               (const double2 *)nullptr, dvi.as<double>());
+            ESPEC_HIP_TRY(lnz_iterations<double>(l, nullptr));
+            ESPEC_HIP_TRY(hipMemset(dinfo.p, 0, sizeof(int)));
+            hipLaunchKernelGGL(lanczs_select, dim3(1), dim3(64), 0, nullptr, m, l.al, l.bt,
+                               de2.as<double>(), il, ns, abstol, dw.as<double>(),
+                               dcoef.as<double>(), dwork.as<double>(), dinfo.as<int>());
+            // the Ritz vector of state k (lanczsc.f:100-106)
+            hipLaunchKernelGGL(ritz_vector, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0,
+                               nullptr, n, m, bufs.basis.as<double>(),
+                               dcoef.as<double>() + (size_t)k * m, dvi.as<double>());
+            ESPEC_HIP_TRY(hipGetLastError());
+            int info = 0;
+            ESPEC_HIP_TRY(hipMemcpy(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost));
+            if (info) return ESPEC_ERR_DIAG;
+            if (ref) {
+                // lanczsc.f:104-109 sums ABS(VI(I) - VIAUX(I)) with I = NREORT + 1,
+                // the index the finished inner loop leaves: VI(41) is still zero
+                // for J <= 40 and final afterwards.
+                double a41 = 0, v41 = 0;
+                ESPEC_HIP_TRY(hipMemcpy(&a41, daux.as<double>() + m, sizeof(double),
+                                        hipMemcpyDeviceToHost));
+                ESPEC_HIP_TRY(hipMemcpy(&v41, dvi.as<double>() + m, sizeof(double),
+                                        hipMemcpyDeviceToHost));
+                sum = 0;
+                for (int j = 1; j <= n; ++j) sum = sum + std::fabs((j <= m ? 0.0 : v41) - a41);
+            } else {
+                hipLaunchKernelGGL(diff_sum, dim3(1), dim3(1024), 0, nullptr, n, dvi.as<double>(),
+                                   daux.as<double>(), dsum.as<double>());
+                ESPEC_HIP_TRY(hipGetLastError());
+                ESPEC_HIP_TRY(hipMemcpy(&sum, dsum.p, sizeof(double), hipMemcpyDeviceToHost));
+            }
+            double wk = 0;
+            ESPEC_HIP_TRY(hipMemcpy(&wk, dw.as<double>() + k, sizeof(double),
+                                    hipMemcpyDeviceToHost));
+            wstate[k] = wk;
+            if (log && nl < logcap) {
+                double *r = log + 4 * (size_t)nl;
+                r[0] = k + 1;
+                r[1] = kk;
+                r[2] = wk;
+                r[3] = sum;
+            }
+            ++nl;
+        } while (sum > abstol && kk <= maxint);
+        if (nlog) *nlog = nl;
+        if (kk >= maxint) return ESPEC_ERR_DIAG;  // "didn't achieve the convergence"
+        // EIGVC(:,K) = VI / ECNORM(N, VI) (lanczsc.f:128-131)
+        hipLaunchKernelGGL(normalise, dim3(1), dim3(1024), 0, nullptr, n, dvi.as<double>());
+        ESPEC_HIP_TRY(hipGetLastError());
+        ESPEC_HIP_TRY(hipMemcpy(eigvc + (size_t)k * n, dvi.p, sizeof(double) * n,
+                                hipMemcpyDeviceToHost));
+    }
+    if (nlog) *nlog = nl;
+    if (!ref) {
+        // each state's value from the restart that converged it
+        for (int k = 0; k < ns; ++k) eigvl[k] = wstate[k];
+        return ESPEC_OK;
+    }
+    // EIGVL as the last DSTEVX left it, and DSTEVX writes EIGVC(1:40, 1:M)
+    // on every restart, so the columns of earlier states end with rows 1..40
+    // replaced by the last restart's tridiagonal eigenvectors.
+    ESPEC_HIP_TRY(hipMemcpy(eigvl, dw.p, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    for (int k = 0; k + 1 < ns; ++k)
+        ESPEC_HIP_TRY(hipMemcpy(eigvc + (size_t)k * n, dcoef.as<double>() + (size_t)k * m,
+                                sizeof(double) * m, hipMemcpyDeviceToHost));
+    return ESPEC_OK;
 }
 
 int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort, double abstol,

@@ -7,7 +7,8 @@ module espec_capi
   implicit none
 
   integer, parameter :: ESPEC_FP64 = 0, ESPEC_MIXED = 1, ESPEC_EXACT = 0, ESPEC_REF = 2
-  integer, parameter :: ESPEC_OK = 0, ESPEC_ERR_ARG = 2, ESPEC_ERR_UNSUPPORTED = 6
+  integer, parameter :: ESPEC_OK = 0, ESPEC_ERR_ARG = 2, ESPEC_ERR_DIAG = 5, &
+                        ESPEC_ERR_UNSUPPORTED = 6
   integer, parameter :: ESPEC_ABSORB_NONE = 0, ESPEC_ABSORB_VOPTIC = 1, ESPEC_ABSORB_SMOOTHW = 2
   integer, parameter :: ESPEC_ABG_SMOOTHW = 1, ESPEC_ABG_SMRS = 2, ESPEC_ABG_VOPTIC_LINEAR = 3, &
                         ESPEC_ABG_VOPTIC_KOSLOFF = 4, ESPEC_ABG_VOPTIC_POLIN = 5
@@ -306,6 +307,26 @@ module espec_capi
       import :: c_int, c_double
       integer(c_int), value :: ndim, il, iu, nreort
       integer(c_int) :: np(*)
+      real(c_double), value :: abstol
+      real(c_double) :: shm(*), vpot(*), eigvl(*), eigvc(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_lanczsg(ndim, np, il, iu, maxint, abstol, shm, vpot, eigvl, eigvc, &
+                           log, logcap, nlog, flags) bind(C, name='espec_lanczsg') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, il, iu, maxint, logcap, flags
+      integer(c_int) :: np(*), nlog
+      real(c_double), value :: abstol
+      real(c_double) :: shm(*), vpot(*), eigvl(*), eigvc(*), log(*)
+      integer(c_int) :: rc
+    end function
+
+    function espec_mtrxdiag(ndim, np, il, iu, abstol, shm, vpot, eigvl, eigvc, ldz, m) &
+        bind(C, name='espec_mtrxdiag') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, il, iu, ldz
+      integer(c_int) :: np(*), m
       real(c_double), value :: abstol
       real(c_double) :: shm(*), vpot(*), eigvl(*), eigvc(*)
       integer(c_int) :: rc

@@ -563,6 +563,108 @@ subroutine LANCZS(CHANGE, DIM, IL, IU, INFO, NREORT, LMTREORT, N, NSEED, MXDCT, 
 1033 format(7X,I2,A2,1X,'state hasn`t been calculating correctly!')
 end subroutine LANCZS
 
+! lanczsc.f:1-153: restarted 40-step Lanczos for eigenstates IL..IU on the
+! device (NREORT is set to 40 as the reference does).  The per-restart lines
+! (FORMAT 1031-1033) are printed after the device returns; a state that does
+! not converge within MAXINT restarts stops the program as the reference does.
+! As at espec.f:913-915, LNZVC holds the eigenvectors on return and EIGVC
+! the eigenvector columns, whose rows 1..40 hold for earlier states what the
+! last restart's DSTEVX wrote in ESPEC_MODE=REF.
+subroutine LANCZSG(CHANGE, DIM, IL, IU, INFO, NREORT, LMTREORT, N, NSEED, MXDCT, ABSTOL, IWK, &
+                   MAXINT, NP, AL, BT, EIGVL, SHM, V0, VI, VPOT, WK1, WK2, VAR, LNZVC, EIGVC)
+  use espec_capi
+  implicit none
+  logical :: CHANGE
+  character(len=*) :: DIM
+  integer :: IL, IU, INFO, NREORT, LMTREORT, N, NSEED, MXDCT, MAXINT
+  real(8) :: ABSTOL
+  integer :: IWK(*), NP(*)
+  real(8) :: AL(*), BT(*), EIGVL(*), SHM(*), V0(*), VI(*), VPOT(*), WK1(*), WK2(*), VAR(*)
+  real(8) :: LNZVC(MXDCT, *), EIGVC(MXDCT, *)
+  integer :: ndim, ns, k, rc, nl, logcap, kk
+  real(8), allocatable :: vc(:), lg(:)
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  if (ndim == 0 .or. ndim == 3) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  if (NREORT > LMTREORT) then  ! lanczsc.f:69-75
+    write(*,*) 'Error NREORT > LMTREORT'
+    write(*,*) 'NREORT=', NREORT, '  LMTREORT=', LMTREORT
+    NREORT = LMTREORT
+    write(*,*) 'Changing NREORT to correct value!'
+    write(*,*) 'NREORT=', NREORT, '  LMTREORT=', LMTREORT
+  end if
+  NSEED = 1
+  NREORT = 40
+  ns = IU - IL + 1
+  logcap = ns * (MAXINT + 2)
+  allocate(vc(N * ns), lg(4 * logcap))
+  nl = 0
+  rc = espec_lanczsg(ndim, NP, IL, IU, MAXINT, ABSTOL, SHM, VPOT, EIGVL, vc, lg, logcap, nl, &
+                     mode_flags())
+  do k = 1, min(nl, logcap)
+    kk = nint(lg(4 * k - 2))
+    if (kk < 10) then
+      write(*,1031) kk, lg(4 * k - 1), lg(4 * k)
+    else if (kk < 100) then
+      write(*,1032) kk, lg(4 * k - 1), lg(4 * k)
+    else if (kk < 1000) then
+      write(*,1033) kk, lg(4 * k - 1), lg(4 * k)
+    else
+      write(*,*) '<<<>>> Too many interactions <<<>>>'
+      write(*,*) 'Stopping program!'
+      stop
+    end if
+  end do
+  if (rc == ESPEC_ERR_DIAG) then
+    write(*,*) '<<<>>> Eigenvector didn''t achieve the ', 'convergence <<<>>>'
+    write(*,*) 'Stopping eSPec!'
+    stop
+  else if (rc /= 0) then
+    write(*,*) '<<<>>> LANCZSG failed on the device, code ', rc
+    stop
+  end if
+  do k = 1, ns
+    EIGVC(1:N, k) = vc((k - 1) * N + 1:k * N)
+    LNZVC(1:N, k) = vc((k - 1) * N + 1:k * N)
+  end do
+  deallocate(vc, lg)
+1031 format(5X,'Interaction',1X,I1,';',1X,'Energy =',E16.8,'/a.u.; ','WF converg. =',E16.8,';')
+1032 format(5X,'Interaction',1X,I2,';',1X,'Energy =',E16.8,'/a.u.; ','WF converg. =',E16.8,';')
+1033 format(5X,'Interaction',1X,I3,';',1X,'Energy =',E16.8,'/a.u.; ','WF converg. =',E16.8,';')
+end subroutine LANCZSG
+
+! mtrxdiag.f:1-89: eigenpairs IL..IU of the dense FD Hamiltonian (the matrix
+! of ap1d.f / ap2d.f) by a device syevx: bisection to ABSTOL, inverse
+! iteration.  Eigenvectors agree with DSPEVX's up to sign.  DIM .2DCT and
+! .3D return INFO = ESPEC_ERR_UNSUPPORTED (the reference builds no matrix
+! for .3D); N above 16384 returns it too (the dense matrix alone is 2 GiB).
+subroutine MTRXDIAG(DIM, IL, IU, INFO, MXDCT, N, ABSTOL, IWORK, NP, EIGVL, SHM, VPOT, WORK, WK, &
+                    EIGVC)
+  use espec_capi
+  implicit none
+  character(len=*) :: DIM
+  integer :: IL, IU, INFO, MXDCT, N
+  real(8) :: ABSTOL
+  integer :: IWORK(*), NP(*)
+  real(8) :: EIGVL(*), SHM(*), VPOT(*), WORK(*), WK(*)
+  real(8) :: EIGVC(MXDCT, *)
+  integer :: ndim, m
+  INFO = 0
+  ndim = dim_of(DIM)
+  if (ndim == 0 .or. ndim == 3) then
+    write(*,'(A)') '<<<>>> Matrix generation error <<<>>>'
+    write(*,*) DIM
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  m = 0
+  INFO = espec_mtrxdiag(ndim, NP, IL, IU, ABSTOL, SHM, VPOT, EIGVL, EIGVC, MXDCT, m)
+end subroutine MTRXDIAG
+
 ! fft3d.f:1
 ! absorb.f:1-130: VABC of TPABSOR on the grid, on the device (N is not
 ! read; the grid is NP).  DIM and TPABSOR are matched on their leading
@@ -1059,6 +1161,7 @@ subroutine PPSOD(ABSORB, PRTVEFF, PRTEIGVC2, PRTPULS, DIM, EFC, PRTCRL, TPABSOR,
       write(*,1021) t, et(s + 1), et0(s + 1), cr(s + 1), ci(s + 1), (ct(j), j = 1, NPR + 1), &
                     nr(s + 1)
     end do
+    flush(6)
   end if
   deallocate(w, cr, ci, nr, et, et0, pr)
 1011 format(/,8X,'*t/fs*',5X,'*ET/a.u.*',6X,'*E0/a.u.*',6X,'*Pr(t)*',4X,'*Pi(t)*',3X, &

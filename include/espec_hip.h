@@ -343,6 +343,33 @@ int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort,
                  double abstol, const double *shm, const double *vpot,
                  double *eigvl, double *eigvc);
 
+/* Replaces LANCZSG (lanczsc.f:1-153): states il..iu one at a time by
+ * restarted Lanczos: from a RAN vector (NSEED = 1, the first N values
+ * discarded), a 40-step recurrence with full Gram-Schmidt (lanczsc.f sets
+ * NREORT = 40), DSTEVX('V','I') eigenpairs il..iu of the tridiagonal
+ * (bisection to abstol), the Ritz vector of the state as the next start,
+ * until the change of the Ritz vector is below abstol, for at most maxint
+ * restarts.  ESPEC_REF measures the change as the reference does (its sum
+ * reads the 41st component for every J, lanczsc.f:104-109, and rarely
+ * falls below abstol); ESPEC_EXACT sums |VI - VIAUX| over the grid with the
+ * sign of the previous vector matched.  eigvl[iu-il+1] are the Ritz values
+ * of the last restart (as DSTEVX leaves EIGVL); eigvc [N][iu-il+1] the
+ * normalised vectors.  log (may be NULL): logcap records (state, restart,
+ * energy, change) as the reference prints them.  ESPEC_ERR_DIAG where the
+ * reference stops after maxint restarts.  Runs on the device. */
+int espec_lanczsg(int ndim, const int *np, int il, int iu, int maxint, double abstol,
+                  const double *shm, const double *vpot, double *eigvl, double *eigvc,
+                  double *log, int logcap, int *nlog, int flags);
+
+/* Replaces MTRXDIAG (mtrxdiag.f:1-89): eigenpairs il..iu of the dense FD
+ * Hamiltonian that AP1D/AP2D pack (ap1d.f, ap2d.f), by DSPEVX's method on
+ * the device: Householder tridiagonalisation, bisection to abstol, inverse
+ * iteration (rocSOLVER syevx on a matrix our kernel writes).  1-D and 2-D,
+ * N <= 16384.  eigvc column k at eigvc + k ldz; m receives the count. */
+int espec_mtrxdiag(int ndim, const int *np, int il, int iu, double abstol,
+                   const double *shm, const double *vpot, double *eigvl,
+                   double *eigvc, int ldz, int *m);
+
 /* Replaces FCORREL(N,U1,U2) (fcorrel.f:1): sum u1*u2 in FP64 on device. */
 int espec_fcorrel(int n, const double *u1, const double *u2, double *out);
 /* No counterpart in the reference: espec_fcorrel on device buffers that

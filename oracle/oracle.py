@@ -355,6 +355,57 @@ class Oracle:
         self.lib.orc_tridiag_bisect(m, _f64(d), _f64(np.append(e, 0.0)), abstol, w)
         return w
 
+    def lanczsg(self, np_, il, iu, maxint, abstol, shm, vpot, mode=REF):
+        """LANCZSG (lanczsc.f:1-153), composed from the restated LNZ and
+        bisection: per state a RAN start (NSEED = 1, the first N values
+        discarded), 40-step Lanczos with full Gram-Schmidt, the selected
+        eigenvalue by bisection, its tridiagonal eigenvector by inverse
+        iteration (largest component positive, dstein.f), the Ritz vector
+        as the next start.  REF measures the change with the 41st component
+        as lanczsc.f:104-109 does; EXACT sums |VI - s VIAUX| with s the sign
+        of <VI, VIAUX>.  Returns (info, eigvl, eigvc[ns][n], log)."""
+        n = int(np.prod(np_))
+        m, ns = 40, iu - il + 1
+        vals = self.ran(1, n * (ns + 1))[0]
+        log, vecs, wk = [], [], []
+        w = z_all = None
+        for k in range(ns):
+            vi = vals[(k + 1) * n:(k + 2) * n].copy()
+            kk = 0
+            while True:
+                kk += 1
+                aux = vi.copy()
+                al, bt, basis = self.lnz(np_, m, m, shm, vpot, vi)
+                w = self.tridiag_bisect(al, bt[:m - 1], abstol)[il - 1:iu]
+                # DSTEIN's vectors: unit length, largest component positive
+                t = np.diag(al) + np.diag(bt[:m - 1], 1) + np.diag(bt[:m - 1], -1)
+                z_all = np.linalg.eigh(t)[1][:, il - 1:iu].copy()
+                for c in range(ns):
+                    if z_all[np.argmax(np.abs(z_all[:, c])), c] < 0:
+                        z_all[:, c] = -z_all[:, c]
+                vi = basis.T @ z_all[:, k]
+                if mode == REF:
+                    ssum = 0.0
+                    for j in range(1, n + 1):
+                        ssum += abs((0.0 if j <= m else vi[m]) - aux[m])
+                else:
+                    sg = 1.0 if vi @ aux >= 0 else -1.0
+                    ssum = float(np.abs(vi - sg * aux).sum())
+                log.append((k + 1, kk, w[k], ssum))
+                if not (ssum > abstol and kk <= maxint):
+                    break
+            if kk >= maxint:
+                return 1, w, np.array(vecs), log
+            vecs.append(vi / np.linalg.norm(vi))
+            wk.append(w[k])
+        vecs = np.array(vecs)
+        if mode == REF:
+            # EIGVL and rows 1..40 of EIGVC as the last DSTEVX left them
+            # (it writes EIGVC(1:40, 1:M) after earlier columns were stored)
+            vecs[:ns - 1, :m] = z_all[:, :ns - 1].T
+            return 0, w, vecs, log
+        return 0, np.array(wk), vecs, log
+
     def ran(self, nseed, count):
         s = _ip(nseed)
         out = [self.lib.orc_ran(C.byref(s)) for _ in range(count)]
@@ -426,6 +477,8 @@ class Reference:
         L.ref_hwhm.argtypes = [_ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_spectrumtd.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp, _d, _d]
         L.ref_ran.argtypes = [P(_ip)]
+        L.ref_lanczsg.argtypes = [_ip, _i, _ip, _ip, _ip, _dp, _d, _d, _d, _d, _ip, P(_ip)]
+        L.ref_mtrxdiag.argtypes = [_ip, _i, _ip, _ip, _dp, _d, _d, _d, _d, _ip, P(_ip)]
         L.ref_absorbingbc.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d, P(_dp)]
         L.ref_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
         L.ref_ef.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
@@ -595,6 +648,27 @@ class Reference:
         s = _ip(nseed)
         out = [self.lib.ref_ran(C.byref(s)) for _ in range(count)]
         return np.array(out), s.value
+
+    def lanczsg(self, np_, il, iu, maxint, abstol, shm, vpot):
+        """LANCZSG as built; the reference stops the process when a state does
+        not converge, so callers pick cases that do (or fork first)."""
+        n = int(np.prod(np_))
+        ns = iu - il + 1
+        ev, vc = np.zeros(ns), np.zeros(n * ns)
+        info = _ip(0)
+        self.lib.ref_lanczsg(len(np_), _np3(np_), il, iu, maxint, abstol, _v3(shm, 0.0),
+                             _f64(vpot), ev, vc, n, C.byref(info))
+        return info.value, ev, vc.reshape(ns, n)
+
+    def mtrxdiag(self, np_, il, iu, abstol, shm, vpot):
+        n = int(np.prod(np_))
+        ns = iu - il + 1
+        assert ns <= 20  # EIGVC(MXDCT, 20) in mtrxdiag.f
+        ev, vc = np.zeros(ns), np.zeros(n * ns)
+        info = _ip(0)
+        self.lib.ref_mtrxdiag(len(np_), _np3(np_), il, iu, abstol, _v3(shm, 0.0), _f64(vpot),
+                              ev, vc, n, C.byref(info))
+        return info.value, ev, vc.reshape(ns, n)
 
     def lanczs(self, np_, il, iu, nreort, abstol, shm, vpot):
         n = int(np.prod(np_))

@@ -596,4 +596,73 @@ contains
     deallocate(iwk, al, bt, v0, vi, wk1, wk2, lnzvc, lanc, ev)
   end subroutine ref_lanczs
 
+  ! LANCZSG (lanczsc.f:1-153): restarted 40-step Lanczos, eigenstates il..iu.
+  ! The per-restart log goes to unit 6 (FORMAT 1031-1033), flushed on return.
+  ! The reference STOPs when a state does not converge within maxint.
+  subroutine ref_lanczsg(ndim, np, il, iu, maxint, abstol, shm, vpot, eigvl, &
+                         eigvc, ldv, info) bind(C, name='ref_lanczsg')
+    integer(c_int), value :: ndim, il, iu, maxint, ldv
+    integer(c_int) :: np(3)
+    real(c_double), value :: abstol
+    real(c_double) :: shm(3), vpot(*), eigvl(*), eigvc(ldv, *)
+    integer(c_int) :: info
+    character(len=8) :: dims
+    integer :: n, i, nseed, nr, inf
+    integer, allocatable :: iwk(:)
+    real(8), allocatable :: al(:), bt(:), v0(:), vi(:), wk1(:), wk2(:), &
+                            lnzvc(:,:), ev(:), zz(:,:)
+    real(8) :: var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    allocate(iwk(5*n + 256), al(n + 64), bt(n + 64), v0(n), vi(n + 64), wk1(5*n + 256), &
+             wk2(5*n + 256), lnzvc(ldv, max(40, iu - il + 1)), &
+             zz(ldv, max(40, iu - il + 1)), ev(n + 64))
+    var = 0d0
+    nseed = 1
+    nr = 40
+    inf = 0
+    call dimstr(ndim, dims)
+    ! espec.f:913-915 passes its EIGVC as LANCZSG's LNZVC (Krylov storage,
+    ! the eigenvectors on return) and LANCZ as LANCZSG's EIGVC.
+    call LANCZSG(.true., dims, il, iu, inf, nr, nr, n, nseed, ldv, abstol, iwk, &
+                 maxint, np, al, bt, ev, shm, v0, vi, vpot, wk1, wk2, var, lnzvc, zz)
+    flush(6)
+    info = inf
+    do i = 1, iu - il + 1
+      eigvl(i) = ev(i)
+      eigvc(1:n, i) = lnzvc(1:n, i)
+    end do
+    deallocate(iwk, al, bt, v0, vi, wk1, wk2, lnzvc, zz, ev)
+  end subroutine ref_lanczsg
+
+  ! MTRXDIAG (mtrxdiag.f:1-89): dense DSPEVX of the packed FD Hamiltonian.
+  subroutine ref_mtrxdiag(ndim, np, il, iu, abstol, shm, vpot, eigvl, eigvc, ldv, &
+                          info) bind(C, name='ref_mtrxdiag')
+    integer(c_int), value :: ndim, il, iu, ldv
+    integer(c_int) :: np(3)
+    real(c_double), value :: abstol
+    real(c_double) :: shm(3), vpot(*), eigvl(*), eigvc(ldv, *)
+    integer(c_int) :: info
+    character(len=8) :: dims
+    integer :: n, i, inf
+    integer, allocatable :: iwk(:)
+    real(8), allocatable :: work(:), wk(:), ev(:), z(:,:)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    allocate(iwk(5*n), work(8*n), wk(8*n), ev(n), z(ldv, 20))
+    inf = 0
+    call dimstr(ndim, dims)
+    call MTRXDIAG(dims, il, iu, inf, ldv, n, abstol, iwk, np, ev, shm, vpot, work, wk, z)
+    info = inf
+    do i = 1, iu - il + 1
+      eigvl(i) = ev(i)
+      eigvc(1:n, i) = z(1:n, i)
+    end do
+    deallocate(iwk, work, wk, ev, z)
+  end subroutine ref_mtrxdiag
+
 end module espec_ref_harness

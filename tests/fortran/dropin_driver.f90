@@ -1,9 +1,11 @@
-! Host program for the drop-in test: calls PSPOFFT / PSODFFT / PLNZ with the
-! argument lists eSPec's main program uses (espec.f:1373-1376, 1474-1477,
-! 1499-1502) and the LC/TF rule of espec.f:1324-1353, against whatever
-! library provides those symbols (here libespec_f.so).
+! Host program for the drop-in test: calls PSPOFFT / PSODFFT / PLNZ / PPLNZ /
+! LANCZSG / MTRXDIAG with the argument lists eSPec's main program uses
+! (espec.f:906-915, 1373-1376, 1474-1477, 1499-1502) and the LC/TF rule of
+! espec.f:1324-1353, against whatever library provides those symbols (here
+! libespec_f.so).
 !
-! stdin:  method ('PFSPO', 'PFSOD', 'PSIL'), DIM keyword, N
+! stdin:  method ('PFSPO', 'PFSOD', 'PSIL', 'PPSIL', 'LANCZSG', 'MTRXDIAG'),
+!         DIM keyword, N
 !         TI TF DT MF MP ABSTOL TOL
 !         NP(1:3) SH(1:3) SM(1:3) SHM(1:3)
 !         N lines: VPOT U V
@@ -29,6 +31,10 @@ program dropin_driver
   do i = 1, n
     read(*,*) vpot(i), u(i), v(i)
   end do
+  if (trim(method) == 'LANCZSG' .or. trim(method) == 'MTRXDIAG') then
+    call eigenstates()
+    stop
+  end if
   ! espec.f:1324-1353 with *CHANGE .YES: LC from 2^MF samples, TF adjusted
   do
     sc = 2.0d0 * (tf - ti) / dt
@@ -75,4 +81,35 @@ program dropin_driver
     write(*,'(A,I8,2ES26.17)') 'C ', i, lnzvc(i, mpr), lnzvc(i, mpi)
   end do
   write(*,'(A,I4)') 'INFO ', info
+
+contains
+
+  ! LANCZSG / MTRXDIAG with espec.f:906-915's argument lists; for these
+  ! methods MF is MAXINT and MP is IU (IL = 1).  Prints 'E ' K EIGVL(K) and
+  ! 'V ' K I EIGVC(I,K) (LANCZSG: the LNZVC dummy, espec's EIGVC).
+  subroutine eigenstates()
+    real(8), allocatable :: ev(:), egv(:, :), lz(:, :), al(:), bt(:)
+    integer :: k, j, nreort, nseed, ns
+    ns = mp
+    allocate(ev(max(n, 64)), egv(n, max(40, ns)), lz(n, max(40, ns)), al(64), bt(64))
+    egv = 0
+    lz = 0
+    var = 0
+    nreort = 40
+    nseed = 1
+    info = 0
+    if (trim(method) == 'LANCZSG') then
+      call LANCZSG(.true., dim, 1, ns, info, nreort, 40, n, nseed, n, abstol, iwork, mf, np, &
+                   al, bt, ev, shm, w1, w2, vpot, w3, w4, var, lz, egv)
+    else
+      call MTRXDIAG(dim, 1, ns, info, n, n, abstol, iwork, np, ev, shm, vpot, w1, w2, lz)
+    end if
+    do k = 1, ns
+      write(*,'(A,I4,ES26.17)') 'E ', k, ev(k)
+      do j = 1, n
+        write(*,'(A,I4,I8,ES26.17)') 'V ', k, j, lz(j, k)
+      end do
+    end do
+    write(*,'(A,I4)') 'INFO ', info
+  end subroutine eigenstates
 end program dropin_driver

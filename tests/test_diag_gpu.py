@@ -1,0 +1,209 @@
+"""Initial eigenstates on the device: LANCZSG (lanczsc.f) and MTRXDIAG
+(mtrxdiag.f), through the C ABI and through the Fortran drop-ins.
+
+LANCZSG in REF against the oracle (pinned to the reference build in
+tests/test_oracle_diag.py): the same restarts, the change measure of
+lanczsc.f:104-109, energies within the bisection tolerance ABSTOL and the
+same vectors; EXACT against the dense eigensolution.  MTRXDIAG against the
+reference's DSPEVX (where the reference build travelled) and the dense
+eigensolution; eigenvectors are compared up to sign, which neither method
+fixes the same way.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import espec_amd as E
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "fortran", "dropin_driver.x")
+
+
+def oscillator(n, half=8.0):
+    x = np.linspace(-half, half, n)
+    h = x[1] - x[0]
+    return [1.0 / (24 * h * h)], 0.5 * x * x
+
+
+def oscillator_2d(nx, ny, half=7.0, ky=1.3):
+    x, y = np.linspace(-half, half, nx), np.linspace(-half, half, ny)
+    xx, yy = np.meshgrid(x, y)
+    return ([1 / (24 * (x[1] - x[0]) ** 2), 1 / (24 * (y[1] - y[0]) ** 2)],
+            (0.5 * xx * xx + 0.5 * ky * yy * yy).ravel())
+
+
+def fd_dense(shape, shm, v):
+    n = len(v)
+    h = np.zeros((n, n))
+    nx = shape[0]
+    for j in range(n):
+        h[j, j] = 30 * sum(shm) + v[j]
+        for d, c in ((1, -16.0), (2, 1.0)):
+            if j % nx >= d:
+                h[j, j - d] = h[j - d, j] = c * shm[0]
+            if len(shape) == 2 and j >= d * nx:
+                h[j, j - d * nx] = h[j - d * nx, j] = c * shm[1]
+    return h
+
+
+def aligned(a, b):
+    return np.abs(a - b * np.sign(a @ b)).max()
+
+
+@pytest.mark.parametrize("n", [400, 600])
+def test_lanczsg_ref_matches_oracle(gpu, orc, n):
+    import oracle
+    shm, v = oscillator(n)
+    print(f"case: lanczsg REF n={n}", flush=True)
+    w, vc, log = E.lanczsg([n], 1, 1, 60, 1e-6, shm, v, E.REF)
+    info, ow, ovc, olog = orc.lanczsg([n], 1, 1, 60, 1e-6, shm, v, oracle.REF)
+    assert info == 0
+    assert len(log) == len(olog)
+    for r, o in zip(log, olog):
+        assert (int(r[0]), int(r[1])) == (o[0], o[1])
+        assert abs(r[2] - o[2]) <= 1e-6
+        assert abs(r[3] - o[3]) <= 1e-6 * abs(o[3]) + 1e-13
+    assert abs(w[0] - ow[0]) <= 1e-6
+    assert np.abs(vc[0] - ovc[0]).max() <= 1e-11
+
+
+def test_lanczsg_ref_2d_matches_oracle(gpu, orc):
+    import oracle
+    shm, v = oscillator_2d(48, 40)
+    print("case: lanczsg REF 48x40", flush=True)
+    w, vc, log = E.lanczsg([48, 40], 1, 1, 60, 1e-6, shm, v, E.REF)
+    info, ow, ovc, olog = orc.lanczsg([48, 40], 1, 1, 60, 1e-6, shm, v, oracle.REF)
+    assert info == 0 and len(log) == len(olog)
+    assert abs(w[0] - ow[0]) <= 1e-6
+    assert np.abs(vc[0] - ovc[0]).max() <= 1e-11
+
+
+def test_lanczsg_exact_is_the_ground_state(gpu):
+    n = 300
+    shm, v = oscillator(n, 7.0)
+    print("case: lanczsg EXACT n=300", flush=True)
+    w, vc, log = E.lanczsg([n], 1, 1, 400, 1e-7, shm, v, E.EXACT)
+    assert log[-1][3] <= 1e-7
+    e, z = np.linalg.eigh(fd_dense((n,), shm, v))
+    assert abs(w[0] - e[0]) <= 1e-7
+    assert aligned(vc[0], z[:, 0]) <= 1e-7
+    assert abs(np.linalg.norm(vc[0]) - 1) <= 1e-14
+
+
+def test_lanczsg_two_states_ref_layout(gpu, orc):
+    # IU = 2 in REF: EIGVL and rows 1..40 of the first column are what the
+    # last restart's DSTEVX wrote (lanczsc.f:98 writes EIGVC(1:40, 1:2))
+    import oracle
+    shm, v = oscillator_2d(40, 36)
+    print("case: lanczsg REF two states 40x36", flush=True)
+    info, ow, ovc, olog = orc.lanczsg([40, 36], 1, 2, 120, 1e-6, shm, v, oracle.REF)
+    if info:
+        with pytest.raises(E.EspecError) as ei:
+            E.lanczsg([40, 36], 1, 2, 120, 1e-6, shm, v, E.REF)
+        assert ei.value.code == E.ESPEC_ERR_DIAG and len(ei.value.log) == len(olog)
+        return
+    w, vc, log = E.lanczsg([40, 36], 1, 2, 120, 1e-6, shm, v, E.REF)
+    assert len(log) == len(olog)
+    assert np.abs(w - ow).max() <= 1e-6
+    assert np.abs(vc[0, :40] - ovc[0, :40]).max() <= 1e-9
+    assert np.abs(vc[1] - ovc[1]).max() <= 1e-9
+
+
+def test_lanczsg_stops_without_convergence(gpu):
+    shm, v = oscillator(400)
+    print("case: lanczsg maxint=2", flush=True)
+    with pytest.raises(E.EspecError) as ei:
+        E.lanczsg([400], 1, 1, 2, 1e-6, shm, v, E.REF)
+    assert ei.value.code == E.ESPEC_ERR_DIAG
+    assert len(ei.value.log) == 3  # restarts 1..MAXINT+1, as lanczsc.f:121-126
+
+
+@pytest.mark.parametrize("shape", [(300,), (18, 14), (64, 40)])
+def test_mtrxdiag_matches_dense_and_reference(gpu, shape):
+    rng = np.random.default_rng(11)
+    n = int(np.prod(shape))
+    shm = [0.7, 0.45][:len(shape)]
+    v = rng.uniform(0, 4, n)
+    print(f"case: mtrxdiag {shape}", flush=True)
+    ev, vc = E.mtrxdiag(list(shape), 1, 6, 1e-13, shm, v)
+    e, z = np.linalg.eigh(fd_dense(shape, shm, v))
+    assert np.abs(ev - e[:6]).max() <= 1e-11 * max(1.0, abs(e[5]))
+    for k in range(6):
+        assert aligned(vc[k], z[:, k]) <= 1e-9
+    try:
+        from oracle import Reference
+        R = Reference()
+    except OSError:
+        return
+    info, rev, rvc = R.mtrxdiag(list(shape), 1, 6, 1e-13, shm, v)
+    assert info == 0
+    assert np.abs(ev - rev).max() <= 1e-11 * max(1.0, abs(e[5]))
+    for k in range(6):
+        assert aligned(vc[k], rvc[k]) <= 1e-9
+
+
+def test_mtrxdiag_rejects_beyond_dense_limit(gpu):
+    with pytest.raises(E.EspecError) as ei:
+        E.mtrxdiag([129, 128], 1, 1, 0.0, [1.0, 1.0], np.zeros(129 * 128))
+    assert ei.value.code == E.ESPEC_ERR_UNSUPPORTED
+
+
+def driver(method, n, maxint, iu, abstol, shm, v, mode="REF"):
+    lines = [f"{method} .1D {n}", f"0.0 1.0 0.1 {maxint} {iu} {abstol:.17e} 3.0",
+             f"{n} 1 1 1.0 1.0 1.0 1.0 1.0 1.0 {shm[0]:.17e} 0.0 0.0"]
+    lines += [f"{x:.17e} 0.0 0.0" for x in v]
+    r = subprocess.run([DRIVER], input="\n".join(lines) + "\n", capture_output=True,
+                       timeout=240, env=dict(os.environ, ESPEC_MODE=mode))
+    out = r.stdout.decode("latin-1")
+    ev, vc, info = {}, {}, None
+    for ln in out.splitlines():
+        p = ln.split()
+        if ln.startswith("E "):
+            ev[int(p[1])] = float(p[2])
+        elif ln.startswith("V "):
+            vc.setdefault(int(p[1]), []).append(float(p[3]))
+        elif ln.startswith("INFO"):
+            info = int(p[1])
+    return r.returncode, out, ev, {k: np.array(x) for k, x in vc.items()}, info
+
+
+def test_dropin_lanczsg_prints_the_reference_log(gpu, orc):
+    import oracle
+    n = 400
+    shm, v = oscillator(n)
+    print("case: drop-in LANCZSG n=400", flush=True)
+    rc, out, ev, vc, info = driver("LANCZSG", n, 60, 1, 1e-6, shm, v)
+    assert rc == 0 and info == 0, out[-2000:]
+    _, ow, ovc, olog = orc.lanczsg([n], 1, 1, 60, 1e-6, shm, v, oracle.REF)
+    lines = [ln for ln in out.splitlines() if "Interaction" in ln]
+    assert len(lines) == len(olog)
+    assert lines[0].startswith("     Interaction 1; Energy =")
+    assert abs(ev[1] - ow[0]) <= 1e-6
+    assert np.abs(vc[1] - ovc[0]).max() <= 1e-11
+
+
+def test_dropin_lanczsg_stops_like_the_reference(gpu):
+    shm, v = oscillator(400)
+    print("case: drop-in LANCZSG maxint=2", flush=True)
+    rc, out, ev, vc, info = driver("LANCZSG", 400, 2, 1, 1e-6, shm, v)
+    assert "convergence" in out and "Stopping eSPec!" in out
+    assert info is None and not ev
+    assert len([ln for ln in out.splitlines() if "Interaction" in ln]) == 3
+
+
+def test_dropin_mtrxdiag(gpu):
+    rng = np.random.default_rng(2)
+    n = 250
+    shm = [0.8]
+    v = rng.uniform(0, 3, n)
+    print("case: drop-in MTRXDIAG n=250", flush=True)
+    rc, out, ev, vc, info = driver("MTRXDIAG", n, 1, 4, 1e-13, shm, v)
+    assert rc == 0 and info == 0, out[-2000:]
+    e, z = np.linalg.eigh(fd_dense((n,), shm, v))
+    for k in range(1, 5):
+        assert abs(ev[k] - e[k - 1]) <= 1e-11
+        assert aligned(vc[k], z[:, k - 1]) <= 1e-9

@@ -86,7 +86,9 @@ def test_ppsod_sample_energies_and_projections(gpu):
     rng = np.random.default_rng(4)
     w = rng.standard_normal((3, 96))
     dt, tf = 2e-5, 20 * 2e-5
-    g = E.ppsod(c["np_"], dt, 0.0, tf, 20, c["shm"], c["u"], c["v"], c["vp"], c["dm"], pulse(1),
+    # 21 steps (the start-up and NINT(19.5) = 20 more): the one sample after
+    # the first is at the last step, whose VEFF is the one returned
+    g = E.ppsod(c["np_"], dt, 0.0, tf, 21, c["shm"], c["u"], c["v"], c["vp"], c["dm"], pulse(1),
                 proj=w, energies=True)
     assert len(g["cr"]) == 2
     psi = g["u"] + 1j * g["v"]
