@@ -664,6 +664,8 @@ class Reference:
         n = int(np.prod(np_))
         ns = iu - il + 1
         assert ns <= 20  # EIGVC(MXDCT, 20) in mtrxdiag.f
+        # AP(N*(N+1)/2) is an automatic (stack) array in mtrxdiag.f
+        assert n * (n + 1) * 4 < 4 << 20, "packed matrix would overflow the stack"
         ev, vc = np.zeros(ns), np.zeros(n * ns)
         info = _ip(0)
         self.lib.ref_mtrxdiag(len(np_), _np3(np_), il, iu, abstol, _v3(shm, 0.0), _f64(vpot),
