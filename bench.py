@@ -1,29 +1,33 @@
 #!/usr/bin/env python3
 """Throughput of the split-operator FFT propagation on one MI355X.
 
-Workload: the 256x256 coupled-Morse surface (two Morse modes, De=0.17 Eh,
-a=1.0/bohr, masses 1836 and 918 m_e, bilinear coupling 0.005 q1 q2), product
-Gaussian packet (sigma 0.2 bohr, displaced +0.3 bohr), dt = 1.0 a.u., C(t)
-every step.  A "step" is one split-operator time step of the packet.
+Headline workload: the batch of 2048 uneven packets (seed 4): 1-D grids N
+in {128, 256, 512, 1024} and 2-D grids 64^2, 128^2, 256^2, Morse De in
+[0.1, 0.2] Eh, a in [0.8, 1.4] /bohr, masses in [500, 2000] m_e, Gaussian
+packets around each minimum, dt = 0.5 a.u., C(t) every step, every packet
+with its own potential, state and masses.  A "step" advances every packet
+by one split-operator time step.  Packets are grouped by grid into
+device-resident plans (launch shapes from the autotuner) that run side by
+side on their own streams; inputs are resident in HBM before the timed
+region starts and only the propagation loop is timed.
 
 Reported: grid-point-steps/s of the mixed path (complex64 state in HBM,
-FP64 arithmetic in the kernels) as `value`, the FP64 path beside it, the
-largest |C(t) - C_oracle(t)| over the timed run, a roofline object for the
-dominant kernel (HIP events on the plan's stream) and the FP64 C oracle
-timed on the host as the CPU baseline.  Inputs are resident in HBM before
-the timed region starts; only the propagation loop is timed.
+FP64 arithmetic) as `value`, the FP64 path beside it, |C(t) - C_oracle(t)|
+of one packet per grid class over the whole timed run, a roofline object
+for the dominant kernel (the row pass of the 256^2 class: HIP events on
+its plan's stream, 2*8 + 4 + 8 = 28 algorithmic bytes per grid point per
+launch), the per-step roofline over the whole batch, and the FP64 C oracle
+timed on the host over all packets (one thread per packet, all host
+cores) as the CPU baseline.
 
-Also reported, under "batched": the 2048-packet uneven batch (1-D and 2-D
-grids, per-packet potentials, states and masses; 1000 steps of dt = 0.5
-a.u.) on the mixed and FP64 paths, with its whole-job rate against the HBM
-peak at 28 B per grid-point-step and a C(t) check of sampled packets.
-Under "config3": the 128^3 surface, 1000 steps, mixed and FP64.
-Under "krylov_controller": the learned Krylov-order/step controller
-(trained on the device) against fixed-order short-iterative Lanczos on the
-256x256 surface, wall time and |dC| against the order-20 run.
+Also reported: "config2" (the 256x256 coupled-Morse packet alone, mixed and
+FP64), "config3" (the 128^3 surface), "krylov_controller" (the learned
+Krylov-order/step controller against fixed-order short-iterative Lanczos
+on the 256x256 surface, with the Lanczos roofline at (m^2 + 6m + 1) * 16
+bytes per grid-point-step).
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 100] [--batch-steps 1000]
-                    [--config3-steps 1000] [--krylov-steps 2000]
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--config2-steps 2000]
+                    [--config3-steps 1000] [--krylov-steps 2000] [--cpu-steps 48]
 """
 import argparse
 import json
@@ -111,38 +115,64 @@ def kernel_names(shape, np0, np1, storage="float"):
     return rows, cols
 
 
-def batched(E, configs, steps, check):
-    """The batched uneven-packet workload: 2048 seeded packets (1-D N in
-    {128..1024}, 2-D 64^2..256^2), each with its own potential, state and
-    masses, dt = 0.5 a.u., `steps` steps, C(t) every step.  Packets are
-    grouped by grid into device-resident plans on their own streams; the
-    timed region is one run of every group (inputs resident).  With
-    `check`, the C(t) of one packet per 2-D class and of one 1-D packet is
-    compared with the FP64 oracle over the whole run."""
+def cpu_batch(packets, steps, threads):
+    """The FP64 C oracle over every packet of the batch, `steps` steps each,
+    one host thread per packet at a time (the oracle call releases the
+    interpreter lock): grid-point-steps/s and seconds."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, EXACT
+    O = Oracle()
+
+    def one(p):
+        O.pspofft(p.np, p.dt_fs, 0.0, (steps - 1) * p.dt_fs, 1, p.sh, p.sm, p.u, p.v, p.vpot,
+                  EXACT)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, sorted(packets, key=lambda p: -int(np.prod(p.np)))))
+    sec = time.perf_counter() - t0
+    return sum(int(np.prod(p.np)) for p in packets) * steps / sec, sec
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def batched(E, configs, steps, warmup, check, profile_steps):
+    """The 2048-packet batch on the mixed and FP64 paths: timed run, per-group
+    kernel timings (HIP events on each group's stream, the group alone) and,
+    with `check`, C(t) of one packet per grid class against the FP64 oracle
+    over the whole run."""
     from espec_amd.batch import SpoBatch
     pk = configs.config4(count=2048, nsteps=steps, seed=4)
     out = {"packets": len(pk), "steps": steps, "lc": 1}
     for name, flags in (("mixed", E.MIXED | E.EXACT), ("fp64", E.FP64 | E.EXACT)):
-        b = SpoBatch(pk, flags=flags)
-        b.run(8)
-        b.sync()
-        b.reset()
+        b = SpoBatch(pk, flags=flags, maxsamp=steps + max(warmup, profile_steps) + 2)
+        if warmup:
+            b.run(warmup)
+            b.sync()
+            b.reset()
         sec = b.timed_run(steps)
         gps = b.points * steps / sec
-        out[name] = {"seconds": sec, "value": gps, "unit": "grid-point-steps/s",
-                     "algorithmic_GBps": gps * 28.0 / 1e9 if name == "mixed" else gps * 56.0 / 1e9,
-                     "groups": [{"grid": g["np"], "packets": len(g["idx"]),
-                                 "shape": g["plan"].shape()} for g in b.groups]}
-        log(f"[bench] batched {name}: {sec:.3f} s, {gps:.3e} gp-steps/s")
+        bpp = 28.0 if name == "mixed" else 56.0
+        storage = "float" if name == "mixed" else "double"
+        res = {"seconds": sec, "value": gps, "unit": "grid-point-steps/s",
+               "ms_per_step": sec * 1e3 / steps, "algorithmic_GBps": gps * bpp / 1e9,
+               "bytes_per_gp_step": bpp, "groups": []}
+        log(f"[bench] batch {name}: {sec:.3f} s, {gps:.3e} gp-steps/s")
         if check and name == "mixed":
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             from oracle import Oracle, EXACT
             O = Oracle()
             picks = {}
             for i, p in enumerate(pk):
-                key = tuple(p.np) if len(p.np) == 2 else ("1-D",)
-                picks.setdefault(key, i)
-            worst = {}
+                picks.setdefault(tuple(p.np), i)
+            worst, drift = {}, {}
             for key, i in picks.items():
                 p = pk[i]
                 o = O.pspofft(p.np, p.dt_fs, 0.0, (steps - 1) * p.dt_fs, 1, p.sh, p.sm, p.u, p.v,
@@ -152,13 +182,73 @@ def batched(E, configs, steps, check):
                 d = max(np.abs(o["cr"][:n] - g["cr"][:n]).max(),
                         np.abs(o["ci"][:n] - g["ci"][:n]).max())
                 worst["x".join(map(str, p.np))] = float(d)
-            out["max_abs_dC_mixed_sample"] = worst
-            log(f"[bench] batched mixed |dC| sample: {worst}")
+                drift["x".join(map(str, p.np))] = float(np.abs(g["norm"][:n] - 1).max())
+            out["max_abs_dC_mixed_vs_oracle"] = worst
+            out["norm_drift_mixed"] = drift
+            log(f"[bench] batch mixed |dC| per class: {worst}")
+        for g in b.groups:
+            plan = g["plan"]
+            shape = plan.shape()
+            ent = {"grid": g["np"], "packets": len(g["idx"]), "points": g["n"] * len(g["idx"]),
+                   "shape": shape, "resident": bool(shape["lds"])}
+            if profile_steps:
+                plan.reset()
+                prof = plan.profile(profile_steps)
+                plan.reset()
+                names = (["spo_lds_fast"] if shape["lds"] else
+                         list(kernel_names(shape, g["np"][0], g["np"][-1], storage)))
+                # the LDS-resident kernel runs all profile_steps steps in one
+                # launch; the pass kernels run once per step
+                per = profile_steps if shape["lds"] else 1
+                ent["kernels"] = [{"kernel": names[k] if k < len(names) else f"kind{k}",
+                                   "ms": m, "ms_per_step": m / per, "bytes": by,
+                                   "GBps": by / (m * 1e-3) / 1e9 if m > 0 else None}
+                                  for k, (m, by) in enumerate(prof)]
+            res["groups"].append(ent)
+        out[name] = res
         b.close()
     m = out["mixed"]["value"]
     out["step_roofline"] = {"bytes_per_gp_step": 28.0, "achieved_GBps": m * 28.0 / 1e9,
                             "frac": m * 28.0 / HBM_PEAK}
     out["mixed_over_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
+
+
+def config2(E, configs, K, W, check):
+    """The 256x256 coupled-Morse packet alone, mixed and FP64, with the row
+    pass's HIP-event timing and, with `check`, |dC| against the oracle."""
+    c = configs.config2(nsteps=K)
+    sec_mix, res_mix, plan_mix = timed_run(E, c, E.MIXED | E.EXACT, K, W)
+    sec_64, res_64, plan_64 = timed_run(E, c, E.FP64 | E.EXACT, K, W)
+    plan_mix.reset()
+    prof = plan_mix.profile(200)
+    shape = plan_mix.shape()
+    inst = kernel_names(shape, c.np[0], c.np[1])
+    out = {"workload": "256x256 coupled-Morse split-operator FFT (2-D), one packet",
+           "steps": K, "launch_shape": shape,
+           "mixed": {"value": c.n * K / sec_mix, "ms_per_step": sec_mix * 1e3 / K},
+           "fp64": {"value": c.n * K / sec_64, "ms_per_step": sec_64 * 1e3 / K},
+           "mixed_over_fp64": sec_64 / sec_mix,
+           "kernels": [{"kernel": inst[k], "ms": m, "bytes": by, "GBps": by / (m * 1e-3) / 1e9}
+                       for k, (m, by) in enumerate(prof)]}
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle, EXACT
+        t0 = time.perf_counter()
+        o = Oracle().pspofft(c.np, c.dt_fs, 0.0, (K - 1) * c.dt_fs, 1, c.sh, c.sm, c.u, c.v,
+                             c.vpot, EXACT)
+        t_cpu = time.perf_counter() - t0
+        n = min(len(o["cr"]), len(res_mix["cr"]))
+        out["max_abs_dC"] = {
+            k: float(max(np.abs(o["cr"][:n] - r["cr"][:n]).max(),
+                         np.abs(o["ci"][:n] - r["ci"][:n]).max()))
+            for k, r in (("mixed", res_mix), ("fp64", res_64))}
+        out["norm_drift"] = {"mixed": float(np.abs(res_mix["norm"][:n] - 1).max()),
+                             "fp64": float(np.abs(res_64["norm"][:n] - 1).max())}
+        out["cpu_oracle_1_thread"] = {"value": c.n * K / t_cpu, "seconds": t_cpu}
+    plan_mix.close()
+    plan_64.close()
+    log(f"[bench] config2: mixed {out['mixed']['value']:.3e}, fp64 {out['fp64']['value']:.3e}")
     return out
 
 
@@ -250,7 +340,16 @@ def krylov_controller(E, configs, steps):
         t1 = time.perf_counter()
         f.run(steps)
         f.sync()
-        out["fixed"][str(m)] = {"seconds": time.perf_counter() - t1, "max_abs_dC": dC(f.get(0))}
+        sec = time.perf_counter() - t1
+        # Lanczos order m with full Gram-Schmidt: (m^2 + 6m + 1) state-sized
+        # streams of 16 bytes (complex FP64) per grid-point-step
+        bpp = (m * m + 6 * m + 1) * 16.0
+        gps = c.n * steps / sec
+        out["fixed"][str(m)] = {"seconds": sec, "max_abs_dC": dC(f.get(0)),
+                                "value": gps, "unit": "grid-point-steps/s",
+                                "roofline": {"bytes_per_gp_step": bpp,
+                                             "achieved_GBps": gps * bpp / 1e9,
+                                             "frac": gps * bpp / HBM_PEAK}}
         f.close()
     a = plan(21)
     a.adaptive(steps, params, tol=tol, m_fix=12)  # warm-up: captures the step graphs it meets
@@ -279,127 +378,100 @@ def krylov_controller(E, configs, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
-    ap.add_argument("--profile-steps", type=int, default=200)
-    ap.add_argument("--batch-steps", type=int, default=1000,
-                    help="steps of the batched 2048-packet workload (0 skips it)")
+    ap.add_argument("--steps", type=int, default=1000, help="timed steps of the batch")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
+    ap.add_argument("--profile-steps", type=int, default=50,
+                    help="steps of the per-group HIP-event kernel timing (0 skips it)")
+    ap.add_argument("--cpu-steps", type=int, default=48,
+                    help="steps of every packet in the CPU baseline sample")
+    ap.add_argument("--config2-steps", type=int, default=2000,
+                    help="steps of the single 256x256 packet (0 skips it)")
     ap.add_argument("--krylov-steps", type=int, default=2000,
                     help="steps of the learned Krylov controller comparison (0 skips it)")
     ap.add_argument("--config3-steps", type=int, default=1000,
                     help="steps of the 128^3 workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the propagation of one packet (or a batch of "
-              "independent packets) runs on a single MI355X and no multi-GPU path is "
-              "built; run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the propagation of a batch of independent packets "
+              "(or of one packet) runs on a single MI355X and no multi-GPU path is built; "
+              "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import espec_amd as E
     from espec_amd import configs
 
-    c = configs.config2(nsteps=args.steps)
-    npts = c.n
     K, W = args.steps, args.warmup
-
-    log(f"[bench] mixed path: {c.meta['name']} {K} steps (warmup {W})")
-    sec_mix, res_mix, plan_mix = timed_run(E, c, E.MIXED | E.EXACT, K, W)
-    ev_ms_mix, launches = plan_mix.timing()
-    log(f"[bench] mixed: {sec_mix * 1e3:.2f} ms host, {ev_ms_mix:.2f} ms events, {launches} launches")
-    log(f"[bench] FP64 path")
-    sec_64, res_64, plan_64 = timed_run(E, c, E.FP64 | E.EXACT, K, W)
-    log(f"[bench] fp64: {sec_64 * 1e3:.2f} ms host")
-
-    # Per-kernel durations (timing-only HIP events on the plan's stream
-    # around every launch of a 200-step run, no graph).
-    plan_mix.reset()
-    prof = plan_mix.profile(args.profile_steps)
-    plan_mix.reset()
-    shape = plan_mix.shape()
-    inst = kernel_names(shape, c.np[0], c.np[1])
-    descr = ["x pass: inverse x, V phases, C(t), forward x, Kx",
-             "y pass: forward y, Ky, inverse y"]
-    kinds = [dict(kernel=inst[i] if len(prof) > 1 else "spo_lds_fast",
-                  role=descr[i] if len(prof) > 1 else "whole step in LDS", ms=m, bytes=b)
-             for i, (m, b) in enumerate(prof)]
-    # The row pass carries every byte of the per-grid-point-step model
-    # (state read + write, V, psi0/LC: 2*8 + 4 + 8/LC for complex64); the
-    # strided pass only adds the multi-pass round trip, which the model
-    # does not count.  The roofline kernel is therefore the row pass (or
-    # the single LDS-resident step kernel).
-    dom = kinds[0]
-    achieved = dom["bytes"] / (dom["ms"] * 1e-3)
-    log(f"[bench] kernels: " + ", ".join(f"{k['kernel']} {k['ms'] * 1e3:.2f} us" for k in kinds))
+    log(f"[bench] batch of 2048 packets: {K} steps (warmup {W})")
+    bt = batched(E, configs, K, W, not args.no_cpu, args.profile_steps)
+    mix = bt["mixed"]
 
     ms_bw, by_bw = E.bw_probe(1 << 30, 10)
     copy_bw = by_bw / (ms_bw * 1e-3)
     log(f"[bench] copy kernel: {copy_bw / 1e9:.0f} GB/s")
 
+    # Dominant kernel: the longest per-step kernel of the batch, timed with
+    # HIP events on its group's stream (the group alone).
+    dom = None
+    for g in mix["groups"]:
+        for k in g.get("kernels", []):
+            if dom is None or k["ms_per_step"] > dom[1]["ms_per_step"]:
+                dom = (g, k)
+    roof = None
+    if dom is not None:
+        g, k = dom
+        achieved = k["bytes"] / (k["ms"] * 1e-3)
+        roof = {"bound": "hbm", "kernel": k["kernel"], "group": "x".join(map(str, g["grid"])),
+                "launch_ms": k["ms"], "bytes_per_launch": k["bytes"],
+                "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK,
+                "traffic": pmc_traffic("batch_2048_packets", k["kernel"])}
+        log(f"[bench] dominant kernel {k['kernel']} ({roof['group']}): {k['ms'] * 1e3:.1f} us, "
+            f"{achieved / 1e9:.0f} GB/s")
+
     out = {
         "metric": METRIC,
-        "value": npts * K / sec_mix,
+        "value": mix["value"],
         "unit": "grid-point-steps/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": sec_mix * 1e3 / K,
+        "ms_per_step": mix["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "storage": "complex64",
         "data": "synthetic",
-        "config": {"workload": "256x256 coupled-Morse split-operator FFT (2-D), mixed path",
-                   "grid": c.np, "dt_au": 1.0, "lc": 1, "steps": K,
-                   "launches_per_run": launches, "launch_shape": plan_mix.shape(),
-                   "launch_shape_pinned": True},
-        "fp64": {"value": npts * K / sec_64, "ms_per_step": sec_64 * 1e3 / K,
-                 "mixed_over_fp64": sec_64 / sec_mix},
-        "kernels": kinds,
+        "config": {"workload": "2048 uneven packets (1-D N 128..1024, 2-D 64^2..256^2, seed 4), "
+                               "split-operator FFT, mixed path",
+                   "packets": bt["packets"], "grid_points": int(mix["value"] * mix["seconds"] / K),
+                   "dt_au": 0.5, "lc": 1, "steps": K},
+        "fp64": {"value": bt["fp64"]["value"], "ms_per_step": bt["fp64"]["ms_per_step"],
+                 "algorithmic_GBps": bt["fp64"]["algorithmic_GBps"]},
+        "mixed_over_fp64": bt["mixed_over_fp64"],
+        "roofline": roof,
+        "step_roofline": bt["step_roofline"],
         "copy_kernel_GBps": copy_bw / 1e9,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom["kernel"],
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": pmc_traffic("config2_mixed", dom["kernel"]),
-        },
-        "timed_region_us_per_launch": ev_ms_mix * 1e3 / max(launches, 1),
-        # whole-step view: gp-steps/s x (2*8 + 4 + 8/LC) bytes against the peak
-        "step_roofline": {"bytes_per_gp_step": 28.0,
-                          "achieved_GBps": npts * K / sec_mix * 28.0 / 1e9,
-                          "frac": npts * K / sec_mix * 28.0 / HBM_PEAK},
+        "groups": {"mixed": mix["groups"], "fp64": bt["fp64"]["groups"]},
     }
+    for key in ("max_abs_dC_mixed_vs_oracle", "norm_drift_mixed"):
+        if key in bt:
+            out[key] = bt[key]
 
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from oracle import Oracle, EXACT
-        O = Oracle()
-        tf = (K - 1) * c.dt_fs  # the DO loop runs (TF-TI)/DT + 1 steps
-        log(f"[bench] CPU oracle (FP64 C, 1 thread): {K} steps of the same packet")
-        t0 = time.perf_counter()
-        o = O.pspofft(c.np, c.dt_fs, 0.0, tf, 1, c.sh, c.sm, c.u, c.v, c.vpot, EXACT)
-        t_cpu = time.perf_counter() - t0
-        n = min(len(o["cr"]), len(res_mix["cr"]))
-        dmix = max(np.abs(o["cr"][:n] - res_mix["cr"][:n]).max(),
-                   np.abs(o["ci"][:n] - res_mix["ci"][:n]).max())
-        d64 = max(np.abs(o["cr"][:n] - res_64["cr"][:n]).max(),
-                  np.abs(o["ci"][:n] - res_64["ci"][:n]).max())
-        out["max_abs_dC"] = {"mixed": float(dmix), "fp64": float(d64), "samples": int(n)}
-        out["norm_drift"] = {"mixed": float(np.abs(res_mix["norm"][:n] - 1).max()),
-                             "fp64": float(np.abs(res_64["norm"][:n] - 1).max())}
-        out["cpu_baseline"] = {"value": npts * K / t_cpu, "unit": "grid-point-steps/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"the same 256x256 packet, {K} steps, FP64 C oracle "
-                                         f"(oracle/espec_oracle.c), {t_cpu:.1f} s"}
-        log(f"[bench] CPU: {t_cpu:.1f} s; max|dC| mixed {dmix:.2e} fp64 {d64:.2e}")
-    plan_mix.close()
-    plan_64.close()
-    if args.batch_steps > 0:
-        out["batched"] = batched(E, configs, args.batch_steps, not args.no_cpu)
+        pk = configs.config4(count=2048, nsteps=args.cpu_steps, seed=4)
+        th = host_threads()
+        log(f"[bench] CPU oracle (FP64 C): 2048 packets x {args.cpu_steps} steps on {th} threads")
+        v, sec = cpu_batch(pk, args.cpu_steps, th)
+        out["cpu_baseline"] = {"value": v, "unit": "grid-point-steps/s", "cores": th,
+                               "kind": "port",
+                               "sample": f"all 2048 packets of the batch, {args.cpu_steps} steps "
+                                         f"each, FP64 C oracle (oracle/espec_oracle.c), one "
+                                         f"thread per packet on {th} host threads, {sec:.1f} s"}
+        log(f"[bench] CPU: {sec:.1f} s, {v:.3e} gp-steps/s")
+    if args.config2_steps > 0:
+        out["config2"] = config2(E, configs, args.config2_steps, 100, not args.no_cpu)
     if args.config3_steps > 0:
         out["config3"] = config3(E, configs, args.config3_steps, not args.no_cpu)
     if args.krylov_steps > 0:
