@@ -95,7 +95,9 @@ def kernel_names(shape, np0, np1, storage="float"):
     arguments stripped) of the row-pass and strided-pass kernels that a
     launch shape selects for steps 2..K."""
     pht = "true" if shape.get("vph") else "false"
-    if shape["rows_pen"] == 2:
+    if shape["rows_pen"] == 3:
+        rows = f"pass_rows_pen_sp<{np0 // 16}, true, true, {pht}>"
+    elif shape["rows_pen"] == 2:
         rows = f"pass_rows_wide<{storage}, {np0}, true, true, {pht}>"
     elif shape["rows_pen"]:
         rows = f"pass_rows_pen<{storage}, {np0 // 16}, true, true, {pht}>"

@@ -18,6 +18,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include <hip/hip_ext.h>
 
@@ -464,6 +465,98 @@ __global__ __launch_bounds__(64, 2) void pass_rows_pen(PassArgs a)
     }
 }
 
+// Row pass on pencils for complex64 storage with single-precision
+// transforms: the inverse and forward line FFTs and the kinetic phase run
+// in FP32 with split (hi + lo) constants, as the strided pass does, while
+// the V phase, the 1/N scale and the C(t) and norm sums stay in FP64 (a
+// fixed per-point rounding of the phase would accumulate coherently over
+// the steps).  Half the registers and exchange slots of pass_rows_pen, so
+// more wavefronts hide the memory latency.
+template <int M, bool INV, bool FWD, bool PHT>
+__global__ __launch_bounds__(64, 2) void pass_rows_pen_sp(PassArgs a)
+{
+    constexpr int C = 64 / M, L = 16 * M;
+    __shared__ float2 xs[C * pencil::xslots<M>()];
+    const int j = threadIdx.x % M, l = threadIdx.x / M;
+    float2 *xl = xs + l * pencil::xslots<M>();
+    const int total = a.xblocks * a.npk;
+    const long long g = INV ? *a.dstep : 0;
+    const bool samp = INV && (g % a.lc) == 0;
+    const double b = a.b;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const int pk = w / a.xblocks, blk = w - pk * a.xblocks;
+        const long long po = (long long)pk * a.n + (long long)(blk * C + l) * L + j;
+        const float2 *gx = reinterpret_cast<const float2 *>(a.x) + po;
+        const float *gv = reinterpret_cast<const float *>(a.vpot) + po;
+        const float2 *g0 = reinterpret_cast<const float2 *>(a.psi0) + po;
+        float2 v[16], q0[16];
+        float cv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = gx[r * M];
+        if (INV) pencil::inverse<M, -1>(v, j, xl, a.twr_s);
+        // V and psi0 after the transform: fewer registers live across it
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            if (!PHT) cv[r] = gv[r * M];
+            if (samp) q0[r] = g0[r * M];
+        }
+        double cr = 0, ci = 0, nr = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            double2 ph;
+            if (PHT) {
+                ph = a.vph[po + r * M];
+            } else {
+                double sn, cs;
+                pencil::sincos_reduced(a.cst2 * (double)cv[r], &sn, &cs);
+                ph = {cs, sn};
+            }
+            double2 x = {(double)v[r].x, (double)v[r].y};
+            if (INV) {
+                x = cmulc(x, ph);
+                x.x *= b;
+                x.y *= b;
+                if (samp) {
+                    const double px = q0[r].x, py = q0[r].y;
+                    cr += px * x.x + py * x.y;
+                    ci += px * x.y - py * x.x;
+                    nr += x.x * x.x + x.y * x.y;
+                }
+            }
+            if (FWD) x = cmulc(x, ph);
+            v[r] = {(float)x.x, (float)x.y};
+            // one point's sincos at a time: interleaved, the sixteen
+            // evaluations need more registers than the occupancy allows
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // keep the forward transform's twiddle and kinetic loads below the
+        // phase loop: hoisted above it they hold ~120 registers across it
+        asm volatile("" ::: "memory");
+        if (FWD) {
+            pencil::forward<M, 1>(v, j, xl, a.twr_s);
+            const float4 *kt = a.kreg_s + (long long)pk * a.ktps + j;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = pencil::cmulc_w(v[r], kt[r * M]);
+        }
+        float2 *gy = reinterpret_cast<float2 *>(a.x) + po;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) gy[r * M] = v[r];
+        if (samp) {
+            cr = wave_sum(cr);
+            ci = wave_sum(ci);
+            nr = wave_sum(nr);
+            const long long nc = g / a.lc;
+            if (threadIdx.x == 0 && nc < a.maxsamp) {
+                double *c = a.part + (((long long)nc * a.npk + pk) * a.xblocks + blk) * 3;
+                c[0] = cr;
+                c[1] = ci;
+                c[2] = nr;
+            }
+        }
+        __syncthreads();  // exchange slots are rewritten by the next item
+    }
+}
+
 // Strided pass on pencils: lane = c + C j, line c of the item's C
 // consecutive columns, lane j within the line; arithmetic in precision S.
 // PF: load the next item's state while transforming the current one.
@@ -826,6 +919,26 @@ hipError_t rows(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, double 
 #undef ESPEC_WIDE_ROWS2
         return hipErrorInvalidValue;
     }
+    if (p.shape.rows_pen == 3) {
+        if constexpr (std::is_same<R, float>::value) {
+            a.twr_s = static_cast<const float4 *>(p.twr_s) + 2 * p.twr_off[0];
+            a.kreg_s = static_cast<const float4 *>(p.kreg_s) + p.kax_off[0];
+            const dim3 pg(act);
+#define ESPEC_PEN_ROWS_SP2(M_, P_)                                                          \
+    if (p.np[0] == 16 * M_ && pht == P_) {                                                  \
+        if (inv && fwd) return launch(pass_rows_pen_sp<M_, true, true, P_>, pg, 64, 0, s, a); \
+        if (inv) return launch(pass_rows_pen_sp<M_, true, false, P_>, pg, 64, 0, s, a);       \
+        return launch(pass_rows_pen_sp<M_, false, true, P_>, pg, 64, 0, s, a);                \
+    }
+#define ESPEC_PEN_ROWS_SP(M_) ESPEC_PEN_ROWS_SP2(M_, true) ESPEC_PEN_ROWS_SP2(M_, false)
+            ESPEC_PEN_ROWS_SP(4)
+            ESPEC_PEN_ROWS_SP(8)
+            ESPEC_PEN_ROWS_SP(16)
+#undef ESPEC_PEN_ROWS_SP
+#undef ESPEC_PEN_ROWS_SP2
+        }
+        return hipErrorInvalidValue;
+    }
     if (p.shape.rows_pen) {
         a.twr = static_cast<const double2 *>(p.twr) + p.twr_off[0];
         a.kreg = static_cast<const double2 *>(p.kreg) + p.kax_off[0];
@@ -1053,7 +1166,7 @@ bool rows_ok(const SpoPassPlan &p, const PassShape &s)
         return pen_len(p.np[0]) && s.rows == 256 / p.np[0] && s.rows_nt == 64 &&
                (p.n / p.np[0]) % s.rows == 0 &&
                (p.vph != nullptr || p.phase_max < pencil::kSincosMax);
-    if (s.rows_pen > 2) return false;
+    if (s.rows_pen > 3 || (s.rows_pen == 3 && !p.mixed)) return false;
     if (s.rows_pen)
         return pen_len(p.np[0]) && s.rows == 1024 / p.np[0] && s.rows_nt == 64 &&
                (p.n / p.np[0]) % s.rows == 0 &&
@@ -1158,7 +1271,7 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
     // {rows, threads, pencil}
     const int rc_[][3] = {{1, 64, 0},  {2, 64, 0},  {4, 64, 0},  {1, 256, 0},
                           {2, 256, 0}, {4, 256, 0}, {8, 256, 0}, {1024 / p.np[0], 64, 1},
-                          {256 / p.np[0], 64, 2}};
+                          {256 / p.np[0], 64, 2}, {1024 / p.np[0], 64, 3}};
     for (auto &c : rc_) {
         PassShape sh = p.shape;
         sh.rows = c[0];

@@ -15,7 +15,8 @@ struct PassShape {
     int xmap = 1;                  // contiguous strided tiles per XCD
     int grid = 0;                  // workgroups per pass (0: one per work item)
     int rows_pen = 0;              // row pass on register pencils (rows = 64 / (np0 / 16)),
-                                   // 2: one 256-point line per wavefront (rows = 1)
+                                   // 2: one 256-point line per wavefront (rows = 1),
+                                   // 3: as 1 with FP32 transforms (complex64 storage)
     int tile_pen = 0;              // strided passes on register pencils: 1 one wavefront per
                                    // item, 2 four wavefronts sharing an LDS-staged tile,
                                    // 3 one 256-point column per wavefront

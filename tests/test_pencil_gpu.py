@@ -24,11 +24,16 @@ SHAPES = {
     "wide_rows": dict(rows_pen=2, tile_pen=1, pen_pf=1, grid=0, xmap=1),
     "wide_cols": dict(rows_pen=1, tile_pen=3, pen_pf=0, grid=0, xmap=0),
 }
+# FP32-transform row pass: complex64 storage only
+SHAPES_MIXED = {
+    "pen_sp": dict(rows_pen=3, tile_pen=1, pen_pf=0, grid=0, xmap=1),
+    "pen_sp_loop": dict(rows_pen=3, tile_pen=1, pen_pf=1, grid=24, xmap=1),
+}
 
 
 def pinned(plan, name, np0):
-    kw = dict(SHAPES[name])
-    if kw.get("rows_pen") == 1:
+    kw = dict(SHAPES.get(name) or SHAPES_MIXED[name])
+    if kw.get("rows_pen") in (1, 3):
         kw.update(rows=1024 // np0, rows_nt=64)
     elif kw.get("rows_pen") == 2:
         kw.update(rows=256 // np0, rows_nt=64)
@@ -61,7 +66,33 @@ def test_config2_shapes(gpu, orc, shape, flags, tol):
     plan.close()
 
 
-@pytest.mark.parametrize("shape", ["pen", "pen_tile", "wide"])
+@pytest.mark.parametrize("shape", list(SHAPES_MIXED))
+def test_config2_fp32_row_transforms(gpu, orc, shape):
+    from oracle import EXACT
+    c = configs.config2()
+    nst = 40
+    print(f"case: config2 mixed shape={shape}", flush=True)
+    plan = E.SpoPlan(c.np, c.dt_fs, 1, c.sh, c.sm, c.vpot[None], c.u[None], c.v[None],
+                     maxsamp=nst + 1, flags=E.MIXED | E.EXACT)
+    pinned(plan, shape, c.np[0])
+    plan.run(nst)
+    g = plan.get(0)
+    o = orc.pspofft(c.np, c.dt_fs, 0.0, (nst - 1) * c.dt_fs, 1, c.sh, c.sm, c.u, c.v, c.vpot,
+                    EXACT)
+    assert dC(g, o) <= 1e-6
+    plan.close()
+
+
+def test_fp32_row_transforms_refused_for_fp64(gpu):
+    c = configs.config2()
+    plan = E.SpoPlan(c.np, c.dt_fs, 1, c.sh, c.sm, c.vpot[None], c.u[None], c.v[None],
+                     maxsamp=4, flags=E.FP64 | E.EXACT)
+    with pytest.raises(E.EspecError):
+        plan.set_shape(rows_pen=3, rows=1024 // c.np[0], rows_nt=64)
+    plan.close()
+
+
+@pytest.mark.parametrize("shape", ["pen", "pen_tile", "wide", "pen_sp"])
 def test_config2_phase_from_v(gpu, orc, shape):
     """Mixed plans above 64 MB form exp(-i cst2 V) in the row pass from V
     (table-based FP64 sincos) instead of reading a phase table: 80 copies
