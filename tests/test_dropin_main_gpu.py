@@ -5,9 +5,9 @@ inputs: its printed C(t) table, correlation function and spectrum against
 the published results (tests/golden, extracted by make_fixtures.py).
 
 Everything before and after the propagation (input reader, potentials,
-MTRXDIAG, the file handling of espec.f) is the reference's own code; the
-propagator, the initial LANCZS states, C(t) assembly pieces, HWHM and
-SPECTRUMTD are the drop-in's.
+the file handling of espec.f) is the reference's own code; the initial
+eigenstates (MTRXDIAG, LANCZS), the propagator, C(t) assembly pieces,
+HWHM and SPECTRUMTD are the drop-in's.
 
 The reference is the same v0.7 program built entirely from the reference's
 sources (tests/golden/v07_*, written by make_ref_fixtures.py from
