@@ -350,13 +350,18 @@ int espec_lanczs(int ndim, const int *np, int il, int iu, int nreort,
  * (bisection to abstol), the Ritz vector of the state as the next start,
  * until the change of the Ritz vector is below abstol, for at most maxint
  * restarts.  ESPEC_REF measures the change as the reference does (its sum
- * reads the 41st component for every J, lanczsc.f:104-109, and rarely
- * falls below abstol); ESPEC_EXACT sums |VI - VIAUX| over the grid with the
- * sign of the previous vector matched.  eigvl[iu-il+1] are the Ritz values
- * of the last restart (as DSTEVX leaves EIGVL); eigvc [N][iu-il+1] the
- * normalised vectors.  log (may be NULL): logcap records (state, restart,
- * energy, change) as the reference prints them.  ESPEC_ERR_DIAG where the
- * reference stops after maxint restarts.  Runs on the device. */
+ * reads the 41st component for every J, lanczsc.f:104-109, so it falls
+ * below abstol only when grid point 41 is in the state's tail) and returns
+ * what the reference returns: eigvl[iu-il+1] the Ritz values of the last
+ * restart (as DSTEVX leaves EIGVL) and, for every state but the last, rows
+ * 0..39 of its vector replaced by the last restart's tridiagonal
+ * eigenvector (DSTEVX writes EIGVC(1:40, 1:M) after those columns were
+ * stored).  ESPEC_EXACT sums |VI - s VIAUX| over the grid (s the sign of
+ * <VI, VIAUX>) and returns each state's own converged value and vector.
+ * eigvc [iu-il+1][N]: the normalised vectors, one row per state.  log (may
+ * be NULL): up to logcap records (state, restart, energy, change) as the
+ * reference prints them; *nlog receives the count.  ESPEC_ERR_DIAG where
+ * the reference stops after maxint restarts.  Runs on the device. */
 int espec_lanczsg(int ndim, const int *np, int il, int iu, int maxint, double abstol,
                   const double *shm, const double *vpot, double *eigvl, double *eigvc,
                   double *log, int logcap, int *nlog, int flags);
