@@ -24,7 +24,8 @@ Also reported: "config2" (the 256x256 coupled-Morse packet alone, mixed and
 FP64), "config3" (the 128^3 surface), "krylov_controller" (the learned
 Krylov-order/step controller against fixed-order short-iterative Lanczos
 on the 256x256 surface, with the Lanczos roofline at (m^2 + 6m + 1) * 16
-bytes per grid-point-step).
+bytes per grid-point-step), "krylov_controller_1d" (the same on the 1-D
+Morse surface, 4096 steps of dt = 0.5 a.u.).
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--config2-steps 2000]
                     [--config3-steps 1000] [--krylov-steps 2000] [--cpu-steps 48]
@@ -296,7 +297,7 @@ def config3(E, configs, steps, check):
     return out
 
 
-def krylov_controller(E, configs, steps):
+def krylov_controller(E, configs, steps, surface="config2"):
     """The learned Krylov-order/step controller on short-iterative Lanczos
     propagation of the 256x256 surface (FD Hamiltonian, dt = 1 a.u., C(t)
     every step, FP64).  The network is trained on the device from traces
@@ -309,7 +310,10 @@ def krylov_controller(E, configs, steps):
     from espec_amd import krylov
     tol = 1e-6 / steps
     t0 = time.perf_counter()
-    fam = krylov.family(count=32, n=256, nsteps=0, seed=5, groups=4)
+    if surface == "config1":
+        fam = krylov.family1d(count=32, nsteps=0, seed=7, groups=4)
+    else:
+        fam = krylov.family(count=32, n=256, nsteps=0, seed=5, groups=4)
     traces = [krylov.trace_packets([p for p in fam if p.meta["group"] == g], 400, mprop=mp)
               for g, mp in zip(range(4), (12, 14, 16, 21))]
     x, t = krylov.dataset(traces, tol)
@@ -317,7 +321,7 @@ def krylov_controller(E, configs, steps):
     t_train = time.perf_counter() - t0
     log(f"[bench] controller: {len(x)} samples, traces + training {t_train:.1f} s, "
         f"loss {hist[0]:.2e} -> {hist[-1]:.2e}")
-    c = configs.config2(nsteps=steps)
+    c = getattr(configs, surface)(nsteps=steps)
     shm = krylov.shm_of(c)
 
     def plan(m):
@@ -331,7 +335,9 @@ def krylov_controller(E, configs, steps):
     def dC(g):
         return float(max(np.abs(g["cr"] - g20["cr"]).max(), np.abs(g["ci"] - g20["ci"]).max()))
 
-    out = {"workload": "256x256 coupled-Morse SIL, FD Hamiltonian, dt 1 a.u., FP64",
+    out = {"workload": ("512-point 1-D Morse SIL, FD Hamiltonian, dt 0.5 a.u., FP64"
+                        if surface == "config1" else
+                        "256x256 coupled-Morse SIL, FD Hamiltonian, dt 1 a.u., FP64"),
            "steps": steps, "budget_per_dt": tol, "train_samples": int(len(x)),
            "train_seconds": t_train, "fixed": {}}
     for m in (10, 11, 12, 13, 14):
@@ -478,6 +484,8 @@ def main():
         out["config3"] = config3(E, configs, args.config3_steps, not args.no_cpu)
     if args.krylov_steps > 0:
         out["krylov_controller"] = krylov_controller(E, configs, args.krylov_steps)
+        out["krylov_controller_1d"] = krylov_controller(E, configs, 2 * args.krylov_steps,
+                                                        "config1")
     print(json.dumps(out))
 
 

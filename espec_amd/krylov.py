@@ -26,7 +26,7 @@ budget (the inputs are the previous step's, and a miss costs a fallback).
 """
 import numpy as np
 
-from . import FP64, EXACT, SilPlan, MLP_NPAR, mlp_train, configs
+from . import FP64, EXACT, SHFS, SilPlan, MLP_NPAR, mlp_train, configs
 
 M0, M1 = 4, 20
 SCALES = (0.5, 1.0, 2.0)
@@ -86,6 +86,30 @@ def family(count=32, n=128, nsteps=3000, seed=5, groups=4):
         for _ in range(count // groups):
             p = configs.packet2d(n, 0.17 * u(2), 1.0 * u(2), m, 0.3 * u(2), 0.2 * u(2), 1.0,
                                  nsteps, coupling=0.005 * u())
+            p.meta["group"] = g
+            out.append(p)
+    return out
+
+
+def family1d(count=32, nsteps=0, seed=7, groups=4):
+    """Training surfaces for the 1-D Morse workload: its grid (512 points on
+    [0.5, 6.0] bohr), De, a, the minimum, the packet's offset from it and
+    its width varied within +-20% of 0.1744 Eh, 1.1 /bohr, 1.4 bohr, 0.4
+    bohr and 0.15 bohr, dt = 0.5 a.u.; the mass (918 m_e +-20%) per group
+    of count/groups packets."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u = lambda: rng.uniform(0.8, 1.2)
+    x = np.linspace(0.5, 6.0, 512)
+    h = x[1] - x[0]
+    out = []
+    for g in range(groups):
+        m = 918.0 * u()
+        for _ in range(count // groups):
+            xe = 1.4 * u()
+            v = configs.morse(x - xe, 0.1744 * u(), 1.1 * u())
+            psi = configs.gaussian(x, xe + 0.4 * u(), 0.15 * u())
+            psi = psi / np.sqrt(np.sum(psi * psi))
+            p = configs.Packet([512], [h], [m], v, psi, np.zeros(512), 0.5 * SHFS, nsteps, 1)
             p.meta["group"] = g
             out.append(p)
     return out

@@ -153,3 +153,54 @@ def test_trained_controller_accuracy(gpu, orc):
     assert st["steps"] >= nst // 2 and len(log) >= st["steps"] - 1
     # an attempt over its budget is retried at the largest order or replaced
     assert st["fallbacks"] <= int(np.sum(log[:, 3] > log[:, 4]))
+
+
+def test_trained_controller_gain_on_1d_morse(gpu, orc):
+    """What the trained network gains over the method without it, on the 1-D
+    Morse workload (512 points, dt = 0.5 a.u.): trained on traces of
+    config-1-style surfaces (+-20%), the controller propagates the workload
+    itself within 1e-6 of the oracle's order-20 propagation, and spends
+    fewer Krylov iterations per time step than the cheapest fixed order
+    that is as accurate.  Wall times are printed: per step the adaptive
+    loop also waits for its estimates on the host."""
+    import time
+    from oracle import EXACT
+    nst = 400
+    tol = 1e-6 / nst
+    fam = krylov.family1d(count=8, nsteps=0, seed=7, groups=2)
+    print("case: controller trained on 8 1-D Morse traces of 300 steps", flush=True)
+    traces = [krylov.trace_packets([p for p in fam if p.meta["group"] == g], 300, mprop=mp)
+              for g, mp in ((0, 12), (1, 21))]
+    x, t = krylov.dataset(traces, tol)
+    params, hist = krylov.train(x, t, epochs=40, lr=3e-3, seed=3)
+    c = configs.config1(nsteps=nst)
+    shm = krylov.shm_of(c)
+    o = orc.plnz(c.np, 20, 0.0, c.dt_fs, 0.0, (nst - 1) * c.dt_fs, 1, shm, c.u, c.v, c.vpot,
+                 EXACT)
+
+    def dC(g):
+        k = min(len(g["cr"]), len(o["cr"]))
+        assert k == nst + 1
+        return max(np.abs(g["cr"][:k] - o["cr"][:k]).max(), np.abs(g["ci"][:k] - o["ci"][:k]).max())
+
+    plan = E.SilPlan(c.np, 21, 0.0, c.dt_fs, 1, shm, c.vpot, c.u, c.v, maxsamp=nst + 1)
+    t0 = time.perf_counter()
+    st = plan.adaptive(nst, params, tol=tol, m_fix=12)
+    ta = time.perf_counter() - t0
+    da = dC(plan.get(0))
+    plan.close()
+    assert da <= 1e-6
+    fixed = {}
+    for m in range(8, 21):
+        f = E.SilPlan(c.np, m, 0.0, c.dt_fs, 1, shm, c.vpot, c.u, c.v, maxsamp=nst + 1)
+        t0 = time.perf_counter()
+        f.run(nst)
+        f.sync()
+        fixed[m] = (dC(f.get(0)), time.perf_counter() - t0)
+        f.close()
+    m_eq = min(m for m, (d, _) in fixed.items() if d <= max(da, 1e-12))
+    per_dt = st["krylov_sum"] / nst  # Krylov iterations per base step of the adaptive run
+    print(f"  stats {st}  |dC| {da:.1e}; {per_dt:.2f} iterations per dt against fixed order "
+          f"{m_eq} (|dC| {fixed[m_eq][0]:.1e}); wall {ta:.3f} s against {fixed[m_eq][1]:.3f} s",
+          flush=True)
+    assert per_dt < 0.9 * m_eq
