@@ -99,6 +99,9 @@ def _load():
     L.espec_ppsod.argtypes = [ip, i, dp, dp, dp, ip, d, d, d, d, d, ip, dp, dp, dp, dp, dp, dp,
                               ip, ip, vp, ip, vp, vp, vp, vp, d, d, vp, vp, vp, vp, ip, P(ip),
                               ip]
+    L.espec_s2ppsod.argtypes = [ip, i, dp, dp, dp, ip, d, dp, dp, dp, dp, d, d, d, d, ip, dp,
+                                dp, dp, dp, dp, dp, ip, ip, vp, ip, dp, vp, ip, P(ip), P(ip),
+                                vp, vp, vp, ip]
     L.espec_fftnt.argtypes = [d, d, ip, ip, ip, ip]
     L.espec_sil_plan_set_absorb.argtypes = [vp, ip, vp]
     L.espec_sil_plan_create.argtypes = [P(vp), ip, i, ip, ip, dp, dp, ip, d, d, d, vp, ip, ip]
@@ -304,6 +307,33 @@ def ppsod(np_, dt, ti, tf, lc, shm, u1, v1, vp, dm, pulse, flags=FP64 | EXACT,
     if npr:
         out["proj"] = pr[:k, :npr]
     return out
+
+
+def s2ppsod(np_, dt, ti, tf, lc, shm, xi, sh, sm1, aa, u1, v1, vp, dm, pulse, prtcrl=2,
+            tol=1e30, flags=FP64 | EXACT, absorb=ABSORB_NONE, vabc=None):
+    """S2PPSOD (s2ppsod.f:1-1143): two surfaces coupled by the coordinate
+    operator, FD leapfrog on the device.  u1, v1, vp, dm [2n] (surface A,
+    then B); pulse = (efc, e0, t0, td, tp, omg, sni, kl); prtcrl 0 .NO,
+    1 .PARTIAL, 2 .YES.  Returns dict(tab [rows][12] as printed, info, u, v,
+    uprev, vprev, veff)."""
+    L = _load()
+    n = int(np.prod(np_))
+    maxrows = int((tf - ti) / dt) + 4
+    tab = np.zeros(12 * maxrows)
+    u, v = _f64(u1).copy(), _f64(v1).copy()
+    up, vq, ve = np.zeros(2 * n), np.zeros(2 * n), np.zeros(2 * n)
+    va = None if vabc is None else _f64(vabc)
+    nr, inf = C.c_int(0), C.c_int(0)
+    efc, e0, t0, td, tp, omg, sni, kl = pulse
+    rc = L.espec_s2ppsod(len(np_), _np3(np_), dt, ti, tf, lc, _v3(shm, 0.0), xi, sh, sm1, aa, u,
+                         v, _f64(vp), _f64(dm), efc, e0, t0, td, tp, omg, sni, kl, absorb,
+                         None if va is None else va.ctypes.data, prtcrl, tol, tab.ctypes.data,
+                         maxrows, C.byref(nr), C.byref(inf), up.ctypes.data, vq.ctypes.data,
+                         ve.ctypes.data, flags)
+    _check(rc, "S2PPSOD")
+    k = min(nr.value, maxrows)
+    return dict(tab=tab[:12 * k].reshape(k, 12), info=inf.value, u=u, v=v, uprev=up, vprev=vq,
+                veff=ve)
 
 
 def psodfft(np_, dt, ti, tf, lc, sh, sm, u1, v1, vpot, flags=FP64 | EXACT, absorb=ABSORB_NONE,

@@ -93,6 +93,51 @@ __global__ __launch_bounds__(256) void fdsod_step(FdGrid g, double2 *prev, const
     }
 }
 
+// S2PPSOD's step (s2ppsod.f:260-458, 460-656, 762-975 with TPCRS =
+// '.CORD'): PSODAUX on surface A (points [0, n)) and on surface B (the
+// next n), then the coordinate coupling with w = w0 XD_i added to the new
+// values from the other surface's psi(t):
+//   UA += w VB(t), VA -= w UB(t), UB += w VA(t), VB -= w UA(t),
+// each sum rounded after PSODAUX's, as the reference's second loop does.
+__global__ __launch_bounds__(256) void fdsod2_step(FdGrid g, double2 *prev, const double2 *cur,
+                                                   const double *vp, const double *dm, double e,
+                                                   double cst, double w0, const double *xd,
+                                                   int absorb, const double *vabc)
+{
+#pragma clang fp contract(off)
+    const int n = g.n;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double2 out[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int o = k * n;
+            const double f = dm[o + i] * e;
+            const double ve = vp[o + i] - f;
+            const double hv = fd_h(g, cur + o, true, ve, i);
+            const double hu = fd_h(g, cur + o, false, ve, i);
+            const double2 a = prev[o + i], b = cur[o + i];
+            double un, vn;
+            if (absorb == ESPEC_ABSORB_VOPTIC) {
+                const double cw = cst * vabc[i];
+                un = a.x + cst * hv - cw * b.x;
+                vn = a.y - cst * hu - cw * b.y;
+            } else if (absorb == ESPEC_ABSORB_SMOOTHW) {
+                un = vabc[i] * (a.x + cst * hv);
+                vn = vabc[i] * (a.y - cst * hu);
+            } else {
+                un = a.x + cst * hv;
+                vn = a.y - cst * hu;
+            }
+            out[k] = {un, vn};
+        }
+        const double w = w0 * xd[i];
+        const double2 ca = cur[i], cb = cur[n + i];
+        const double wvb = w * cb.y, wub = w * cb.x, wva = w * ca.y, wua = w * ca.x;
+        prev[i] = {out[0].x + wvb, out[0].y - wub};
+        prev[n + i] = {out[1].x + wva, out[1].y - wua};
+    }
+}
+
 struct FdSampleArgs {
     FdGrid g;
     const double2 *x;    // psi
@@ -327,6 +372,196 @@ int espec_ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc
                            ddm.as<double>(), ef.back(), dve.as<double>());
         ESPEC_HIP_TRY(hipGetLastError());
         ESPEC_HIP_TRY(hipMemcpy(veff, dve.p, by, hipMemcpyDeviceToHost));
+    }
+    return ESPEC_OK;
+}
+
+// S2PPSOD (s2ppsod.f:1-1143) on the device: the time loop and sampling of
+// the oracle's restatement (two surfaces, the '.CORD' coupling, the fields
+// only when E0 /= 0), the printed rows as the reference prints them.
+int espec_s2ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                  const double *shm, double xi, double sh, double sm1, double aa, double *u,
+                  double *v, const double *vp, const double *dm, int efc, double e0, double t0,
+                  double td, double tp, double omg, double sni, int kl, int absorb,
+                  const double *vabc, int prtcrl, double tol, double *tab, int maxrows,
+                  int *nrows, int *info, double *uprev, double *vprev, double *veff, int flags)
+{
+    if (ndim < 1 || ndim > 3 || !np || !shm || !u || !v || !vp || !dm || lc < 1 ||
+        !(dt != 0) || !(sm1 != 0) || efc < ESPEC_PULSE_NONE || efc > ESPEC_PULSE_SIN2 ||
+        prtcrl < 0 || prtcrl > 2 || maxrows < 0 || (maxrows > 0 && !tab) ||
+        absorb < ESPEC_ABSORB_NONE || absorb > ESPEC_ABSORB_SMOOTHW ||
+        (absorb != ESPEC_ABSORB_NONE && !vabc))
+        return ESPEC_ERR_ARG;
+    if (flags & ESPEC_MIXED) return ESPEC_ERR_UNSUPPORTED;  // FP64 leapfrog
+    FdGrid g{};
+    g.ndim = ndim;
+    long long nn = 1;
+    int st = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (a < ndim && np[a] < 4) return ESPEC_ERR_ARG;
+        g.np[a] = a < ndim ? np[a] : 1;
+        g.st[a] = st;
+        st *= g.np[a];
+        nn *= g.np[a];
+        g.shm[a] = a < ndim ? shm[a] : 0.0;
+    }
+    if (nn > (1LL << 27)) return ESPEC_ERR_ARG;
+    g.n = (int)nn;
+    const int n = g.n;
+    const double shfs = 2.4188843243e-2;
+    const double cst = 2.0 * dt / shfs, csqc = cst * aa / sm1;
+    const bool field = e0 != 0.0;
+    // steps: (field, strength, coupling) of the start-up and of the loop
+    // DO T=TI+2*DT,TF+DT/2,DT (MAX(INT((m2 - m1 + m3)/m3), 0) trips, T += DT)
+    struct Step { double e, cst, w0, t; };
+    std::vector<Step> steps;
+    auto efv = [&](double t) { return field ? espec_ef(efc, e0, t0, td, tp, omg, sni, kl, t) : 0.0; };
+    steps.push_back({efv((ti + dt) / 2.0), cst / 4.0, csqc / 4.0, ti});
+    steps.push_back({efv(ti + dt), cst / 2.0, csqc / 2.0, ti + dt});
+    const double m1 = ti + 2.0 * dt;
+    long long trips = (long long)((tf + dt / 2.0 - m1 + dt) / dt);
+    if (trips < 0) trips = 0;
+    if (trips > (1LL << 26)) return ESPEC_ERR_ARG;
+    double t = m1;
+    for (long long k = 0; k < trips; ++k, t = t + dt) steps.push_back({efv(t), cst, csqc, t});
+    // printed rows: (after step index, quirk, time, field of VEFF); -1: psi0
+    struct Row { int after, quirk; double t, e; };
+    std::vector<Row> rows;
+    if (prtcrl >= 1) rows.push_back({-1, 1, ti, 0.0});
+    if (prtcrl == 2) rows.push_back({1, 2, ti + dt, steps[1].e});
+    int mc = lc == 1 ? 0 : 1;
+    for (size_t k = 2; k < steps.size(); ++k) {
+        if (++mc == lc) mc = 0;
+        if (prtcrl == 2 || (prtcrl == 1 && mc == 0))
+            rows.push_back({(int)k, 0, steps[k].t, steps[k].e});
+    }
+    std::vector<double> xd(n);
+    double x = xi;
+    for (int i = 0; i < n; ++i) {
+        x = x + sh;
+        xd[i] = x;
+    }
+    const size_t by = sizeof(double) * n;
+    std::vector<double> h0(4 * (size_t)n);
+    for (int i = 0; i < 2 * n; ++i) {
+        h0[2 * i] = u[i];
+        h0[2 * i + 1] = v[i];
+    }
+    DevBuf dpsi0, da, db, dvp, ddm, dva, dxd, dpart, dout;
+    ESPEC_HIP_TRY(dpsi0.upload(h0.data(), 4 * by));
+    ESPEC_HIP_TRY(da.upload(h0.data(), 4 * by));
+    ESPEC_HIP_TRY(db.upload(h0.data(), 4 * by));
+    ESPEC_HIP_TRY(dvp.upload(vp, 2 * by));
+    ESPEC_HIP_TRY(ddm.upload(dm, 2 * by));
+    ESPEC_HIP_TRY(dxd.upload(xd.data(), by));
+    if (absorb != ESPEC_ABSORB_NONE) ESPEC_HIP_TRY(dva.upload(vabc, by));
+    const int blocks = std::max(1, std::min(kPpsodBlocks, (n + 255) / 256));
+    const int nslot = 2 * (int)std::max<size_t>(rows.size(), 1);
+    ESPEC_HIP_TRY(dpart.alloc(sizeof(double) * kPpsodVals * (size_t)blocks * nslot));
+    ESPEC_HIP_TRY(dout.alloc(sizeof(double) * kPpsodVals * (size_t)nslot));
+    FdSampleArgs sa{};
+    sa.g = g;
+    sa.energies = 1;
+    sa.npr = 0;
+    sa.part = dpart.as<double>();
+    auto sample = [&](const double2 *xs, double e, int slot) -> hipError_t {
+        for (int k = 0; k < 2; ++k) {
+            FdSampleArgs a = sa;
+            a.x = xs + (size_t)k * n;
+            a.x0 = dpsi0.as<double2>() + (size_t)k * n;
+            a.vp = dvp.as<double>() + (size_t)k * n;
+            a.dm = ddm.as<double>() + (size_t)k * n;
+            a.e = e;
+            a.slot = 2 * slot + k;
+            hipLaunchKernelGGL(fdsod_sample, dim3(blocks), dim3(256), 0, nullptr, a);
+        }
+        return hipGetLastError();
+    };
+    const double *va = absorb != ESPEC_ABSORB_NONE ? dva.as<double>() : nullptr;
+    double2 *prev = da.as<double2>(), *cur = db.as<double2>();
+    size_t ri = 0;
+    auto rows_after = [&](int k, const double2 *xs) -> hipError_t {
+        while (ri < rows.size() && rows[ri].after == k) {
+            hipError_t e = sample(xs, rows[ri].e, (int)ri);
+            if (e != hipSuccess) return e;
+            ++ri;
+        }
+        return hipSuccess;
+    };
+    ESPEC_HIP_TRY(rows_after(-1, dpsi0.as<double2>()));
+    for (size_t k = 0; k < steps.size(); ++k) {
+        hipLaunchKernelGGL(fdsod2_step, dim3(blocks), dim3(256), 0, nullptr, g, prev, cur,
+                           dvp.as<double>(), ddm.as<double>(), steps[k].e, steps[k].cst,
+                           steps[k].w0, dxd.as<double>(), absorb, va);
+        ESPEC_HIP_TRY(hipGetLastError());
+        std::swap(prev, cur);  // cur = psi(t+dt), prev = psi(t)
+        ESPEC_HIP_TRY(rows_after((int)k, cur));
+        // after the start-up psi(t - dt) restarts from psi0 (s2ppsod.f:753-756)
+        if (k == 1) ESPEC_HIP_TRY(hipMemcpy(prev, dpsi0.p, 4 * by, hipMemcpyDeviceToDevice));
+    }
+    if (!rows.empty()) {
+        hipLaunchKernelGGL(fdsod_reduce, dim3((nslot * kPpsodVals + 255) / 256), dim3(256), 0,
+                           nullptr, dpart.as<double>(), blocks, nslot, dout.as<double>());
+        ESPEC_HIP_TRY(hipGetLastError());
+    }
+    std::vector<double> out((size_t)kPpsodVals * nslot), hs(4 * (size_t)n);
+    if (!rows.empty())
+        ESPEC_HIP_TRY(hipMemcpy(out.data(), dout.p, sizeof(double) * out.size(),
+                                hipMemcpyDeviceToHost));
+    int inf = 0;
+    double f0 = 0;
+    for (size_t r = 0; r < rows.size(); ++r) {
+        const double *a = out.data() + (size_t)kPpsodVals * (2 * r);
+        const double *b = a + kPpsodVals;
+        const double cta = a[0] * a[0] + a[1] * a[1], ctb = b[0] * b[0] + b[1] * b[1];
+        if (rows[r].quirk == 1) f0 = a[2] + b[2];  // F0 = FNTA + FNTB of psi0
+        const double fntb = rows[r].quirk == 1 ? b[2] * b[2] : b[2];
+        const double fnt = a[2] + fntb;
+        if (rows[r].quirk == 0 && absorb == ESPEC_ABSORB_NONE &&
+            std::fabs((fnt - f0) / f0) * 1.0e2 > tol)
+            inf = 1;
+        if ((int)r < maxrows) {
+            double *o = tab + 12 * r;
+            o[0] = rows[r].t;
+            o[1] = a[3] + b[3];
+            o[2] = a[4] + b[4];
+            o[3] = a[0];
+            o[4] = a[1];
+            o[5] = cta;
+            o[6] = rows[r].quirk == 2 ? ctb : b[0];
+            o[7] = b[1];
+            o[8] = ctb;
+            o[9] = a[2];
+            o[10] = fntb;
+            o[11] = fnt;
+        }
+    }
+    if (nrows) *nrows = (int)rows.size();
+    if (info) *info = inf;
+    ESPEC_HIP_TRY(hipMemcpy(hs.data(), cur, 4 * by, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 2 * n; ++i) {
+        u[i] = hs[2 * i];
+        v[i] = hs[2 * i + 1];
+    }
+    if (uprev || vprev) {
+        ESPEC_HIP_TRY(hipMemcpy(hs.data(), prev, 4 * by, hipMemcpyDeviceToHost));
+        for (int i = 0; i < 2 * n; ++i) {
+            if (uprev) uprev[i] = hs[2 * i];
+            if (vprev) vprev[i] = hs[2 * i + 1];
+        }
+    }
+    if (veff) {  // VPOT as the reference leaves it: the last CHPOT when E0 /= 0
+        if (field) {
+            DevBuf dve;
+            ESPEC_HIP_TRY(dve.alloc(2 * by));
+            hipLaunchKernelGGL(fdsod_veff, dim3(blocks), dim3(256), 0, nullptr, 2 * n,
+                               dvp.as<double>(), ddm.as<double>(), steps.back().e,
+                               dve.as<double>());
+            ESPEC_HIP_TRY(hipGetLastError());
+            ESPEC_HIP_TRY(hipMemcpy(veff, dve.p, 2 * by, hipMemcpyDeviceToHost));
+        } else {
+            std::copy(vp, vp + 2 * (size_t)n, veff);
+        }
     }
     return ESPEC_OK;
 }

@@ -312,6 +312,19 @@ module espec_capi
       integer(c_int) :: rc
     end function
 
+    function espec_s2ppsod(ndim, np, dt, ti, tf, lc, shm, xi, sh, sm1, aa, u, v, vp, dm, efc, &
+                           e0, t0, td, tp, omg, sni, kl, absorb, vabc, prtcrl, tol, tab, &
+                           maxrows, nrows, info, uprev, vprev, veff, flags) &
+        bind(C, name='espec_s2ppsod') result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: ndim, lc, efc, kl, absorb, prtcrl, maxrows, flags
+      integer(c_int) :: np(*), nrows, info
+      real(c_double), value :: dt, ti, tf, xi, sh, sm1, aa, e0, t0, td, tp, omg, sni, tol
+      real(c_double) :: shm(*), u(*), v(*), vp(*), dm(*), vabc(*), tab(*), uprev(*), vprev(*), &
+                        veff(*)
+      integer(c_int) :: rc
+    end function
+
     function espec_lanczsg(ndim, np, il, iu, maxint, abstol, shm, vpot, eigvl, eigvc, &
                            log, logcap, nlog, flags) bind(C, name='espec_lanczsg') result(rc)
       import :: c_int, c_double

@@ -1034,6 +1034,103 @@ subroutine FFTNT(X, Y, N, M, ITYPE)
   if (rc /= 0) write(*,'(A,I3)') '<<<>>> FFTNT on the device failed, code ', rc
 end subroutine FFTNT
 
+! s2ppsod.f:1-1143: two surfaces coupled by the coordinate operator (the
+! routine's active TPCRS = '.CORD'), FD leapfrog with a pulse, on the
+! device.  U1, V1, VPOT, DM hold surface A in 1..N and B in N+1..2N.  The
+! routine's own diagnostic lines and its table (FORMAT 1011/1021, with the
+! reference's FNTB**2 at TI and CTB in the CRB column after the start-up)
+! are printed; U2, V2 return psi(TF), U1, V1 psi(TF - DT) and VPOT the last
+! VEFF.  The files (PRTVEFF, PRTEIGVC2, PRTPULS, .LASTWP) are not written:
+! INFO = 6.  An unknown EFC gives INFO = 2; an unknown TPABSOR with ABSORB
+! stops as PSODAUX does.
+subroutine S2PPSOD(ABSORB, PRTVEFF, PRTEIGVC2, PRTPULS, DIM, EFC, PRTCRL, TPABSOR, INFO, LC, &
+                   MP, MPI, MPR, MXDCT, N, NSHOT, NPR, KP, NISPG, KL, E0, T0, TD, TP, OMG, SNI, &
+                   DT, TI, TF, TOL, DELQ, QC, AA, NP, ND, SH, SHM, U1, U2, V1, V2, XI, XP, XF, &
+                   SM, DM, VPOT, VABC, WORK, VAR, LNZVC)
+  use espec_capi
+  implicit none
+  logical :: ABSORB, PRTVEFF, PRTEIGVC2, PRTPULS
+  character(len=*) :: DIM, EFC, PRTCRL, TPABSOR
+  integer :: INFO, LC, MP, MPI, MPR, MXDCT, N, NSHOT, NPR, KP, NISPG, KL
+  real(8) :: DT, TI, TF, TOL, E0, T0, TD, TP, OMG, SNI, DELQ, QC, AA
+  integer :: NP(*), ND(*)
+  real(8) :: SH(*), SHM(*), U1(*), U2(*), V1(*), V2(*), XI(*), XP(*), XF(*), SM(*), DM(*)
+  real(8) :: VPOT(*), VABC(*), WORK(*), VAR(*), LNZVC(MXDCT, *)
+  integer :: ndim, efk, kind, prt, maxrows, nrows, inf, rc, r
+  real(8) :: cst, csqc
+  real(8), allocatable :: vp(:), tab(:)
+
+  INFO = 0
+  ndim = dim_of(DIM)
+  efk = pulse_of(EFC)
+  cst = 2.0d0 * DT / 2.4188843243d-2
+  csqc = cst * AA / SM(1)
+  ! s2ppsod.f:130-132: the routine's own diagnostic output
+  write(*,*) 'oi', csqc, 0d0, 0d0
+  write(*,*) DELQ, QC, AA
+  write(*,*) SH(1), (XF(1) - XI(1)) / (N - 1), SHM(1), csqc, SHM(1) * 30, AA
+  flush(6)
+  if (ndim == 0 .or. PRTVEFF .or. PRTEIGVC2 .or. PRTPULS .or. starts(PRTCRL, '.LASTWP')) then
+    INFO = ESPEC_ERR_UNSUPPORTED
+    return
+  end if
+  if (efk < 0) then
+    write(*,'(A)') '<<<>>> Desired laser pulse wasn''t found. <<<>>>'
+    INFO = ESPEC_ERR_ARG
+    return
+  end if
+  kind = ESPEC_ABSORB_NONE
+  if (ABSORB) then  ! sod.f:55-120
+    if (starts(TPABSOR, '.VOPTIC')) then
+      kind = ESPEC_ABSORB_VOPTIC
+    else if (starts(TPABSOR, '.SMOOTHW') .or. starts(TPABSOR, '.SMRS')) then
+      kind = ESPEC_ABSORB_SMOOTHW
+    else
+      write(*,*) '<<<>>> SOD propagation error <<<>>>'
+      error stop
+    end if
+  end if
+  prt = 0
+  if (starts(PRTCRL, '.PARTIAL')) prt = 1
+  if (starts(PRTCRL, '.YES')) prt = 2
+  maxrows = int((TF - TI) / DT) + 4
+  allocate(vp(2 * N), tab(12 * maxrows))
+  vp = VPOT(1:2 * N)
+  ! psi0 comes in U1, V1 (s2ppsod.f:143-149); U2, V2 carry it to the device
+  U2(1:2 * N) = U1(1:2 * N)
+  V2(1:2 * N) = V1(1:2 * N)
+  nrows = 0
+  inf = 0
+  rc = espec_s2ppsod(ndim, NP, DT, TI, TF, LC, SHM, XI(1), SH(1), SM(1), AA, U2, V2, vp, DM, &
+                     efk, E0, T0, TD, TP, OMG, SNI, KL, kind, VABC, prt, TOL, tab, maxrows, &
+                     nrows, inf, U1, V1, VPOT, ESPEC_FP64)
+  deallocate(vp)
+  if (rc /= 0) then
+    INFO = rc
+    deallocate(tab)
+    return
+  end if
+  INFO = inf
+  if (prt > 0) then
+    write(*,*)
+    if (DIM == '.1D') then
+      write(*,*) 'Auto-correlation function:'
+    else
+      write(*,*) 'Partial auto-correlation function:'
+    end if
+    write(*,*) '=================================='
+    write(*,1011)
+    do r = 1, min(nrows, maxrows)
+      write(*,1021) tab(12 * (r - 1) + 1:12 * r)
+    end do
+  end if
+  deallocate(tab)
+  flush(6)
+1011 format(/,8X,'*t/fs*',6X,'*ET/a.u.*',6X,'*E0/a.u.*',6X,'*CIA*',6X,'*CRA*',6X,'*CTA*',8X, &
+            '*CIB*',6X,'*CRB*',6X,'*CTB*',8X,'*FTA*',8X,'*FTB*',8X,'*FT*')
+1021 format(1X,F14.6,3X,2(E12.6,3X),2(F8.5,3X),E10.4,3X,2(F8.5,3X),E10.4,3X,2(E10.4,3X),F8.6)
+end subroutine S2PPSOD
+
 ! ppsod.f:1-548: the pulse-driven finite-difference leapfrog (PSODAUX,
 ! sod.f:3-122) on the device (espec_ppsod), with the reference's table
 ! (FORMAT 1011/1012/1021: t, ET, E0, Pr, Pi, |c|^2, the populations of

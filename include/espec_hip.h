@@ -252,6 +252,28 @@ int espec_pplnz_absorb(int ndim, const int *np, int mp, double abstol, double dt
                        double sni, int kl, double *veff, double *cr, double *ci,
                        double *nrm, int maxsamp, int *nsamp, int flags, int absorb,
                        const double *vabc);
+/* Replaces S2PPSOD (s2ppsod.f:1-1143): two surfaces A and B (u, v, vp, dm
+ * [2N], A first) propagated by the FD leapfrog of PSODAUX (sod.f:3-122)
+ * with VEFF = VP - DM E(T) per surface when e0 != 0, coupled by the
+ * routine's active coordinate coupling (TPCRS = '.CORD', s2ppsod.f:95):
+ * after each step UA += w VB(t), VA -= w UB(t), UB += w VA(t), VB -= w UA(t)
+ * with w = c (CST aa / sm1) XD_i, c = 1/4, 1/2, 1 for the quarter, half and
+ * full steps and XD_i = xi + (i+1) sh accumulated as the reference does.
+ * vabc [N] serves both surfaces (s2ppsod.f passes VABC(1) for both).
+ * prtcrl 0 .NO, 1 .PARTIAL, 2 .YES selects the printed rows; tab
+ * [maxrows][12] receives them as the reference prints them (T, ET, ET0,
+ * CRA, CIA, CTA, CRB, CIB, CTB, FNTA, FNTB, FNT, with its FNTB**2 at TI and
+ * CTB in the CRB column after the start-up), *nrows their count, *info 1
+ * where the norm drifts by more than tol percent (no absorber).  u, v
+ * return psi(TF); uprev, vprev (may be NULL) psi(TF - DT); veff (may be
+ * NULL) [2N] VPOT as the reference leaves it.  FP64 only. */
+int espec_s2ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                  const double *shm, double xi, double sh, double sm1, double aa, double *u,
+                  double *v, const double *vp, const double *dm, int efc, double e0, double t0,
+                  double td, double tp, double omg, double sni, int kl, int absorb,
+                  const double *vabc, int prtcrl, double tol, double *tab, int maxrows,
+                  int *nrows, int *info, double *uprev, double *vprev, double *veff, int flags);
+
 /* ---------------------------------------------------------------------
  * Pulse-driven finite-difference leapfrog.  Replaces PPSOD (ppsod.f:1-548)
  * with its step PSODAUX (sod.f:3-122): psi(t+dt) = psi(t-dt) - i CST H psi(t),

@@ -1471,3 +1471,172 @@ int orc_ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
     free(u0);
     return nc + 1;
 }
+
+/* s2ppsod.f:1-1143: two surfaces A and B coupled by the coordinate
+ * operator, each advanced by PSODAUX with its own VEFF = V - DM E(T), with
+ * the coupling the routine selects by assignment (TPCRS = '.CORD',
+ * s2ppsod.f:95; CSQC = CST AA / SM(1), s2ppsod.f:127): after both
+ * surfaces' PSODAUX of a step,
+ *   U2A += w V1B, V2A -= w U1B, U2B += w V1A, V2B -= w U1A   (point by point)
+ * with w = (CSQC/4) XD, (CSQC/2) XD, CSQC XD for the quarter, half and full
+ * steps and XD accumulated from XI(1) by SH(1) before each use, so point i
+ * (0-based) sees XI(1) + (i+1) SH(1) (s2ppsod.f:434-444, 632-642,
+ * 951-961).  U1/V1 then hold psi(t) (PSODAUX swaps, sod.f:88-121).  The
+ * fields are taken at (TI+DT)/2, TI+DT and each T of DO T=TI+2*DT,TF+DT/2,DT
+ * (s2ppsod.f:760), and only when E0 /= 0 (s2ppsod.f:260-267, 460-467,
+ * 762-775).
+ *
+ * The printed table (FORMAT 1021) goes to tab[row][12] = (T, ET, ET0, CRA,
+ * CIA, CTA, CRB, CIB, CTB, FNTA, FNTB, FNT) as printed:
+ *   - at TI when PRTCRL is .YES or .PARTIAL, with FNTB = FNRB**2
+ *     (s2ppsod.f:186);
+ *   - after the start-up only when PRTCRL is .YES (the .PARTIAL test
+ *     compares the reset MC with LC, s2ppsod.f:660-663), with CTB printed in
+ *     the CRB column (s2ppsod.f:711);
+ *   - in the loop every LC steps for .PARTIAL, every step for .YES; there,
+ *     without absorber, INFO = 1 when the total norm has drifted by more
+ *     than TOL percent (s2ppsod.f:1030-1034).
+ * prtcrl: 0 .NO, 1 .PARTIAL, 2 .YES.  u, v [2n] return U2, V2 (psi(TF));
+ * u1, v1 (may be NULL) U1, V1.  Returns the number of rows. */
+struct s2ctx {
+    int ndim, n, absorb;
+    const int *np;
+    const double *shm, *vp, *dm, *vabc, *xd;
+    double *u0, *v0, *u1, *v1, *ve, *u, *v, *tab;
+    double f0, tol;
+    int rows, maxrows, *info;
+};
+
+static void s2_row(struct s2ctx *c, double t, int quirk)
+{
+    const int n = c->n;
+    double s[2][6];
+    for (int k = 0; k < 2; ++k) {
+        const int o = k * n;
+        double cr, ci;
+        corr(n, c->u0 + o, c->v0 + o, c->u + o, c->v + o, &cr, &ci);
+        s[k][0] = cr;
+        s[k][1] = ci;
+        s[k][2] = cr * cr + ci * ci;
+        s[k][3] = orc_fcorrel(n, c->u + o, c->u + o) + orc_fcorrel(n, c->v + o, c->v + o);
+        s[k][4] = orc_eigenerg(c->ndim, c->np, c->shm, c->u + o, c->ve + o) +
+                  orc_eigenerg(c->ndim, c->np, c->shm, c->v + o, c->ve + o);
+        s[k][5] = orc_eigenerg(c->ndim, c->np, c->shm, c->u + o, c->vp + o) +
+                  orc_eigenerg(c->ndim, c->np, c->shm, c->v + o, c->vp + o);
+    }
+    const double fntb = quirk == 1 ? s[1][3] * s[1][3] : s[1][3];
+    const double fnt = s[0][3] + fntb;
+    if (quirk == 0 && c->absorb == ORC_ABS_NONE && fabs((fnt - c->f0) / c->f0) * 1.0e2 > c->tol)
+        *c->info = 1;
+    if (c->rows < c->maxrows) {
+        double *r = c->tab + 12 * (size_t)c->rows;
+        r[0] = t;
+        r[1] = s[0][4] + s[1][4];
+        r[2] = s[0][5] + s[1][5];
+        r[3] = s[0][0];
+        r[4] = s[0][1];
+        r[5] = s[0][2];
+        r[6] = quirk == 2 ? s[1][2] : s[1][0];
+        r[7] = s[1][1];
+        r[8] = s[1][2];
+        r[9] = s[0][3];
+        r[10] = fntb;
+        r[11] = fnt;
+    }
+    ++c->rows;
+}
+
+static void s2_step(struct s2ctx *c, double e, int field, double cstk, double w0)
+{
+    const int n = c->n;
+    for (int k = 0; k < 2; ++k) {
+        const int o = k * n;
+        if (field) chpot(n, c->vp + o, c->dm + o, e, c->ve + o);
+        orc_psodaux(c->ndim, c->np, cstk, c->shm, c->u1 + o, c->u + o, c->v1 + o, c->v + o,
+                    c->ve + o, c->absorb, c->vabc);
+    }
+    double *u = c->u, *v = c->v;
+    const double *u1 = c->u1, *v1 = c->v1;
+    for (int i = 0; i < n; ++i) {
+        const int j = n + i;
+        const double w = w0 * c->xd[i];
+        u[i] = u[i] + w * v1[j];
+        v[i] = v[i] - w * u1[j];
+        u[j] = u[j] + w * v1[i];
+        v[j] = v[j] - w * u1[i];
+    }
+}
+
+int orc_s2ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                const double *shm, double xi, double sh, double sm1, double aa, double *u,
+                double *v, const double *vp, const double *dm, int efc, double e0, double t0,
+                double td, double tp, double omg, double sni, int kl, int absorb,
+                const double *vabc, int prtcrl, double tol, double *tab, int maxrows,
+                int *info, double *u1out, double *v1out)
+{
+    const int n = ntot_of(ndim, np), n2 = 2 * n;
+    const double cst = 2.0 * dt / ORC_SHFS, csqc = cst * aa / sm1;
+    double *buf = malloc(sizeof(double) * 5 * (size_t)n2 + sizeof(double) * n);
+    struct s2ctx c;
+    c.ndim = ndim;
+    c.n = n;
+    c.absorb = absorb;
+    c.np = np;
+    c.shm = shm;
+    c.vp = vp;
+    c.dm = dm;
+    c.vabc = vabc;
+    c.u0 = buf;
+    c.v0 = buf + n2;
+    c.u1 = buf + 2 * n2;
+    c.v1 = buf + 3 * n2;
+    c.ve = buf + 4 * n2;
+    double *xd = buf + 5 * n2;
+    c.xd = xd;
+    c.u = u;
+    c.v = v;
+    c.tab = tab;
+    c.tol = tol;
+    c.rows = 0;
+    c.maxrows = maxrows;
+    c.info = info;
+    memcpy(c.u0, u, sizeof(double) * n2);
+    memcpy(c.v0, v, sizeof(double) * n2);
+    memcpy(c.u1, u, sizeof(double) * n2);
+    memcpy(c.v1, v, sizeof(double) * n2);
+    memcpy(c.ve, vp, sizeof(double) * n2);
+    double x = xi;
+    for (int i = 0; i < n; ++i) {
+        x = x + sh;
+        xd[i] = x;
+    }
+    *info = 0;
+    c.f0 = (orc_fcorrel(n, u, u) + orc_fcorrel(n, v, v)) +
+           (orc_fcorrel(n, u + n, u + n) + orc_fcorrel(n, v + n, v + n));
+    const int field = e0 != 0.0;
+    double t = ti;
+    if (prtcrl >= 1) s2_row(&c, t, 1);
+    s2_step(&c, orc_ef(efc, e0, t0, td, tp, omg, sni, kl, (t + dt) / 2.0), field, cst / 4.0,
+            csqc / 4.0);
+    s2_step(&c, orc_ef(efc, e0, t0, td, tp, omg, sni, kl, t + dt), field, cst / 2.0, csqc / 2.0);
+    int mc = 1;
+    t = ti + dt;
+    if (mc == lc) mc = 0;
+    if (prtcrl == 2) s2_row(&c, t, 2);
+    memcpy(c.u1, c.u0, sizeof(double) * n2);
+    memcpy(c.v1, c.v0, sizeof(double) * n2);
+    /* DO T=TI+2*DT,TF+DT/2,DT: MAX(INT((m2 - m1 + m3)/m3), 0) trips, T += DT */
+    const double m1 = ti + 2.0 * dt;
+    long long trips = (long long)((tf + dt / 2.0 - m1 + dt) / dt);
+    if (trips < 0) trips = 0;
+    t = m1;
+    for (long long k = 0; k < trips; ++k, t = t + dt) {
+        s2_step(&c, orc_ef(efc, e0, t0, td, tp, omg, sni, kl, t), field, cst, csqc);
+        if (++mc == lc) mc = 0;
+        if (prtcrl == 2 || (prtcrl == 1 && mc == 0)) s2_row(&c, t, 0);
+    }
+    if (u1out) memcpy(u1out, c.u1, sizeof(double) * n2);
+    if (v1out) memcpy(v1out, c.v1, sizeof(double) * n2);
+    free(buf);
+    return c.rows;
+}

@@ -218,6 +218,12 @@ void orc_psodaux(int ndim, const int *np, double cst, const double *shm, double 
                  const double *vabc);
 /* PPSOD (ppsod.f:1-548) time loop: returns the number of C(t) samples
  * (sample 0 = C(TI), then every LC steps); u, v receive psi(TF). */
+int orc_s2ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
+                const double *shm, double xi, double sh, double sm1, double aa, double *u,
+                double *v, const double *vp, const double *dm, int efc, double e0, double t0,
+                double td, double tp, double omg, double sni, int kl, int absorb,
+                const double *vabc, int prtcrl, double tol, double *tab, int maxrows,
+                int *info, double *u1out, double *v1out);
 int orc_ppsod(int ndim, const int *np, double dt, double ti, double tf, int lc,
               const double *shm, double *u, double *v, const double *vp, const double *dm,
               int efc, double e0, double t0, double td, double tp, double omg, double sni,

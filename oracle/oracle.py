@@ -102,6 +102,10 @@ class Oracle:
         L.orc_ppsod.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _ip, _dp, _dp,
                                 _dp, _dp, _dp, _dp, _ip, _ip, _d, _d, _d, _d, _d, _ip]
         L.orc_ppsod.restype = _ip
+        L.orc_s2ppsod.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _dp, _dp, _dp, _dp, _d, _d,
+                                  _d, _d, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _d, _ip,
+                                  _dp, _d, _ip, C.POINTER(_ip), _d, _d]
+        L.orc_s2ppsod.restype = _ip
         L.orc_absorbingbc.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d, C.POINTER(_dp)]
         L.orc_absorbingbc.restype = _ip
         L.orc_init_cond.argtypes = [_ip, _i, _ip, _d, _d, _d, _d, _d, _d, _d]
@@ -305,6 +309,28 @@ class Oracle:
                                va, ve, cr, ci, nr, ns)
         return dict(cr=cr[:k], ci=ci[:k], norm=nr[:k], u=u, v=v, veff=ve)
 
+    def s2ppsod(self, np_, dt, ti, tf, lc, shm, xi, sh, sm1, aa, u, v, vp, dm, pulse,
+                prtcrl=2, tol=1e30, absorb=ABS_NONE, vabc=None, maxrows=None):
+        """S2PPSOD (s2ppsod.f:1-1143) with its .CORD coupling: u, v, vp, dm
+        [2n] (surface A then B); dict(tab [rows][12] as printed (T, ET, ET0,
+        CRA, CIA, CTA, CRB, CIB, CTB, FNTA, FNTB, FNT), info, u, v (psi(TF)),
+        u1, v1)."""
+        n = int(np.prod(np_))
+        if maxrows is None:
+            maxrows = int((tf - ti) / dt) + 4
+        tab = np.zeros(12 * maxrows)
+        u, v = _f64(u).copy(), _f64(v).copy()
+        u1, v1 = np.zeros(2 * n), np.zeros(2 * n)
+        va = np.zeros(n) if vabc is None else _f64(vabc)
+        info = _ip(0)
+        efc, e0, t0, td, tp, omg, sni, kl = pulse
+        rows = self.lib.orc_s2ppsod(len(np_), _np3(np_), dt, ti, tf, lc, _v3(shm, 0.0), xi, sh,
+                                    sm1, aa, u, v, _f64(vp), _f64(dm), efc, e0, t0, td, tp, omg,
+                                    sni, kl, absorb, va, prtcrl, tol, tab, maxrows,
+                                    C.byref(info), u1, v1)
+        return dict(tab=tab[:12 * min(rows, maxrows)].reshape(-1, 12), info=info.value, u=u, v=v,
+                    u1=u1, v1=v1)
+
     def plnz_step(self, np_, mp, abstol, cst, shm, u, v, vpot, mode=REF):
         u, v = _f64(u).copy(), _f64(v).copy()
         info = self.lib.orc_plnz_step(len(np_), _np3(np_), mp, abstol, cst,
@@ -465,6 +491,9 @@ class Reference:
                                        _d, _d, _d, _ip, P(_ip)]
         L.ref_ppsod.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _d, _d, _d, _d, _ip, _dp, _dp,
                                 _dp, _dp, _dp, _dp, _ip, _ip, _d, _d, _d, _d, _ip, P(_ip)]
+        L.ref_s2ppsod.argtypes = [_ip, _i, _dp, _dp, _dp, _ip, _d, _dp, _dp, _dp, _dp, _d, _d,
+                                  _d, _d, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _d, _ip,
+                                  _dp, _d, _d, P(_ip)]
         L.ref_fft.argtypes = [_d, _d, _ip, _ip, _ip, _ip]
         L.ref_fftnt.argtypes = [_d, _d, _ip, _ip, _ip]
         L.ref_digt.argtypes = [_ip, _i, _d, _d, _d, P(_dp)]
@@ -543,6 +572,21 @@ class Reference:
                                   _v3(shm, 0.0), u, v, _f64(vpot), _f64(mask), cr, ci, nsamp,
                                   C.byref(info))
         return dict(cr=cr, ci=ci, u=u, v=v, info=info.value)
+
+    def s2ppsod(self, np_, dt, ti, tf, lc, shm, xi, sh, sm1, aa, u, v, vp, dm, pulse,
+                prtcrl=2, tol=1e30, absorb=0, vabc=None):
+        """S2PPSOD as built: prints its table on unit 6 (flushed); returns
+        dict(info, u, v (U2, V2), u1, v1)."""
+        n = int(np.prod(np_))
+        u, v = _f64(u).copy(), _f64(v).copy()
+        u1, v1 = np.zeros(2 * n), np.zeros(2 * n)
+        va = np.ones(n) if vabc is None else _f64(vabc)
+        info = _ip(0)
+        efc, e0, t0, td, tp, omg, sni, kl = pulse
+        self.lib.ref_s2ppsod(len(np_), _np3(np_), dt, ti, tf, lc, _v3(shm, 0.0), xi, sh, sm1, aa,
+                             u, v, _f64(vp), _f64(dm), efc, e0, t0, td, tp, omg, sni, kl, absorb,
+                             va, prtcrl, tol, u1, v1, C.byref(info))
+        return dict(info=info.value, u=u, v=v, u1=u1, v1=v1)
 
     def ppsod(self, np_, dt, ti, tf, lc, shm, u, v, vp, dm, pulse, absorb=0, vabc=None,
               nsamp=None):

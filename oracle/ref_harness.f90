@@ -665,4 +665,60 @@ contains
     deallocate(iwk, work, wk, ev, z)
   end subroutine ref_mtrxdiag
 
+  ! S2PPSOD (s2ppsod.f:1-1143): two coupled surfaces, u/v/vp/dm [2n]
+  ! (surface A then B), vabc [n]; prtcrl 0 .NO, 1 .PARTIAL, 2 .YES.  The
+  ! table goes to unit 6 (flushed on return); u, v receive U2, V2 and
+  ! u1o, v1o U1, V1.  No files: PRTVEFF, PRTEIGVC2 and PRTPULS are false.
+  subroutine ref_s2ppsod(ndim, np, dt, ti, tf, lc, shm, xi1, sh1, sm1, aa, u, v, vp, dm, &
+                         efc, e0, t0, td, tp, omg, sni, kl, absorb, vabcin, prtcrl, tol, &
+                         u1o, v1o, info) bind(C, name='ref_s2ppsod')
+    integer(c_int), value :: ndim, lc, efc, kl, absorb, prtcrl
+    integer(c_int) :: np(3)
+    real(c_double), value :: dt, ti, tf, xi1, sh1, sm1, aa, e0, t0, td, tp, omg, sni, tol
+    real(c_double) :: shm(3), u(*), v(*), vp(*), dm(*), vabcin(*), u1o(*), v1o(*)
+    integer(c_int) :: info
+    character(len=8) :: dims, efcs, tpab, prt
+    integer :: n, i, nd(3), inf
+    real(8), allocatable :: vabc(:), lnzvc(:,:), u1(:), u2(:), v1(:), v2(:), work(:), vpot(:), &
+                            dmm(:)
+    real(8) :: xi(3), xf(3), xp(3), sh(3), sm(3), var(4)
+    n = np(1)
+    do i = 2, ndim
+      n = n*np(i)
+    end do
+    allocate(vabc(n), lnzvc(n, 4), u1(2*n), u2(2*n), v1(2*n), v2(2*n), work(2*n), &
+             vpot(2*n), dmm(2*n))
+    lnzvc = 0d0
+    vabc = 1d0
+    if (absorb /= 0) vabc = vabcin(1:n)
+    tpab = '.NONE   '
+    if (absorb == 1) tpab = '.VOPTIC '
+    if (absorb == 2) tpab = '.SMOOTHW'
+    prt = '.NO     '
+    if (prtcrl == 1) prt = '.PARTIAL'
+    if (prtcrl == 2) prt = '.YES    '
+    vpot = vp(1:2*n)
+    dmm = dm(1:2*n)
+    u1 = u(1:2*n); v1 = v(1:2*n); u2 = 0d0; v2 = 0d0
+    xi = 0d0; xf = 0d0; xp = 0d0; sh = 0d0; sm = 1d0; var = 0d0; nd = 0
+    xi(1) = xi1
+    sh(1) = sh1
+    sm(1) = sm1
+    xf(1) = xi1 + (np(1) - 1)*sh1
+    inf = 0
+    call dimstr(ndim, dims)
+    call efcstr(efc, efcs)
+    call S2PPSOD(absorb /= 0, .false., .false., .false., dims, efcs, prt, tpab, inf, lc, 2, 4, &
+                 3, n, n, 1000000, 0, 0, 1, kl, e0, t0, td, tp, omg, sni, dt, ti, tf, tol, &
+                 1d0, 0d0, aa, np, nd, sh, shm, u1, u2, v1, v2, xi, xp, xf, sm, dmm, vpot, &
+                 vabc, work, var, lnzvc)
+    flush(6)
+    info = inf
+    u(1:2*n) = u2
+    v(1:2*n) = v2
+    u1o(1:2*n) = u1
+    v1o(1:2*n) = v1
+    deallocate(vabc, lnzvc, u1, u2, v1, v2, work, vpot, dmm)
+  end subroutine ref_s2ppsod
+
 end module espec_ref_harness

@@ -23,7 +23,7 @@ OBJ = os.path.join(ROOT, "oracle", "_ref", "obj", "espec.o")
 REPLACED = ["pspofft_", "pspofftaux_", "psodfft_", "psodfftaux_", "plnz_", "plnzaux_", "lnz_",
             "reort_", "lanczs_", "au_", "au_1d_", "au_2d_", "au_3d_", "fft_", "fftnt_", "digt_",
             "fcorrel_", "ecnorm_", "efft4_", "eigenerg_", "ran_", "hwhm_", "spectrumtd_",
-            "pplnz_", "absorbingbc_", "ppsod_", "lanczsg_", "mtrxdiag_"]
+            "pplnz_", "absorbingbc_", "ppsod_", "lanczsg_", "mtrxdiag_", "s2ppsod_"]
 
 
 @pytest.fixture(scope="module")
