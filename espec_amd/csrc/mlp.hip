@@ -1,7 +1,9 @@
 // Kernels of the Krylov-step controller network (layout in mlp.hpp):
 // inference, the gradient of one minibatch, and device-resident training
-// with Adam.  bfloat16 operands, FP32 sums; the FP32 master parameters are
-// updated and their bfloat16 image refreshed after every minibatch.
+// with Adam.  Every layer product and every weight gradient is a chain of
+// 16x16x16 bfloat16 matrix-core products with FP32 accumulation
+// (v_mfma_f32_16x16x16_bf16); the FP32 master parameters are updated and
+// their bfloat16 image refreshed after every minibatch.
 //
 // No reference counterpart: the controller picks the Krylov order and the
 // time-step scale of the short-iterative Lanczos propagator (sil.hip),
@@ -32,39 +34,92 @@ __device__ __forceinline__ float sigm(float y) { return 1.0f / (1.0f + expf(-y))
 
 __device__ __forceinline__ float w(const uint16_t *wb, int i) { return bf2f(wb[i]); }
 
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// d + A B for one 16x16x16 block on the matrix cores.  a(i, k) and b(k, j)
+// return bfloat16-representable values; lane l supplies A's row i = l % 16
+// and B's column j = l % 16 at k = 4 (l / 16) + v, v < 4.  On return,
+// element v of lane l holds D's row 4 (l / 16) + v, column l % 16.
+template <class FA, class FB>
+__device__ __forceinline__ f32x4 mma(f32x4 d, FA a, FB b)
+{
+    const int l = threadIdx.x & 63, r = l & 15, q = (l >> 4) * 4;
+    bf16x4 x, y;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        x[v] = (short)f2bf(a(r, q + v));
+        y[v] = (short)f2bf(b(q + v, r));
+    }
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, y, d, 0, 0, 0);
+}
+
 }  // namespace
 
-// One wavefront per sample: lane j is hidden unit j of both layers.
+// One wavefront per 16 samples: the three layers as matrix-core products
+// over [16 samples x 64 units] tiles (inputs padded from 8 to 16, outputs
+// from 2 to 16 with zeros), activations exchanged through LDS.
 __global__ __launch_bounds__(64) void mlp_infer(const uint16_t *wb, const float *feat, float *out,
                                                 int n)
 {
-    __shared__ float h1s[kMlpHid];
-    const int s = blockIdx.x;
-    if (s >= n) return;
-    const int j = threadIdx.x;
-    float z = 0;
-    for (int k = 0; k < kMlpIn; ++k) z += w(wb, kMlpW1 + j * kMlpIn + k) * bfr(feat[s * kMlpIn + k]);
-    h1s[j] = bfr(tanhf(z + w(wb, kMlpB1 + j)));
-    __syncthreads();
-    float z2 = 0;
-    for (int k = 0; k < kMlpHid; ++k) z2 += w(wb, kMlpW2 + j * kMlpHid + k) * h1s[k];
-    const float h2 = bfr(tanhf(z2 + w(wb, kMlpB2 + j)));
-    for (int o = 0; o < kMlpOut; ++o) {
-        const float y = wave_sum_f(w(wb, kMlpW3 + o * kMlpHid + j) * h2);
-        if (j == 0) out[s * kMlpOut + o] = sigm(y + w(wb, kMlpB3 + o));
+    __shared__ float hs[16][kMlpHid + 1];
+    const int s0 = blockIdx.x * 16, l = threadIdx.x, c = l & 15, r0 = (l >> 4) * 4;
+#pragma unroll
+    for (int nt = 0; nt < kMlpHid / 16; ++nt) {
+        f32x4 d = {0, 0, 0, 0};
+        d = mma(d,
+                [&](int i, int k) {
+                    return (k < kMlpIn && s0 + i < n) ? bfr(feat[(long long)(s0 + i) * kMlpIn + k])
+                                                      : 0.0f;
+                },
+                [&](int k, int j) { return k < kMlpIn ? w(wb, kMlpW1 + (nt * 16 + j) * kMlpIn + k) : 0.0f; });
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = nt * 16 + c;
+            hs[r0 + v][j] = bfr(tanhf(d[v] + w(wb, kMlpB1 + j)));
+        }
     }
+    __syncthreads();
+    f32x4 d2[kMlpHid / 16] = {};
+#pragma unroll
+    for (int kt = 0; kt < kMlpHid / 16; ++kt)
+#pragma unroll
+        for (int nt = 0; nt < kMlpHid / 16; ++nt)
+            d2[nt] = mma(d2[nt], [&](int i, int k) { return hs[i][kt * 16 + k]; },
+                         [&](int k, int j) { return w(wb, kMlpW2 + (nt * 16 + j) * kMlpHid + kt * 16 + k); });
+    __syncthreads();  // every read of the first layer's activations is done
+#pragma unroll
+    for (int nt = 0; nt < kMlpHid / 16; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = nt * 16 + c;
+            hs[r0 + v][j] = bfr(tanhf(d2[nt][v] + w(wb, kMlpB2 + j)));
+        }
+    __syncthreads();
+    f32x4 y = {0, 0, 0, 0};
+#pragma unroll
+    for (int kt = 0; kt < kMlpHid / 16; ++kt)
+        y = mma(y, [&](int i, int k) { return hs[i][kt * 16 + k]; },
+                [&](int k, int j) { return j < kMlpOut ? w(wb, kMlpW3 + j * kMlpHid + kt * 16 + k) : 0.0f; });
+    if (c < kMlpOut)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int sm = s0 + r0 + v;
+            if (sm < n) out[(long long)sm * kMlpOut + c] = sigm(y[v] + w(wb, kMlpB3 + c));
+        }
 }
 
 namespace {
 
-constexpr int kTrainNT = 256;
+constexpr int kTrainNT = 256;  // four wavefronts
 constexpr int B = kMlpBatch;
+constexpr int NT16 = kMlpHid / 16;
 
 struct MlpLds {
     float x[B][kMlpIn];
-    float h1[B][kMlpHid];
-    float h2[B][kMlpHid];   // after the output layer's backward: dz1
-    float d2[B][kMlpHid];   // dz2
+    float h1[B][kMlpHid + 1];
+    float h2[B][kMlpHid + 1];  // after the output layer's backward: dz1
+    float d2[B][kMlpHid + 1];  // dz2
     float dy[B][kMlpOut];
     float red[kTrainNT / 64];
 };
@@ -72,50 +127,80 @@ struct MlpLds {
 // Forward and backward of one minibatch (rows idx[0..B) of x, t) for the
 // loss mean((sigmoid(y) - t)^2) over the B x 2 outputs.  Every operand of
 // a product is a bfloat16 value (weights, inputs, activations and the
-// back-propagated dz), every sum FP32.  grad (global) receives dL/dparam;
-// returns the loss in thread 0.
+// back-propagated dz), every sum FP32.  Wavefront q takes samples
+// 16q..16q+15 in the forward pass and the dz1 product, and the q-th
+// 16-column (or 16-row) block of each weight gradient.  grad (global)
+// receives dL/dparam; returns the loss in thread 0.
 __device__ float minibatch(const uint16_t *wb, const float *X, const float *T, const int *idx,
                            float *grad, MlpLds &L)
 {
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, q = t >> 6, l = t & 63, c = l & 15, r0 = (l >> 4) * 4;
+    const int R = 16 * q;
     for (int e = t; e < B * kMlpIn; e += kTrainNT) {
         const int b = e / kMlpIn, k = e % kMlpIn;
         L.x[b][k] = bfr(X[(long long)idx[b] * kMlpIn + k]);
     }
     __syncthreads();
-    for (int e = t; e < B * kMlpHid; e += kTrainNT) {
-        const int b = e / kMlpHid, j = e % kMlpHid;
-        float z = 0;
-        for (int k = 0; k < kMlpIn; ++k) z += w(wb, kMlpW1 + j * kMlpIn + k) * L.x[b][k];
-        L.h1[b][j] = bfr(tanhf(z + w(wb, kMlpB1 + j)));
+    // forward, samples R..R+15
+#pragma unroll
+    for (int nt = 0; nt < NT16; ++nt) {
+        f32x4 d = {0, 0, 0, 0};
+        d = mma(d, [&](int i, int k) { return k < kMlpIn ? L.x[R + i][k] : 0.0f; },
+                [&](int k, int j) { return k < kMlpIn ? w(wb, kMlpW1 + (nt * 16 + j) * kMlpIn + k) : 0.0f; });
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = nt * 16 + c;
+            L.h1[R + r0 + v][j] = bfr(tanhf(d[v] + w(wb, kMlpB1 + j)));
+        }
     }
     __syncthreads();
-    for (int e = t; e < B * kMlpHid; e += kTrainNT) {
-        const int b = e / kMlpHid, j = e % kMlpHid;
-        float z = 0;
-        for (int k = 0; k < kMlpHid; ++k) z += w(wb, kMlpW2 + j * kMlpHid + k) * L.h1[b][k];
-        L.h2[b][j] = bfr(tanhf(z + w(wb, kMlpB2 + j)));
+    {
+        f32x4 d2[NT16] = {};
+#pragma unroll
+        for (int kt = 0; kt < NT16; ++kt)
+#pragma unroll
+            for (int nt = 0; nt < NT16; ++nt)
+                d2[nt] = mma(d2[nt], [&](int i, int k) { return L.h1[R + i][kt * 16 + k]; },
+                             [&](int k, int j) { return w(wb, kMlpW2 + (nt * 16 + j) * kMlpHid + kt * 16 + k); });
+#pragma unroll
+        for (int nt = 0; nt < NT16; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = nt * 16 + c;
+                L.h2[R + r0 + v][j] = bfr(tanhf(d2[nt][v] + w(wb, kMlpB2 + j)));
+            }
     }
     __syncthreads();
     float lacc = 0;
-    for (int e = t; e < B * kMlpOut; e += kTrainNT) {
-        const int b = e / kMlpOut, o = e % kMlpOut;
-        float y = 0;
-        for (int j = 0; j < kMlpHid; ++j) y += w(wb, kMlpW3 + o * kMlpHid + j) * L.h2[b][j];
-        const float p = sigm(y + w(wb, kMlpB3 + o));
-        const float r = p - T[(long long)idx[b] * kMlpOut + o];
-        lacc += r * r;
-        L.dy[b][o] = bfr(2.0f * r / (float)(B * kMlpOut) * p * (1.0f - p));
+    {
+        f32x4 y = {0, 0, 0, 0};
+#pragma unroll
+        for (int kt = 0; kt < NT16; ++kt)
+            y = mma(y, [&](int i, int k) { return L.h2[R + i][kt * 16 + k]; },
+                    [&](int k, int j) { return j < kMlpOut ? w(wb, kMlpW3 + j * kMlpHid + kt * 16 + k) : 0.0f; });
+        if (c < kMlpOut)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int b = R + r0 + v;
+                const float p = sigm(y[v] + w(wb, kMlpB3 + c));
+                const float r = p - T[(long long)idx[b] * kMlpOut + c];
+                lacc += r * r;
+                L.dy[b][c] = bfr(2.0f * r / (float)(B * kMlpOut) * p * (1.0f - p));
+            }
     }
     lacc = wave_sum_f(lacc);
-    if ((t & 63) == 0) L.red[t >> 6] = lacc;
+    if (l == 0) L.red[q] = lacc;
     __syncthreads();
-    // output layer
-    for (int e = t; e < kMlpOut * kMlpHid; e += kTrainNT) {
-        const int o = e / kMlpHid, j = e % kMlpHid;
-        float g = 0;
-        for (int b = 0; b < B; ++b) g += L.dy[b][o] * L.h2[b][j];
-        grad[kMlpW3 + e] = g;
+    // output layer: dW3 = dy^T h2 (rows o < 2 of a 16-row block), block q of its columns
+    {
+        f32x4 d = {0, 0, 0, 0};
+#pragma unroll
+        for (int kt = 0; kt < B / 16; ++kt)
+            d = mma(d, [&](int i, int k) { return i < kMlpOut ? L.dy[kt * 16 + k][i] : 0.0f; },
+                    [&](int k, int j) { return L.h2[kt * 16 + k][R + j]; });
+        if (r0 == 0)
+#pragma unroll
+            for (int v = 0; v < kMlpOut; ++v) grad[kMlpW3 + v * kMlpHid + R + c] = d[v];
     }
     if (t < kMlpOut) {
         float g = 0;
@@ -129,31 +214,47 @@ __device__ float minibatch(const uint16_t *wb, const float *X, const float *T, c
         L.d2[b][j] = bfr(d * (1.0f - L.h2[b][j] * L.h2[b][j]));
     }
     __syncthreads();
-    // second hidden layer
-    for (int e = t; e < kMlpHid * kMlpHid; e += kTrainNT) {
-        const int j = e / kMlpHid, k = e % kMlpHid;
-        float g = 0;
-        for (int b = 0; b < B; ++b) g += L.d2[b][j] * L.h1[b][k];
-        grad[kMlpW2 + e] = g;
+    // second hidden layer: dW2 = dz2^T h1 (row block q), dz1 = dz2 W2 (sample block q)
+    {
+        f32x4 g[NT16] = {}, d1[NT16] = {};
+#pragma unroll
+        for (int kt = 0; kt < B / 16; ++kt)
+#pragma unroll
+            for (int nt = 0; nt < NT16; ++nt)
+                g[nt] = mma(g[nt], [&](int i, int k) { return L.d2[kt * 16 + k][R + i]; },
+                            [&](int k, int j) { return L.h1[kt * 16 + k][nt * 16 + j]; });
+#pragma unroll
+        for (int kt = 0; kt < NT16; ++kt)
+#pragma unroll
+            for (int nt = 0; nt < NT16; ++nt)
+                d1[nt] = mma(d1[nt], [&](int i, int k) { return L.d2[R + i][kt * 16 + k]; },
+                             [&](int k, int j) { return w(wb, kMlpW2 + (kt * 16 + k) * kMlpHid + nt * 16 + j); });
+#pragma unroll
+        for (int nt = 0; nt < NT16; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = R + r0 + v, k = nt * 16 + c;
+                grad[kMlpW2 + i * kMlpHid + k] = g[nt][v];
+                // dz1 overwrites h2 (rows R..R+15 are this wavefront's alone here)
+                L.h2[i][k] = bfr(d1[nt][v] * (1.0f - L.h1[i][k] * L.h1[i][k]));
+            }
     }
     for (int j = t; j < kMlpHid; j += kTrainNT) {
         float g = 0;
         for (int b = 0; b < B; ++b) g += L.d2[b][j];
         grad[kMlpB2 + j] = g;
     }
-    for (int e = t; e < B * kMlpHid; e += kTrainNT) {  // dz1 overwrites h2
-        const int b = e / kMlpHid, k = e % kMlpHid;
-        float d = 0;
-        for (int j = 0; j < kMlpHid; ++j) d += w(wb, kMlpW2 + j * kMlpHid + k) * L.d2[b][j];
-        L.h2[b][k] = bfr(d * (1.0f - L.h1[b][k] * L.h1[b][k]));
-    }
     __syncthreads();
-    // first hidden layer
-    for (int e = t; e < kMlpHid * kMlpIn; e += kTrainNT) {
-        const int k = e / kMlpIn, i = e % kMlpIn;
-        float g = 0;
-        for (int b = 0; b < B; ++b) g += L.h2[b][k] * L.x[b][i];
-        grad[kMlpW1 + e] = g;
+    // first hidden layer: dW1 = dz1^T x (row block q, inputs padded to 16)
+    {
+        f32x4 d = {0, 0, 0, 0};
+#pragma unroll
+        for (int kt = 0; kt < B / 16; ++kt)
+            d = mma(d, [&](int i, int k) { return L.h2[kt * 16 + k][R + i]; },
+                    [&](int k, int j) { return j < kMlpIn ? L.x[kt * 16 + k][j] : 0.0f; });
+        if (c < kMlpIn)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) grad[kMlpW1 + (R + r0 + v) * kMlpIn + c] = d[v];
     }
     for (int k = t; k < kMlpHid; k += kTrainNT) {
         float g = 0;
@@ -163,7 +264,7 @@ __device__ float minibatch(const uint16_t *wb, const float *X, const float *T, c
     __syncthreads();  // grad complete and visible to the block
     float loss = 0;
     if (t == 0)
-        for (int q = 0; q < kTrainNT / 64; ++q) loss += L.red[q];
+        for (int k = 0; k < kTrainNT / 64; ++k) loss += L.red[k];
     return loss / (float)(B * kMlpOut);
 }
 
@@ -214,7 +315,7 @@ __global__ __launch_bounds__(kTrainNT) void mlp_train_kernel(AdamArgs a, const f
 hipError_t launch_mlp_infer(const uint16_t *wb, const float *feat, float *out, int n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mlp_infer, dim3(n), dim3(64), 0, s, wb, feat, out, n);
+    hipLaunchKernelGGL(mlp_infer, dim3((n + 15) / 16), dim3(64), 0, s, wb, feat, out, n);
     return hipGetLastError();
 }
 
