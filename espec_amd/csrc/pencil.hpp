@@ -214,23 +214,28 @@ __device__ __forceinline__ void twiddle_table(float2 *v, const float4 *__restric
     for (int e = 1; e < CNT; ++e) v[O + e] = ctw_w<SIGN>(v[O + e], tt[e * M]);
 }
 
-// sin and cos of th for |th| < kSincosMax: Cody-Waite reduction by pi/2
-// in three FMA steps (pi/2 split as P1 + P2 + P3, each rounded to double),
-// then the Taylor series of sin to r^15 and of cos to r^16 on |r| <= pi/4
-// (truncation below 5e-17), and the quadrant.  Cheaper in registers than a
-// general sincos, whose large-argument reduction the callers never need.
+// sin and cos of th for |th| < kSincosMax: Cody-Waite reduction by pi in
+// three FMA steps (pi split as P1 + P2 + P3, each rounded to double) to
+// th = k pi + r, |r| <= pi/2, then the Taylor series of sin to r^19 and of
+// cos to r^20 (truncation below 3e-16) and the sign (-1)^k of both.  The
+// half-period reduction leaves one sign flip where a quarter-period one
+// needs a four-way select; the two extra terms cost less than the select
+// and the 64-bit quadrant.  Cheaper in registers than a general sincos,
+// whose large-argument reduction the callers never need.
 constexpr double kSincosMax = 1.0e6;
 __device__ __forceinline__ void sincos_reduced(double th, double *sn, double *cs)
 {
-    constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17,
-                     P3 = -1.4973849048591698e-33, TWO_OVER_PI = 0.6366197723675814;
-    const double k = rint(th * TWO_OVER_PI);
+    constexpr double P1 = 3.141592653589793, P2 = 1.2246467991473532e-16,
+                     P3 = -2.9947698097183397e-33, INV_PI = 0.3183098861837907;
+    const double k = rint(th * INV_PI);
     double r = fma(-k, P1, th);
     r = fma(-k, P2, r);
     r = fma(-k, P3, r);
     const double r2 = r * r;
     // 1/n! for the odd (sin) and even (cos) terms
-    double ps = 1.0 / 1307674368000.0;                       // 1/15!
+    double ps = 1.0 / 121645100408832000.0;                  // 1/19!
+    ps = fma(ps, r2, -1.0 / 355687428096000.0);              // 1/17!
+    ps = fma(ps, r2, 1.0 / 1307674368000.0);                 // 1/15!
     ps = fma(ps, r2, -1.0 / 6227020800.0);                   // 1/13!
     ps = fma(ps, r2, 1.0 / 39916800.0);                      // 1/11!
     ps = fma(ps, r2, -1.0 / 362880.0);                       // 1/9!
@@ -238,7 +243,9 @@ __device__ __forceinline__ void sincos_reduced(double th, double *sn, double *cs
     ps = fma(ps, r2, -1.0 / 120.0);                          // 1/5!
     ps = fma(ps, r2, 1.0 / 6.0);                             // 1/3!
     const double s = fma(-ps * r2, r, r);                    // r - r^3/3! + ...
-    double pc = 1.0 / 20922789888000.0;                      // 1/16!
+    double pc = 1.0 / 2432902008176640000.0;                 // 1/20!
+    pc = fma(pc, r2, -1.0 / 6402373705728000.0);             // 1/18!
+    pc = fma(pc, r2, 1.0 / 20922789888000.0);                // 1/16!
     pc = fma(pc, r2, -1.0 / 87178291200.0);                  // 1/14!
     pc = fma(pc, r2, 1.0 / 479001600.0);                     // 1/12!
     pc = fma(pc, r2, -1.0 / 3628800.0);                      // 1/10!
@@ -247,9 +254,10 @@ __device__ __forceinline__ void sincos_reduced(double th, double *sn, double *cs
     pc = fma(pc, r2, 1.0 / 24.0);                            // 1/4!
     pc = fma(pc, r2, -0.5);                                  // 1/2!
     const double c = fma(pc, r2, 1.0);
-    const int q = (int)(long long)k & 3;
-    *sn = q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
-    *cs = q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+    // |k| < 2^19 here, so the conversion to int is exact; bit 63 flips both signs
+    const unsigned long long flip = (unsigned long long)((int)k & 1) << 63;
+    *sn = __longlong_as_double(__double_as_longlong(s) ^ (long long)flip);
+    *cs = __longlong_as_double(__double_as_longlong(c) ^ (long long)flip);
 }
 
 // LDS slot of exchange element (k1, n2) of a line: row k1 of M entries,
