@@ -297,6 +297,65 @@ def config3(E, configs, steps, check):
     return out
 
 
+def config2_sil(E, configs, steps, check):
+    """Short-iterative Lanczos at fixed order 10 (PLNZ with ABSTOL 0, plnz.f:109-131)
+    on the 256x256 coupled-Morse surface, dt = 1 a.u., C(t) every step, FP64
+    and mixed (complex64 Krylov basis, FP64 alpha/beta and Gram sums); with
+    `check`, |dC| of the first 100 steps against the FP64 oracle."""
+    from espec_amd import krylov
+    c = configs.config2(nsteps=steps)
+    shm = krylov.shm_of(c)
+    m = 10
+    out = {"workload": "256x256 coupled-Morse SIL, FD Hamiltonian, fixed order 10, dt 1 a.u.",
+           "steps": steps, "order": m}
+    res = {}
+    for name, fl in (("fp64", E.FP64 | E.EXACT), ("mixed", E.MIXED | E.EXACT)):
+        f = E.SilPlan(c.np, m, 0.0, c.dt_fs, 1, shm, c.vpot, c.u, c.v, maxsamp=steps + 1,
+                      flags=fl)
+        f.run(16)  # warm-up: captures the step graph
+        f.reset()
+        f.sync()
+        t1 = time.perf_counter()
+        f.run(steps)
+        f.sync()
+        sec = time.perf_counter() - t1
+        res[name] = f.get(0)
+        f.close()
+        # (m^2 + 6m + 1) state-sized streams per grid-point-step, 16 bytes
+        # per complex FP64 element and 8 per complex64 one
+        bpp = (m * m + 6 * m + 1) * (16.0 if name == "fp64" else 8.0)
+        gps = c.n * steps / sec
+        out[name] = {"seconds": sec, "value": gps, "unit": "grid-point-steps/s",
+                     "ms_per_step": sec * 1e3 / steps,
+                     "roofline": {"bytes_per_gp_step": bpp, "achieved_GBps": gps * bpp / 1e9,
+                                  "frac": gps * bpp / HBM_PEAK}}
+    out["mixed_over_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle, EXACT
+        k = min(100, steps)
+        o = Oracle().plnz(c.np, m, 0.0, c.dt_fs, 0.0, (k - 1) * c.dt_fs, 1, shm, c.u, c.v,
+                          c.vpot, EXACT)
+        n = min(len(o["cr"]), k + 1)
+        out["max_abs_dC_vs_oracle_first_steps"] = {
+            name: float(max(np.abs(o["cr"][:n] - r["cr"][:n]).max(),
+                            np.abs(o["ci"][:n] - r["ci"][:n]).max()))
+            for name, r in res.items()}
+        out["oracle_steps"] = n - 1
+        # Order 10 at dt = 1 a.u. amplifies rounding on this grid: the
+        # oracle's own departure from a run started at psi0 perturbed by
+        # 1e-16 (relative, random) over the same steps.
+        rng = np.random.default_rng(1)
+        up = c.u * (1.0 + 1e-16 * rng.standard_normal(c.u.size))
+        op = Oracle().plnz(c.np, m, 0.0, c.dt_fs, 0.0, (k - 1) * c.dt_fs, 1, shm, up, c.v,
+                           c.vpot, EXACT)
+        out["oracle_self_departure_1e-16_start"] = float(
+            max(np.abs(o["cr"][:n] - op["cr"][:n]).max(), np.abs(o["ci"][:n] - op["ci"][:n]).max()))
+    log(f"[bench] config2 SIL order {m}: fp64 {out['fp64']['ms_per_step']:.3f} ms/step, "
+        f"mixed {out['mixed']['ms_per_step']:.3f} ms/step")
+    return out
+
+
 def krylov_controller(E, configs, steps, surface="config2"):
     """The learned Krylov-order/step controller on short-iterative Lanczos
     propagation of the 256x256 surface (FD Hamiltonian, dt = 1 a.u., C(t)
@@ -399,6 +458,8 @@ def main():
                     help="steps of the learned Krylov controller comparison (0 skips it)")
     ap.add_argument("--config3-steps", type=int, default=1000,
                     help="steps of the 128^3 workload (0 skips it)")
+    ap.add_argument("--sil-steps", type=int, default=2000,
+                    help="steps of the fixed-order Lanczos run on the 256x256 surface (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagation of a batch of independent packets "
@@ -482,6 +543,8 @@ def main():
         out["config2"] = config2(E, configs, args.config2_steps, 100, not args.no_cpu)
     if args.config3_steps > 0:
         out["config3"] = config3(E, configs, args.config3_steps, not args.no_cpu)
+    if args.sil_steps > 0:
+        out["config2_sil"] = config2_sil(E, configs, args.sil_steps, not args.no_cpu)
     if args.krylov_steps > 0:
         out["krylov_controller"] = krylov_controller(E, configs, args.krylov_steps)
         out["krylov_controller_1d"] = krylov_controller(E, configs, 2 * args.krylov_steps,
