@@ -297,7 +297,7 @@ def config3(E, configs, steps, check):
     return out
 
 
-def config2_sil(E, configs, steps, check):
+def config2_sil(E, configs, steps, check, extra_flags=0):
     """Short-iterative Lanczos at fixed order 10 (PLNZ with ABSTOL 0, plnz.f:109-131)
     on the 256x256 coupled-Morse surface, dt = 1 a.u., C(t) every step, FP64
     and mixed (complex64 Krylov basis, FP64 alpha/beta and Gram sums); with
@@ -307,11 +307,12 @@ def config2_sil(E, configs, steps, check):
     shm = krylov.shm_of(c)
     m = 10
     out = {"workload": "256x256 coupled-Morse SIL, FD Hamiltonian, fixed order 10, dt 1 a.u.",
-           "steps": steps, "order": m}
+           "steps": steps, "order": m,
+           "launch": "direct kernel launches" if extra_flags & E.NO_GRAPH else "step graphs"}
     res = {}
     for name, fl in (("fp64", E.FP64 | E.EXACT), ("mixed", E.MIXED | E.EXACT)):
         f = E.SilPlan(c.np, m, 0.0, c.dt_fs, 1, shm, c.vpot, c.u, c.v, maxsamp=steps + 1,
-                      flags=fl)
+                      flags=fl | extra_flags)
         f.run(16)  # warm-up: captures the step graph
         f.reset()
         f.sync()
@@ -460,6 +461,9 @@ def main():
                     help="steps of the 128^3 workload (0 skips it)")
     ap.add_argument("--sil-steps", type=int, default=2000,
                     help="steps of the fixed-order Lanczos run on the 256x256 surface (0 skips it)")
+    ap.add_argument("--sil-no-graph", action="store_true",
+                    help="run the fixed-order Lanczos section with direct kernel launches "
+                         "(ESPEC_NO_GRAPH) instead of step graphs, for kernel tracers")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagation of a batch of independent packets "
@@ -471,6 +475,13 @@ def main():
     from espec_amd import configs
 
     K, W = args.steps, args.warmup
+    if K <= 0:
+        # Profiling runs of the other sections only: no headline.
+        out = {"metric": METRIC, "value": None, "unit": "grid-point-steps/s", "n_gpus": 1,
+               "steps": 0, "warmup": W, "note": "batch skipped (--steps 0)"}
+        extra_sections(E, configs, args, out)
+        print(json.dumps(out))
+        return
     log(f"[bench] batch of 2048 packets: {K} steps (warmup {W})")
     bt = batched(E, configs, K, W, not args.no_cpu, args.profile_steps)
     mix = bt["mixed"]
@@ -539,17 +550,23 @@ def main():
                                          f"each, FP64 C oracle (oracle/espec_oracle.c), one "
                                          f"thread per packet on {th} host threads, {sec:.1f} s"}
         log(f"[bench] CPU: {sec:.1f} s, {v:.3e} gp-steps/s")
+    extra_sections(E, configs, args, out)
+    print(json.dumps(out))
+
+
+def extra_sections(E, configs, args, out):
+    """The single-packet, Lanczos and controller sections of the bench line."""
     if args.config2_steps > 0:
         out["config2"] = config2(E, configs, args.config2_steps, 100, not args.no_cpu)
     if args.config3_steps > 0:
         out["config3"] = config3(E, configs, args.config3_steps, not args.no_cpu)
     if args.sil_steps > 0:
-        out["config2_sil"] = config2_sil(E, configs, args.sil_steps, not args.no_cpu)
+        out["config2_sil"] = config2_sil(E, configs, args.sil_steps, not args.no_cpu,
+                                         E.NO_GRAPH if args.sil_no_graph else 0)
     if args.krylov_steps > 0:
         out["krylov_controller"] = krylov_controller(E, configs, args.krylov_steps)
         out["krylov_controller_1d"] = krylov_controller(E, configs, 2 * args.krylov_steps,
                                                         "config1")
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -22,6 +22,7 @@ FP64, MIXED = 0x0, 0x1
 EXACT, REF = 0x0, 0x2
 PER_PACKET_MASS = 0x4
 MULTIPASS = 0x8
+NO_GRAPH = 0x10  # Lanczos plans: launch each step's kernels directly (no graph replays)
 # Absorbing boundaries (TPABSOR with ABSORB=.TRUE.; spofft.f:65-118, 162-176,
 # sodfft.f:66-72, 108-116): VABC is an imaginary potential or a mask.
 ABSORB_NONE, ABSORB_VOPTIC, ABSORB_SMOOTHW = 0, 1, 2

@@ -101,77 +101,101 @@ __device__ int tridiag_ql(int m, double *d, double *e, double *z)
     return 0;
 }
 
-// The same QL run by one whole wavefront, d, e and z in LDS: every lane
-// computes the rotation sequence (identical scalars and uniform control
-// flow, each lane writing the same d and e values), and lane k applies
-// each rotation to row k of the eigenvector columns it couples.  The
-// rotations dominate the scalar version's time (O(m^2) per sweep).
-__device__ int tridiag_ql_wave(int m, double *d, double *e, double *z)
+// Register read of lane i's value (i wave-uniform).
+__device__ __forceinline__ double lane_get(double v, int i)
 {
-    const int k = threadIdx.x & 63;
-    for (int i = k; i < m * m; i += 64) z[i] = (i % (m + 1)) == 0 ? 1.0 : 0.0;
-    e[m - 1] = 0;
-    for (int l = 0; l < m; ++l) {
-        int iter = 0, mm;
-        do {
-            for (mm = l; mm < m - 1; ++mm) {
-                double dd = fabs(d[mm]) + fabs(d[mm + 1]);
-                if (fabs(e[mm]) <= 2.2204460492503131e-16 * dd) break;
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffLL), i);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), i);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The QL of tridiag_ql run by one wavefront with the tridiagonal spread
+// over the lanes: lane i holds d_i and e_i, so every operand of the
+// rotation chain is a register read of one lane (readlane) rather than an
+// LDS round trip, and the split point of each sweep comes from one ballot
+// instead of a scan.  Lane k applies each rotation to row k of Z, which it
+// keeps in LDS (zr[j 64 + k], kMaxKrylov * 64 entries per wavefront): the
+// row's loads are issued before the rotation's scalar chain, which hides
+// their latency.  The same operations in the same order as tridiag_ql, so
+// the results are those of the scalar form bit for bit.  On entry
+// d and e (LDS, m entries) hold the matrix; on return d holds the
+// eigenvalues in ascending order and z[j m + k] component k of eigenvector
+// j.  Returns l + 1 when eigenvalue l has not converged after 60
+// iterations (d and z are then written unsorted), else 0.
+__device__ int tridiag_ql_wave(int m, double *d, double *e, double *z, double *zr)
+{
+    constexpr double eps = 2.2204460492503131e-16;
+    const int lane = threadIdx.x & 63;
+    double dv = lane < m ? d[lane] : 0.0;
+    double ev = lane < m - 1 ? e[lane] : 0.0;  // e[m-1] = 0
+    for (int j = 0; j < m; ++j) zr[j * 64 + lane] = j == lane ? 1.0 : 0.0;
+    int info = 0;
+    for (int l = 0; l < m && !info; ++l) {
+        int iter = 0;
+        for (;;) {
+            // mm: the first index >= l whose off-diagonal is negligible
+            const double dn = __shfl_down(dv, 1, 64);
+            const bool neg = lane >= l && lane < m - 1 && fabs(ev) <= eps * (fabs(dv) + fabs(dn));
+            const unsigned long long bal = __ballot(neg);
+            const int mm = bal ? __ffsll((long long)bal) - 1 : m - 1;
+            if (mm == l) break;
+            if (++iter > 60) {
+                info = l + 1;
+                break;
             }
-            if (mm != l) {
-                if (++iter > 60) return l + 1;
-                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = hypot(g, 1.0);
-                g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-                double s = 1, c = 1, p = 0;
-                int i;
-                for (i = mm - 1; i >= l; --i) {
-                    double f = s * e[i], bb = c * e[i];
-                    r = hypot(f, g);
-                    e[i + 1] = r;
-                    if (r == 0) {
-                        d[i + 1] -= p;
-                        e[mm] = 0;
-                        break;
-                    }
-                    s = f / r;
-                    c = g / r;
-                    g = d[i + 1] - p;
-                    r = (d[i] - g) * s + 2.0 * c * bb;
-                    p = s * r;
-                    d[i + 1] = g + p;
-                    g = c * r - bb;
-                    if (k < m) {
-                        const double zf = z[(i + 1) * m + k];
-                        const double zi = z[i * m + k];
-                        z[(i + 1) * m + k] = s * zi + c * zf;
-                        z[i * m + k] = c * zi - s * zf;
-                    }
+            const double dl = lane_get(dv, l), el = lane_get(ev, l);
+            double g = (lane_get(dv, l + 1) - dl) / (2.0 * el);
+            double r = hypot(g, 1.0);
+            g = lane_get(dv, mm) - dl + el / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+            double s = 1, c = 1, p = 0;
+            int i;
+            for (i = mm - 1; i >= l; --i) {
+                const double zi = zr[i * 64 + lane], zf = zr[(i + 1) * 64 + lane];
+                const double ei = lane_get(ev, i);
+                const double f = s * ei, bb = c * ei;
+                r = hypot(f, g);
+                if (lane == i + 1) ev = r;
+                if (r == 0) {
+                    if (lane == i + 1) dv -= p;
+                    if (lane == mm) ev = 0;
+                    break;
                 }
-                if (r == 0 && i >= l) continue;
-                d[l] -= p;
-                e[l] = g;
-                e[mm] = 0;
+                s = f / r;
+                c = g / r;
+                g = lane_get(dv, i + 1) - p;
+                r = (lane_get(dv, i) - g) * s + 2.0 * c * bb;
+                p = s * r;
+                if (lane == i + 1) dv = g + p;
+                g = c * r - bb;
+                zr[(i + 1) * 64 + lane] = s * zi + c * zf;
+                zr[i * 64 + lane] = c * zi - s * zf;
             }
-        } while (mm != l);
-    }
-    for (int i = 0; i < m - 1; ++i) {
-        int q = i;
-        for (int j = i + 1; j < m; ++j)
-            if (d[j] < d[q]) q = j;
-        if (q != i) {
-            const double t = d[i];
-            const double dq = d[q];
-            d[q] = t;
-            d[i] = dq;
-            if (k < m) {
-                const double zt = z[i * m + k];
-                z[i * m + k] = z[q * m + k];
-                z[q * m + k] = zt;
+            if (r == 0 && i >= l) continue;
+            if (lane == l) {
+                dv -= p;
+                ev = g;
             }
+            if (lane == mm) ev = 0;
         }
     }
-    return 0;
+    // Ascending order: eigenvalue j (lane j) and eigenvector j go to the
+    // rank of d_j (ties, which an unreduced tridiagonal does not have,
+    // keep index order).
+    int rank = lane;
+    if (!info) {
+        rank = 0;
+        for (int q = 0; q < m; ++q) {
+            const double dq = lane_get(dv, q);
+            rank += (dq < dv || (dq == dv && q < lane)) ? 1 : 0;
+        }
+    }
+    if (lane < m) d[rank] = dv;
+    for (int j = 0; j < m; ++j) {
+        const int rj = __builtin_amdgcn_readlane(rank, j);
+        if (lane < m) z[rj * m + lane] = zr[j * 64 + lane];
+    }
+    return info;
 }
 
 __device__ void tridiag_bounds(int m, const double *d, const double *e, double abstol, double *e2,
@@ -356,6 +380,26 @@ __device__ double sum_parts1(const LnzArgs &a, const double *p, double *red)
     const double t = sh;
     __syncthreads();  // red and sh are reused by the caller
     return t;
+}
+
+// Totals of k values over the nb block partials p[b kPart + j] that the
+// previous kernel wrote (put_parts), in last_block's fixed order (block
+// b's to thread b, then block_sum_k); thread 0 receives them in tot.
+template <int NT, int K>
+__device__ __forceinline__ void sum_parts_k(const LnzArgs &a, const double *p, int k, double *tot,
+                                            double *red)
+{
+    double v[K];
+    const double *q = p + (long long)threadIdx.x * kPart;
+    const bool mine = (int)threadIdx.x < a.nb;
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = (mine && j < k) ? q[j] : 0.0;
+    block_sum_k<NT>(v, k, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (j < k) tot[j] = v[j];
+    }
 }
 
 // Publish this block's k partials (thread 0 holds them after block_sum_k).
@@ -561,19 +605,16 @@ struct SilArgs {
     const double *mask;   // [npk][n] .SMOOTHW mask applied to the new state, or null
 };
 
-// Projection c_j = <v_j, start> (DGEMV 'T', sil.f:86-87), then in the last
-// block the tridiagonal eigensolve (sil.f:82) and the coefficients of
-// sil.f:88-117 / 136-166.
+// Block partials of the projection c_j = <v_j, start> (DGEMV 'T',
+// sil.f:86-87) of the m basis vectors of start vector `start` (Re psi for
+// even recurrences, Im psi for odd ones), into part slot (vec, 0).
 template <typename S, int NT>
-__global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
+__device__ __forceinline__ void project_start(const LnzArgs &a, const void *start, int m,
+                                              double *red)
 {
-    __shared__ double red[(NT / 64) * kPart];
-    __shared__ double d[kMaxKrylov], e[kMaxKrylov], z[kMaxKrylov * kMaxKrylov], e2[kMaxKrylov];
-    __shared__ double bnd[4];
-    const LnzArgs &a = s.l;
-    const int vec = blockIdx.y, m = a.m;
+    const int vec = blockIdx.y;
     const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.ncols * a.n;
-    const V2<S> *x0 = reinterpret_cast<const V2<S> *>(s.x0) + (long long)(vec >> 1) * a.n;
+    const V2<S> *x0 = reinterpret_cast<const V2<S> *>(start) + (long long)(vec >> 1) * a.n;
     double c[kMaxKrylov];
     for (int j = 0; j < kMaxKrylov; ++j) c[j] = 0;
     for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
@@ -584,18 +625,45 @@ __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
             if (j < m) c[j] += (double)B[(long long)j * a.n + p] * u;
     }
     block_sum_k<NT>(c, m, red);
+    put_parts(a.part + (long long)vec * a.nchunk * a.nb * kPart, c, m);
+}
+
+// The projection's block partials (sil_coeffs sums them: the kernel
+// boundary orders them, so no block needs a device-scope fence).
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void sil_project(SilArgs s)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    project_start<S, NT>(s.l, s.x0, s.l.m, red);
+}
+
+// One workgroup per recurrence: the projection's total, the tridiagonal
+// eigensolve (sil.f:82) and the coefficients of sil.f:88-117 / 136-166.
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    __shared__ double d[kMaxKrylov], e[kMaxKrylov], z[kMaxKrylov * kMaxKrylov], e2[kMaxKrylov];
+    __shared__ double zr[kMaxKrylov * 64];
+    __shared__ double bnd[4];
+    const LnzArgs &a = s.l;
+    const int vec = blockIdx.y, m = a.m;
     double tot[kMaxKrylov];
-    if (!last_block<NT>(a, vec, 0, c, m, tot, red)) return;
+    sum_parts_k<NT, kMaxKrylov>(a, a.part + (long long)vec * a.nchunk * a.nb * kPart, m, tot, red);
     double *coef = s.coef + (long long)vec * 2 * m;
     if (a.dead[vec]) {
         if (threadIdx.x < 2 * m) coef[threadIdx.x] = 0;
         return;
     }
+    __shared__ double tq[kMaxKrylov], pcs[kMaxKrylov], pss[kMaxKrylov];
     if (threadIdx.x == 0) {
         for (int j = 0; j < m; ++j) {
             d[j] = a.al[vec * m + j];
             e[j] = j < m - 1 ? a.bt[vec * m + j] : 0;
         }
+#pragma unroll
+        for (int j = 0; j < kMaxKrylov; ++j)  // constant indices keep tot in registers
+            if (j < m) tq[j] = tot[j];
         if (s.abstol > 0) {
             double pm, gl, gu, at;
             tridiag_bounds(m, d, e, s.abstol, e2, &pm, &gl, &gu, &at);
@@ -613,39 +681,37 @@ __global__ __launch_bounds__(NT) void sil_coeffs(SilArgs s)
         lam_b = tridiag_bisect_one(m, d, e2, bnd[0], bnd[1], bnd[2], bnd[3], threadIdx.x);
     __syncthreads();
     if (threadIdx.x < 64) {
-        const int bad = tridiag_ql_wave(m, d, e, z);
+        const int bad = tridiag_ql_wave(m, d, e, z, zr);
         if (bad && threadIdx.x == 0) *s.info = bad;
     }
     __syncthreads();
     if (s.abstol > 0 && threadIdx.x < m) d[threadIdx.x] = lam_b;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double u2[kMaxKrylov], pc[kMaxKrylov], ps[kMaxKrylov];
-        for (int j = 0; j < m; ++j) {  // DGEMV 'T': u2 = Q^T c
-            double t = 0;
-            for (int i = 0; i < m; ++i) t += z[j * m + i] * tot[i];
-            u2[j] = t;
+    // Thread j takes eigenvalue j, then thread i row i of the products;
+    // every sum runs in ascending order, as the serial DGEMVs do.
+    const int k = threadIdx.x;
+    if (k < m) {
+        double u2 = 0;  // DGEMV 'T': (Q^T c)_k
+        for (int i = 0; i < m; ++i) u2 += z[k * m + i] * tq[i];
+        const double ang = d[k] * s.cst;
+        const double cs = cos(ang), sn = sin(ang);
+        if ((vec & 1) == 0) {  // Re psi: sil.f:95-100
+            pcs[k] = cs * u2;
+            pss[k] = -sn * (s.ref ? cs * u2 : u2);
+        } else {               // Im psi: sil.f:143-148
+            pss[k] = sn * u2;
+            pcs[k] = cs * u2;
         }
+    }
+    __syncthreads();
+    if (k < m) {  // DGEMV 'N': coefficients = Q * (...)
+        double tc = 0, ts = 0;
         for (int j = 0; j < m; ++j) {
-            const double ang = d[j] * s.cst;
-            const double cs = cos(ang), sn = sin(ang);
-            if ((vec & 1) == 0) {  // Re psi: sil.f:95-100
-                pc[j] = cs * u2[j];
-                ps[j] = -sn * (s.ref ? pc[j] : u2[j]);
-            } else {               // Im psi: sil.f:143-148
-                ps[j] = sn * u2[j];
-                pc[j] = cs * u2[j];
-            }
+            tc += pcs[j] * z[j * m + k];
+            ts += pss[j] * z[j * m + k];
         }
-        for (int i = 0; i < m; ++i) {  // DGEMV 'N': coefficients = Q * (...)
-            double tc = 0, ts = 0;
-            for (int j = 0; j < m; ++j) {
-                tc += pc[j] * z[j * m + i];
-                ts += ps[j] * z[j * m + i];
-            }
-            coef[i] = tc;
-            coef[m + i] = ts;
-        }
+        coef[k] = tc;
+        coef[m + k] = ts;
     }
 }
 
@@ -978,14 +1044,16 @@ __device__ double krylov_est(int m, const double *lam, const double *z, double b
     return b0 * bm * sqrt(sr * sr + si * si);
 }
 
-// As leading_ql, by one whole wavefront on LDS arrays.
-__device__ int leading_ql_wave(const LnzArgs &a, int vec, int m, double *d, double *e, double *z)
+// As leading_ql, by one whole wavefront on LDS arrays (zr: the
+// wavefront's rows of Z, kMaxKrylov * 64 entries).
+__device__ int leading_ql_wave(const LnzArgs &a, int vec, int m, double *d, double *e, double *z,
+                               double *zr)
 {
     for (int j = 0; j < m; ++j) {  // every lane stores the same values
         d[j] = a.al[vec * a.m + j];
         e[j] = j < m - 1 ? a.bt[vec * a.m + j] : 0;
     }
-    return tridiag_ql_wave(m, d, e, z);
+    return tridiag_ql_wave(m, d, e, z, zr);
 }
 
 // Leading m x m tridiagonal of recurrence vec into (d, e); QL in place.
@@ -1031,33 +1099,31 @@ __device__ double krylov_diff(int m, const double *lm, const double *zm, int M, 
     return b0 * sqrt(s);
 }
 
-// Projection onto the basis (as sil_coeffs), eigensolves of T_mact and
-// T_{mact-1} (two wavefronts), the coefficient sets of order mact, and the
-// step's error estimate: the larger of beta0 |y_{mact-1} - y_mact| at the
-// step's end and the residual estimate of order mact.  Also err(4, tfeat),
-// a controller input.
+// Block partials of the projection onto the first mact basis vectors
+// (as sil_project); ctl_coeffs sums them.
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void ctl_project(CtlArgs c)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    project_start<S, NT>(c.l, c.x0, c.mact, red);
+}
+
+// One workgroup per recurrence: the projection's total, eigensolves of
+// T_mact and T_{mact-1} (two wavefronts), the coefficient sets of order
+// mact, and the step's error estimate: the larger of beta0 |y_{mact-1} -
+// y_mact| at the step's end and the residual estimate of order mact.  Also
+// err(4, tfeat), a controller input.
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void ctl_coeffs(CtlArgs c)
 {
     __shared__ double red[(NT / 64) * kPart];
     __shared__ double d[2][kMaxKrylov], ee[2][kMaxKrylov], z[2][kMaxKrylov * kMaxKrylov];
+    __shared__ double zr[2][kMaxKrylov * 64];
     __shared__ int bad;
     const LnzArgs &a = c.l;
     const int vec = blockIdx.y, m = c.mact, ms = a.m;
-    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.ncols * a.n;
-    const V2<S> *x0 = reinterpret_cast<const V2<S> *>(c.x0) + (long long)(vec >> 1) * a.n;
-    double cp[kMaxKrylov];
-    for (int j = 0; j < kMaxKrylov; ++j) cp[j] = 0;
-    for (int p = blockIdx.x * NT + threadIdx.x; p < a.n; p += a.nb * NT) {
-        const V2<S> v = x0[p];
-        const double u = (vec & 1) ? (double)v.y : (double)v.x;
-#pragma unroll
-        for (int j = 0; j < kMaxKrylov; ++j)
-            if (j < m) cp[j] += (double)B[(long long)j * a.n + p] * u;
-    }
-    block_sum_k<NT>(cp, m, red);
     double tot[kMaxKrylov];
-    if (!last_block<NT>(a, vec, 0, cp, m, tot, red)) return;
+    sum_parts_k<NT, kMaxKrylov>(a, a.part + (long long)vec * a.nchunk * a.nb * kPart, m, tot, red);
     double *coef = c.coef + (long long)vec * 4 * ms;
     if (a.dead[vec]) {
         for (int i = threadIdx.x; i < 4 * ms; i += NT) coef[i] = 0;
@@ -1073,12 +1139,14 @@ __global__ __launch_bounds__(NT) void ctl_coeffs(CtlArgs c)
     __shared__ double ce[2][kMaxKrylov], se[2][kMaxKrylov];  // cos, sin(lam_j tend): T_m, T_{m-1}
     __shared__ double yr[2][kMaxKrylov], yi[2][kMaxKrylov];  // exp(-i tend T) e1: T_m, T_{m-1}
     if (threadIdx.x == 0)
-        for (int j = 0; j < m; ++j) tq_[j] = tot[j];
+#pragma unroll
+        for (int j = 0; j < kMaxKrylov; ++j)  // constant indices keep tot in registers
+            if (j < m) tq_[j] = tot[j];
     if (threadIdx.x < 64) {  // wavefront 0: T_m; wavefront 1: T_{m-1}
-        const int r = leading_ql_wave(a, vec, m, d[0], ee[0], z[0]);
+        const int r = leading_ql_wave(a, vec, m, d[0], ee[0], z[0], zr[0]);
         if (threadIdx.x == 0) bad = r;
     } else if (threadIdx.x < 128) {
-        leading_ql_wave(a, vec, m - 1, d[1], ee[1], z[1]);
+        leading_ql_wave(a, vec, m - 1, d[1], ee[1], z[1], zr[1]);
     } else if (threadIdx.x == 128) {
         double d4[4], e4[4], z4[16];
         leading_ql(a, vec, 4, d4, e4, z4);
@@ -1321,6 +1389,7 @@ using namespace espec;
 struct espec_sil_plan {
     int ndim = 1, np[3] = {1, 1, 1}, n = 0, npk = 0, m = 0, lc = 1, maxsamp = 0;
     bool mixed = false, ref = false;
+    bool nograph = false;  // ESPEC_NO_GRAPH: launch each step's kernels directly
     double cst = 0, abstol = 0;
     long long steps_done = 0;
     int nb = 0;
@@ -1333,7 +1402,7 @@ struct espec_sil_plan {
     long long nfield = 0;
     LnzBufs lb;
     SilArgs sa{};
-    // step graphs of 8 steps and of one step (runs replay graphs only)
+    // step graphs of 8 steps and of one step (runs replay graphs unless nograph)
     hipGraphExec_t gexec = nullptr, gexec1 = nullptr;
     hipGraph_t graph = nullptr, graph1 = nullptr;
     // adaptive and trace runs: second state buffer, coefficient sets,
@@ -1439,7 +1508,8 @@ hipError_t sil_step(espec_sil_plan &P, hipStream_t s)
     if (e != hipSuccess) return e;
     SilArgs sa = P.sa;
     sa.x0 = P.x.p;
-    hipLaunchKernelGGL((sil_coeffs<S, 256>), grid, dim3(256), 0, s, sa);
+    hipLaunchKernelGGL((sil_project<S, 256>), grid, dim3(256), 0, s, sa);
+    hipLaunchKernelGGL((sil_coeffs<S, 256>), dim3(1, l.nvec), dim3(256), 0, s, sa);
     hipLaunchKernelGGL((sil_combine<S, 256>), dim3(P.nb, P.npk), dim3(256), 0, s, sa,
                        (V2<S> *)P.x.p, (const V2<S> *)P.psi0.p, P.cbuf.as<double>(), P.maxsamp,
                        P.lc, P.dstep.as<long long>());
@@ -1469,6 +1539,11 @@ int sil_run_typed(espec_sil_plan &P, int nsteps)
     hipLaunchKernelGGL(sil_set_step, dim3(1), dim3(1), 0, P.stream, P.dstep.as<long long>(),
                        P.steps_done);
     ESPEC_HIP_TRY(hipGetLastError());
+    if (P.nograph) {
+        for (int k = 0; k < nsteps; ++k) ESPEC_HIP_TRY(sil_step<S>(P, P.stream));
+        P.steps_done += nsteps;
+        return ESPEC_OK;
+    }
     const int gsteps = 8;
     int left = nsteps;
     if (left >= gsteps) {
@@ -1546,7 +1621,8 @@ hipError_t ctl_launches(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb,
         lf.it = iters - 1;
         hipLaunchKernelGGL(lnz_finish<256>, dim3(1, l.nvec), dim3(256), 0, s, lf);
     }
-    hipLaunchKernelGGL((ctl_coeffs<S, 256>), grid, dim3(256), 0, s, c);
+    hipLaunchKernelGGL((ctl_project<S, 256>), grid, dim3(256), 0, s, c);
+    hipLaunchKernelGGL((ctl_coeffs<S, 256>), dim3(1, grid.y), dim3(256), 0, s, c);
     hipLaunchKernelGGL(ctl_feat, dim3((P.npk + 63) / 64), dim3(64), 0, s, c, P.cfeat.as<float>(),
                        P.npk);
     if (wb) {
@@ -1783,6 +1859,7 @@ int espec_sil_plan_create(espec_sil_plan **out, int ndim, const int *np, int npk
     P->maxsamp = maxsamp;
     P->mixed = (flags & ESPEC_MIXED) != 0;
     P->ref = (flags & ESPEC_REF) != 0;
+    P->nograph = (flags & ESPEC_NO_GRAPH) != 0;
     P->cst = dt / kShfs;
     P->abstol = abstol;
     const int n = P->n, nv = 2 * npk;

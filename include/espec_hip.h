@@ -45,6 +45,11 @@ extern "C" {
  * when a packet fits one workgroup's LDS (batches of many small packets,
  * where the LDS-resident kernel has one workgroup per packet). */
 #define ESPEC_MULTIPASS 0x8
+/* Short-iterative Lanczos plans only: launch every kernel of a step
+ * directly on the plan's stream instead of replaying captured step graphs
+ * (same kernels, same results; for kernel tracers that do not follow
+ * graph launches). */
+#define ESPEC_NO_GRAPH 0x10
 
 enum {
     ESPEC_OK = 0,

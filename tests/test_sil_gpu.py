@@ -129,3 +129,25 @@ def test_sil_plan_batched(gpu, orc):
         assert dC(g, o) <= TOL64, k
         assert np.abs(g["u"] - o["u"]).max() <= TOL64
     P.close()
+
+
+@pytest.mark.parametrize("flags", [E.FP64 | E.EXACT, E.MIXED | E.EXACT])
+def test_no_graph_runs_match_graph_runs(flags):
+    """ESPEC_NO_GRAPH launches the same kernels in the same order as the
+    step graphs: identical C(t) and state, bit for bit, over a run that
+    the graphs cover with one 8-step and several 1-step replays."""
+    c = coupled_2d()
+    dt = 0.05
+    outs = []
+    for extra in (0, E.NO_GRAPH):
+        p = E.SilPlan(c["np_"], 8, 0.0, dt, 1, c["shm"], c["vpot"], c["u"], c["v"],
+                      maxsamp=32, flags=flags | extra)
+        p.run(11)
+        p.run(3)
+        p.sync()
+        outs.append(p.get(0))
+        p.close()
+    a, b = outs
+    assert len(a["cr"]) == len(b["cr"]) == 15
+    for k in ("cr", "ci", "norm", "u", "v"):
+        assert np.array_equal(a[k], b[k]), k
