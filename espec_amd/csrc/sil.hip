@@ -440,11 +440,13 @@ __global__ __launch_bounds__(NT) void lnz_init(LnzArgs a, const V2<S> *x, const 
 }
 
 // lnz.f:89-108: v_i = w / beta, w = H v_i - beta v_{i-1}, alpha_i = <v_i, w>.
+// Iteration i's three steps are device functions (i, and the chunk c of
+// the projection, as arguments) so that lnz_block can run a whole run of
+// iterations in one launch; the kernels below wrap them.
 template <typename S, int NT>
-__global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
+__device__ __forceinline__ void apply_it(const LnzArgs &a, int i, int bfrom, double *red)
 {
-    __shared__ double red[(NT / 64) * kPart];
-    const int vec = blockIdx.y, i = a.it;
+    const int vec = blockIdx.y;
     const long long vo = (long long)vec * a.n;
     const S *win = reinterpret_cast<const S *>((i & 1) ? a.w1 : a.w0) + vo;
     S *wout = reinterpret_cast<S *>((i & 1) ? a.w0 : a.w1) + vo;
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
     // carried as zeros so nothing reads a stale beta.
     bool dead;
     double bprev;
-    if (i > 0 && a.bfrom) {
+    if (i > 0 && bfrom) {
         dead = a.dead[vec];
         bprev = a.bt[vec * a.m + i - 1];
     } else {
@@ -501,10 +503,9 @@ __global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
 // columns of the Gram window, kChunk columns per grid.z slice (the residual
 // itself is updated by lnz_update, so every slice reads the same w).
 template <typename S, int NT>
-__global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
+__device__ __forceinline__ void project_it(const LnzArgs &a, int i, int c, double *red)
 {
-    __shared__ double red[(NT / 64) * kPart];
-    const int vec = blockIdx.y, i = a.it, c = blockIdx.z;
+    const int vec = blockIdx.y;
     const int nj = min(i + 1, a.nreort);
     const int j0 = c * kChunk;
     if (j0 >= nj) return;
@@ -529,11 +530,9 @@ __global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
 
 // w <- (w - alpha v_i) - sum_j r_j v_j (reort.f:69-78), beta_i = |w|.
 template <typename S, int NT>
-__global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
+__device__ __forceinline__ void update_it(const LnzArgs &a, int i, double *red, double *rs)
 {
-    __shared__ double red[(NT / 64) * kPart];
-    __shared__ double rs[kMaxReort];
-    const int vec = blockIdx.y, i = a.it;
+    const int vec = blockIdx.y;
     const int nj = min(i + 1, a.nreort);
     const long long vo = (long long)vec * a.n;
     S *w = reinterpret_cast<S *>((i & 1) ? a.w0 : a.w1) + vo;
@@ -586,12 +585,65 @@ __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
 
 // beta of the last iteration of a run (no lnz_apply follows to form it).
 template <int NT>
+__device__ __forceinline__ void finish_it(const LnzArgs &a, int i, double *red)
+{
+    const int vec = blockIdx.y;
+    const double t = sum_parts1<NT>(a, a.pb + (long long)vec * a.nb * kPart, red);
+    if (threadIdx.x == 0) a.bt[vec * a.m + i] = sqrt(t);
+}
+
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    apply_it<S, NT>(a, a.it, a.bfrom, red);
+}
+
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    project_it<S, NT>(a, a.it, blockIdx.z, red);
+}
+
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    __shared__ double rs[kMaxReort];
+    update_it<S, NT>(a, a.it, red, rs);
+}
+
+template <int NT>
 __global__ __launch_bounds__(NT) void lnz_finish(LnzArgs a)
 {
     __shared__ double red[(NT / 64) * kPart];
-    const int vec = blockIdx.y;
-    const double t = sum_parts1<NT>(a, a.pb + (long long)vec * a.nb * kPart, red);
-    if (threadIdx.x == 0) a.bt[vec * a.m + a.it] = sqrt(t);
+    finish_it<NT>(a, a.it, red);
+}
+
+// Iterations i0 .. i1-1 of every recurrence in one launch, one workgroup
+// per recurrence, for grids of one block (nb = 1, n <= 256): the same
+// steps, sums and order as the kernels above, with a workgroup barrier
+// where those have a kernel boundary (the partials of a one-block grid are
+// read back by the thread that wrote them).  bfrom as in lnz_apply for
+// iteration i0; with fin, also beta of iteration i1-1 (lnz_finish).
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void lnz_block(LnzArgs a, int i0, int i1, int fin)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    __shared__ double rs[kMaxReort];
+    for (int i = i0; i < i1; ++i) {
+        apply_it<S, NT>(a, i, i == i0 ? a.bfrom : 0, red);
+        __syncthreads();
+        const int nj = min(i + 1, a.nreort);
+        for (int c = 0; c * kChunk < nj; ++c) {
+            project_it<S, NT>(a, i, c, red);
+            __syncthreads();  // thread 0 may still read red in the last sum
+        }
+        update_it<S, NT>(a, i, red, rs);
+        __syncthreads();
+    }
+    if (fin) finish_it<NT>(a, i1 - 1, red);
 }
 
 struct SilArgs {
@@ -1458,6 +1510,12 @@ int blocks_for(int n) { return std::max(1, std::min(256, (n + 255) / 256)); }
 template <typename S>
 hipError_t lnz_iterations(const LnzArgs &l, hipStream_t s)
 {
+    if (l.nb == 1) {  // one launch for all m iterations and the last beta
+        LnzArgs lb = l;
+        lb.bfrom = 0;
+        hipLaunchKernelGGL((lnz_block<S, 256>), dim3(1, l.nvec), dim3(256), 0, s, lb, 0, l.m, 1);
+        return hipGetLastError();
+    }
     dim3 grid(l.nb, l.nvec), gridp(l.nb, l.nvec, l.nchunk);
     for (int i = 0; i < l.m; ++i) {
         LnzArgs li = l;
@@ -1609,17 +1667,24 @@ hipError_t ctl_launches(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb,
         if (have == 0)
             hipLaunchKernelGGL((lnz_init<S, 256>), grid, dim3(256), 0, s, l, (const V2<S> *)P.x.p,
                                (const S *)nullptr);
-        for (int i = have; i < iters; ++i) {
-            LnzArgs li = l;
-            li.it = i;
-            li.bfrom = i == have && have > 0;  // extending: beta_{have-1} is in bt
-            hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
-            hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
-            hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
+        if (l.nb == 1) {  // one launch (extending: beta_{have-1} is in bt)
+            LnzArgs lb = l;
+            lb.bfrom = have > 0;
+            hipLaunchKernelGGL((lnz_block<S, 256>), dim3(1, l.nvec), dim3(256), 0, s, lb, have,
+                               iters, 1);
+        } else {
+            for (int i = have; i < iters; ++i) {
+                LnzArgs li = l;
+                li.it = i;
+                li.bfrom = i == have && have > 0;  // extending: beta_{have-1} is in bt
+                hipLaunchKernelGGL((lnz_apply<S, 256>), grid, dim3(256), 0, s, li);
+                hipLaunchKernelGGL((lnz_project<S, 256>), gridp, dim3(256), 0, s, li);
+                hipLaunchKernelGGL((lnz_update<S, 256>), grid, dim3(256), 0, s, li);
+            }
+            LnzArgs lf = l;
+            lf.it = iters - 1;
+            hipLaunchKernelGGL(lnz_finish<256>, dim3(1, l.nvec), dim3(256), 0, s, lf);
         }
-        LnzArgs lf = l;
-        lf.it = iters - 1;
-        hipLaunchKernelGGL(lnz_finish<256>, dim3(1, l.nvec), dim3(256), 0, s, lf);
     }
     hipLaunchKernelGGL((ctl_project<S, 256>), grid, dim3(256), 0, s, c);
     hipLaunchKernelGGL((ctl_coeffs<S, 256>), dim3(1, grid.y), dim3(256), 0, s, c);
