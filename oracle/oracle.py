@@ -660,8 +660,16 @@ class Reference:
         return u, v
 
     def au(self, np_, shm, vpot, u):
-        hu = np.zeros(int(np.prod(np_)))
-        self.lib.ref_au(len(np_), _np3(np_), _v3(shm, 0.0), _f64(vpot), _f64(u), hu)
+        # AU_3D reads US up to 2 NP(1) NP(2) points before and after the
+        # grid and multiplies those values by a zero mask (au3d.f:82-112):
+        # u travels inside a zero margin of that width, so that 0 times
+        # whatever lies past the array cannot be 0 * NaN.
+        n = int(np.prod(np_))
+        pad = 2 * int(np.prod(np_[:2])) if len(np_) == 3 else 0
+        ub = np.zeros(n + 2 * pad)
+        ub[pad:pad + n] = _f64(u)
+        hu = np.zeros(n)
+        self.lib.ref_au(len(np_), _np3(np_), _v3(shm, 0.0), _f64(vpot), ub[pad:], hu)
         return hu
 
     def efft4(self, np_, u, v, tc, vpot):
