@@ -479,6 +479,8 @@ def main():
         # Profiling runs of the other sections only: no headline.
         out = {"metric": METRIC, "value": None, "unit": "grid-point-steps/s", "n_gpus": 1,
                "steps": 0, "warmup": W, "note": "batch skipped (--steps 0)"}
+        ms_bw, by_bw = E.bw_probe(1 << 30, 10)  # also the counters' calibration kernel
+        out["copy_kernel_GBps"] = by_bw / (ms_bw * 1e-3) / 1e9
         extra_sections(E, configs, args, out)
         print(json.dumps(out))
         return
