@@ -357,7 +357,7 @@ def config2_sil(E, configs, steps, check, extra_flags=0):
     return out
 
 
-def krylov_controller(E, configs, steps, surface="config2"):
+def krylov_controller(E, configs, steps, surface="config2", extra_flags=0):
     """The learned Krylov-order/step controller on short-iterative Lanczos
     propagation of the 256x256 surface (FD Hamiltonian, dt = 1 a.u., C(t)
     every step, FP64).  The network is trained on the device from traces
@@ -374,7 +374,8 @@ def krylov_controller(E, configs, steps, surface="config2"):
         fam = krylov.family1d(count=32, nsteps=0, seed=7, groups=4)
     else:
         fam = krylov.family(count=32, n=256, nsteps=0, seed=5, groups=4)
-    traces = [krylov.trace_packets([p for p in fam if p.meta["group"] == g], 400, mprop=mp)
+    traces = [krylov.trace_packets([p for p in fam if p.meta["group"] == g], 400, mprop=mp,
+                                   flags=E.FP64 | E.EXACT | extra_flags)
               for g, mp in zip(range(4), (12, 14, 16, 21))]
     x, t = krylov.dataset(traces, tol)
     params, hist = krylov.train(x, t, epochs=40, lr=3e-3, seed=3)
@@ -385,7 +386,8 @@ def krylov_controller(E, configs, steps, surface="config2"):
     shm = krylov.shm_of(c)
 
     def plan(m):
-        return E.SilPlan(c.np, m, 0.0, c.dt_fs, 1, shm, c.vpot, c.u, c.v, maxsamp=steps + 1)
+        return E.SilPlan(c.np, m, 0.0, c.dt_fs, 1, shm, c.vpot, c.u, c.v, maxsamp=steps + 1,
+                         flags=E.FP64 | E.EXACT | extra_flags)
 
     ref = plan(20)
     ref.run(steps)
@@ -462,8 +464,11 @@ def main():
     ap.add_argument("--sil-steps", type=int, default=2000,
                     help="steps of the fixed-order Lanczos run on the 256x256 surface (0 skips it)")
     ap.add_argument("--sil-no-graph", action="store_true",
-                    help="run the fixed-order Lanczos section with direct kernel launches "
-                         "(ESPEC_NO_GRAPH) instead of step graphs, for kernel tracers")
+                    help="run the Lanczos sections (fixed order and the controller's traces, "
+                         "fixed and adaptive runs) with direct kernel launches (ESPEC_NO_GRAPH) "
+                         "instead of step graphs, for kernel tracers")
+    ap.add_argument("--krylov-1d", type=int, default=1,
+                    help="also run the controller comparison on the 1-D Morse surface (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the propagation of a batch of independent packets "
@@ -558,17 +563,18 @@ def main():
 
 def extra_sections(E, configs, args, out):
     """The single-packet, Lanczos and controller sections of the bench line."""
+    ng = E.NO_GRAPH if args.sil_no_graph else 0
     if args.config2_steps > 0:
         out["config2"] = config2(E, configs, args.config2_steps, 100, not args.no_cpu)
     if args.config3_steps > 0:
         out["config3"] = config3(E, configs, args.config3_steps, not args.no_cpu)
     if args.sil_steps > 0:
-        out["config2_sil"] = config2_sil(E, configs, args.sil_steps, not args.no_cpu,
-                                         E.NO_GRAPH if args.sil_no_graph else 0)
+        out["config2_sil"] = config2_sil(E, configs, args.sil_steps, not args.no_cpu, ng)
     if args.krylov_steps > 0:
-        out["krylov_controller"] = krylov_controller(E, configs, args.krylov_steps)
-        out["krylov_controller_1d"] = krylov_controller(E, configs, 2 * args.krylov_steps,
-                                                        "config1")
+        out["krylov_controller"] = krylov_controller(E, configs, args.krylov_steps, "config2", ng)
+        if args.krylov_1d:
+            out["krylov_controller_1d"] = krylov_controller(E, configs, 2 * args.krylov_steps,
+                                                            "config1", ng)
 
 
 if __name__ == "__main__":

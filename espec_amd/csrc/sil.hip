@@ -1700,7 +1700,8 @@ hipError_t ctl_launches(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb,
     return hipGetLastError();
 }
 
-// The step's launches replay from a graph captured on first use: one
+// The step's launches replay from a graph captured on first use (unless
+// the plan has ESPEC_NO_GRAPH, which launches them directly): one
 // graph per (order, iterations already run, coefficient sets, step, sample
 // flag, network, which state buffer is current); the sample index comes
 // from the device clock set just before.
@@ -1712,7 +1713,7 @@ hipError_t ctl_step(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb, lon
     iters = std::max(iters, c.mact);
     hipLaunchKernelGGL(sil_set_step, dim3(1), dim3(1), 0, s, P.dstep.as<long long>(), g0 - 1);
     const int tcode = c.tau == P.cst ? 0 : (c.tau == 0.5 * P.cst ? 1 : -1);
-    if (tcode < 0) return ctl_launches<S>(P, c, wb, emit, have, iters, s);
+    if (tcode < 0 || P.nograph) return ctl_launches<S>(P, c, wb, emit, have, iters, s);
     const unsigned long long key = (unsigned long long)c.mact | ((unsigned long long)have << 8) |
                                    ((unsigned long long)iters << 32) |
                                    ((unsigned long long)c.nsub << 16) |

@@ -17,10 +17,14 @@
  * Flags select the arithmetic:
  *   ESPEC_FP64   complex128 state, FP64 everything;
  *   ESPEC_MIXED  complex64 state in HBM (psi, psi0, Krylov basis), rounded
- *                once per launch or pass; every kernel computes in FP64
- *                (FFT stages, phases, stencils, norm, C(t), Lanczos
- *                alpha/beta and Gram coefficients): FP32 twiddle and phase
- *                arithmetic accumulates coherently and misses 1e-6 in C(t);
+ *                once per launch or pass; the strided transforms of the
+ *                multi-pass split-operator plans compute in FP32 with every
+ *                constant split into float hi + lo, everything else in FP64
+ *                (row passes and LDS-resident steps, the potential phase,
+ *                stencils, norm, C(t), Lanczos alpha/beta and Gram
+ *                coefficients): a constant or phase rounded once to float
+ *                would scale every step the same way, accumulate
+ *                coherently and miss 1e-6 in C(t);
  *   ESPEC_REF    the reference's exact semantics (f32-rounded 2*pi in the
  *                FFT twiddles, the accumulated kinetic grid, the
  *                non-unitary kinetic and Lanczos recombination updates)
@@ -46,9 +50,9 @@ extern "C" {
  * where the LDS-resident kernel has one workgroup per packet). */
 #define ESPEC_MULTIPASS 0x8
 /* Short-iterative Lanczos plans only: launch every kernel of a step
- * directly on the plan's stream instead of replaying captured step graphs
- * (same kernels, same results; for kernel tracers that do not follow
- * graph launches). */
+ * (fixed-order, adaptive and trace steps) directly on the plan's stream
+ * instead of replaying captured step graphs (same kernels, same results;
+ * for kernel tracers that do not follow graph launches). */
 #define ESPEC_NO_GRAPH 0x10
 
 enum {

@@ -119,6 +119,35 @@ def test_adaptive_without_network_is_the_fixed_order_step(gpu, flags):
     f.close()
 
 
+def test_adaptive_no_graph_matches_graph_steps(gpu):
+    """ESPEC_NO_GRAPH: the adaptive loop (no network, fixed order 10) and a
+    short trace launch their kernels directly and give the graph-replayed
+    results bit for bit."""
+    pk = _packet2d()[:2]
+    shm = krylov.shm_of(pk[0])
+    vp = np.stack([p.vpot for p in pk])
+    u = np.stack([p.u for p in pk])
+    nst = 12
+    print("case: adaptive and trace, graphs vs ESPEC_NO_GRAPH, 2x64^2", flush=True)
+    runs = []
+    for extra in (0, E.NO_GRAPH):
+        a = E.SilPlan(pk[0].np, 21, 0.0, pk[0].dt_fs, 1, shm, vp, u, None, maxsamp=nst + 1,
+                      flags=E.FP64 | E.EXACT | extra)
+        st = a.adaptive(nst, None, m_fix=10)
+        res = [a.get(k) for k in range(2)]
+        a.reset()
+        tr = a.trace(4, 12)
+        runs.append((st, res, tr))
+        a.close()
+    (s0, r0, t0), (s1, r1, t1) = runs
+    assert s0["steps"] == s1["steps"] == nst
+    for ga, gb in zip(r0, r1):
+        for k in ("cr", "ci", "norm", "u", "v"):
+            assert np.array_equal(ga[k], gb[k]), k
+    for x, y in zip(t0, t1):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
 def test_trained_controller_accuracy(gpu, orc):
     """Train on traces of a small family (states advanced at orders 12 and
     21), then propagate a packet the network has not seen: every accepted
