@@ -16,7 +16,9 @@ Pieces:
   label of step k+1, with Adam on the device (``espec_amd.mlp_train``);
 * use: ``SilPlan.adaptive(ndt, params)`` lets the network choose m and
   s = dt scale before every step; a step whose residual estimate misses
-  the budget is redone with the fixed-order step (the fallback and judge).
+  the budget is redone with the fixed-order step (the fallback and judge);
+* check: ``split_operator_check`` compares a finished run's C(t) with the
+  split-operator FFT propagator's on the same packet.
 
 Decision rule (the device host loop, sil.hip): k = ceil(4 + 16 o1) and
 s = 2^round(2 o2 - 1), so the training targets are t1 = (m - 4) / 16 and
@@ -26,7 +28,7 @@ budget (the inputs are the previous step's, and a miss costs a fallback).
 """
 import numpy as np
 
-from . import FP64, EXACT, SHFS, SilPlan, MLP_NPAR, mlp_train, configs
+from . import FP64, EXACT, SHFS, SilPlan, SpoPlan, MLP_NPAR, mlp_train, configs
 
 M0, M1 = 4, 20
 SCALES = (0.5, 1.0, 2.0)
@@ -113,6 +115,35 @@ def family1d(count=32, nsteps=0, seed=7, groups=4):
             p.meta["group"] = g
             out.append(p)
     return out
+
+
+def split_operator_reference(packet, nsamp, flags=FP64 | EXACT):
+    """C(t) of `packet` from the split-operator FFT propagator, nsamp
+    samples every dt (sample 0 = C(0))."""
+    p = SpoPlan(packet.np, packet.dt_fs, 1, packet.sh, packet.sm, packet.vpot, packet.u,
+                packet.v, maxsamp=nsamp, flags=flags)
+    try:
+        p.run(nsamp - 1)
+        return p.get(0, psi=False)
+    finally:
+        p.close()
+
+
+def split_operator_check(packet, res, flags=FP64 | EXACT, ref=None):
+    """Independent check of a Lanczos propagation of `packet` (C(t) samples
+    every dt in res["cr"], res["ci"], sample 0 = C(0)): the same packet
+    propagated by the split-operator FFT propagator (spectral kinetic
+    operator, PSPOFFT's step) over the same samples; returns
+    max_t |C(t) - C_spo(t)|.  The Lanczos propagator applies the
+    fourth-order finite-difference H of AU, so a converged run departs
+    from the split-operator one by the two discretisations' difference
+    only; a failed Krylov step shows as a departure well beyond that.
+    ref: a split_operator_reference result to reuse."""
+    ns = len(res["cr"])
+    g = ref if ref is not None else split_operator_reference(packet, ns, flags)
+    k = min(ns, len(g["cr"]))
+    return float(max(np.abs(np.asarray(res["cr"][:k]) - g["cr"][:k]).max(),
+                     np.abs(np.asarray(res["ci"][:k]) - g["ci"][:k]).max()))
 
 
 def trace_packets(packets, nsteps, mp=21, mprop=None, flags=FP64 | EXACT):

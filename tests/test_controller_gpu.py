@@ -184,6 +184,26 @@ def test_trained_controller_accuracy(gpu, orc):
     assert st["fallbacks"] <= int(np.sum(log[:, 3] > log[:, 4]))
 
 
+def test_split_operator_check_flags_a_failed_krylov_run(gpu):
+    """The split-operator comparison of a run: converged fixed orders (16,
+    20) depart from the split-operator FFT propagation by the same
+    discretisation difference, while order 4, which is unstable at this
+    dt, departs by far more."""
+    p = krylov.family(count=1, n=64, nsteps=0, seed=99, groups=1)[0]
+    shm = krylov.shm_of(p)
+    nst = 200
+    print("case: split-operator check of fixed orders 4, 16, 20 on 64^2, 200 steps", flush=True)
+    d = {}
+    for m in (4, 16, 20):
+        s = E.SilPlan(p.np, m, 0.0, p.dt_fs, 1, shm, p.vpot, p.u, p.v, maxsamp=nst + 1)
+        s.run(nst)
+        d[m] = krylov.split_operator_check(p, s.get(0, psi=False))
+        s.close()
+    print(f"  |C - C_spo|: {d}", flush=True)
+    assert abs(d[16] - d[20]) <= 1e-6
+    assert d[4] >= 10 * d[20]
+
+
 def test_trained_controller_gain_on_1d_morse(gpu, orc):
     """What the trained network gains over the method without it, on the 1-D
     Morse workload (512 points, dt = 0.5 a.u.): trained on traces of
