@@ -597,6 +597,44 @@ class _Plan:
             pass
 
 
+class SpoBatchGraph:
+    """Split-operator plans advanced together by one graph launch (no
+    reference counterpart: replaces the job farm of tools&scrips/parallel.x).
+    The graph records every kernel of `nsteps` steps of each plan from its
+    current step, one branch per plan; run() replays it once and is valid
+    while every plan is still at the step it was captured at (reset() the
+    plans to replay from step 0)."""
+
+    def __init__(self, plans, nsteps):
+        self._L = _load()
+        self.plans = list(plans)
+        self.nsteps = nsteps
+        self._h = C.c_void_p()
+        arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        _check(self._L.espec_spo_batch_create(C.byref(self._h), arr, len(self.plans), nsteps),
+               "spo_batch_create")
+
+    def run(self):
+        _check(self._L.espec_spo_batch_run(self._h), "spo_batch_run")
+
+    def sync(self):
+        """Wait for the last run; its graph's duration in ms (HIP events)."""
+        ms = C.c_double()
+        _check(self._L.espec_spo_batch_sync(self._h, C.byref(ms)), "spo_batch_sync")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.espec_spo_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class SilPlan(_Plan):
     """Device-resident batch of short-iterative Lanczos packets on one grid
     (PLNZ's time loop, plnz.f:109-131, for npk packets at once)."""

@@ -4,8 +4,10 @@ Packets are grouped by grid (shape and spacing); each group is one
 device-resident plan (SpoPlan, SodPlan or SilPlan) holding all of its
 packets with their own potentials, initial states and masses.  run()
 enqueues every group on its own stream, so the groups execute side by
-side; sync() waits for all of them.  This replaces the reference's job
-farm (tools&scrips/parallel.x), which starts one espec.x per packet.
+side; sync() waits for all of them.  graph(nsteps) instead records the
+next nsteps steps of every group as one HIP graph (one branch per group),
+which a single launch replays.  This replaces the reference's job farm
+(tools&scrips/parallel.x), which starts one espec.x per packet.
 
 A 2-D or 3-D group whose packets fit one workgroup's LDS can run either
 on the LDS-resident kernel (one workgroup per packet, every step in LDS)
@@ -18,7 +20,7 @@ from collections import OrderedDict
 
 import numpy as np
 
-from . import MIXED, EXACT, MULTIPASS, SpoPlan
+from . import MIXED, EXACT, MULTIPASS, SpoBatchGraph, SpoPlan
 
 
 def _step_time(plan, nsteps=40):
@@ -99,6 +101,19 @@ class SpoBatch:
         t0 = time.perf_counter()
         self.run(nsteps)
         self.sync()
+        return time.perf_counter() - t0
+
+    def graph(self, nsteps):
+        """Every group's next nsteps steps recorded as one graph
+        (SpoBatchGraph): one launch then advances the whole batch."""
+        return SpoBatchGraph([g["plan"] for g in self.groups], nsteps)
+
+    def timed_graph_run(self, graph):
+        """Wall time of one replay of `graph` (from graph()), inputs resident."""
+        self.sync()
+        t0 = time.perf_counter()
+        graph.run()
+        graph.sync()
         return time.perf_counter() - t0
 
     def get(self, i, psi=False):

@@ -1111,6 +1111,30 @@ int capture_typed(SpoPassPlan &p, int j)
     return ESPEC_OK;
 }
 
+// The run's kernels launched one by one on s (no step-graph replays): for
+// a caller that captures the whole run into a graph of its own, where a
+// replay of the plan's step graphs cannot be recorded.
+template <typename R>
+int enqueue_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s)
+{
+    int launches = 0;
+    hipLaunchKernelGGL(set_step, dim3(1), dim3(1), 0, s, p.dstep, step0);
+    ESPEC_HIP_TRY(hipGetLastError());
+    ESPEC_HIP_TRY(one_step<R>(p, false, true, s, &launches));
+    for (int k = 1; k < nsteps; ++k) ESPEC_HIP_TRY(one_step<R>(p, true, true, s, &launches));
+    ESPEC_HIP_TRY(one_step<R>(p, true, false, s, &launches));
+    long long lo = step0 / p.lc + 1, hi = (step0 + nsteps) / p.lc + 1;
+    lo = std::min<long long>(lo, p.maxsamp);
+    hi = std::min<long long>(hi, p.maxsamp);
+    if (hi > lo) {
+        int cnt = (int)(hi - lo) * p.npk;
+        hipLaunchKernelGGL(reduce_parts, dim3((cnt + 255) / 256), dim3(256), 0, s, p.part, p.cbuf,
+                           p.npk, p.xblocks, p.maxsamp, (int)lo, (int)hi);
+        ESPEC_HIP_TRY(hipGetLastError());
+    }
+    return ESPEC_OK;
+}
+
 template <typename R>
 int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *launches)
 {
@@ -1705,6 +1729,15 @@ int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_
     if (nsteps <= 0) return ESPEC_OK;
     return p.mixed ? run_typed<float>(p, nsteps, step0, s, launches)
                    : run_typed<double>(p, nsteps, step0, s, launches);
+}
+
+int spo_pass_enqueue(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s)
+{
+    if (lc != p.lc) drop_graph(p);  // the plan's own step graphs carry lc
+    p.lc = lc;
+    if (nsteps <= 0) return ESPEC_OK;
+    return p.mixed ? enqueue_typed<float>(p, nsteps, step0, s)
+                   : enqueue_typed<double>(p, nsteps, step0, s);
 }
 
 int spo_pass_set_shape(SpoPassPlan &p, const PassShape &s)

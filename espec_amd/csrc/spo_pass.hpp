@@ -65,6 +65,9 @@ int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    double phase_max, int tune, hipStream_t s);
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s,
                  int *launches);
+// spo_pass_run's kernels launched one by one on s, for capture into a
+// caller's graph.
+int spo_pass_enqueue(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s);
 int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s, double *ms,
                      double *bytes);
 void spo_pass_release(SpoPassPlan &p);

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <vector>
 
 #include "api_common.hpp"
 #include "axis.hpp"
@@ -429,6 +430,176 @@ int espec_spo_plan_profile(espec_spo_plan *P, int nsteps, double *ms, double *by
         bytes[k] = b[k];
     }
     *nkinds = kinds < maxkinds ? kinds : maxkinds;
+    return ESPEC_OK;
+}
+
+}  // extern "C"
+
+// A batch of split-operator plans run as one graph: each plan's whole run
+// of nsteps (every kernel of it, launched one by one) on its own branch,
+// forked from and joined to one capture stream, so that one hipGraphLaunch
+// advances every packet of every plan.  The captured kernels carry each
+// plan's step count at capture, so the graph replays that range of steps.
+struct espec_spo_batch {
+    std::vector<espec_spo_plan *> plans;
+    std::vector<long long> step0;
+    int nsteps = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr, t0 = nullptr, t1 = nullptr;
+    std::vector<hipEvent_t> ready;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float last_ms = 0;
+    ~espec_spo_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ready) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Plan P's run of nsteps from step0, every kernel launched on s.
+int plan_enqueue(espec_spo_plan *P, int nsteps, long long step0, hipStream_t s)
+{
+    if (!P->lds) return spo_pass_enqueue(P->pp, nsteps, step0, P->lc, s);
+    const int chunk = std::max(1, (1 << 24) / P->n);
+    SpoLdsArgs la = P->la;
+    for (int done = 0; done < nsteps;) {
+        const int k = std::min(chunk, nsteps - done);
+        la.nsteps = k;
+        la.step0 = step0 + done;
+        ESPEC_HIP_TRY(launch_spo_lds(la, P->mixed, P->ref, P->nt, s));
+        done += k;
+    }
+    return ESPEC_OK;
+}
+
+int batch_capture(espec_spo_batch &B)
+{
+    const int np = (int)B.plans.size();
+    std::vector<hipStream_t> br(np, nullptr);
+    std::vector<hipEvent_t> joins(np, nullptr);
+    hipEvent_t fork = nullptr;
+    hipStream_t cs = nullptr;
+    int rc = ESPEC_OK;
+    auto fail = [&](hipError_t e) {
+        if (e != hipSuccess && rc == ESPEC_OK) rc = ESPEC_ERR_HIP;
+        return e != hipSuccess;
+    };
+    if (!fail(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) &&
+        !fail(hipEventCreateWithFlags(&fork, hipEventDisableTiming))) {
+        for (int i = 0; i < np && rc == ESPEC_OK; ++i) {
+            if (!fail(hipStreamCreateWithFlags(&br[i], hipStreamNonBlocking)))
+                fail(hipEventCreateWithFlags(&joins[i], hipEventDisableTiming));
+        }
+    }
+    if (rc == ESPEC_OK && !fail(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal))) {
+        fail(hipEventRecord(fork, cs));
+        for (int i = 0; i < np && rc == ESPEC_OK; ++i) {
+            if (fail(hipStreamWaitEvent(br[i], fork, 0))) break;
+            const int r = plan_enqueue(B.plans[i], B.nsteps, B.step0[i], br[i]);
+            if (r) {
+                rc = r;
+                break;
+            }
+            if (fail(hipEventRecord(joins[i], br[i]))) break;
+            fail(hipStreamWaitEvent(cs, joins[i], 0));
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        if (rc == ESPEC_OK && !fail(e)) {
+            B.graph = g;
+            fail(hipGraphInstantiate(&B.exec, g, nullptr, nullptr, 0));
+        } else if (g) {
+            (void)hipGraphDestroy(g);
+        }
+    }
+    for (int i = 0; i < np; ++i) {
+        if (joins[i]) (void)hipEventDestroy(joins[i]);
+        if (br[i]) (void)hipStreamDestroy(br[i]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (cs) (void)hipStreamDestroy(cs);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int espec_spo_batch_create(espec_spo_batch **out, espec_spo_plan *const *plans, int nplans,
+                           int nsteps)
+{
+    if (!out || !plans || nplans < 1 || nsteps < 1) return ESPEC_ERR_ARG;
+    *out = nullptr;
+    std::unique_ptr<espec_spo_batch> B(new espec_spo_batch);
+    B->nsteps = nsteps;
+    for (int i = 0; i < nplans; ++i) {
+        espec_spo_plan *P = plans[i];
+        if (!P) return ESPEC_ERR_ARG;
+        for (espec_spo_plan *Q : B->plans)
+            if (Q == P) return ESPEC_ERR_ARG;  // each plan once
+        ESPEC_HIP_TRY(hipStreamSynchronize(P->stream));
+        B->plans.push_back(P);
+        B->step0.push_back(P->steps_done);
+    }
+    ESPEC_HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    ESPEC_HIP_TRY(hipEventCreateWithFlags(&B->done, hipEventDisableTiming));
+    ESPEC_HIP_TRY(hipEventCreate(&B->t0));
+    ESPEC_HIP_TRY(hipEventCreate(&B->t1));
+    for (int i = 0; i < nplans; ++i) {
+        hipEvent_t e = nullptr;
+        ESPEC_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        B->ready.push_back(e);
+    }
+    const int rc = batch_capture(*B);
+    if (rc) return rc;
+    *out = B.release();
+    return ESPEC_OK;
+}
+
+int espec_spo_batch_run(espec_spo_batch *B)
+{
+    if (!B || !B->exec) return ESPEC_ERR_ARG;
+    for (size_t i = 0; i < B->plans.size(); ++i)
+        if (B->plans[i]->steps_done != B->step0[i]) return ESPEC_ERR_ARG;  // not the captured range
+    // the graph starts after every plan's pending work and the plans' own
+    // streams (get, sync, later runs) continue after the graph
+    for (size_t i = 0; i < B->plans.size(); ++i) {
+        ESPEC_HIP_TRY(hipEventRecord(B->ready[i], B->plans[i]->stream));
+        ESPEC_HIP_TRY(hipStreamWaitEvent(B->stream, B->ready[i], 0));
+    }
+    ESPEC_HIP_TRY(hipEventRecord(B->t0, B->stream));
+    ESPEC_HIP_TRY(hipGraphLaunch(B->exec, B->stream));
+    ESPEC_HIP_TRY(hipEventRecord(B->t1, B->stream));
+    ESPEC_HIP_TRY(hipEventRecord(B->done, B->stream));
+    for (espec_spo_plan *P : B->plans) {
+        ESPEC_HIP_TRY(hipStreamWaitEvent(P->stream, B->done, 0));
+        P->steps_done += B->nsteps;
+    }
+    return ESPEC_OK;
+}
+
+int espec_spo_batch_sync(espec_spo_batch *B, double *ms)
+{
+    if (!B) return ESPEC_ERR_ARG;
+    ESPEC_HIP_TRY(hipStreamSynchronize(B->stream));
+    float t = 0;
+    if (hipEventElapsedTime(&t, B->t0, B->t1) == hipSuccess) B->last_ms = t;
+    if (ms) *ms = B->last_ms;
+    return ESPEC_OK;
+}
+
+int espec_spo_batch_destroy(espec_spo_batch *B)
+{
+    delete B;
     return ESPEC_OK;
 }
 

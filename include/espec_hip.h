@@ -499,6 +499,20 @@ int espec_spo_plan_reset(espec_spo_plan *plan);
  * k = 1 the strided (y, z) passes; nkinds receives the number of kinds. */
 int espec_spo_plan_profile(espec_spo_plan *plan, int nsteps, double *ms,
                            double *bytes, int maxkinds, int *nkinds);
+/* An uneven batch as one graph (no reference counterpart: it replaces the
+ * job farm of tools&scrips/parallel.x).  create() records every kernel of
+ * nsteps steps of each plan -- one plan per grid, from each plan's current
+ * step -- on its own branch of one HIP graph; run() advances every packet
+ * of every plan by nsteps with a single graph launch (the plans' streams
+ * wait for it) and returns ESPEC_ERR_ARG unless every plan is still at the
+ * step it was captured at (reset() the plans to replay from step 0).
+ * sync() waits and gives the graph's duration in ms (HIP events). */
+typedef struct espec_spo_batch espec_spo_batch;
+int espec_spo_batch_create(espec_spo_batch **batch, espec_spo_plan *const *plans,
+                           int nplans, int nsteps);
+int espec_spo_batch_run(espec_spo_batch *batch);
+int espec_spo_batch_sync(espec_spo_batch *batch, double *ms);
+int espec_spo_batch_destroy(espec_spo_batch *batch);
 
 /* ---------------------------------------------------------------------
  * Batched short-iterative Lanczos plan: npk packets on one grid (PLNZ's

@@ -88,3 +88,32 @@ def test_config4_subset(gpu, orc):
         o = orc.pspofft(p.np, p.dt_fs, 0.0, 59 * p.dt_fs, 1, p.sh, p.sm, p.u, p.v, p.vpot, EXACT)
         assert dC(b.get(i), o) <= 1e-6, (i, p.np)
     b.close()
+
+
+@pytest.mark.parametrize("flags", [E.MIXED | E.EXACT, E.FP64 | E.EXACT])
+def test_batch_graph_matches_streams(gpu, flags):
+    """The uneven mix (LDS-resident 1-D groups, multi-pass 2-D groups) run
+    as one graph launch gives the per-plan stream runs' C(t) and state bit
+    for bit; replaying needs the plans back at the captured step."""
+    pk = configs.config4(count=40, nsteps=37, seed=4)
+    nst = 37
+    print(f"case: batch graph vs streams, {len(pk)} packets, flags={flags}", flush=True)
+    b = SpoBatch(pk, flags=flags, maxsamp=nst + 1)
+    b.run(nst)
+    b.sync()
+    ref = [b.get(i, psi=True) for i in range(len(pk))]
+    b.reset()
+    g = b.graph(nst)
+    for rep in range(2):
+        g.run()
+        ms = g.sync()
+        assert ms > 0
+        for i in range(len(pk)):
+            r = b.get(i, psi=True)
+            for k in ("cr", "ci", "norm", "u", "v"):
+                assert np.array_equal(r[k], ref[i][k]), (rep, i, k)
+        with pytest.raises(E.EspecError):
+            g.run()  # the plans have moved past the captured steps
+        b.reset()
+    g.close()
+    b.close()
