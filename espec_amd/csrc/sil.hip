@@ -440,19 +440,46 @@ __global__ __launch_bounds__(NT) void lnz_init(LnzArgs a, const V2<S> *x, const 
 }
 
 // lnz.f:89-108: v_i = w / beta, w = H v_i - beta v_{i-1}, alpha_i = <v_i, w>.
-// Iteration i's three steps are device functions (i, and the chunk c of
-// the projection, as arguments) so that lnz_block can run a whole run of
-// iterations in one launch; the kernels below wrap them.
+// One recurrence's vectors: the residual ping-pong w[2], the ring of v_i
+// and v_{i-1}, and its basis [ncols][n] (global memory, or LDS in
+// lnz_block_lds).
+template <typename S> struct VecView {
+    S *w0, *w1;      // residual of even / odd iterations' input
+    S *r0, *r1;      // v_i of even / odd iterations
+    S *basis;
+    // selects, not a runtime index: an indexed pointer array would live in
+    // scratch memory
+    __device__ S *w(int k) const { return k ? w1 : w0; }
+    __device__ S *ring(int k) const { return k ? r1 : r0; }
+};
+
+template <typename S>
+__device__ __forceinline__ VecView<S> global_view(const LnzArgs &a, int vec)
+{
+    const long long vo = (long long)vec * a.n;
+    VecView<S> v;
+    v.w0 = reinterpret_cast<S *>(a.w0) + vo;
+    v.w1 = reinterpret_cast<S *>(a.w1) + vo;
+    v.r0 = reinterpret_cast<S *>(a.vring) + vo;
+    v.r1 = reinterpret_cast<S *>(a.vring) + ((long long)a.nvec + vec) * a.n;
+    v.basis = reinterpret_cast<S *>(a.basis) + (long long)vec * a.ncols * a.n;
+    return v;
+}
+
+// Iteration i's three steps are device functions (i, the chunk c of the
+// projection and the recurrence's vectors as arguments) so that lnz_block
+// can run a whole run of iterations in one launch; the kernels below wrap
+// them.
 template <typename S, int NT>
-__device__ __forceinline__ void apply_it(const LnzArgs &a, int i, int bfrom, double *red)
+__device__ __forceinline__ void apply_it(const LnzArgs &a, const VecView<S> &V, int i, int bfrom,
+                                         double *red)
 {
     const int vec = blockIdx.y;
-    const long long vo = (long long)vec * a.n;
-    const S *win = reinterpret_cast<const S *>((i & 1) ? a.w1 : a.w0) + vo;
-    S *wout = reinterpret_cast<S *>((i & 1) ? a.w0 : a.w1) + vo;
-    S *vcur = reinterpret_cast<S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
-    const S *vprev = reinterpret_cast<const S *>(a.vring) + ((long long)((i + 1) & 1) * a.nvec + vec) * a.n;
-    S *B = reinterpret_cast<S *>(a.basis) + (long long)vec * a.ncols * a.n;
+    const S *win = V.w(i & 1);
+    S *wout = V.w((i & 1) ^ 1);
+    S *vcur = V.ring(i & 1);
+    const S *vprev = V.ring((i + 1) & 1);
+    S *B = V.basis;
     const double *vp = a.vpot + ((long long)(vec >> a.vshift)) * a.n;
     // beta_{i-1} (beta0 at i = 0) from the partials of the kernel before,
     // or from bt when this launch extends an earlier run.  A zero start
@@ -503,17 +530,17 @@ __device__ __forceinline__ void apply_it(const LnzArgs &a, int i, int bfrom, dou
 // columns of the Gram window, kChunk columns per grid.z slice (the residual
 // itself is updated by lnz_update, so every slice reads the same w).
 template <typename S, int NT>
-__device__ __forceinline__ void project_it(const LnzArgs &a, int i, int c, double *red)
+__device__ __forceinline__ void project_it(const LnzArgs &a, const VecView<S> &V, int i, int c,
+                                           double *red)
 {
     const int vec = blockIdx.y;
     const int nj = min(i + 1, a.nreort);
     const int j0 = c * kChunk;
     if (j0 >= nj) return;
     const int jn = min(kChunk, nj - j0);
-    const long long vo = (long long)vec * a.n;
-    const S *w = reinterpret_cast<const S *>((i & 1) ? a.w0 : a.w1) + vo;
-    const S *vcur = reinterpret_cast<const S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
-    const S *B = reinterpret_cast<const S *>(a.basis) + ((long long)vec * a.ncols + j0) * a.n;
+    const S *w = V.w((i & 1) ^ 1);
+    const S *vcur = V.ring(i & 1);
+    const S *B = V.basis + (long long)j0 * a.n;
     const double al = sum_parts1<NT>(a, a.pa + (long long)vec * a.nb * kPart, red);
     if (c == 0 && blockIdx.x == 0 && threadIdx.x == 0) a.al[vec * a.m + i] = al;
     double r[kChunk];
@@ -530,14 +557,14 @@ __device__ __forceinline__ void project_it(const LnzArgs &a, int i, int c, doubl
 
 // w <- (w - alpha v_i) - sum_j r_j v_j (reort.f:69-78), beta_i = |w|.
 template <typename S, int NT>
-__device__ __forceinline__ void update_it(const LnzArgs &a, int i, double *red, double *rs)
+__device__ __forceinline__ void update_it(const LnzArgs &a, const VecView<S> &V, int i, double *red,
+                                          double *rs)
 {
     const int vec = blockIdx.y;
     const int nj = min(i + 1, a.nreort);
-    const long long vo = (long long)vec * a.n;
-    S *w = reinterpret_cast<S *>((i & 1) ? a.w0 : a.w1) + vo;
-    const S *vcur = reinterpret_cast<const S *>(a.vring) + ((long long)(i & 1) * a.nvec + vec) * a.n;
-    const S *B = reinterpret_cast<const S *>(a.basis) + (long long)vec * a.ncols * a.n;
+    S *w = V.w((i & 1) ^ 1);
+    const S *vcur = V.ring(i & 1);
+    const S *B = V.basis;
     const double al = a.al[vec * a.m + i];
     // Gram coefficient j: wavefront j mod (NT/64) sums the nb partials of
     // lnz_project, lanes in a fixed order, then a wavefront shuffle tree.
@@ -596,14 +623,14 @@ template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_apply(LnzArgs a)
 {
     __shared__ double red[(NT / 64) * kPart];
-    apply_it<S, NT>(a, a.it, a.bfrom, red);
+    apply_it<S, NT>(a, global_view<S>(a, blockIdx.y), a.it, a.bfrom, red);
 }
 
 template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_project(LnzArgs a)
 {
     __shared__ double red[(NT / 64) * kPart];
-    project_it<S, NT>(a, a.it, blockIdx.z, red);
+    project_it<S, NT>(a, global_view<S>(a, blockIdx.y), a.it, blockIdx.z, red);
 }
 
 template <typename S, int NT>
@@ -611,7 +638,7 @@ __global__ __launch_bounds__(NT) void lnz_update(LnzArgs a)
 {
     __shared__ double red[(NT / 64) * kPart];
     __shared__ double rs[kMaxReort];
-    update_it<S, NT>(a, a.it, red, rs);
+    update_it<S, NT>(a, global_view<S>(a, blockIdx.y), a.it, red, rs);
 }
 
 template <int NT>
@@ -628,21 +655,72 @@ __global__ __launch_bounds__(NT) void lnz_finish(LnzArgs a)
 // read back by the thread that wrote them).  bfrom as in lnz_apply for
 // iteration i0; with fin, also beta of iteration i1-1 (lnz_finish).
 template <typename S, int NT>
+__device__ __forceinline__ void block_iterations(const LnzArgs &a, const VecView<S> &V, int i0,
+                                                 int i1, double *red, double *rs)
+{
+    for (int i = i0; i < i1; ++i) {
+        apply_it<S, NT>(a, V, i, i == i0 ? a.bfrom : 0, red);
+        __syncthreads();
+        const int nj = min(i + 1, a.nreort);
+        for (int c = 0; c * kChunk < nj; ++c) {
+            project_it<S, NT>(a, V, i, c, red);
+            __syncthreads();  // thread 0 may still read red in the last sum
+        }
+        update_it<S, NT>(a, V, i, red, rs);
+        __syncthreads();
+    }
+}
+
+template <typename S, int NT>
 __global__ __launch_bounds__(NT) void lnz_block(LnzArgs a, int i0, int i1, int fin)
 {
     __shared__ double red[(NT / 64) * kPart];
     __shared__ double rs[kMaxReort];
-    for (int i = i0; i < i1; ++i) {
-        apply_it<S, NT>(a, i, i == i0 ? a.bfrom : 0, red);
-        __syncthreads();
-        const int nj = min(i + 1, a.nreort);
-        for (int c = 0; c * kChunk < nj; ++c) {
-            project_it<S, NT>(a, i, c, red);
-            __syncthreads();  // thread 0 may still read red in the last sum
-        }
-        update_it<S, NT>(a, i, red, rs);
-        __syncthreads();
+    block_iterations<S, NT>(a, global_view<S>(a, blockIdx.y), i0, i1, red, rs);
+    if (fin) finish_it<NT>(a, i1 - 1, red);
+}
+
+// LDS bytes of lnz_block_lds for grid size n and ncols basis columns.
+size_t lnz_lds_bytes(int n, int ncols, size_t elem) { return (size_t)(4 + ncols) * n * elem; }
+
+// lnz_block with the recurrence's vectors (residuals, v ring, basis) held
+// in LDS for the whole run: they are loaded once, every step reads and
+// writes them there instead of making a round trip to L2 per phase, and
+// they are stored back at the end, so that the kernels that follow (and a
+// later extension of the run) find them in global memory as lnz_block
+// leaves them.  Same arithmetic and order as lnz_block, bit for bit.
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void lnz_block_lds(LnzArgs a, int i0, int i1, int fin)
+{
+    __shared__ double red[(NT / 64) * kPart];
+    __shared__ double rs[kMaxReort];
+    extern __shared__ double smem_l[];
+    const int n = a.n;
+    const VecView<S> G = global_view<S>(a, blockIdx.y);
+    S *base = reinterpret_cast<S *>(smem_l);
+    VecView<S> L;
+    L.w0 = base;
+    L.w1 = base + n;
+    L.r0 = base + 2 * n;
+    L.r1 = base + 3 * n;
+    L.basis = base + 4 * n;
+    const long long nb = (long long)a.ncols * n;
+    for (int p = threadIdx.x; p < n; p += NT) {
+        L.w0[p] = G.w0[p];
+        L.w1[p] = G.w1[p];
+        L.r0[p] = G.r0[p];
+        L.r1[p] = G.r1[p];
     }
+    for (long long p = threadIdx.x; p < nb; p += NT) L.basis[p] = G.basis[p];
+    __syncthreads();
+    block_iterations<S, NT>(a, L, i0, i1, red, rs);
+    for (int p = threadIdx.x; p < n; p += NT) {
+        G.w0[p] = L.w0[p];
+        G.w1[p] = L.w1[p];
+        G.r0[p] = L.r0[p];
+        G.r1[p] = L.r1[p];
+    }
+    for (long long p = threadIdx.x; p < nb; p += NT) G.basis[p] = L.basis[p];
     if (fin) finish_it<NT>(a, i1 - 1, red);
 }
 
@@ -1507,14 +1585,31 @@ bool set_grid(LnzArgs &l, int ndim, const int *np, const double *shm)
 int blocks_for(int n) { return std::max(1, std::min(256, (n + 255) / 256)); }
 
 // The m iterations of LNZ for every recurrence of l, on stream s.
+// Iterations i0 .. i1-1 and the last beta of a one-block grid in one
+// launch: in LDS when the recurrence's vectors fit beside the kernel's
+// static arrays within the default 64 KiB (no attribute call, which a
+// stream capture may not record), else on global memory.
+constexpr size_t kBlockLdsMax = 56 * 1024;
+
+template <typename S>
+hipError_t launch_block(const LnzArgs &lb, int i0, int i1, hipStream_t s)
+{
+    const size_t lds = lnz_lds_bytes(lb.n, lb.ncols, sizeof(S));
+    if (lds <= kBlockLdsMax)
+        hipLaunchKernelGGL((lnz_block_lds<S, 256>), dim3(1, lb.nvec), dim3(256), lds, s, lb, i0,
+                           i1, 1);
+    else
+        hipLaunchKernelGGL((lnz_block<S, 256>), dim3(1, lb.nvec), dim3(256), 0, s, lb, i0, i1, 1);
+    return hipGetLastError();
+}
+
 template <typename S>
 hipError_t lnz_iterations(const LnzArgs &l, hipStream_t s)
 {
     if (l.nb == 1) {  // one launch for all m iterations and the last beta
         LnzArgs lb = l;
         lb.bfrom = 0;
-        hipLaunchKernelGGL((lnz_block<S, 256>), dim3(1, l.nvec), dim3(256), 0, s, lb, 0, l.m, 1);
-        return hipGetLastError();
+        return launch_block<S>(lb, 0, l.m, s);
     }
     dim3 grid(l.nb, l.nvec), gridp(l.nb, l.nvec, l.nchunk);
     for (int i = 0; i < l.m; ++i) {
@@ -1670,8 +1765,8 @@ hipError_t ctl_launches(espec_sil_plan &P, const CtlArgs &c, const uint16_t *wb,
         if (l.nb == 1) {  // one launch (extending: beta_{have-1} is in bt)
             LnzArgs lb = l;
             lb.bfrom = have > 0;
-            hipLaunchKernelGGL((lnz_block<S, 256>), dim3(1, l.nvec), dim3(256), 0, s, lb, have,
-                               iters, 1);
+            hipError_t e = launch_block<S>(lb, have, iters, s);
+            if (e != hipSuccess) return e;
         } else {
             for (int i = have; i < iters; ++i) {
                 LnzArgs li = l;
