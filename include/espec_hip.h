@@ -506,7 +506,9 @@ int espec_spo_plan_profile(espec_spo_plan *plan, int nsteps, double *ms,
  * of every plan by nsteps with a single graph launch (the plans' streams
  * wait for it) and returns ESPEC_ERR_ARG unless every plan is still at the
  * step it was captured at (reset() the plans to replay from step 0).
- * sync() waits and gives the graph's duration in ms (HIP events). */
+ * sync() waits and gives the graph's duration in ms (HIP events).  The
+ * plans must outlive the batch; the graph keeps the launch shapes the
+ * plans had at create(). */
 typedef struct espec_spo_batch espec_spo_batch;
 int espec_spo_batch_create(espec_spo_batch **batch, espec_spo_plan *const *plans,
                            int nplans, int nsteps);
