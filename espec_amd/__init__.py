@@ -765,7 +765,7 @@ def mlp_train(params, x, t, epochs, lr=1e-3, seed=1):
 
 
 _SHAPE_KEYS = ("lds", "rows", "rows_nt", "tile", "tile_nt", "spread", "xmap", "grid", "rows_pen",
-               "tile_pen", "pen_pf", "vph")
+               "tile_pen", "pen_pf", "vph", "fused")
 
 
 class SpoPlan(_Plan):
@@ -799,7 +799,7 @@ class SpoPlan(_Plan):
 
     def shape(self):
         """Launch shape: dict(lds, rows, rows_nt, tile, tile_nt, spread, xmap, grid,
-        rows_pen, tile_pen, pen_pf, vph)."""
+        rows_pen, tile_pen, pen_pf, vph, fused)."""
         a = np.zeros(len(_SHAPE_KEYS), dtype=np.int32)
         _check(self._L.espec_spo_plan_shape(self._h, a), "spo_plan_shape")
         return dict(zip(_SHAPE_KEYS, a.tolist()))

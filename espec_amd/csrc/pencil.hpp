@@ -270,14 +270,31 @@ template <int M> __device__ __forceinline__ int xslot(int k1, int n2)
 // Exchange slots per line.
 template <int M> constexpr int xslots() { return 17 * M; }
 
-// Natural -> spectral, sign SIGN.  xl: this line's exchange slots, which
-// only this wavefront touches during the transform.
+// Exchange element i (= k1 M + n2) of a line.  A pointer is the line's own
+// block of xslots() slots (xslot's padding); a Strided view is a line whose
+// elements lie pitch slots apart in a larger LDS array (a column of an
+// LDS-resident grid), exchanged in place.
+template <typename V> __device__ __forceinline__ V &xat(V *xl, int i) { return xl[i + (i >> 4)]; }
+template <typename V> struct Strided {
+    V *p;
+    int pitch;
+};
+template <typename V> __device__ __forceinline__ V &xat(const Strided<V> &xl, int i)
+{
+    return xl.p[i * xl.pitch];
+}
+
+// Natural -> spectral, sign SIGN.  xl: this line's exchange slots (a
+// pointer or a Strided view, see xat), which only this wavefront touches
+// during the transform.  The exchange may overlay the slots the line was
+// read from: one wavefront's LDS accesses complete in issue order, and
+// every write here depends on all sixteen inputs.
 // tw: double2 data take exp(2 pi i m / N), m in [0, N), and form the
 // powers they need; float2 data take the split table of every twiddle,
 // entry k1 M + j = W^{j k1} for the forward, then at N + (q M + n2) M + j
 // W^{(j Q + q) n2} for the inverse.
-template <int M, int SIGN, typename V>
-__device__ __forceinline__ void forward(V (&v)[16], int j, V *xl,
+template <int M, int SIGN, typename V, typename X>
+__device__ __forceinline__ void forward(V (&v)[16], int j, X xl,
                                         const typename ConstOf<V>::T *__restrict__ tw)
 {
     constexpr int Q = 16 / M;
@@ -287,20 +304,20 @@ __device__ __forceinline__ void forward(V (&v)[16], int j, V *xl,
     else
         twiddle_powers<SIGN, 16, 0>(v, tw, j);
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) xl[xslot<M>(k1, j)] = v[k1];
+    for (int k1 = 0; k1 < 16; ++k1) xat(xl, k1 * M + j) = v[k1];
     wave_sync();
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
-        for (int n2 = 0; n2 < M; ++n2) v[q * M + n2] = xl[xslot<M>(j * Q + q, n2)];
+        for (int n2 = 0; n2 < M; ++n2) v[q * M + n2] = xat(xl, (j * Q + q) * M + n2);
 #pragma unroll
     for (int q = 0; q < Q; ++q) dft<M, SIGN, 0>(v + q * M);
 }
 
 // Spectral -> natural, sign SIGN (exact inverse of forward<M, -SIGN> up to
 // the factor N).
-template <int M, int SIGN, typename V>
-__device__ __forceinline__ void inverse(V (&v)[16], int j, V *xl,
+template <int M, int SIGN, typename V, typename X>
+__device__ __forceinline__ void inverse(V (&v)[16], int j, X xl,
                                         const typename ConstOf<V>::T *__restrict__ tw)
 {
     constexpr int Q = 16 / M;
@@ -315,10 +332,10 @@ __device__ __forceinline__ void inverse(V (&v)[16], int j, V *xl,
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
-        for (int n2 = 0; n2 < M; ++n2) xl[xslot<M>(j * Q + q, n2)] = v[q * M + n2];
+        for (int n2 = 0; n2 < M; ++n2) xat(xl, (j * Q + q) * M + n2) = v[q * M + n2];
     wave_sync();
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) v[k1] = xl[xslot<M>(k1, j)];
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = xat(xl, k1 * M + j);
     dft16<SIGN, 0>(v);
 }
 

@@ -777,6 +777,180 @@ __global__ __launch_bounds__(64) void pass_strided_wide(PassArgs a)
     if (a.last && blockIdx.x == 0 && threadIdx.x == 0) *a.dstep += 1;
 }
 
+// ------------------------------------------------ LDS-resident pencils
+// Two-dimensional grids of L0 x L1 points (L = 16 M, axis 0 contiguous)
+// whose state fits one workgroup's LDS in storage precision: one
+// workgroup per packet keeps the grid in LDS for every step of a launch,
+// so per step HBM carries only V (or its phase table) and psi0, 12 B per
+// point in complex64 instead of the 44 B of a row pass and a strided pass.
+// Each step is the arithmetic of those two passes on the LDS copy: the row
+// pass of pass_rows_pen_sp (complex64; pass_rows_pen for complex128) on
+// every row, then the strided pass of pass_strided_pen on every column,
+// with every pencil exchange done in place in the line's own slots.  The
+// state is therefore bit for bit the multi-pass plan's with those kernels,
+// and C(t) is too: the partial sums are formed per row block of C0 rows
+// and added in block order, as reduce_parts adds them.
+// LDS layout: row y at y RP, element x at x + x / 16 (xat's padding); the
+// pitch RP = 17 L0 / 16 + 1 puts the rows that one column exchange reads
+// together in different banks.
+struct FusedArgs {
+    int n, npk, nsteps, lc, maxsamp, ktps;
+    long long step0;
+    void *x;
+    const void *psi0, *vpot;
+    const double2 *vph;
+    double *cbuf;
+    double b, cst2;
+    const double2 *twr[2], *kreg[2];    // FP64 pencil tables of axes 0 and 1
+    const float4 *twr_s[2], *kreg_s[2];  // their split forms (complex64 storage)
+};
+
+template <typename S> __device__ __forceinline__ const PenW<S> *fz_twr(const FusedArgs &a, int ax);
+template <> __device__ __forceinline__ const double2 *fz_twr<double>(const FusedArgs &a, int ax) { return a.twr[ax]; }
+template <> __device__ __forceinline__ const float4 *fz_twr<float>(const FusedArgs &a, int ax) { return a.twr_s[ax]; }
+template <typename S> __device__ __forceinline__ const PenW<S> *fz_kreg(const FusedArgs &a, int ax);
+template <> __device__ __forceinline__ const double2 *fz_kreg<double>(const FusedArgs &a, int ax) { return a.kreg[ax]; }
+template <> __device__ __forceinline__ const float4 *fz_kreg<float>(const FusedArgs &a, int ax) { return a.kreg_s[ax]; }
+
+constexpr int fz_pitch(int l0) { return l0 + l0 / 16 + 1; }
+
+// NW wavefronts per workgroup; row blocks and column groups are dealt to
+// them round robin.  The whole launch advances nsteps steps from step0
+// (the state in HBM is in natural order before and after).
+template <typename S, int M0, int M1, int NW, bool PHT>
+__global__ __launch_bounds__(64 * NW, 2) void pass_fused2d(FusedArgs a)
+{
+    using VS = V2<S>;
+    constexpr int L0 = 16 * M0, L1 = 16 * M1, C0 = 64 / M0, C1 = 64 / M1;
+    constexpr int RP = fz_pitch(L0), NB = L1 / C0, NT = 64 * NW;
+    extern __shared__ double smem_d[];
+    VS *xs = reinterpret_cast<VS *>(smem_d);
+    double *red = reinterpret_cast<double *>(xs + L1 * RP);  // [NB][3] row-block sums
+    const int pk = blockIdx.x, wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const long long po = (long long)pk * (L0 * L1);
+    VS *gx = reinterpret_cast<VS *>(a.x) + po;
+    for (int i = threadIdx.x; i < L0 * L1; i += NT) {
+        const int y = i / L0, x = i % L0;
+        xs[y * RP + x + (x >> 4)] = gx[i];
+    }
+    __syncthreads();
+    const PenW<S> *tw0 = fz_twr<S>(a, 0), *tw1 = fz_twr<S>(a, 1);
+    const PenW<S> *kt0 = fz_kreg<S>(a, 0) + (long long)pk * a.ktps;
+    const PenW<S> *kt1 = fz_kreg<S>(a, 1) + (long long)pk * a.ktps;
+    const double b = a.b;
+    for (int st = 0;; ++st) {
+        const bool inv = st > 0, fwd = st < a.nsteps;
+        const long long g = a.step0 + st;
+        const bool samp = inv && (g % a.lc) == 0;
+        // Row phase: [inverse x of step g]  B e^{-iV dt/2}  (C(t) sample)
+        // e^{-iV dt/2}  [forward x, e^{-iTx dt} of step g + 1].
+        {
+            const int j = lane % M0, l = lane / M0;
+            for (int blk = wv; blk < NB; blk += NW) {
+                const int y = blk * C0 + l;
+                VS *xl = xs + y * RP;
+                const long long pr = po + (long long)y * L0 + j;
+                VS v[16], q0[16];
+                S cv[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r] = pencil::xat(xl, r * M0 + j);
+                pencil::wave_sync();  // every lane's reads before the exchange writes
+                if (inv) pencil::inverse<M0, -1>(v, j, xl, tw0);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    if (!PHT) cv[r] = reinterpret_cast<const S *>(a.vpot)[pr + r * M0];
+                    if (samp) q0[r] = reinterpret_cast<const VS *>(a.psi0)[pr + r * M0];
+                }
+                double cr = 0, ci = 0, nr = 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    double2 ph;
+                    if (PHT) {
+                        ph = a.vph[pr + r * M0];
+                    } else {
+                        double sn, cs;
+                        pencil::sincos_reduced(a.cst2 * (double)cv[r], &sn, &cs);
+                        ph = {cs, sn};
+                    }
+                    double2 x = {(double)v[r].x, (double)v[r].y};
+                    if (inv) {
+                        x = cmulc(x, ph);
+                        x.x *= b;
+                        x.y *= b;
+                        if (samp) {
+                            const double px = q0[r].x, py = q0[r].y;
+                            cr += px * x.x + py * x.y;
+                            ci += px * x.y - py * x.x;
+                            nr += x.x * x.x + x.y * x.y;
+                        }
+                    }
+                    if (fwd) x = cmulc(x, ph);
+                    v[r] = {(S)x.x, (S)x.y};
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                asm volatile("" ::: "memory");
+                if (fwd) {
+                    pencil::forward<M0, 1>(v, j, xl, tw0);
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) v[r] = pencil::cmulc_w(v[r], kt0[j + r * M0]);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) pencil::xat(xl, r * M0 + j) = v[r];
+                if (samp) {
+                    cr = wave_sum(cr);
+                    ci = wave_sum(ci);
+                    nr = wave_sum(nr);
+                    if (lane == 0) {
+                        red[3 * blk] = cr;
+                        red[3 * blk + 1] = ci;
+                        red[3 * blk + 2] = nr;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (samp && threadIdx.x == 0) {
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int k = 0; k < NB; ++k) {
+                s0 += red[3 * k];
+                s1 += red[3 * k + 1];
+                s2 += red[3 * k + 2];
+            }
+            const long long nc = g / a.lc;
+            if (nc < a.maxsamp) {
+                double *c = a.cbuf + ((long long)pk * a.maxsamp + nc) * 3;
+                c[0] = s0;
+                c[1] = s1;
+                c[2] = s2;
+            }
+        }
+        if (!fwd) break;
+        // Column phase: forward y, e^{-iTy dt}, inverse y.
+        {
+            const int c = lane % C1, j = lane / C1;
+            for (int cb = wv; cb < L0 / C1; cb += NW) {
+                const int x = cb * C1 + c;
+                const pencil::Strided<VS> col{xs + x + (x >> 4), RP};
+                VS v[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r] = pencil::xat(col, r * M1 + j);
+                pencil::forward<M1, 1>(v, j, col, tw1);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r] = pencil::cmulc_w(v[r], kt1[j + r * M1]);
+                pencil::wave_sync();  // the forward exchange's reads before the inverse's writes
+                pencil::inverse<M1, -1>(v, j, col, tw1);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) pencil::xat(col, r * M1 + j) = v[r];
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < L0 * L1; i += NT) {
+        const int y = i / L0, x = i % L0;
+        gx[i] = xs[y * RP + x + (x >> 4)];
+    }
+}
+
 __global__ void set_step(long long *d, long long v) { *d = v; }
 // One-dimensional grids have no strided pass to advance the step counter.
 __global__ void add_step(long long *d) { *d += 1; }
@@ -833,8 +1007,8 @@ void apply_shape(SpoPassPlan &p)
 // measured duration excludes queue gaps and marker packets.
 thread_local hipEvent_t tl_ev_start = nullptr, tl_ev_stop = nullptr;
 
-template <typename K>
-hipError_t launch(K kern, dim3 grid, int nt, size_t lds, hipStream_t s, const PassArgs &a)
+template <typename K, typename A>
+hipError_t launch(K kern, dim3 grid, int nt, size_t lds, hipStream_t s, const A &a)
 {
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
@@ -1060,6 +1234,88 @@ hipError_t strided(const SpoPassPlan &p, int axis, bool last, hipStream_t s)
     return hipErrorInvalidValue;
 }
 
+// LDS bytes of pass_fused2d for this grid: the padded state and the
+// row-block sums.
+size_t fused_lds(const SpoPassPlan &p)
+{
+    const size_t elem = p.mixed ? sizeof(float2) : sizeof(double2);
+    const int nb = p.np[1] / (64 / (p.np[0] / 16));
+    return elem * (size_t)p.np[1] * fz_pitch(p.np[0]) + sizeof(double) * 3 * nb;
+}
+
+// Grids with a pass_fused2d instance: 64^2 and 128^2 in complex64, 64^2
+// in complex128 (128^2 complex128 needs 272 KiB).
+bool fused_grid(const SpoPassPlan &p)
+{
+    if (p.ndim != 2) return false;
+    if (p.np[0] == 64 && p.np[1] == 64) return true;
+    return p.mixed && p.np[0] == 128 && p.np[1] == 128;
+}
+
+bool fused_ok(const SpoPassPlan &p, const PassShape &s)
+{
+    if (!s.fused) return true;
+    return (s.fused == 4 || s.fused == 8) && fused_grid(p) && !p.mask &&
+           (p.vph != nullptr || p.phase_max < pencil::kSincosMax) && fused_lds(p) <= 160 * 1024;
+}
+
+// One launch of pass_fused2d: nsteps steps of every packet from step0.
+template <typename R>
+hipError_t fused_launch(const SpoPassPlan &p, int nsteps, long long step0, hipStream_t s)
+{
+    FusedArgs f{};
+    f.n = p.n;
+    f.npk = p.npk;
+    f.nsteps = nsteps;
+    f.lc = p.lc;
+    f.maxsamp = p.maxsamp;
+    f.ktps = p.kax_pstride;
+    f.step0 = step0;
+    f.x = p.x;
+    f.psi0 = p.psi0;
+    f.vpot = p.vpot;
+    f.vph = static_cast<const double2 *>(p.vph);
+    f.cbuf = p.cbuf;
+    f.b = p.b;
+    f.cst2 = p.cst2;
+    for (int a = 0; a < 2; ++a) {
+        f.twr[a] = static_cast<const double2 *>(p.twr) + p.twr_off[a];
+        f.kreg[a] = static_cast<const double2 *>(p.kreg) + p.kax_off[a];
+        f.twr_s[a] = static_cast<const float4 *>(p.twr_s) + 2 * p.twr_off[a];
+        f.kreg_s[a] = static_cast<const float4 *>(p.kreg_s) + p.kax_off[a];
+    }
+    const size_t lds = fused_lds(p);
+    const bool pht = p.vph != nullptr;
+    const int nw = p.shape.fused;
+    const dim3 grid(p.npk);
+#define ESPEC_FUSED2(M0_, M1_, NW_)                                                              \
+    if (p.np[0] == 16 * M0_ && p.np[1] == 16 * M1_ && nw == NW_)                                  \
+        return pht ? launch(pass_fused2d<R, M0_, M1_, NW_, true>, grid, 64 * NW_, lds, s, f)       \
+                   : launch(pass_fused2d<R, M0_, M1_, NW_, false>, grid, 64 * NW_, lds, s, f);
+    ESPEC_FUSED2(4, 4, 4)
+    ESPEC_FUSED2(4, 4, 8)
+    if constexpr (std::is_same<R, float>::value) {
+        ESPEC_FUSED2(8, 8, 4)
+        ESPEC_FUSED2(8, 8, 8)
+    }
+#undef ESPEC_FUSED2
+    return hipErrorInvalidValue;
+}
+
+// A run of the fused shape: launches of about 2^24 point-steps per packet.
+template <typename R>
+int fused_run(const SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *launches)
+{
+    const int chunk = std::max(1, (1 << 24) / p.n);
+    for (int done = 0; done < nsteps;) {
+        const int k = std::min(chunk, nsteps - done);
+        ESPEC_HIP_TRY(fused_launch<R>(p, k, step0 + done, s));
+        ++*launches;
+        done += k;
+    }
+    return ESPEC_OK;
+}
+
 template <typename R>
 hipError_t one_step(const SpoPassPlan &p, bool inv, bool fwd, hipStream_t s, int *launches)
 {
@@ -1118,6 +1374,7 @@ template <typename R>
 int enqueue_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s)
 {
     int launches = 0;
+    if (p.shape.fused) return fused_run<R>(p, nsteps, step0, s, &launches);
     hipLaunchKernelGGL(set_step, dim3(1), dim3(1), 0, s, p.dstep, step0);
     ESPEC_HIP_TRY(hipGetLastError());
     ESPEC_HIP_TRY(one_step<R>(p, false, true, s, &launches));
@@ -1138,6 +1395,7 @@ int enqueue_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s)
 template <typename R>
 int run_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, int *launches)
 {
+    if (p.shape.fused) return fused_run<R>(p, nsteps, step0, s, launches);
     hipLaunchKernelGGL(set_step, dim3(1), dim3(1), 0, s, p.dstep, step0);
     ESPEC_HIP_TRY(hipGetLastError());
     ++*launches;
@@ -1453,6 +1711,37 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
         }
         p.shape = best_sh;
     }
+    // The LDS-resident form where the grid has one: launches of 16 steps
+    // against the final multi-pass step, per step.  Its samples go past
+    // maxsamp, so the tuning writes none.
+    if (fused_grid(p)) {
+        const PassShape keep = p.shape;
+        apply_shape(p);
+        ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
+        int dummy = 0;
+        best_ms = 1e30f;
+        int rc = time_launches(s, reps, [&]() { return one_step<R>(p, true, true, s, &dummy); }, &best_ms);
+        if (rc) return rc;
+        constexpr int kSteps = 16;
+        const long long past = (long long)p.maxsamp * p.lc;
+        int best_nw = 0;
+        for (int nw : {4, 8}) {
+            PassShape sh = keep;
+            sh.fused = nw;
+            if (!fused_ok(p, sh)) continue;
+            p.shape = sh;
+            ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
+            float ms = 0;
+            rc = time_launches(s, 2, [&]() { return fused_launch<R>(p, kSteps, past, s); }, &ms);
+            if (rc) return rc;
+            if (clearly_faster(ms / kSteps, best_ms)) {
+                best_ms = ms / kSteps;
+                best_nw = nw;
+            }
+        }
+        p.shape = keep;
+        p.shape.fused = best_nw;
+    }
     apply_shape(p);
     ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
     ESPEC_HIP_TRY(hipStreamSynchronize(s));
@@ -1468,6 +1757,28 @@ int tune_typed(SpoPassPlan &p, hipStream_t s)
 template <typename R>
 int profile_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, double *ms, double *bytes)
 {
+    if (p.shape.fused) {
+        // One launch of nsteps: psi read and written once, V every step,
+        // psi0 at every sample step.
+        hipEvent_t e0, e1;
+        ESPEC_HIP_TRY(hipEventCreate(&e0));
+        ESPEC_HIP_TRY(hipEventCreate(&e1));
+        tl_ev_start = e0;
+        tl_ev_stop = e1;
+        const hipError_t e = fused_launch<R>(p, nsteps, step0, s);
+        tl_ev_start = tl_ev_stop = nullptr;
+        ESPEC_HIP_TRY(e);
+        ESPEC_HIP_TRY(hipStreamSynchronize(s));
+        float t = 0;
+        ESPEC_HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double vs = sizeof(V2<R>), rs = sizeof(R);
+        ms[0] = t;
+        bytes[0] = (double)p.n * p.npk * (2 * vs + nsteps * rs + vs * ((double)nsteps / p.lc));
+        ms[1] = bytes[1] = 0;
+        return ESPEC_OK;
+    }
     std::vector<hipEvent_t> ev;
     // Events recorded by the kernel dispatch itself (start, stop).
     auto arm = [&]() -> hipError_t {
@@ -1714,6 +2025,7 @@ int spo_pass_prepare(SpoPassPlan &p, int lc)
 {
     if (lc != p.lc) drop_graph(p);
     p.lc = lc;
+    if (p.shape.fused) return ESPEC_OK;  // no step graphs: one launch per chunk of steps
     for (int j = 0; j < SpoPassPlan::kGraphLevels; ++j) {
         const int rc = p.mixed ? capture_typed<float>(p, j) : capture_typed<double>(p, j);
         if (rc) return rc;
@@ -1742,7 +2054,7 @@ int spo_pass_enqueue(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStr
 
 int spo_pass_set_shape(SpoPassPlan &p, const PassShape &s)
 {
-    if (!rows_ok(p, s) || (p.ndim > 1 && !tile_ok(p, s))) return ESPEC_ERR_ARG;
+    if (!rows_ok(p, s) || (p.ndim > 1 && !tile_ok(p, s)) || !fused_ok(p, s)) return ESPEC_ERR_ARG;
     if (s.spread < 1 || s.spread > 8 || (s.spread & (s.spread - 1)) || s.grid < 0 ||
         (s.grid % 8) != 0)
         return ESPEC_ERR_ARG;

@@ -21,6 +21,9 @@ struct PassShape {
                                    // item, 2 four wavefronts sharing an LDS-staged tile,
                                    // 3 one 256-point column per wavefront
     int pen_pf = 0;                // pencil strided passes prefetch their next work item
+    int fused = 0;                 // 4 or 8: 2-D grids that fit one workgroup's LDS run
+                                   // whole launches of steps LDS-resident (pass_fused2d),
+                                   // one packet per workgroup of that many wavefronts
 };
 
 struct SpoPassPlan {

@@ -371,6 +371,7 @@ int espec_spo_plan_shape(espec_spo_plan *P, int *shape)
     shape[9] = s.tile_pen;
     shape[10] = s.pen_pf;
     shape[11] = P->lds ? 0 : (P->pp.vph != nullptr);
+    shape[12] = P->lds ? 0 : s.fused;
     return ESPEC_OK;
 }
 
@@ -389,6 +390,7 @@ int espec_spo_plan_set_shape(espec_spo_plan *P, const int *shape)
     s.rows_pen = shape[8];
     s.tile_pen = shape[9];
     s.pen_pf = shape[10];
+    s.fused = shape[12];
     const int rc = spo_pass_set_shape(P->pp, s);
     return rc ? rc : spo_pass_prepare(P->pp, P->lc);
 }
@@ -423,7 +425,7 @@ int espec_spo_plan_profile(espec_spo_plan *P, int nsteps, double *ms, double *by
         int rc = spo_pass_profile(P->pp, nsteps, P->steps_done, P->lc, P->stream, m, b);
         if (rc) return rc;
         P->steps_done += nsteps;
-        kinds = P->ndim > 1 ? 2 : 1;
+        kinds = (P->ndim > 1 && !P->pp.shape.fused) ? 2 : 1;
     }
     for (int k = 0; k < kinds && k < maxkinds; ++k) {
         ms[k] = m[k];

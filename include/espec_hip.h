@@ -605,15 +605,18 @@ int espec_mlp_grad(const float *params, const float *x, const float *t,
 int espec_mlp_train(float *params, const float *x, const float *t, int n,
                     int epochs, float lr, unsigned seed, float *loss);
 
-/* Launch shape chosen for the plan, shape[12]: 1 if LDS-resident, rows per
+/* Launch shape chosen for the plan, shape[13]: 1 if LDS-resident, rows per
  * row-pass workgroup, its threads, strided tile width, its threads, XCD
  * spread, XCD-contiguous tile map, workgroups per pass (0: one per row block
  * or tile; else a grid-stride loop), register-pencil row pass, register-pencil
  * strided passes, prefetch of the next work item in the pencil strided
- * passes, and 1 if the row pass reads an FP64 V phase table (no reference
- * counterpart; for reports). */
+ * passes, 1 if the row pass reads an FP64 V phase table, and the fused
+ * form (0, or 4 or 8 wavefronts per workgroup: a 2-D grid of 64^2, or
+ * 128^2 in complex64, runs the row and strided passes' arithmetic on a
+ * copy of each packet held in one workgroup's LDS for a whole launch of
+ * steps) (no reference counterpart; for reports). */
 int espec_spo_plan_shape(espec_spo_plan *plan, int *shape);
-/* Pin a multi-pass plan's launch shape (shape[12] as above; shape[0] and
+/* Pin a multi-pass plan's launch shape (shape[13] as above; shape[0] and
  * shape[11] are ignored) in place of the autotuner's choice, e.g. to test one kernel
  * family.  ESPEC_ERR_ARG if the grid cannot use that shape,
  * ESPEC_ERR_UNSUPPORTED for an LDS-resident plan (no reference counterpart). */
