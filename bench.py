@@ -210,11 +210,14 @@ def batched(E, configs, steps, warmup, check, profile_steps):
                 plan.reset()
                 prof = plan.profile(profile_steps)
                 plan.reset()
+                m0, m1 = g["np"][0] // 16, g["np"][-1] // 16
                 names = (["spo_lds_fast"] if shape["lds"] else
+                         [f"pass_fused2d<{storage}, {m0}, {m1}, {shape['fused']}>"]
+                         if shape["fused"] else
                          list(kernel_names(shape, g["np"][0], g["np"][-1], storage)))
-                # the LDS-resident kernel runs all profile_steps steps in one
+                # the LDS-resident kernels run all profile_steps steps in one
                 # launch; the pass kernels run once per step
-                per = profile_steps if shape["lds"] else 1
+                per = profile_steps if shape["lds"] or shape["fused"] else 1
                 ent["kernels"] = [{"kernel": names[k] if k < len(names) else f"kind{k}",
                                    "ms": m, "ms_per_step": m / per, "bytes": by,
                                    "GBps": by / (m * 1e-3) / 1e9 if m > 0 else None}

@@ -781,8 +781,9 @@ __global__ __launch_bounds__(64) void pass_strided_wide(PassArgs a)
 // Two-dimensional grids of L0 x L1 points (L = 16 M, axis 0 contiguous)
 // whose state fits one workgroup's LDS in storage precision: one
 // workgroup per packet keeps the grid in LDS for every step of a launch,
-// so per step HBM carries only V (or its phase table) and psi0, 12 B per
-// point in complex64 instead of the 44 B of a row pass and a strided pass.
+// so per step HBM carries only the FP64 V phase table and psi0, 24 B per
+// point in complex64 instead of the 44 B of a row pass and a strided pass
+// (the table, not V and a sincos: the kernel is bound by its arithmetic).
 // Each step is the arithmetic of those two passes on the LDS copy: the row
 // pass of pass_rows_pen_sp (complex64; pass_rows_pen for complex128) on
 // every row, then the strided pass of pass_strided_pen on every column,
@@ -797,10 +798,10 @@ struct FusedArgs {
     int n, npk, nsteps, lc, maxsamp, ktps;
     long long step0;
     void *x;
-    const void *psi0, *vpot;
-    const double2 *vph;
+    const void *psi0;
+    const double2 *vph;  // exp(-i cst2 V) as (cos, sin), FP64
     double *cbuf;
-    double b, cst2;
+    double b;
     const double2 *twr[2], *kreg[2];    // FP64 pencil tables of axes 0 and 1
     const float4 *twr_s[2], *kreg_s[2];  // their split forms (complex64 storage)
 };
@@ -817,7 +818,7 @@ constexpr int fz_pitch(int l0) { return l0 + l0 / 16 + 1; }
 // NW wavefronts per workgroup; row blocks and column groups are dealt to
 // them round robin.  The whole launch advances nsteps steps from step0
 // (the state in HBM is in natural order before and after).
-template <typename S, int M0, int M1, int NW, bool PHT>
+template <typename S, int M0, int M1, int NW>
 __global__ __launch_bounds__(64 * NW, 2) void pass_fused2d(FusedArgs a)
 {
     using VS = V2<S>;
@@ -851,27 +852,17 @@ __global__ __launch_bounds__(64 * NW, 2) void pass_fused2d(FusedArgs a)
                 VS *xl = xs + y * RP;
                 const long long pr = po + (long long)y * L0 + j;
                 VS v[16], q0[16];
-                S cv[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) v[r] = pencil::xat(xl, r * M0 + j);
                 pencil::wave_sync();  // every lane's reads before the exchange writes
                 if (inv) pencil::inverse<M0, -1>(v, j, xl, tw0);
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    if (!PHT) cv[r] = reinterpret_cast<const S *>(a.vpot)[pr + r * M0];
+                for (int r = 0; r < 16; ++r)
                     if (samp) q0[r] = reinterpret_cast<const VS *>(a.psi0)[pr + r * M0];
-                }
                 double cr = 0, ci = 0, nr = 0;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    double2 ph;
-                    if (PHT) {
-                        ph = a.vph[pr + r * M0];
-                    } else {
-                        double sn, cs;
-                        pencil::sincos_reduced(a.cst2 * (double)cv[r], &sn, &cs);
-                        ph = {cs, sn};
-                    }
+                    const double2 ph = a.vph[pr + r * M0];
                     double2 x = {(double)v[r].x, (double)v[r].y};
                     if (inv) {
                         x = cmulc(x, ph);
@@ -1245,18 +1236,19 @@ size_t fused_lds(const SpoPassPlan &p)
 
 // Grids with a pass_fused2d instance: 64^2 and 128^2 in complex64, 64^2
 // in complex128 (128^2 complex128 needs 272 KiB).
-bool fused_grid(const SpoPassPlan &p)
+bool fused_grid(const SpoPassPlan &p) { return spo_pass_fused_grid(p.ndim, p.np, p.mixed); }
+
+// The phase table the fused form reads, if any.
+const double2 *fused_table(const SpoPassPlan &p)
 {
-    if (p.ndim != 2) return false;
-    if (p.np[0] == 64 && p.np[1] == 64) return true;
-    return p.mixed && p.np[0] == 128 && p.np[1] == 128;
+    return static_cast<const double2 *>(p.vph ? p.vph : p.vph_fz);
 }
 
 bool fused_ok(const SpoPassPlan &p, const PassShape &s)
 {
     if (!s.fused) return true;
     return (s.fused == 4 || s.fused == 8) && fused_grid(p) && !p.mask &&
-           (p.vph != nullptr || p.phase_max < pencil::kSincosMax) && fused_lds(p) <= 160 * 1024;
+           fused_table(p) != nullptr && fused_lds(p) <= 160 * 1024;
 }
 
 // One launch of pass_fused2d: nsteps steps of every packet from step0.
@@ -1273,11 +1265,9 @@ hipError_t fused_launch(const SpoPassPlan &p, int nsteps, long long step0, hipSt
     f.step0 = step0;
     f.x = p.x;
     f.psi0 = p.psi0;
-    f.vpot = p.vpot;
-    f.vph = static_cast<const double2 *>(p.vph);
+    f.vph = fused_table(p);
     f.cbuf = p.cbuf;
     f.b = p.b;
-    f.cst2 = p.cst2;
     for (int a = 0; a < 2; ++a) {
         f.twr[a] = static_cast<const double2 *>(p.twr) + p.twr_off[a];
         f.kreg[a] = static_cast<const double2 *>(p.kreg) + p.kax_off[a];
@@ -1285,13 +1275,12 @@ hipError_t fused_launch(const SpoPassPlan &p, int nsteps, long long step0, hipSt
         f.kreg_s[a] = static_cast<const float4 *>(p.kreg_s) + p.kax_off[a];
     }
     const size_t lds = fused_lds(p);
-    const bool pht = p.vph != nullptr;
     const int nw = p.shape.fused;
     const dim3 grid(p.npk);
-#define ESPEC_FUSED2(M0_, M1_, NW_)                                                              \
-    if (p.np[0] == 16 * M0_ && p.np[1] == 16 * M1_ && nw == NW_)                                  \
-        return pht ? launch(pass_fused2d<R, M0_, M1_, NW_, true>, grid, 64 * NW_, lds, s, f)       \
-                   : launch(pass_fused2d<R, M0_, M1_, NW_, false>, grid, 64 * NW_, lds, s, f);
+    if (!f.vph) return hipErrorInvalidValue;
+#define ESPEC_FUSED2(M0_, M1_, NW_)                                     \
+    if (p.np[0] == 16 * M0_ && p.np[1] == 16 * M1_ && nw == NW_)         \
+        return launch(pass_fused2d<R, M0_, M1_, NW_>, grid, 64 * NW_, lds, s, f);
     ESPEC_FUSED2(4, 4, 4)
     ESPEC_FUSED2(4, 4, 8)
     if constexpr (std::is_same<R, float>::value) {
@@ -1841,11 +1830,12 @@ int profile_typed(SpoPassPlan &p, int nsteps, long long step0, hipStream_t s, do
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   const void *vph, const double *mask, double *cbuf, int maxsamp,
-                   double phase_max, int tune, hipStream_t s)
+                   const void *vph, const void *vph_fz, const double *mask, double *cbuf,
+                   int maxsamp, double phase_max, int tune, hipStream_t s)
 {
     p.phase_max = phase_max;
     p.mask = mask;
+    p.vph_fz = vph_fz;
     // taxes holds ndim kinetic axes for one shared set of masses, or
     // ndim per packet (packet-major) when the masses differ per packet.
     const int nset = (int)taxes.size() / ndim;
@@ -2062,6 +2052,13 @@ int spo_pass_set_shape(SpoPassPlan &p, const PassShape &s)
     p.shape = s;
     apply_shape(p);
     return ESPEC_OK;
+}
+
+bool spo_pass_fused_grid(int ndim, const int *np, bool mixed)
+{
+    if (ndim != 2) return false;
+    if (np[0] == 64 && np[1] == 64) return true;
+    return mixed && np[0] == 128 && np[1] == 128;
 }
 
 void spo_pass_release(SpoPassPlan &p)

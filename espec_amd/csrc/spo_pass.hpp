@@ -36,6 +36,8 @@ struct SpoPassPlan {
     void *x = nullptr;
     const void *psi0 = nullptr, *vpot = nullptr;  // storage precision
     const void *vph = nullptr;  // FP64 V phase table, or null for V + sincos
+    const void *vph_fz = nullptr;  // the same table for the fused form only, where the
+                                   // multi-pass kernels form the phase from V
     const double *mask = nullptr;  // .SMOOTHW mask [npk][n] (forces the LDS-stage row pass)
     double *cbuf = nullptr;
     void *tw = nullptr;    // double2 twiddles, axis a at tw_off[a], ntw[a] entries
@@ -64,8 +66,10 @@ struct SpoPassPlan {
 int spo_pass_setup(SpoPassPlan &p, int ndim, const int *np, int npk, bool mixed,
                    const std::vector<FftPlan> &fp, const std::vector<std::vector<double>> &taxes,
                    double cst1, double cst2, void *x, const void *psi0, const void *vpot,
-                   const void *vph, const double *mask, double *cbuf, int maxsamp,
-                   double phase_max, int tune, hipStream_t s);
+                   const void *vph, const void *vph_fz, const double *mask, double *cbuf,
+                   int maxsamp, double phase_max, int tune, hipStream_t s);
+// True if the grid has an LDS-resident fused form (PassShape::fused).
+bool spo_pass_fused_grid(int ndim, const int *np, bool mixed);
 int spo_pass_run(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStream_t s,
                  int *launches);
 // spo_pass_run's kernels launched one by one on s, for capture into a

@@ -96,7 +96,11 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         // .VOPTIC always uses the table: its damping rides on it.
         const bool table = !P.mixed || P.absorb == ESPEC_ABSORB_VOPTIC ||
                            (size_t)n * npk * sizeof(double2) <= (size_t)64 << 20;
-        if (table) {
+        // The LDS-resident fused form moves no state through HBM within a
+        // launch, so there a table load beats the sincos at any batch size:
+        // grids with that form get a table for it alone.
+        const bool fz_table = !table && spo_pass_fused_grid(P.ndim, P.np, P.mixed);
+        if (table || fz_table) {
             std::vector<double> vph = vphase();
             ESPEC_HIP_TRY(P.vph.upload(vph.data(), sizeof(double) * vph.size()));
         }
@@ -108,7 +112,8 @@ int build_tables(espec_spo_plan &P, const double *sh, const double *sm, const do
         double vmax = 0;
         for (size_t i = 0; i < (size_t)n * npk; ++i) vmax = std::max(vmax, std::fabs(vpot[i]));
         return spo_pass_setup(P.pp, P.ndim, P.np, npk, P.mixed, fp, taxes, cst1, cst2, P.x.p,
-                              P.psi0.p, P.vst.p, table ? P.vph.p : nullptr, P.mask.as<double>(),
+                              P.psi0.p, P.vst.p, table ? P.vph.p : nullptr,
+                              fz_table ? P.vph.p : nullptr, P.mask.as<double>(),
                               P.cbuf.as<double>(), P.maxsamp, std::fabs(cst2) * vmax, 1,
                               P.stream);
     }
@@ -370,7 +375,7 @@ int espec_spo_plan_shape(espec_spo_plan *P, int *shape)
     shape[8] = s.rows_pen;
     shape[9] = s.tile_pen;
     shape[10] = s.pen_pf;
-    shape[11] = P->lds ? 0 : (P->pp.vph != nullptr);
+    shape[11] = P->lds ? 0 : (P->pp.vph != nullptr || (s.fused && P->pp.vph_fz != nullptr));
     shape[12] = P->lds ? 0 : s.fused;
     return ESPEC_OK;
 }
