@@ -2011,7 +2011,7 @@ int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStr
                    : profile_typed<double>(p, nsteps, step0, s, ms, bytes);
 }
 
-int spo_pass_prepare(SpoPassPlan &p, int lc)
+int spo_pass_prepare(SpoPassPlan &p, int lc, hipStream_t s, bool prime)
 {
     if (lc != p.lc) drop_graph(p);
     p.lc = lc;
@@ -2019,6 +2019,14 @@ int spo_pass_prepare(SpoPassPlan &p, int lc)
     for (int j = 0; j < SpoPassPlan::kGraphLevels; ++j) {
         const int rc = p.mixed ? capture_typed<float>(p, j) : capture_typed<double>(p, j);
         if (rc) return rc;
+    }
+    if (prime) {
+        // 63 steps in all; the row passes write only row-block partials,
+        // which a run rewrites before it reduces them.
+        for (int j = 0; j < SpoPassPlan::kGraphLevels; ++j) ESPEC_HIP_TRY(hipGraphLaunch(p.gexec[j], s));
+        const size_t bytes = (size_t)p.n * p.npk * (p.mixed ? sizeof(float2) : sizeof(double2));
+        ESPEC_HIP_TRY(hipMemcpyAsync(p.x, p.psi0, bytes, hipMemcpyDeviceToDevice, s));
+        ESPEC_HIP_TRY(hipStreamSynchronize(s));
     }
     return ESPEC_OK;
 }

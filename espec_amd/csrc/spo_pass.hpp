@@ -80,8 +80,11 @@ int spo_pass_profile(SpoPassPlan &p, int nsteps, long long step0, int lc, hipStr
 void spo_pass_release(SpoPassPlan &p);
 // Replace the tuned launch shape (ESPEC_ERR_ARG if the grid cannot use it).
 int spo_pass_set_shape(SpoPassPlan &p, const PassShape &s);
-// Capture the per-step launch graph for sampling interval lc ahead of the
-// first run (a run captures it itself when needed).
-int spo_pass_prepare(SpoPassPlan &p, int lc);
+// Capture the per-step launch graphs for sampling interval lc ahead of the
+// first run (a run captures them itself when needed).  With prime (only
+// while psi equals psi0), replay each once on s and restore psi from
+// psi0: a graph's first replay costs more than the next ones, and a short
+// timed run should not pay for it.
+int spo_pass_prepare(SpoPassPlan &p, int lc, hipStream_t s, bool prime);
 
 }  // namespace espec

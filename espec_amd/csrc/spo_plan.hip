@@ -231,7 +231,7 @@ int espec_spo_plan_create_absorb(espec_spo_plan **out, int ndim, const int *np, 
                       : build_tables<double>(*P, sh, sm, vpot, u, v, vabc);
     if (rc) return rc;
     if (!P->lds) {
-        rc = spo_pass_prepare(P->pp, lc);
+        rc = spo_pass_prepare(P->pp, lc, P->stream, true);
         if (rc) return rc;
     }
     SpoLdsArgs &a = P->la;
@@ -397,7 +397,7 @@ int espec_spo_plan_set_shape(espec_spo_plan *P, const int *shape)
     s.pen_pf = shape[10];
     s.fused = shape[12];
     const int rc = spo_pass_set_shape(P->pp, s);
-    return rc ? rc : spo_pass_prepare(P->pp, P->lc);
+    return rc ? rc : spo_pass_prepare(P->pp, P->lc, P->stream, P->steps_done == 0);
 }
 
 int espec_spo_plan_profile(espec_spo_plan *P, int nsteps, double *ms, double *bytes, int maxkinds,
