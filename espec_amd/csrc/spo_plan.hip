@@ -210,7 +210,9 @@ int espec_spo_plan_create_absorb(espec_spo_plan **out, int ndim, const int *np, 
     P->absorb = absorb;
     if (P->ref && P->mixed) return ESPEC_ERR_UNSUPPORTED;
     P->b = 1.0 / P->n;
-    P->nt = P->n <= 1024 ? 64 : 256;
+    // One wavefront per packet up to 256 points; four above (a 1024-point
+    // 1-D batch of 260 packets: 15.4 -> 7.9 us per step).
+    P->nt = P->n <= 256 ? 64 : 256;
     // LDS-resident when the FP64 grid fits in one workgroup's LDS; the V
     // phase table is staged next to it when that fits too.
     const size_t lds_cap = 160 * 1024 - 1024;
