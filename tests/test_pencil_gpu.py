@@ -158,3 +158,28 @@ def test_batch_per_packet_masses(gpu, orc, shape, flags, tol):
                         p.vpot, EXACT)
         assert dC(plan.get(k), o) <= tol, k
     plan.close()
+
+
+def test_shape_change_mid_run_keeps_the_state(gpu):
+    """A plan's step graphs are replayed once when they are captured, and
+    psi restored from psi0, only while the plan is at step 0: pinning a
+    shape after 7 steps must continue the same propagation (bit for bit
+    against a plan pinned to that shape from the start)."""
+    c = configs.config2()
+    nst = 20
+    print("case: config2 mixed shape change mid-run", flush=True)
+    mk = lambda: E.SpoPlan(c.np, c.dt_fs, 1, c.sh, c.sm, c.vpot[None], c.u[None], c.v[None],
+                           maxsamp=nst + 1, flags=E.MIXED | E.EXACT)
+    a, b = mk(), mk()
+    pinned(a, "pen_tile", c.np[0])
+    a.run(7)
+    a.run(nst - 7)
+    pinned(b, "pen_tile", c.np[0])
+    b.run(7)
+    pinned(b, "pen_tile", c.np[0])  # captures the step graphs again, at step 7
+    b.run(nst - 7)
+    ga, gb = a.get(0), b.get(0)
+    for key in ("cr", "ci", "u", "v"):
+        assert np.array_equal(ga[key], gb[key]), key
+    a.close()
+    b.close()
