@@ -607,7 +607,9 @@ int espec_sod_plan_create_absorb(espec_sod_plan **out, int ndim, const int *np, 
     // ESPEC_REF: the reference's .SMOOTHW branch is unreachable (sodfft.f:108).
     P->absorb = (P->ref && absorb == ESPEC_ABSORB_SMOOTHW) ? ESPEC_ABSORB_NONE : absorb;
     if (P->ref && P->mixed) return ESPEC_ERR_UNSUPPORTED;
-    P->nt = P->n <= 1024 ? 64 : 256;
+    // One wavefront per packet up to 256 points; four above (a 1024-point
+    // 1-D batch of 260 packets: 22.1 -> 11.1 us per step).
+    P->nt = P->n <= 256 ? 64 : 256;
     P->lds = P->n <= kSodLdsMax;
     if (!P->lds && P->ref) return ESPEC_ERR_UNSUPPORTED;
     P->la.cst = 2.0 * dt / kShfs;  // psodfft.f:80
