@@ -552,7 +552,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        # transforms in FP32 with split constants (FP64 on the LDS-resident
+        # 1-D kernels), V phase, 1/N and the C(t) and norm sums in FP64
+        "dtype": "f32+f64",
         "storage": "complex64",
         "data": "synthetic",
         "config": {"workload": "2048 uneven packets (1-D N 128..1024, 2-D 64^2..256^2, seed 4), "
