@@ -52,9 +52,15 @@ class SpoBatch:
         for i, p in enumerate(packets):
             key = (tuple(p.np), tuple(np.round(p.sh, 15)), p.dt_fs)
             groups.setdefault(key, []).append(i)
+        # Plans (and their streams) are created smallest first, the largest
+        # last: measured on the 2048-packet workload, a first 20-step run
+        # took 0.31 ms per step in this order against 0.32-0.33 ms in order
+        # of appearance and 0.36 ms largest first.
+        order = sorted(groups, key=lambda k: (len(groups[k]) * int(np.prod(k[0])), k))
         self.groups = []
         self.where = {}
-        for key, idx in groups.items():
+        for key in order:
+            idx = groups[key]
             p0 = packets[idx[0]]
             n = int(np.prod(p0.np))
             ms = maxsamp if maxsamp is not None else p0.nsteps // lc + 1
